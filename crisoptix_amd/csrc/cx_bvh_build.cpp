@@ -1,0 +1,229 @@
+// Binned-SAH binary BVH build on the host (once per scene; replaces the triangle GAS and the
+// instance IAS builds of src/Mesh.cpp:54-135, src/Model.cpp:376-457, src/Scene.cpp:73-174).
+//
+// 32 centroid bins per axis, leaves of at most `maxLeaf` triangles.  Below depth 24 the
+// split switches to the object median of the widest centroid axis, which bounds the tree
+// depth by 24 + log2(n) and so the traversal stack (STACK_DEPTH) for any n < 2^24.
+// Child boxes are padded outward by |coord| * 2^-15 so the device slab test stays
+// conservative for every triangle the watertight test can report.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "cx_internal.h"
+
+namespace cx {
+
+namespace {
+
+struct Box {
+    float lo[3], hi[3];
+    void reset() {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = INFINITY;
+            hi[a] = -INFINITY;
+        }
+    }
+    void grow(const float* p) {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], p[a]);
+            hi[a] = std::max(hi[a], p[a]);
+        }
+    }
+    void grow(const Box& b) {
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], b.lo[a]);
+            hi[a] = std::max(hi[a], b.hi[a]);
+        }
+    }
+    float area() const {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0.0f)) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct Builder {
+    const float* pos;
+    const uint32_t* idx;
+    uint32_t maxLeaf;
+    std::vector<Box> triBox;
+    std::vector<float> cent;   // [T][3]
+    std::vector<uint32_t> order;
+    BvhHost* out;
+    uint32_t maxDepth = 0;
+
+    Box range_box(uint32_t b, uint32_t e) const {
+        Box bx;
+        bx.reset();
+        for (uint32_t i = b; i < e; i++) bx.grow(triBox[order[i]]);
+        return bx;
+    }
+
+    static void pad(Box& b) {
+        for (int a = 0; a < 3; a++) {
+            const float m = std::max(std::fabs(b.lo[a]), std::fabs(b.hi[a]));
+            const float e = m * 3.0517578125e-5f + 1e-20f;
+            b.lo[a] -= e;
+            b.hi[a] += e;
+        }
+    }
+
+    int32_t emit_leaf(uint32_t b, uint32_t e) {
+        const uint32_t first = (uint32_t)(out->tris.size() / 3);
+        for (uint32_t i = b; i < e; i++) {
+            const uint32_t t = order[i];
+            for (int k = 0; k < 3; k++) {
+                const float* v = pos + 3 * (size_t)idx[3 * (size_t)t + k];
+                float w = 0.0f;
+                if (k == 0) std::memcpy(&w, &t, 4);
+                out->tris.push_back(make_float4(v[0], v[1], v[2], w));
+            }
+        }
+        const uint32_t code = (first << 4) | (e - b);
+        return (int32_t)~code;
+    }
+
+    // Chooses a split of [b, e); returns mid or 0 for "make a leaf".
+    uint32_t split(uint32_t b, uint32_t e, int depth) {
+        const uint32_t n = e - b;
+        Box cb;
+        cb.reset();
+        for (uint32_t i = b; i < e; i++) cb.grow(&cent[3 * (size_t)order[i]]);
+        int axis = 0;
+        float ext = cb.hi[0] - cb.lo[0];
+        for (int a = 1; a < 3; a++)
+            if (cb.hi[a] - cb.lo[a] > ext) {
+                ext = cb.hi[a] - cb.lo[a];
+                axis = a;
+            }
+        auto median = [&](int ax) {
+            const uint32_t mid = b + n / 2;
+            std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e, [&](uint32_t x, uint32_t y) {
+                const float cx = cent[3 * (size_t)x + ax], cy = cent[3 * (size_t)y + ax];
+                return cx < cy || (cx == cy && x < y);
+            });
+            return mid;
+        };
+        if (!(ext > 0.0f)) return n <= 15 ? 0 : b + n / 2;   // coincident centroids: split by index
+        if (depth >= 24) return median(axis);
+        constexpr int NB = 32;
+        float bestCost = INFINITY;
+        int bestAxis = -1, bestBin = -1;
+        for (int a = 0; a < 3; a++) {
+            const float lo = cb.lo[a], w = cb.hi[a] - cb.lo[a];
+            if (!(w > 0.0f)) continue;
+            Box bins[NB];
+            uint32_t cnt[NB] = {0};
+            for (int k = 0; k < NB; k++) bins[k].reset();
+            const float sc = NB / w;
+            for (uint32_t i = b; i < e; i++) {
+                const uint32_t t = order[i];
+                int k = (int)((cent[3 * (size_t)t + a] - lo) * sc);
+                k = std::min(NB - 1, std::max(0, k));
+                bins[k].grow(triBox[t]);
+                cnt[k]++;
+            }
+            float rA[NB];
+            uint32_t rC[NB];
+            Box acc;
+            acc.reset();
+            uint32_t c = 0;
+            for (int k = NB - 1; k > 0; k--) {
+                acc.grow(bins[k]);
+                c += cnt[k];
+                rA[k] = acc.area();
+                rC[k] = c;
+            }
+            acc.reset();
+            c = 0;
+            for (int k = 0; k < NB - 1; k++) {
+                acc.grow(bins[k]);
+                c += cnt[k];
+                if (c == 0 || rC[k + 1] == 0) continue;
+                const float cost = acc.area() * c + rA[k + 1] * rC[k + 1];
+                if (cost < bestCost) {
+                    bestCost = cost;
+                    bestAxis = a;
+                    bestBin = k;
+                }
+            }
+        }
+        const Box nb = range_box(b, e);
+        const float leafCost = nb.area() * n;
+        if (n <= maxLeaf && !(bestCost < leafCost - 2.0f * nb.area())) return 0;
+        if (bestAxis < 0) return median(axis);
+        const float lo = cb.lo[bestAxis], sc = NB / (cb.hi[bestAxis] - cb.lo[bestAxis]);
+        auto it = std::partition(order.begin() + b, order.begin() + e, [&](uint32_t t) {
+            int k = (int)((cent[3 * (size_t)t + bestAxis] - lo) * sc);
+            k = std::min(NB - 1, std::max(0, k));
+            return k <= bestBin;
+        });
+        const uint32_t mid = (uint32_t)(it - order.begin());
+        if (mid == b || mid == e) return median(axis);
+        return mid;
+    }
+
+    // Returns the reference of the subtree over [b, e).
+    int32_t build(uint32_t b, uint32_t e, int depth) {
+        maxDepth = std::max(maxDepth, (uint32_t)depth);
+        const uint32_t mid = (e - b <= 1) ? 0 : split(b, e, depth);
+        if (mid == 0) return emit_leaf(b, e);
+        const uint32_t node = (uint32_t)(out->nodes.size() / 4);
+        out->nodes.resize(out->nodes.size() + 4);
+        const int32_t l = build(b, mid, depth + 1);
+        const int32_t r = build(mid, e, depth + 1);
+        Box lb = range_box(b, mid), rb = range_box(mid, e);
+        pad(lb);
+        pad(rb);
+        float4* n = &out->nodes[4 * (size_t)node];
+        n[0] = make_float4(lb.lo[0], lb.hi[0], lb.lo[1], lb.hi[1]);
+        n[1] = make_float4(rb.lo[0], rb.hi[0], rb.lo[1], rb.hi[1]);
+        n[2] = make_float4(lb.lo[2], lb.hi[2], rb.lo[2], rb.hi[2]);
+        int32_t refs[4] = {l, r, 0, 0};
+        std::memcpy(&n[3], refs, 16);
+        return (int32_t)node;
+    }
+};
+
+}  // namespace
+
+bool build_bvh(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t maxLeaf, BvhHost& out,
+               std::string& err) {
+    out.nodes.clear();
+    out.tris.clear();
+    out.root = 0;
+    out.maxDepth = 0;
+    if (numTris == 0) return true;
+    if (numTris >= (1u << 27)) {
+        err = "scene has too many triangles for the leaf encoding (limit 2^27)";
+        return false;
+    }
+    Builder B;
+    B.pos = pos;
+    B.idx = idx;
+    B.maxLeaf = std::max(1u, std::min(15u, maxLeaf));
+    B.out = &out;
+    B.triBox.resize(numTris);
+    B.cent.resize(3 * (size_t)numTris);
+    B.order.resize(numTris);
+    out.nodes.reserve(8 * (size_t)numTris / std::max(1u, B.maxLeaf));
+    out.tris.reserve(3 * (size_t)numTris);
+    for (uint32_t t = 0; t < numTris; t++) {
+        B.order[t] = t;
+        Box bx;
+        bx.reset();
+        for (int k = 0; k < 3; k++) bx.grow(pos + 3 * (size_t)idx[3 * (size_t)t + k]);
+        B.triBox[t] = bx;
+        for (int a = 0; a < 3; a++) B.cent[3 * (size_t)t + a] = 0.5f * (bx.lo[a] + bx.hi[a]);
+    }
+    out.root = B.build(0, numTris, 0);
+    out.maxDepth = B.maxDepth;
+    if (out.maxDepth + 1 >= (uint32_t)STACK_DEPTH) {
+        err = "BVH deeper than the traversal stack";
+        return false;
+    }
+    return true;
+}
+
+}  // namespace cx

@@ -1,0 +1,713 @@
+// C ABI of the frame loop (include/crisoptix.h): context, scene upload + BVH, the prefix-entry
+// search structure, and the renderer that runs Renderer::LaunchFrame's phase order
+// (src/Renderer.cpp:209-406) as HIP kernels on the caller's stream with no device sync.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "cx_internal.h"
+
+using namespace cx;
+
+struct cx_ctx {
+    int device = 0;
+    std::string err;
+};
+
+struct cx_scene {
+    cx_ctx* ctx = nullptr;
+    DScene d{};
+    std::vector<void*> allocs;
+    cx_scene_info info{};
+};
+
+struct cx_entries {
+    cx_ctx* ctx = nullptr;
+    uint32_t capacity = 0;
+    EntryBuild E{};
+    EntryTree T{};
+    void* sortTemp = nullptr;
+    size_t sortTempBytes = 0;
+    std::vector<void*> allocs;
+};
+
+struct cx_renderer {
+    cx_ctx* ctx = nullptr;
+    const cx_scene* scene = nullptr;
+    uint32_t W = 0, H = 0, P = 0;
+    cx_launch_params params{};
+    uint64_t frameRngState = 0;
+    uint64_t frameIdx = 0;
+    uint32_t front = 0, back = 0;
+    uint4* prefixRes[2] = {nullptr, nullptr};
+    uint4* suffixRes[2] = {nullptr, nullptr};
+    uint4* gbuf = nullptr;
+    uint32_t* neighIdx = nullptr;
+    float* neighDist = nullptr;
+    uint32_t* neighCount = nullptr;
+    uint32_t kCap = 0;
+    unsigned long long* counters = nullptr;
+    cx_entries* entries = nullptr;
+    cx_camera cam{};
+    float* out = nullptr;
+    bool timing = false;
+    hipEvent_t ev[CX_PASS_COUNT + 1] = {};
+    bool evMade = false;
+    bool evValid = false;
+    bool passRan[CX_PASS_COUNT] = {};
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int set_err(cx_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_err(cx_ctx* ctx, hipError_t e, const char* what) {
+    return set_err(ctx, e == hipErrorOutOfMemory ? CX_ERR_OUT_OF_MEMORY : CX_ERR_HIP,
+                   std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define CX_HIP(ctx, call)                                  \
+    do {                                                   \
+        hipError_t _e = (call);                            \
+        if (_e != hipSuccess) return hip_err(ctx, _e, #call); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(std::vector<void*>& allocs, T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return e;
+    allocs.push_back(q);
+    *p = (T*)q;
+    return hipSuccess;
+}
+
+void free_all(std::vector<void*>& allocs) {
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+}
+
+// PCG32 step and seeding on the host (include/util/random.h:52-68), for the per-frame random.
+uint32_t pcg_next(uint64_t& s) {
+    const uint64_t old = s;
+    s = old * PCG_MULT + PCG_INC;
+    const uint32_t xs = (uint32_t)(((old >> 18) ^ old) >> 27);
+    const uint32_t rot = (uint32_t)(old >> 59);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+
+uint64_t pcg_seed(uint64_t initstate) {
+    uint64_t s = 0;
+    pcg_next(s);
+    s += initstate;
+    pcg_next(s);
+    return s;
+}
+
+uint32_t tree_levels(uint32_t capacity, uint32_t leafSize) {
+    const uint32_t leaves = std::max(1u, (capacity + leafSize - 1) / leafSize);
+    uint32_t l = 0;
+    while ((1u << l) < leaves) l++;
+    return l;
+}
+
+int entries_alloc(cx_ctx* ctx, uint32_t capacity, cx_entries** out) {
+    cx_entries* en = new (std::nothrow) cx_entries();
+    if (!en) return set_err(ctx, CX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    en->ctx = ctx;
+    en->capacity = capacity;
+    const uint32_t n = std::max(1u, capacity);
+    en->T.leafSize = 8;
+    en->T.levels = tree_levels(n, en->T.leafSize);
+    const size_t nodes = (size_t(2) << en->T.levels) - 1;
+    hipError_t e = hipSuccess;
+    auto A = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    A(dalloc(en->allocs, &en->E.pos, n));
+    A(dalloc(en->allocs, &en->E.valid, n));
+    A(dalloc(en->allocs, &en->E.keys, n));
+    A(dalloc(en->allocs, &en->E.vals, n));
+    A(dalloc(en->allocs, &en->E.keysAlt, n));
+    A(dalloc(en->allocs, &en->E.valsAlt, n));
+    A(dalloc(en->allocs, &en->E.numValid, 1));
+    A(dalloc(en->allocs, &en->T.entrySorted, n));
+    A(dalloc(en->allocs, &en->T.entryPixel, n));
+    A(dalloc(en->allocs, &en->T.lo, nodes));
+    A(dalloc(en->allocs, &en->T.hi, nodes));
+    en->T.numValid = en->E.numValid;
+    en->sortTempBytes = sort_temp_bytes(n);
+    if (e == hipSuccess) {
+        e = hipMalloc(&en->sortTemp, std::max<size_t>(en->sortTempBytes, 16));
+        if (e == hipSuccess) en->allocs.push_back(en->sortTemp);
+    }
+    if (e != hipSuccess) {
+        free_all(en->allocs);
+        delete en;
+        return hip_err(ctx, e, "entries allocation");
+    }
+    *out = en;
+    return CX_OK;
+}
+
+void set_quantisation(EntryBuild& E, const float* mn, const float* mx) {
+    E.lo = V3{mn[0], mn[1], mn[2]};
+    float sc[3];
+    for (int a = 0; a < 3; a++) {
+        const float w = mx[a] - mn[a];
+        sc[a] = w > 0.0f ? 1023.0f / w : 0.0f;
+    }
+    E.scale = V3{sc[0], sc[1], sc[2]};
+}
+
+FrameArgs make_args(cx_renderer* r) {
+    FrameArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.S = r->scene->d;
+    A.lp = r->params;
+    A.camPos = V3{r->cam.pos[0], r->cam.pos[1], r->cam.pos[2]};
+    A.camU = V3{r->cam.U[0], r->cam.U[1], r->cam.U[2]};
+    A.camV = V3{r->cam.V[0], r->cam.V[1], r->cam.V[2]};
+    A.camW = V3{r->cam.W[0], r->cam.W[1], r->cam.W[2]};
+    A.W = r->W;
+    A.H = r->H;
+    A.P = r->P;
+    A.front = r->front;
+    A.back = r->back;
+    const cx_launch_params& lp = r->params;
+    A.copyPrefixToBack = ((lp.rendererType == CX_RESTIR_PT && lp.prefixEnableSpatial) ||
+                          (lp.rendererType == CX_CONDITIONAL_RESTIR && lp.suffixEnableSpatial))
+                             ? 1u
+                             : 0u;
+    for (int b = 0; b < 2; b++) {
+        A.prefixRes[b] = r->prefixRes[b];
+        A.suffixRes[b] = r->suffixRes[b];
+    }
+    A.gbuf = r->gbuf;
+    A.out = r->out;
+    A.tree = r->entries->T;
+    A.neighIdx = r->neighIdx;
+    A.neighDist = r->neighDist;
+    A.neighCount = r->neighCount;
+    A.k = lp.gatherM > 1 ? (uint32_t)(lp.gatherM - 1) : 0u;
+    A.counters = r->counters;
+    return A;
+}
+
+int check_params(cx_renderer* r, const cx_launch_params& p) {
+    if (p.rendererType < 0 || p.rendererType > 2) return set_err(r->ctx, CX_ERR_INVALID_ARG, "bad rendererType");
+    if (p.prefixSpatialCount < 0 || p.prefixSpatialCount > 16 || p.suffixSpatialCount < 0 ||
+        p.suffixSpatialCount > 16)
+        return set_err(r->ctx, CX_ERR_INVALID_ARG, "spatial neighbour count must be in [0, 16] (settings.h:24)");
+    if (p.gatherM < 1 || p.gatherM > 32)
+        return set_err(r->ctx, CX_ERR_INVALID_ARG, "gatherM must be in [1, 32] (Renderer.cpp:462)");
+    if (p.gatherN < 1) return set_err(r->ctx, CX_ERR_INVALID_ARG, "gatherN must be >= 1");
+    if (p.maxPathLen < 0 || p.prefixLen < 0 || p.maxPathLen > 255)
+        return set_err(r->ctx, CX_ERR_INVALID_ARG, "bad path length");
+    if (p.windowRadius > 1024) return set_err(r->ctx, CX_ERR_INVALID_ARG, "windowRadius too large");
+    return CX_OK;
+}
+
+int ensure_neighbors(cx_renderer* r, uint32_t k) {
+    if (k <= r->kCap) return CX_OK;
+    if (r->neighIdx) {
+        (void)hipFree(r->neighIdx);
+        (void)hipFree(r->neighDist);
+        r->allocs.erase(std::remove(r->allocs.begin(), r->allocs.end(), (void*)r->neighIdx), r->allocs.end());
+        r->allocs.erase(std::remove(r->allocs.begin(), r->allocs.end(), (void*)r->neighDist), r->allocs.end());
+        r->neighIdx = nullptr;
+        r->neighDist = nullptr;
+    }
+    CX_HIP(r->ctx, dalloc(r->allocs, &r->neighIdx, (size_t)r->P * k));
+    CX_HIP(r->ctx, dalloc(r->allocs, &r->neighDist, (size_t)r->P * k));
+    r->kCap = k;
+    return CX_OK;
+}
+
+void rec(cx_renderer* r, int i, hipStream_t s) {
+    if (r->timing) (void)hipEventRecord(r->ev[i], s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cx_abi_version(void) { return CX_ABI_VERSION; }
+
+int cx_ctx_create(int device, cx_ctx** out) {
+    if (!out) return CX_ERR_INVALID_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CX_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return CX_ERR_INVALID_ARG;
+    if (hipSetDevice(device) != hipSuccess) return CX_ERR_HIP;
+    cx_ctx* c = new (std::nothrow) cx_ctx();
+    if (!c) return CX_ERR_OUT_OF_MEMORY;
+    c->device = device;
+    *out = c;
+    return CX_OK;
+}
+
+int cx_ctx_destroy(cx_ctx* ctx) {
+    delete ctx;
+    return CX_OK;
+}
+
+const char* cx_last_error(const cx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
+    if (!ctx || !d || !out) return set_err(ctx, CX_ERR_INVALID_ARG, "null argument");
+    if (d->numTriangles > 0 && (!d->positions || !d->normals || !d->indices || !d->triMaterial))
+        return set_err(ctx, CX_ERR_INVALID_ARG, "missing geometry arrays");
+    if (d->numEmitters == 0 || !d->emitterFaces || !d->emitterFaceOffset || !d->emitterColor)
+        return set_err(ctx, CX_ERR_INVALID_ARG, "at least one emitter is required (sample_emitter.h:5-6)");
+    if (d->numMaterials == 0 || !d->materials) return set_err(ctx, CX_ERR_INVALID_ARG, "no materials");
+    for (uint32_t t = 0; t < d->numTriangles; t++) {
+        if (d->triMaterial[t] >= d->numMaterials) return set_err(ctx, CX_ERR_INVALID_ARG, "material id out of range");
+        for (int k = 0; k < 3; k++)
+            if (d->indices[3 * (size_t)t + k] >= d->numVertices)
+                return set_err(ctx, CX_ERR_INVALID_ARG, "vertex index out of range");
+    }
+    const uint32_t numFaces = d->emitterFaceOffset[d->numEmitters];
+    for (uint32_t e = 0; e < d->numEmitters; e++)
+        if (d->emitterFaceOffset[e + 1] <= d->emitterFaceOffset[e])
+            return set_err(ctx, CX_ERR_INVALID_ARG, "every emitter needs at least one face");
+    BvhHost bvh;
+    std::string err;
+    if (!build_bvh(d->positions, d->indices, d->numTriangles, d->bvhMaxLeafSize ? d->bvhMaxLeafSize : 4, bvh, err))
+        return set_err(ctx, CX_ERR_BUILD, err);
+    cx_scene* s = new (std::nothrow) cx_scene();
+    if (!s) return set_err(ctx, CX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    s->ctx = ctx;
+    hipError_t e = hipSuccess;
+    auto up = [&](const void* src, size_t count, size_t elem) -> void* {
+        if (e != hipSuccess) return nullptr;
+        void* q = nullptr;
+        e = hipMalloc(&q, std::max<size_t>(count * elem, 16));
+        if (e != hipSuccess) return nullptr;
+        s->allocs.push_back(q);
+        if (count) e = hipMemcpy(q, src, count * elem, hipMemcpyHostToDevice);
+        return q;
+    };
+    std::vector<DMaterial> mats(d->numMaterials);
+    for (uint32_t m = 0; m < d->numMaterials; m++) {
+        const cx_material& cm = d->materials[m];
+        mats[m].diffRough = make_float4(cm.diffColor[0], cm.diffColor[1], cm.diffColor[2], cm.roughness);
+        mats[m].spec = make_float4(cm.specF0[0], cm.specF0[1], cm.specF0[2], 0.0f);
+        mats[m].emissive = make_float4(cm.emissiveColor[0], cm.emissiveColor[1], cm.emissiveColor[2], 0.0f);
+    }
+    const float4* nodesSrc = bvh.nodes.empty() ? nullptr : bvh.nodes.data();
+    s->d.nodes = (const float4*)up(nodesSrc, bvh.nodes.size(), sizeof(float4));
+    s->d.tris = (const float4*)up(bvh.tris.data(), bvh.tris.size(), sizeof(float4));
+    s->d.pos = (const float*)up(d->positions, 3 * (size_t)d->numVertices, sizeof(float));
+    s->d.nrm = (const float*)up(d->normals, 3 * (size_t)d->numVertices, sizeof(float));
+    s->d.idx = (const uint32_t*)up(d->indices, 3 * (size_t)d->numTriangles, sizeof(uint32_t));
+    s->d.triMat = (const uint32_t*)up(d->triMaterial, d->numTriangles, sizeof(uint32_t));
+    s->d.mats = (const DMaterial*)up(mats.data(), mats.size(), sizeof(DMaterial));
+    s->d.emitFaces = (const float*)up(d->emitterFaces, 9 * (size_t)numFaces, sizeof(float));
+    s->d.emitOff = (const uint32_t*)up(d->emitterFaceOffset, d->numEmitters + 1, sizeof(uint32_t));
+    s->d.emitColor = (const float*)up(d->emitterColor, 3 * (size_t)d->numEmitters, sizeof(float));
+    if (e != hipSuccess) {
+        free_all(s->allocs);
+        delete s;
+        return hip_err(ctx, e, "scene upload");
+    }
+    s->d.numTris = d->numTriangles;
+    s->d.numEmitters = d->numEmitters;
+    s->d.root = bvh.root;
+    // Scene::GetSceneSize (src/Scene.cpp:48-71): bbox of all vertices and the origin.
+    float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t v = 0; v < d->numVertices; v++)
+        for (int a = 0; a < 3; a++) {
+            const float c = d->positions[3 * (size_t)v + a];
+            mn[a] = (c < mn[a]) ? c : mn[a];
+            mx[a] = (mx[a] < c) ? c : mx[a];
+        }
+    const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    s->info.sceneSize = std::sqrt(dx * dx + dy * dy + dz * dz);
+    for (int a = 0; a < 3; a++) {
+        s->info.boundsMin[a] = mn[a];
+        s->info.boundsMax[a] = mx[a];
+    }
+    s->info.numTriangles = d->numTriangles;
+    s->info.numVertices = d->numVertices;
+    s->info.numNodes = (uint32_t)(bvh.nodes.size() / 4);
+    s->info.maxDepth = bvh.maxDepth;
+    s->info.numEmitters = d->numEmitters;
+    s->info.numEmitterFaces = numFaces;
+    s->info.deviceBytes = (bvh.nodes.size() + bvh.tris.size()) * sizeof(float4) +
+                          (size_t)d->numVertices * 24 + (size_t)d->numTriangles * 16 + mats.size() * sizeof(DMaterial);
+    *out = s;
+    return CX_OK;
+}
+
+int cx_scene_destroy(cx_scene* s) {
+    if (!s) return CX_OK;
+    free_all(s->allocs);
+    delete s;
+    return CX_OK;
+}
+
+int cx_scene_get_info(const cx_scene* s, cx_scene_info* info) {
+    if (!s || !info) return CX_ERR_INVALID_ARG;
+    *info = s->info;
+    return CX_OK;
+}
+
+int cx_trace_closest(cx_ctx* ctx, const cx_scene* s, const float* rays, uint32_t n, uint32_t* prim, float* t,
+                     float* u, float* v, void* stream) {
+    if (!ctx || !s || (n && (!rays || !prim || !t || !u || !v))) return set_err(ctx, CX_ERR_INVALID_ARG, "null argument");
+    CX_HIP(ctx, launch_trace(s->d, rays, n, 0, prim, t, u, v, nullptr, (hipStream_t)stream));
+    return CX_OK;
+}
+
+int cx_trace_occluded(cx_ctx* ctx, const cx_scene* s, const float* rays, uint32_t n, uint32_t* occ, void* stream) {
+    if (!ctx || !s || (n && (!rays || !occ))) return set_err(ctx, CX_ERR_INVALID_ARG, "null argument");
+    CX_HIP(ctx, launch_trace(s->d, rays, n, 1, nullptr, nullptr, nullptr, nullptr, occ, (hipStream_t)stream));
+    return CX_OK;
+}
+
+int cx_entries_create(cx_ctx* ctx, uint32_t capacity, cx_entries** out) {
+    if (!ctx || !out) return set_err(ctx, CX_ERR_INVALID_ARG, "null argument");
+    return entries_alloc(ctx, capacity, out);
+}
+
+int cx_entries_destroy(cx_entries* en) {
+    if (!en) return CX_OK;
+    free_all(en->allocs);
+    delete en;
+    return CX_OK;
+}
+
+int cx_entries_build(cx_ctx* ctx, cx_entries* en, const float* pos, const float* r, const uint8_t* valid, uint32_t n,
+                     const float* sceneMin, const float* sceneMax, void* stream) {
+    if (!ctx || !en || !sceneMin || !sceneMax || (n && (!pos || !r || !valid)))
+        return set_err(ctx, CX_ERR_INVALID_ARG, "null argument");
+    if (n > en->capacity) return set_err(ctx, CX_ERR_INVALID_ARG, "more entries than the structure's capacity");
+    set_quantisation(en->E, sceneMin, sceneMax);
+    hipStream_t s = (hipStream_t)stream;
+    CX_HIP(ctx, launch_entries_from_arrays(pos, r, valid, n, en->E, s));
+    CX_HIP(ctx, launch_tree_build(en->E, en->T, n, en->sortTemp, en->sortTempBytes, s));
+    return CX_OK;
+}
+
+int cx_entries_query(cx_ctx* ctx, const cx_entries* en, const float* q, uint32_t nq, uint32_t k, uint32_t* idx,
+                     float* dist, uint32_t* count, void* stream) {
+    if (!ctx || !en || (nq && (!q || !idx || !dist || !count))) return set_err(ctx, CX_ERR_INVALID_ARG, "null argument");
+    if (k == 0 || k > 31) return set_err(ctx, CX_ERR_INVALID_ARG, "k must be in [1, 31]");
+    CX_HIP(ctx, launch_entries_query(en->T, q, nq, k, idx, dist, count, (hipStream_t)stream));
+    return CX_OK;
+}
+
+int cx_default_params(const cx_scene* s, uint32_t w, uint32_t h, cx_launch_params* p) {
+    if (!p) return CX_ERR_INVALID_ARG;
+    std::memset(p, 0, sizeof(*p));
+    p->width = w;
+    p->height = h;
+    p->windowRadius = 10;
+    p->neeProb = 0.3f;
+    p->maxPathLen = 8;
+    p->rendererType = CX_PATH_TRACER;
+    p->reconMinDistance = 0.01f * (s ? s->info.sceneSize : 0.0f);
+    p->reconMinRoughness = 0.2f;
+    p->prefixLen = 2;
+    p->prefixEnableTemporal = 1;
+    p->prefixEnableSpatial = 0;
+    p->prefixSpatialCount = 1;
+    p->prefixSpatialRounds = 1;
+    p->suffixEnableTemporal = 1;
+    p->suffixEnableSpatial = 1;
+    p->suffixSpatialCount = 1;
+    p->suffixSpatialRounds = 1;
+    p->gatherN = 1;
+    p->gatherM = 1;
+    p->gatherRadius = 0.01f;
+    p->gatherRadiusType = CX_RADIUS_PATH_LENGTH;
+    p->trackPrefixStats = 0;
+    return CX_OK;
+}
+
+int cx_renderer_create(cx_ctx* ctx, const cx_scene* scene, uint32_t w, uint32_t h, cx_renderer** out) {
+    if (!ctx || !scene || !out) return set_err(ctx, CX_ERR_INVALID_ARG, "null argument");
+    if (w == 0 || h == 0 || (uint64_t)w * h > (1ull << 28))
+        return set_err(ctx, CX_ERR_INVALID_ARG, "image size must be in [1, 2^28] pixels");
+    cx_renderer* r = new (std::nothrow) cx_renderer();
+    if (!r) return set_err(ctx, CX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    r->ctx = ctx;
+    r->scene = scene;
+    r->W = w;
+    r->H = h;
+    r->P = w * h;
+    cx_default_params(scene, w, h, &r->params);
+    r->frameRngState = pcg_seed(0);
+    hipError_t e = hipSuccess;
+    auto A = [&](hipError_t x) {
+        if (e == hipSuccess) e = x;
+    };
+    for (int b = 0; b < 2; b++) {
+        A(dalloc(r->allocs, &r->prefixRes[b], (size_t)PREFIX_CHUNKS * r->P));
+        A(dalloc(r->allocs, &r->suffixRes[b], (size_t)SUFFIX_CHUNKS * r->P));
+    }
+    A(dalloc(r->allocs, &r->gbuf, r->P));
+    A(dalloc(r->allocs, &r->neighCount, r->P));
+    A(dalloc(r->allocs, &r->counters, 8));
+    if (e != hipSuccess) {
+        free_all(r->allocs);
+        delete r;
+        return hip_err(ctx, e, "renderer allocation");
+    }
+    int rc = entries_alloc(ctx, r->P, &r->entries);
+    if (rc == CX_OK) rc = ensure_neighbors(r, 8);
+    if (rc != CX_OK) {
+        if (r->entries) cx_entries_destroy(r->entries);
+        free_all(r->allocs);
+        delete r;
+        return rc;
+    }
+    rc = cx_renderer_reset(r, nullptr);
+    if (rc == CX_OK) {
+        e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) rc = hip_err(ctx, e, "renderer reset");
+    }
+    if (rc != CX_OK) {
+        cx_renderer_destroy(r);
+        return rc;
+    }
+    *out = r;
+    return CX_OK;
+}
+
+int cx_renderer_destroy(cx_renderer* r) {
+    if (!r) return CX_OK;
+    if (r->evMade)
+        for (auto& ev : r->ev) (void)hipEventDestroy(ev);
+    if (r->entries) cx_entries_destroy(r->entries);
+    free_all(r->allocs);
+    delete r;
+    return CX_OK;
+}
+
+int cx_renderer_set_params(cx_renderer* r, const cx_launch_params* p) {
+    if (!r || !p) return CX_ERR_INVALID_ARG;
+    int rc = check_params(r, *p);
+    if (rc != CX_OK) return rc;
+    const uint32_t k = p->gatherM > 1 ? (uint32_t)(p->gatherM - 1) : 0u;
+    rc = ensure_neighbors(r, std::max(k, 1u));
+    if (rc != CX_OK) return rc;
+    const uint32_t random = r->params.random;
+    const uint64_t frameIdx = r->params.frameIdx;
+    r->params = *p;
+    r->params.random = random;
+    r->params.frameIdx = frameIdx;
+    r->params.width = r->W;
+    r->params.height = r->H;
+    return CX_OK;
+}
+
+int cx_renderer_get_params(const cx_renderer* r, cx_launch_params* p) {
+    if (!r || !p) return CX_ERR_INVALID_ARG;
+    *p = r->params;
+    return CX_OK;
+}
+
+int cx_renderer_seed(cx_renderer* r, uint64_t seed) {
+    if (!r) return CX_ERR_INVALID_ARG;
+    r->frameRngState = pcg_seed(seed);
+    return CX_OK;
+}
+
+int cx_renderer_reset(cx_renderer* r, void* stream) {
+    if (!r) return CX_ERR_INVALID_ARG;
+    r->front = 0;
+    r->back = 0;
+    r->frameIdx = 0;
+    FrameArgs A = make_args(r);
+    CX_HIP(r->ctx, launch_reset(A, (hipStream_t)stream));
+    return CX_OK;
+}
+
+int cx_renderer_begin_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, float* out) {
+    if (!r || !cam || !out) return set_err(r ? r->ctx : nullptr, CX_ERR_INVALID_ARG, "null argument");
+    // src/Renderer.cpp:212-232
+    ++r->frameIdx;
+    if (cameraChanged || !r->params.enableAccum) r->frameIdx = 0;
+    r->params.frameIdx = r->frameIdx;
+    r->params.random = pcg_next(r->frameRngState);
+    r->params.width = r->W;
+    r->params.height = r->H;
+    r->cam = *cam;
+    r->out = out;
+    if (r->front == 0) {
+        r->front = 1;
+        r->back = 0;
+    } else {
+        r->front = 0;
+        r->back = 1;
+    }
+    r->params.spatialRoundIdx = 0;
+    return CX_OK;
+}
+
+int cx_pass_launch(cx_renderer* r, cx_pass pass, const cx_launch_params* params, void* stream) {
+    if (!r || !params) return CX_ERR_INVALID_ARG;
+    if (!r->out) return set_err(r->ctx, CX_ERR_INVALID_ARG, "no frame begun (cx_renderer_begin_frame)");
+    int prc = check_params(r, *params);
+    if (prc != CX_OK) return prc;
+    prc = ensure_neighbors(r, std::max(params->gatherM > 1 ? (uint32_t)(params->gatherM - 1) : 0u, 1u));
+    if (prc != CX_OK) return prc;
+    hipStream_t s = (hipStream_t)stream;
+    const cx_launch_params saved = r->params;
+    r->params = *params;
+    r->params.width = r->W;
+    r->params.height = r->H;
+    FrameArgs A = make_args(r);
+    int rc = CX_OK;
+    hipError_t e = hipSuccess;
+    switch (pass) {
+        case CX_PASS_PREFIX_GEN_TEMP:
+        case CX_PASS_PREFIX_SPATIAL:
+        case CX_PASS_SUFFIX_GEN_TEMP:
+        case CX_PASS_SUFFIX_SPATIAL:
+            e = launch_pass(pass, A, s);
+            if (e == hipSuccess && pass == CX_PASS_PREFIX_SPATIAL)
+                e = hipMemcpyAsync(r->prefixRes[r->back], r->prefixRes[r->front], (size_t)PREFIX_CHUNKS * r->P * 16,
+                                   hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess && pass == CX_PASS_SUFFIX_SPATIAL)
+                e = hipMemcpyAsync(r->suffixRes[r->back], r->suffixRes[r->front], (size_t)SUFFIX_CHUNKS * r->P * 16,
+                                   hipMemcpyDeviceToDevice, s);
+            break;
+        case CX_PASS_PREFIX_STORE_ENTRIES: {
+            const float* mn = r->scene->info.boundsMin;
+            const float* mx = r->scene->info.boundsMax;
+            set_quantisation(r->entries->E, mn, mx);
+            e = launch_store_entries(A, r->entries->E, s);
+            break;
+        }
+        case CX_PASS_ENTRIES_BUILD:
+            e = launch_tree_build(r->entries->E, r->entries->T, r->P, r->entries->sortTemp, r->entries->sortTempBytes, s);
+            break;
+        case CX_PASS_FINAL_GATHER: e = launch_pass(pass, A, s); break;
+        default: rc = set_err(r->ctx, CX_ERR_INVALID_ARG, "unknown pass");
+    }
+    r->params = saved;
+    if (e != hipSuccess) return hip_err(r->ctx, e, "pass launch");
+    return rc;
+}
+
+int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, float* out, void* stream) {
+    int rc = cx_renderer_begin_frame(r, cam, cameraChanged, out);
+    if (rc != CX_OK) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    cx_launch_params& lp = r->params;
+    if (r->timing && !r->evMade) {
+        for (auto& ev : r->ev) CX_HIP(r->ctx, hipEventCreate(&ev));
+        r->evMade = true;
+    }
+    for (bool& b : r->passRan) b = false;
+    CX_HIP(r->ctx, launch_zero_counters(r->counters, 8, s));
+    rec(r, 0, s);
+    // src/Renderer.cpp:259-371
+    auto run = [&](cx_pass p) -> int {
+        r->passRan[p] = true;
+        return cx_pass_launch(r, p, &lp, stream);
+    };
+    if ((rc = run(CX_PASS_PREFIX_GEN_TEMP))) return rc;
+    rec(r, 1, s);
+    const bool restir = lp.rendererType == CX_CONDITIONAL_RESTIR || lp.rendererType == CX_RESTIR_PT;
+    if (restir && lp.prefixEnableSpatial) {
+        for (int i = 0; i < lp.prefixSpatialRounds; i++) {
+            lp.spatialRoundIdx = i;
+            if ((rc = run(CX_PASS_PREFIX_SPATIAL))) return rc;
+        }
+    }
+    rec(r, 2, s);
+    const bool cond = lp.rendererType == CX_CONDITIONAL_RESTIR;
+    if (cond) {
+        if ((rc = run(CX_PASS_SUFFIX_GEN_TEMP))) return rc;
+    }
+    rec(r, 3, s);
+    if (cond && lp.suffixEnableSpatial && lp.suffixSpatialCount > 0) {
+        for (int i = 0; i < lp.suffixSpatialRounds; i++) {
+            lp.spatialRoundIdx = i;
+            if ((rc = run(CX_PASS_SUFFIX_SPATIAL))) return rc;
+        }
+    }
+    rec(r, 4, s);
+    if (cond) {
+        if ((rc = run(CX_PASS_PREFIX_STORE_ENTRIES))) return rc;
+    }
+    rec(r, 5, s);
+    if (cond) {
+        if ((rc = run(CX_PASS_ENTRIES_BUILD))) return rc;
+    }
+    rec(r, 6, s);
+    if (cond) {
+        if ((rc = run(CX_PASS_FINAL_GATHER))) return rc;
+    }
+    rec(r, 7, s);
+    r->evValid = r->timing;
+    return CX_OK;
+}
+
+int cx_renderer_enable_timing(cx_renderer* r, int enable) {
+    if (!r) return CX_ERR_INVALID_ARG;
+    r->timing = enable != 0;
+    return CX_OK;
+}
+
+int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* st) {
+    if (!r || !st) return CX_ERR_INVALID_ARG;
+    std::memset(st, 0, sizeof(*st));
+    unsigned long long c[8] = {0};
+    CX_HIP(r->ctx, hipDeviceSynchronize());
+    CX_HIP(r->ctx, hipMemcpy(c, r->counters, sizeof(c), hipMemcpyDeviceToHost));
+    st->closestRays = c[0];
+    st->shadowRays = c[1];
+    st->totalNeighCount = c[2];
+    st->minNeighCount = (uint32_t)std::min<unsigned long long>(c[3], 0xFFFFFFFFull);
+    st->maxNeighCount = (uint32_t)c[4];
+    uint32_t nv = 0;
+    CX_HIP(r->ctx, hipMemcpy(&nv, r->entries->E.numValid, 4, hipMemcpyDeviceToHost));
+    st->numEntries = nv;
+    if (r->evValid) {
+        for (int i = 0; i < CX_PASS_COUNT; i++) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, r->ev[i], r->ev[i + 1]);
+            st->passMs[i] = ms;
+        }
+        (void)hipEventElapsedTime(&st->totalMs, r->ev[0], r->ev[CX_PASS_COUNT]);
+    }
+    return CX_OK;
+}
+
+int cx_renderer_state(cx_renderer* r, cx_state_view* v) {
+    if (!r || !v) return CX_ERR_INVALID_ARG;
+    std::memset(v, 0, sizeof(*v));
+    for (int b = 0; b < 2; b++) {
+        v->prefixRes[b] = r->prefixRes[b];
+        v->suffixRes[b] = r->suffixRes[b];
+    }
+    v->gbuffer = r->gbuf;
+    v->neighIdx = r->neighIdx;
+    v->neighDist = r->neighDist;
+    v->neighCount = r->neighCount;
+    v->entryPos = r->entries->E.pos;
+    v->entryValid = r->entries->E.valid;
+    v->frontBufferIdx = r->front;
+    v->backBufferIdx = r->back;
+    v->prefixChunks = PREFIX_CHUNKS;
+    v->suffixChunks = SUFFIX_CHUNKS;
+    v->k = r->params.gatherM > 1 ? (uint32_t)(r->params.gatherM - 1) : 0u;
+    return CX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+// Internal declarations shared by the kernel translation unit and the host library code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/crisoptix.h"
+#include "cx_restir_dev.h"
+
+namespace cx {
+
+hipError_t launch_pass(int pass, const FrameArgs& A, hipStream_t s);
+hipError_t launch_reset(const FrameArgs& A, hipStream_t s);
+hipError_t launch_zero_counters(unsigned long long* c, uint32_t n, hipStream_t s);
+hipError_t launch_store_entries(const FrameArgs& A, const EntryBuild& E, hipStream_t s);
+hipError_t launch_entries_from_arrays(const float* pos, const float* r, const uint8_t* valid, uint32_t n,
+                                      const EntryBuild& E, hipStream_t s);
+size_t sort_temp_bytes(uint32_t n);
+hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n, void* temp, size_t tempBytes,
+                             hipStream_t s);
+hipError_t launch_entries_query(const EntryTree& T, const float* q, uint32_t nq, uint32_t k, uint32_t* idx,
+                                float* dist, uint32_t* count, hipStream_t s);
+hipError_t launch_trace(const DScene& S, const float* rays, uint32_t n, int anyHit, uint32_t* prim, float* t,
+                        float* u, float* v, uint32_t* occ, hipStream_t s);
+
+// Host BVH build output (binary SAH BVH in the node format of cx_scene_dev.h).
+struct BvhHost {
+    std::vector<float4> nodes;   // 4 per inner node
+    std::vector<float4> tris;    // 3 per triangle slot
+    int32_t root = 0;
+    uint32_t maxDepth = 0;
+};
+
+bool build_bvh(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t maxLeaf, BvhHost& out,
+               std::string& err);
+
+}  // namespace cx

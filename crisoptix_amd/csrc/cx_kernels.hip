@@ -1,0 +1,885 @@
+// HIP kernels of the frame loop.  One kernel per raygen program of the reference
+// (shader/*.cu), one thread per pixel, 128-thread blocks covering 16x8 pixel tiles so a
+// wavefront holds a compact 16x4 screen region.  The BVH traversal stack of each thread is a
+// column of a block-wide LDS array.
+#include "cx_internal.h"
+#include "cx_restir_dev.h"
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+namespace cx {
+
+static constexpr int BLOCK = 128;
+static constexpr int TILE_X = 16;
+static constexpr int TILE_Y = 8;
+
+#define CX_STACK_DECL                                  \
+    __shared__ int s_stack[STACK_DEPTH * BLOCK];       \
+    TraceCtx tc;                                       \
+    tc.stack = s_stack + threadIdx.x;                  \
+    tc.stride = BLOCK;                                 \
+    tc.closest = 0;                                    \
+    tc.shadow = 0;
+
+__device__ __forceinline__ bool tile_pixel(const FrameArgs& A, uint32_t& x, uint32_t& y, uint32_t& p) {
+    x = blockIdx.x * TILE_X + (threadIdx.x % TILE_X);
+    y = blockIdx.y * TILE_Y + (threadIdx.x / TILE_X);
+    p = y * A.W + x;
+    return x < A.W && y < A.H;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One atomic per wavefront for the ray counters.
+__device__ __forceinline__ void flush_counters(const TraceCtx& tc, unsigned long long* counters) {
+    const unsigned long long c = wave_sum((unsigned long long)tc.closest);
+    const unsigned long long s = wave_sum((unsigned long long)tc.shadow);
+    if ((threadIdx.x & 63) == 0 && counters != nullptr) {
+        if (c) atomicAdd(&counters[0], c);
+        if (s) atomicAdd(&counters[1], s);
+    }
+}
+
+__device__ __forceinline__ void write_output(const FrameArgs& A, uint32_t p, V3 rad) {
+    float* o = A.out + 3 * (size_t)p;
+    V3 r = rad;
+    if (A.lp.enableAccum) {
+        const V3 old = V3{o[0], o[1], o[2]};
+        const float blend = 1.0f / (float)(A.lp.frameIdx + 1);
+        r = blend * rad + (1.0f - blend) * old;
+    }
+    o[0] = r.x;
+    o[1] = r.y;
+    o[2] = r.z;
+}
+
+// prefix_gen_temp_reuse.cu:122-126, Camera.h:14-22
+__device__ __forceinline__ void camera_ray(const FrameArgs& A, uint32_t x, uint32_t y, Rng& rng, V3& o, V3& d) {
+    const float jx = rng.f();
+    const float jy = rng.f();
+    float ux = ((float)x + jx) / (float)A.W;
+    float uy = ((float)y + jy) / (float)A.H;
+    ux = 2.0f * ux - 1.0f;
+    uy = 2.0f * uy - 1.0f;
+    o = A.camPos;
+    d = normalize(A.camU * ux + A.camV * uy + A.camW);
+}
+
+__device__ __forceinline__ void store_gbuf(const FrameArgs& A, uint32_t p, uint64_t rng, uint32_t primaryValid,
+                                           uint32_t canonFlags) {
+    A.gbuf[p] = uint4{(uint32_t)rng, (uint32_t)(rng >> 32), primaryValid, canonFlags};
+}
+
+// ------------------------------------------------------------------------------------------
+// prefix_gen_temp_reuse (shader/prefix_gen_temp_reuse.cu:11-193)
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_prefix_gen_temp(FrameArgs A) {
+    CX_STACK_DECL
+    uint32_t x, y, p;
+    if (tile_pixel(A, x, y, p)) {
+        const cx_launch_params& lp = A.lp;
+        Rng rng = rng_seed(tea64(p, lp.random));
+        V3 origin, dir;
+        camera_ray(A, x, y, rng, origin, dir);
+        if (lp.rendererType == CX_CONDITIONAL_RESTIR || lp.rendererType == CX_RESTIR_PT) {
+            const uint32_t oldFlags = A.gbuf[p].w;
+            Prefix canon;
+            canon.primary = packed_invalid();
+            canon.recon = packed_invalid();
+            canon.last = packed_invalid();
+            canon.reconOutDir = v3(0.0f);
+            trace_prefix(A, tc, canon, origin, dir, rng);
+            uint32_t flags = oldFlags & CANON_SUFFIX_VALID;
+            if (canon.valid()) flags |= CANON_PREFIX_VALID;
+            if ((oldFlags & CANON_PRIMARY_VALID) || canon.length() >= 1) flags |= CANON_PRIMARY_VALID;
+            bool validPrimary = false;
+            V3 rad = v3(0.0f);
+            uint4* front = A.prefixRes[A.front];
+            uint4* back = A.prefixRes[A.back];
+            if (canon.valid()) {   // PrefixGenTempReuse
+                const float canonPHat = luminance(canon.f);
+                PrefixRes curr = load_prefix(front, A.P, p);
+                curr.wSum = 0.0f;
+                curr.conf = 0.0f;
+                bool temporal = lp.prefixEnableTemporal != 0;
+                PrefixRes prev;
+                if (temporal) {
+                    prev = load_prefix(back, A.P, p);
+                    const bool skipNee = is_cond(lp) && prev.s.nee();
+                    if (prev.wSum <= 0.0f || !prev.s.valid() || skipNee) temporal = false;
+                }
+                if (!temporal) {
+                    res_update(curr, canon, canonPHat / canon.p, rng, 1.0f);
+                    res_finalize(curr);
+                } else {
+                    float jCanonToPrev = 0.0f, jPrevToCanon = 0.0f;
+                    const V3 fFromCanonOfPrev = shift_prefix(A, tc, prev.s, canon, jPrevToCanon);
+                    const V3 fFromPrevOfCanon = shift_prefix(A, tc, canon, prev.s, jCanonToPrev);
+                    const float pFromCanonOfPrev = luminance(fFromCanonOfPrev) * jPrevToCanon;
+                    const float K = 1.0f;
+                    const float prevMis = neighbor_mis(fFromCanonOfPrev, prev.s.f, jPrevToCanon, K, 1.0f, prev.conf);
+                    const float canonMis =
+                        1.0f + canonical_mis(canon.f, fFromPrevOfCanon, jCanonToPrev, K, 1.0f, prev.conf);
+                    const float canonRis = canonMis * canonPHat / canon.p;
+                    res_update(curr, canon, canonRis, rng, 1.0f);
+                    const float prevRis = prevMis * pFromCanonOfPrev * prev.wSum;
+                    Prefix shifted = prev.s;   // PrefixPath.h:60-81
+                    shifted.f = fFromCanonOfPrev;
+                    shifted.primary = canon.primary;
+                    res_update(curr, shifted, prevRis, rng, prev.conf);
+                    if (curr.wSum > 0.0f) {
+                        curr.wSum /= K + 1.0f;
+                        res_finalize(curr);
+                    }
+                }
+                store_prefix(front, A.P, p, curr);
+                if (A.copyPrefixToBack) store_prefix(back, A.P, p, curr);
+                validPrimary = true;
+                rad = curr.wSum * curr.s.f;
+            } else if (A.copyPrefixToBack) {
+                store_prefix(back, A.P, p, load_prefix(front, A.P, p));
+            }
+            if (is_pt(lp) && !lp.prefixEnableSpatial) write_output(A, p, validPrimary ? rad : v3(0.0f));
+            store_gbuf(A, p, rng.s, validPrimary ? 1u : 0u, flags);
+        } else if (lp.rendererType == CX_PATH_TRACER) {
+            V3 rad = trace_complete_path(A, tc, origin, dir, rng);
+            if (any_bad(rad)) rad = v3(0.0f);
+            write_output(A, p, rad);
+        }
+    }
+    flush_counters(tc, A.counters);
+}
+
+// ------------------------------------------------------------------------------------------
+// prefix_spatial_reuse (shader/prefix_spatial_reuse.cu:11-177).  Neighbours are read from the
+// back buffer, which nothing writes during the pass (pass-entry snapshot); the front->back
+// copy of :162-163 is a separate copy after the kernel.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_prefix_spatial(FrameArgs A) {
+    CX_STACK_DECL
+    uint32_t x, y, p;
+    if (tile_pixel(A, x, y, p)) {
+        const cx_launch_params& lp = A.lp;
+        uint4* front = A.prefixRes[A.front];
+        const uint4* back = A.prefixRes[A.back];
+        V3 rad = v3(0.0f);
+        bool ok = false;
+        const uint4 g = A.gbuf[p];
+        if ((load_prefix_flags(front, A.P, p) >> 16) & 1u) {
+            PrefixRes curr = load_prefix(front, A.P, p);
+            const Prefix currPrefix = curr.s;
+            if ((g.w & CANON_PREFIX_VALID) && (g.w & CANON_PRIMARY_VALID)) {
+                Rng rng;
+                rng.s = (uint64_t)g.x | ((uint64_t)g.y << 32);
+                const float currWSum = curr.wSum;
+                const float currConf = curr.conf;
+                curr.wSum = 0.0f;
+                curr.conf = 0.0f;
+                const int neighCount = lp.prefixSpatialCount;
+                float canonMis = 1.0f;
+                uint32_t validCount = 0, validFlags = 0;
+                const Rng selStart = rng;
+                for (int i = 0; i < neighCount; ++i) {
+                    uint32_t nx, ny;
+                    select_neighbor(x, y, lp.windowRadius, rng, nx, ny);
+                    if (!(nx < A.W && ny < A.H)) continue;
+                    const uint32_t np = ny * A.W + nx;
+                    if (!A.gbuf[np].z) continue;
+                    const uint32_t nf = load_prefix_flags(back, A.P, np);
+                    const bool nvalid = (nf >> 16) & 1u, nnee = (nf >> 17) & 1u;
+                    if (!nvalid || (is_cond(lp) && nnee)) continue;
+                    ++validCount;
+                    validFlags |= 1u << i;
+                }
+                const float K = (float)validCount;
+                Rng sel = selStart;
+                for (int i = 0; i < neighCount; ++i) {
+                    uint32_t nx, ny;
+                    select_neighbor(x, y, lp.windowRadius, sel, nx, ny);   // replays the first loop's draws
+                    if (!((validFlags >> i) & 1u)) continue;
+                    const uint32_t np = ny * A.W + nx;
+                    const PrefixRes nres = load_prefix(back, A.P, np);
+                    float jNC = 0.0f, jCN = 0.0f;
+                    const V3 fFromCanonOfNeigh = shift_prefix(A, tc, nres.s, currPrefix, jNC);
+                    const float pFromCanonOfNeigh = luminance(fFromCanonOfNeigh) * jNC;
+                    const V3 fFromNeighOfCanon = shift_prefix(A, tc, currPrefix, nres.s, jCN);
+                    float nMis = neighbor_mis(fFromCanonOfNeigh, nres.s.f, jNC, K, currConf, nres.conf);
+                    if (bad(nMis)) nMis = 0.0f;
+                    const float nRis = nMis * pFromCanonOfNeigh * nres.wSum;
+                    Prefix shifted = nres.s;
+                    shifted.f = fFromCanonOfNeigh;
+                    shifted.primary = currPrefix.primary;
+                    res_update(curr, shifted, nRis, rng, nres.conf);
+                    canonMis += canonical_mis(currPrefix.f, fFromNeighOfCanon, jCN, K, currConf, nres.conf);
+                }
+                if (bad(canonMis)) canonMis = 0.0f;
+                const float canonRis = canonMis * currWSum * luminance(currPrefix.f);
+                res_update(curr, currPrefix, canonRis, rng, currConf);
+                if (curr.wSum > 0.0f) {
+                    curr.wSum /= K + 1.0f;
+                    res_finalize(curr);
+                }
+                store_prefix(front, A.P, p, curr);
+                A.gbuf[p] = uint4{(uint32_t)rng.s, (uint32_t)(rng.s >> 32), g.z, g.w};
+                ok = true;
+                rad = curr.wSum * curr.s.f;
+            }
+        }
+        if (is_pt(lp) && lp.spatialRoundIdx == lp.prefixSpatialRounds - 1) write_output(A, p, ok ? rad : v3(0.0f));
+    }
+    flush_counters(tc, A.counters);
+}
+
+__device__ __forceinline__ void invalidate(SuffixRes& r) {   // suffix_gen_temp_reuse.cu:12-17
+    r.wSum = 0.0f;
+    r.conf = 0.0f;
+    r.s.set_valid(false);
+}
+
+// ------------------------------------------------------------------------------------------
+// suffix_gen_temp_reuse (shader/suffix_gen_temp_reuse.cu:19-134), with the unconditional
+// front->back copy of :132-133 fused into the store.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_suffix_gen_temp(FrameArgs A) {
+    CX_STACK_DECL
+    uint32_t x, y, p;
+    if (tile_pixel(A, x, y, p)) {
+        const cx_launch_params& lp = A.lp;
+        const uint4 g = A.gbuf[p];
+        uint32_t flags = g.w;
+        Rng rng;
+        rng.s = (uint64_t)g.x | ((uint64_t)g.y << 32);
+        uint4* front = A.suffixRes[A.front];
+        uint4* back = A.suffixRes[A.back];
+        SuffixRes curr = load_suffix(front, A.P, p);
+        const uint32_t pf = load_prefix_flags(A.prefixRes[A.front], A.P, p);
+        if (!((pf >> 16) & 1u) || ((pf >> 17) & 1u)) {
+            invalidate(curr);
+        } else {
+            const PrefixRes pre = load_prefix(A.prefixRes[A.front], A.P, p);
+            Suffix canon;
+            canon.recon = packed_invalid();
+            canon.reconOutDir = v3(0.0f);
+            trace_suffix(A, tc, canon, pre.s, rng);
+            flags = (flags & ~CANON_SUFFIX_VALID) | (canon.valid() ? CANON_SUFFIX_VALID : 0u);
+            if (!canon.valid()) {
+                invalidate(curr);
+            } else {
+                const float canonPHat = luminance(canon.f);
+                curr.wSum = 0.0f;
+                curr.conf = 0.0f;
+                bool temporal = lp.suffixEnableTemporal != 0;
+                SuffixRes prev;
+                if (temporal) {
+                    prev = load_suffix(back, A.P, p);
+                    if (prev.wSum <= 0.0f || !prev.s.valid()) temporal = false;
+                }
+                if (!temporal) {
+                    res_update(curr, canon, canonPHat / canon.p, rng, 1.0f);
+                    res_finalize(curr);
+                } else {
+                    float jCanonToPrev = 0.0f, jPrevToCanon = 0.0f;
+                    const V3 fFromCanonOfPrev = shift_suffix(A, tc, prev.s, canon, jPrevToCanon);
+                    const V3 fFromPrevOfCanon = shift_suffix(A, tc, canon, prev.s, jCanonToPrev);
+                    const float pFromCanonOfPrev = luminance(fFromCanonOfPrev) * jPrevToCanon;
+                    const float K = 1.0f;
+                    const float prevMis = neighbor_mis(fFromCanonOfPrev, prev.s.f, jPrevToCanon, K, 1.0f, prev.conf);
+                    const float canonMis =
+                        1.0f + canonical_mis(canon.f, fFromPrevOfCanon, jCanonToPrev, K, 1.0f, prev.conf);
+                    const float canonRis = canonMis * canonPHat / canon.p;
+                    res_update(curr, canon, canonRis, rng, 1.0f);
+                    const float prevRis = prevMis * pFromCanonOfPrev * prev.wSum;
+                    Suffix shifted = prev.s;   // SuffixPath.h:51-65
+                    shifted.lastPrefix = canon.lastPrefix;
+                    shifted.f = fFromCanonOfPrev;
+                    res_update(curr, shifted, prevRis, rng, prev.conf);
+                    if (curr.wSum > 0.0f) {
+                        curr.wSum /= K + 1.0f;
+                        res_finalize(curr);
+                    }
+                }
+            }
+        }
+        store_suffix(front, A.P, p, curr);
+        store_suffix(back, A.P, p, curr);
+        A.gbuf[p] = uint4{(uint32_t)rng.s, (uint32_t)(rng.s >> 32), g.z, flags};
+    }
+    flush_counters(tc, A.counters);
+}
+
+// ------------------------------------------------------------------------------------------
+// suffix_spatial_reuse (shader/suffix_spatial_reuse.cu:16-167).  Both neighbour loops read the
+// back buffer (equal to the front buffer at pass entry); the copy of :165-166 follows the kernel.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_suffix_spatial(FrameArgs A) {
+    CX_STACK_DECL
+    uint32_t x, y, p;
+    if (tile_pixel(A, x, y, p)) {
+        const cx_launch_params& lp = A.lp;
+        uint4* front = A.suffixRes[A.front];
+        const uint4* back = A.suffixRes[A.back];
+        const uint4 g = A.gbuf[p];
+        SuffixRes curr = load_suffix(front, A.P, p);
+        const uint32_t pf = load_prefix_flags(A.prefixRes[A.front], A.P, p);
+        bool stored = false;
+        if (!((pf >> 16) & 1u) || ((pf >> 17) & 1u) || !curr.s.valid() || !(g.w & CANON_SUFFIX_VALID)) {
+            invalidate(curr);
+        } else {
+            const Suffix currSuffix = curr.s;
+            Rng rng;
+            rng.s = (uint64_t)g.x | ((uint64_t)g.y << 32);
+            const float currWSum = curr.wSum;
+            const float currConf = curr.conf;
+            curr.wSum = 0.0f;
+            curr.conf = 0.0f;
+            const int neighCount = lp.suffixSpatialCount;
+            float canonMis = 1.0f;
+            uint32_t validCount = 0, validFlags = 0;
+            const Rng selStart = rng;
+            for (int i = 0; i < neighCount; ++i) {
+                uint32_t nx, ny;
+                select_neighbor(x, y, lp.windowRadius, rng, nx, ny);
+                if (!(nx < A.W && ny < A.H)) continue;
+                const uint32_t np = ny * A.W + nx;
+                if (!A.gbuf[np].z) continue;
+                if (!((load_suffix_flags(back, A.P, np) >> 16) & 1u)) continue;
+                ++validCount;
+                validFlags |= 1u << i;
+            }
+            const float K = (float)validCount;
+            Rng sel = selStart;
+            const Inter currLast = rehydrate(A.S, currSuffix.lastPrefix);
+            for (int i = 0; i < neighCount; ++i) {
+                uint32_t nx, ny;
+                select_neighbor(x, y, lp.windowRadius, sel, nx, ny);
+                if (!((validFlags >> i) & 1u)) continue;
+                const uint32_t np = ny * A.W + nx;
+                const SuffixRes nres = load_suffix(back, A.P, np);
+                float jNC = 0.0f, jCN = 0.0f;
+                V3 fFromCanonOfNeigh = v3(0.0f);
+                if (currLast.valid) fFromCanonOfNeigh = shift_suffix(A, tc, nres.s, currLast, jNC);
+                const float pFromCanonOfNeigh = luminance(fFromCanonOfNeigh) * jNC;
+                const V3 fFromNeighOfCanon = shift_suffix(A, tc, currSuffix, nres.s, jCN);
+                float nMis = neighbor_mis(fFromCanonOfNeigh, nres.s.f, jNC, K, currConf, nres.conf);
+                if (bad(nMis)) nMis = 0.0f;
+                const float nRis = nMis * pFromCanonOfNeigh * nres.wSum;
+                Suffix shifted = nres.s;
+                shifted.lastPrefix = currSuffix.lastPrefix;
+                shifted.f = fFromCanonOfNeigh;
+                res_update(curr, shifted, nRis, rng, nres.conf);
+                canonMis += canonical_mis(currSuffix.f, fFromNeighOfCanon, jCN, K, currConf, nres.conf);
+            }
+            if (bad(canonMis)) canonMis = 0.0f;
+            const float canonRis = canonMis * currWSum * luminance(currSuffix.f);
+            res_update(curr, currSuffix, canonRis, rng, currConf);
+            if (curr.wSum > 0.0f) {
+                curr.wSum /= K + 1.0f;
+                res_finalize(curr);
+            }
+            A.gbuf[p] = uint4{(uint32_t)rng.s, (uint32_t)(rng.s >> 32), g.z, g.w};
+            stored = true;
+        }
+        (void)stored;
+        store_suffix(front, A.P, p, curr);
+    }
+    flush_counters(tc, A.counters);
+}
+
+// ------------------------------------------------------------------------------------------
+// prefix_store_entries (shader/prefix_store_entries.cu:9-53) + the Morton keys of the entry
+// tree build.  Entry radius = half-width of the entry's box (prefix_search_hit.cu:33-34).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton30(V3 q, V3 lo, V3 scale) {
+    const float fx = fminf(fmaxf((q.x - lo.x) * scale.x, 0.0f), 1023.0f);
+    const float fy = fminf(fmaxf((q.y - lo.y) * scale.y, 0.0f), 1023.0f);
+    const float fz = fminf(fmaxf((q.z - lo.z) * scale.z, 0.0f), 1023.0f);
+    return (expand_bits((uint32_t)fx) << 2) | (expand_bits((uint32_t)fy) << 1) | expand_bits((uint32_t)fz);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_store_entries(FrameArgs A, EntryBuild E) {
+    uint32_t x, y, p;
+    if (!tile_pixel(A, x, y, p)) return;
+    const cx_launch_params& lp = A.lp;
+    const uint32_t pf = load_prefix_flags(A.prefixRes[A.front], A.P, p);
+    const uint32_t sf = load_suffix_flags(A.suffixRes[A.front], A.P, p);
+    bool valid = false;
+    float4 e = float4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (((pf >> 16) & 1u) && ((sf >> 16) & 1u)) {
+        const PrefixRes pre = load_prefix(A.prefixRes[A.front], A.P, p);
+        const Inter last = rehydrate(A.S, pre.s.last);
+        if (last.valid) {
+            float radius = lp.gatherRadius;
+            if (lp.gatherRadiusType == CX_RADIUS_PATH_LENGTH) radius *= pre.s.pathLen / 4.5f;
+            const float minX = last.pos.x - radius, maxX = last.pos.x + radius;
+            e = float4{last.pos.x, last.pos.y, last.pos.z, (maxX - minX) / 2.0f};
+            valid = true;
+        }
+    }
+    E.pos[p] = e;
+    E.valid[p] = valid ? 1 : 0;
+    E.keys[p] = valid ? morton30(V3{e.x, e.y, e.z}, E.lo, E.scale) : 0xFFFFFFFFu;
+    E.vals[p] = p;
+    A.neighCount[p] = 0;
+    if (valid) atomicAdd(E.numValid, 1u);
+}
+
+// Standalone entry upload (cx_entries_build): same record as k_store_entries.
+__global__ void k_entries_from_arrays(const float* pos, const float* r, const uint8_t* valid, uint32_t n,
+                                      EntryBuild E) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool v = valid[i] != 0;
+    const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    const float minX = px - r[i], maxX = px + r[i];
+    const float4 e = float4{px, py, pz, (maxX - minX) / 2.0f};
+    E.pos[i] = v ? e : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    E.valid[i] = v ? 1 : 0;
+    E.keys[i] = v ? morton30(V3{px, py, pz}, E.lo, E.scale) : 0xFFFFFFFFu;
+    E.vals[i] = i;
+    if (v) atomicAdd(E.numValid, 1u);
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n, unsigned long long minInit) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (i == 3) ? minInit : 0ull;
+}
+
+// Leaves of the implicit entry tree: leaf j holds sorted entries [j*L, (j+1)*L) ∩ [0, nValid).
+// Boxes are the union of [pos - r, pos + r], padded outward so the float box test is
+// conservative for every entry whose distance test passes.
+__global__ void k_tree_leaves(EntryTree T, const uint32_t* sortedVals, const float4* entryPos, uint32_t numLeavesMax,
+                              uint32_t leafBase) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= numLeavesMax) return;
+    const uint32_t n = *T.numValid;
+    const uint32_t b = j * T.leafSize;
+    const uint32_t e = min(b + T.leafSize, n);
+    float4 lo = float4{INFINITY, INFINITY, INFINITY, 0.0f}, hi = float4{-INFINITY, -INFINITY, -INFINITY, 0.0f};
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t pix = sortedVals[i];
+        const float4 q = entryPos[pix];
+        T.entrySorted[i] = q;
+        T.entryPixel[i] = pix;
+        const float r = q.w;
+        lo.x = fminf(lo.x, q.x - r);
+        lo.y = fminf(lo.y, q.y - r);
+        lo.z = fminf(lo.z, q.z - r);
+        hi.x = fmaxf(hi.x, q.x + r);
+        hi.y = fmaxf(hi.y, q.y + r);
+        hi.z = fmaxf(hi.z, q.z + r);
+    }
+    if (b < e) {
+        const float ex = fmaxf(fabsf(lo.x), fabsf(hi.x)) * 3.0517578125e-5f + 1e-20f;
+        const float ey = fmaxf(fabsf(lo.y), fabsf(hi.y)) * 3.0517578125e-5f + 1e-20f;
+        const float ez = fmaxf(fabsf(lo.z), fabsf(hi.z)) * 3.0517578125e-5f + 1e-20f;
+        lo.x -= ex; lo.y -= ey; lo.z -= ez;
+        hi.x += ex; hi.y += ey; hi.z += ez;
+    }
+    T.lo[leafBase + j] = lo;
+    T.hi[leafBase + j] = hi;
+}
+
+__global__ void k_tree_level(EntryTree T, uint32_t levelBase, uint32_t count) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    const uint32_t node = levelBase + j;
+    const uint32_t c0 = 2 * node + 1, c1 = 2 * node + 2;
+    const float4 l0 = T.lo[c0], l1 = T.lo[c1], h0 = T.hi[c0], h1 = T.hi[c1];
+    T.lo[node] = float4{fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.0f};
+    T.hi[node] = float4{fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.0f};
+}
+
+// Neighbour query under C2: every entry with |q - pos| <= r is a candidate; the k smallest
+// (distance, index) are kept in ascending order.  Up to KR candidates live in registers.
+template <int KR>
+struct TopK {
+    float d[KR];
+    uint32_t i[KR];
+    uint32_t count;
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int j = 0; j < KR; j++) {
+            d[j] = INFINITY;
+            i[j] = 0xFFFFFFFFu;
+        }
+        count = 0;
+    }
+    __device__ __forceinline__ static bool less(float da, uint32_t ia, float db, uint32_t ib) {
+        return da < db || (da == db && ia < ib);
+    }
+    // Sorted insertion; a candidate not smaller than the k-th entry changes nothing.
+    __device__ __forceinline__ void insert(float cd, uint32_t ci, uint32_t k) {
+#pragma unroll
+        for (int j = KR - 1; j >= 0; j--) {
+            if ((uint32_t)j >= k) continue;
+            if (less(cd, ci, d[j], i[j])) {
+                if (j > 0 && less(cd, ci, d[j - 1], i[j - 1])) {
+                    d[j] = d[j - 1];
+                    i[j] = i[j - 1];
+                } else {
+                    d[j] = cd;
+                    i[j] = ci;
+                }
+            }
+        }
+        if (count < k) count++;
+    }
+};
+
+template <int KR>
+__device__ __forceinline__ uint32_t entries_query(const EntryTree& T, V3 q, uint32_t k, int* stack, int stride,
+                                                  uint32_t* outIdx, float* outDist) {
+    TopK<KR> top;
+    top.init();
+    const uint32_t n = *T.numValid;
+    if (k == 0 || n == 0 || isnan(q.x) || isnan(q.y) || isnan(q.z)) return 0;
+    const uint32_t leafBase = (1u << T.levels) - 1u;
+    int sp = 0;
+    uint32_t node = 0;
+    while (true) {
+        const float4 lo = T.lo[node], hi = T.hi[node];
+        const bool inside = q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y && q.z >= lo.z && q.z <= hi.z;
+        if (inside) {
+            if (node >= leafBase) {
+                const uint32_t j = node - leafBase;
+                const uint32_t b = j * T.leafSize, e = min(b + T.leafSize, n);
+                for (uint32_t s = b; s < e; s++) {
+                    const float4 ep = T.entrySorted[s];
+                    const float dd = distance(q, V3{ep.x, ep.y, ep.z});
+                    if (dd > ep.w) continue;
+                    top.insert(dd, T.entryPixel[s], k);
+                }
+            } else {
+                stack[sp * stride] = (int)(2 * node + 2);
+                ++sp;
+                node = 2 * node + 1;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        node = (uint32_t)stack[sp * stride];
+    }
+#pragma unroll
+    for (int j = 0; j < KR; j++) {
+        if ((uint32_t)j < top.count) {
+            outIdx[j] = top.i[j];
+            outDist[j] = top.d[j];
+        }
+    }
+    return top.count;
+}
+
+template <int KR>
+__global__ __launch_bounds__(BLOCK) void k_entries_query(EntryTree T, const float* q, uint32_t nq, uint32_t k,
+                                                        uint32_t* idx, float* dist, uint32_t* count) {
+    __shared__ int s_stack[STACK_DEPTH * BLOCK];
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nq) return;
+    for (uint32_t i = 0; i < k; i++) {
+        idx[(size_t)j * k + i] = 0xFFFFFFFFu;
+        dist[(size_t)j * k + i] = 0.0f;
+    }
+    count[j] = entries_query<KR>(T, V3{q[3 * j], q[3 * j + 1], q[3 * j + 2]}, k, s_stack + threadIdx.x, BLOCK,
+                                 idx + (size_t)j * k, dist + (size_t)j * k);
+}
+
+// ------------------------------------------------------------------------------------------
+// final_gather (shader/final_gather.cu:69-356)
+// ------------------------------------------------------------------------------------------
+template <int KR>
+__device__ __forceinline__ V3 gather_canon(const FrameArgs& A, TraceCtx& tc, uint32_t p, V3 origin, V3 dir,
+                                           uint32_t k, Rng& rng) {
+    Prefix cp;
+    cp.primary = packed_invalid();
+    cp.recon = packed_invalid();
+    cp.last = packed_invalid();
+    cp.reconOutDir = v3(0.0f);
+    cp.rng = PCG_DEFAULT_STATE;
+    trace_prefix(A, tc, cp, origin, dir, rng);
+    if (!cp.valid()) return v3(0.0f);
+    if (cp.nee()) return cp.f / cp.p;
+    Suffix cs;
+    cs.recon = packed_invalid();
+    cs.reconOutDir = v3(0.0f);
+    trace_suffix(A, tc, cs, cp, rng);
+    if (!cs.valid()) return v3(0.0f);
+    const V3 canonContrib = cp.f * cs.f / (cp.p * cs.p);
+    if (k == 0) return canonContrib;
+    const Inter last = rehydrate(A.S, cs.lastPrefix);
+    uint32_t* nIdx = A.neighIdx + (size_t)p * k;
+    float* nDist = A.neighDist + (size_t)p * k;
+    const uint32_t neighCount = entries_query<KR>(A.tree, last.pos, k, tc.stack, tc.stride, nIdx, nDist);
+    A.neighCount[p] = neighCount;
+    if (A.lp.trackPrefixStats) {
+        atomicAdd(&A.counters[2], (unsigned long long)neighCount);
+        atomicMin(&A.counters[3], (unsigned long long)neighCount);
+        atomicMax(&A.counters[4], (unsigned long long)neighCount);
+    }
+    const uint4* sfront = A.suffixRes[A.front];
+    uint32_t validCount = 0, validFlags = 0;
+    for (uint32_t i = 0; i < neighCount; ++i) {
+        if (!((load_suffix_flags(sfront, A.P, nIdx[i]) >> 16) & 1u)) continue;
+        ++validCount;
+        validFlags |= 1u << i;
+    }
+    V3 suffixContrib = v3(0.0f);
+    const float K = (float)validCount;
+    float canonMis = 1.0f;
+    for (uint32_t i = 0; i < neighCount; ++i) {
+        if (!(validFlags & (1u << i))) continue;
+        const SuffixRes nres = load_suffix(sfront, A.P, nIdx[i]);
+        float jNC = 0.0f, jCN = 0.0f;
+        V3 fFromCanonOfNeigh = v3(0.0f);
+        if (last.valid) fFromCanonOfNeigh = shift_suffix(A, tc, nres.s, last, jNC);
+        const V3 fFromNeighOfCanon = shift_suffix(A, tc, cs, nres.s, jCN);
+        float nMis = neighbor_mis(fFromCanonOfNeigh, nres.s.f, jNC, K, 1.0f, nres.conf);
+        if (bad(nMis)) nMis = 0.0f;
+        const V3 nc = nMis * fFromCanonOfNeigh * nres.wSum * jNC;
+        if (!any_bad(nc)) suffixContrib = suffixContrib + nc;
+        canonMis += canonical_mis(cs.f, fFromNeighOfCanon, jCN, K, 1.0f, nres.conf);
+    }
+    V3 gatherContrib = suffixContrib * cp.f / cp.p;
+    gatherContrib = gatherContrib / (float)(validCount + 1);
+    V3 result = gatherContrib + (canonContrib * canonMis);
+    result = result / (K + 1.0f);
+    if (any_bad(result)) return v3(0.0f);
+    return result;
+}
+
+template <int KR>
+__device__ __forceinline__ V3 gather_not_canon(const FrameArgs& A, TraceCtx& tc, uint32_t p, V3 origin, V3 dir,
+                                               uint32_t k, Rng& rng) {
+    if (k == 0) return v3(0.0f);
+    Inter last;
+    last.valid = false;
+    V3 thr = v3(0.0f);
+    float pp = 0.0f;
+    if (!trace_prefix_fg(A, tc, thr, pp, last, origin, dir, (uint32_t)A.lp.prefixLen, rng)) return v3(0.0f);
+    uint32_t* nIdx = A.neighIdx + (size_t)p * k;
+    float* nDist = A.neighDist + (size_t)p * k;
+    const uint32_t neighCount = entries_query<KR>(A.tree, last.pos, k, tc.stack, tc.stride, nIdx, nDist);
+    A.neighCount[p] = neighCount;
+    if (A.lp.trackPrefixStats) {
+        atomicAdd(&A.counters[2], (unsigned long long)neighCount);
+        atomicMin(&A.counters[3], (unsigned long long)neighCount);
+        atomicMax(&A.counters[4], (unsigned long long)neighCount);
+    }
+    const uint4* sfront = A.suffixRes[A.front];
+    uint32_t validCount = 0, validFlags = 0;
+    for (uint32_t i = 0; i < neighCount; ++i) {
+        if (!((load_suffix_flags(sfront, A.P, nIdx[i]) >> 16) & 1u)) continue;
+        ++validCount;
+        validFlags |= 1u << i;
+    }
+    V3 suffixContrib = v3(0.0f);
+    const float K = (float)validCount;
+    for (uint32_t i = 0; i < neighCount; ++i) {
+        if (!(validFlags & (1u << i))) continue;
+        const SuffixRes nres = load_suffix(sfront, A.P, nIdx[i]);
+        float jNC = 0.0f;
+        const V3 f = shift_suffix(A, tc, nres.s, last, jNC);
+        float nMis = neighbor_mis(f, nres.s.f, jNC, K, 1.0f, nres.conf);
+        if (bad(nMis)) nMis = 0.0f;
+        const V3 nc = nMis * f * nres.wSum * jNC;
+        if (!any_bad(nc)) suffixContrib = suffixContrib + nc;
+    }
+    V3 result = suffixContrib * thr / pp;
+    result = result / (float)A.lp.gatherN;
+    result = result / (K + 1.0f);
+    if (any_bad(result)) return v3(0.0f);
+    return result;
+}
+
+template <int KR>
+__global__ __launch_bounds__(BLOCK) void k_final_gather(FrameArgs A) {
+    CX_STACK_DECL
+    uint32_t x, y, p;
+    if (tile_pixel(A, x, y, p)) {
+        const uint4 g = A.gbuf[p];
+        V3 rad = v3(0.0f);
+        if (g.w & CANON_PRIMARY_VALID) {
+            Rng rng;
+            rng.s = (uint64_t)g.x | ((uint64_t)g.y << 32);
+            V3 origin, dir;
+            camera_ray(A, x, y, rng, origin, dir);
+            const uint32_t k = A.k;
+            rad = rad + gather_canon<KR>(A, tc, p, origin, dir, k, rng);
+            for (int gi = 1; gi < A.lp.gatherN; ++gi) rad = rad + gather_not_canon<KR>(A, tc, p, origin, dir, k, rng);
+            if (any_bad(rad)) rad = v3(0.0f);
+            A.gbuf[p] = uint4{(uint32_t)rng.s, (uint32_t)(rng.s >> 32), g.z, g.w};
+        }
+        write_output(A, p, rad);
+    }
+    flush_counters(tc, A.counters);
+}
+
+// ------------------------------------------------------------------------------------------
+// State reset (the uploads of src/Renderer.cpp:155-199)
+// ------------------------------------------------------------------------------------------
+__global__ void k_reset_state(FrameArgs A) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.P) return;
+    const uint64_t defRng = rng_seed(PCG_DEFAULT_STATE).s;
+    PrefixRes pr;
+    pr.s.rng = defRng;
+    pr.s.flags = 0;
+    pr.s.p = 0.0f;
+    pr.s.f = v3(0.0f);
+    pr.s.pathLen = 0.0f;
+    pr.s.postReconF = v3(0.0f);
+    pr.s.reconOutDir = v3(0.0f);
+    pr.s.primary = packed_invalid();
+    pr.s.recon = packed_invalid();
+    pr.s.last = packed_invalid();
+    pr.wSum = 0.0f;
+    pr.conf = 0.0f;
+    SuffixRes sr;
+    sr.s.rng = defRng;
+    sr.s.flags = 0;
+    sr.s.p = 0.0f;
+    sr.s.f = v3(0.0f);
+    sr.s.postReconF = v3(0.0f);
+    sr.s.reconOutDir = v3(0.0f);
+    sr.s.recon = packed_invalid();
+    sr.s.lastPrefix = packed_invalid();
+    sr.wSum = 0.0f;
+    sr.conf = 0.0f;
+    for (int b = 0; b < 2; b++) {
+        store_prefix(A.prefixRes[b], A.P, p, pr);
+        store_suffix(A.suffixRes[b], A.P, p, sr);
+    }
+    A.gbuf[p] = uint4{(uint32_t)defRng, (uint32_t)(defRng >> 32), 0u, 0u};
+    A.neighCount[p] = 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// Standalone traversal (cx_trace_closest / cx_trace_occluded)
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_trace(DScene S, const float* rays, uint32_t n, int anyHit,
+                                                 uint32_t* prim, float* t, float* u, float* v, uint32_t* occ) {
+    __shared__ int s_stack[STACK_DEPTH * BLOCK];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = rays + 8 * (size_t)i;
+    Hit h;
+    const V3 o = V3{r[0], r[1], r[2]}, d = V3{r[3], r[4], r[5]};
+    if (anyHit) {
+        occ[i] = traverse<true>(S, o, d, r[6], r[7], h, s_stack + threadIdx.x, BLOCK) ? 1u : 0u;
+    } else {
+        traverse<false>(S, o, d, r[6], r[7], h, s_stack + threadIdx.x, BLOCK);
+        const bool hit = h.prim != INVALID_PRIM;
+        prim[i] = h.prim;
+        t[i] = hit ? h.t : 0.0f;
+        u[i] = hit ? h.u : 0.0f;
+        v[i] = hit ? h.v : 0.0f;
+    }
+}
+
+// ==========================================================================================
+// Host launchers
+// ==========================================================================================
+
+static dim3 tile_grid(uint32_t W, uint32_t H) { return dim3((W + TILE_X - 1) / TILE_X, (H + TILE_Y - 1) / TILE_Y); }
+
+hipError_t launch_pass(int pass, const FrameArgs& A, hipStream_t s) {
+    const dim3 grid = tile_grid(A.W, A.H);
+    switch (pass) {
+        case CX_PASS_PREFIX_GEN_TEMP: hipLaunchKernelGGL(k_prefix_gen_temp, grid, dim3(BLOCK), 0, s, A); break;
+        case CX_PASS_PREFIX_SPATIAL: hipLaunchKernelGGL(k_prefix_spatial, grid, dim3(BLOCK), 0, s, A); break;
+        case CX_PASS_SUFFIX_GEN_TEMP: hipLaunchKernelGGL(k_suffix_gen_temp, grid, dim3(BLOCK), 0, s, A); break;
+        case CX_PASS_SUFFIX_SPATIAL: hipLaunchKernelGGL(k_suffix_spatial, grid, dim3(BLOCK), 0, s, A); break;
+        case CX_PASS_FINAL_GATHER:
+            if (A.k <= 8) hipLaunchKernelGGL(k_final_gather<8>, grid, dim3(BLOCK), 0, s, A);
+            else hipLaunchKernelGGL(k_final_gather<32>, grid, dim3(BLOCK), 0, s, A);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reset(const FrameArgs& A, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_state, dim3((A.P + 255) / 256), dim3(256), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_counters(unsigned long long* c, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c, n, 0xFFFFFFFFull);
+    return hipGetLastError();
+}
+
+hipError_t launch_store_entries(const FrameArgs& A, const EntryBuild& E, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, E.numValid, 1u);
+    hipLaunchKernelGGL(k_store_entries, tile_grid(A.W, A.H), dim3(BLOCK), 0, s, A, E);
+    return hipGetLastError();
+}
+
+hipError_t launch_entries_from_arrays(const float* pos, const float* r, const uint8_t* valid, uint32_t n,
+                                      const EntryBuild& E, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, E.numValid, 1u);
+    if (n) hipLaunchKernelGGL(k_entries_from_arrays, dim3((n + 255) / 256), dim3(256), 0, s, pos, r, valid, n, E);
+    return hipGetLastError();
+}
+
+size_t sort_temp_bytes(uint32_t n) {
+    size_t bytes = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                              (uint32_t*)nullptr, n, 0, 32, (hipStream_t)0);
+    return bytes;
+}
+
+// Sort entries by Morton key and build the implicit tree bottom-up, one launch per level.
+hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n, void* temp, size_t tempBytes,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (sort_temp_bytes(n) > tempBytes) return hipErrorInvalidValue;
+    size_t bytes = tempBytes;
+    hipError_t err = rocprim::radix_sort_pairs(temp, bytes, E.keys, E.keysAlt, E.vals, E.valsAlt, n, 0, 32, s);
+    if (err != hipSuccess) return err;
+    const uint32_t leaves = 1u << T.levels;
+    const uint32_t leafBase = leaves - 1u;
+    hipLaunchKernelGGL(k_tree_leaves, dim3((leaves + 255) / 256), dim3(256), 0, s, T, E.valsAlt, E.pos, leaves,
+                       leafBase);
+    for (int l = (int)T.levels - 1; l >= 0; l--) {
+        const uint32_t cnt = 1u << l;
+        const uint32_t base = cnt - 1u;
+        hipLaunchKernelGGL(k_tree_level, dim3((cnt + 255) / 256), dim3(256), 0, s, T, base, cnt);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_entries_query(const EntryTree& T, const float* q, uint32_t nq, uint32_t k, uint32_t* idx,
+                                float* dist, uint32_t* count, hipStream_t s) {
+    if (nq == 0) return hipSuccess;
+    const dim3 grid((nq + BLOCK - 1) / BLOCK);
+    if (k <= 8) hipLaunchKernelGGL(k_entries_query<8>, grid, dim3(BLOCK), 0, s, T, q, nq, k, idx, dist, count);
+    else hipLaunchKernelGGL(k_entries_query<32>, grid, dim3(BLOCK), 0, s, T, q, nq, k, idx, dist, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(const DScene& S, const float* rays, uint32_t n, int anyHit, uint32_t* prim, float* t,
+                        float* u, float* v, uint32_t* occ, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_trace, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, S, rays, n, anyHit, prim, t, u, v,
+                       occ);
+    return hipGetLastError();
+}
+
+}  // namespace cx

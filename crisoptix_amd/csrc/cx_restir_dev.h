@@ -1,0 +1,595 @@
+// Per-pixel records and the ReSTIR path logic on the device: PrefixPath / SuffixPath /
+// Reservoir / RestirGBuffer records in chunked structure-of-arrays form, path generation
+// (include/graph/restir/path_gen.h), the hybrid shifts with reconnection Jacobians
+// (prefix_reuse.h, suffix_reuse.h), pairwise MIS and weighted reservoir updates.
+#pragma once
+
+#include "../../include/crisoptix.h"
+#include "cx_scene_dev.h"
+
+namespace cx {
+
+// ---- records ----
+
+// PrefixPath + reservoir fields (PrefixPath.h:7-132, Reservoir.h:8-51).  The PCG32 of the
+// record is stored as its state only (the stream is always the default one).
+struct Prefix {
+    uint64_t rng;
+    uint32_t flags;
+    float p;
+    V3 f;
+    float pathLen;
+    V3 postReconF;
+    V3 reconOutDir;
+    Packed primary, recon, last;
+    CX_HD uint32_t length() const { return flags & 0xFFu; }
+    CX_HD void set_length(uint32_t l) { flags = (flags & ~0xFFu) | (l & 0xFFu); }
+    CX_HD uint32_t recon_idx() const { return (flags >> 8) & 0xFFu; }
+    CX_HD void set_recon_idx(uint32_t r) { flags = (flags & ~0xFF00u) | ((r & 0xFFu) << 8); }
+    CX_HD bool valid() const { return (flags >> 16) & 1u; }
+    CX_HD void set_valid(bool v) { flags = (flags & ~(1u << 16)) | (v ? (1u << 16) : 0u); }
+    CX_HD bool nee() const { return (flags >> 17) & 1u; }
+    CX_HD void set_nee(bool v) { flags = (flags & ~(1u << 17)) | (v ? (1u << 17) : 0u); }
+};
+
+struct PrefixRes {
+    Prefix s;
+    float wSum, conf;
+};
+
+struct Suffix {   // SuffixPath.h:9-104
+    uint64_t rng;
+    uint32_t flags;
+    float p;
+    V3 f;
+    V3 postReconF;
+    V3 reconOutDir;
+    Packed recon, lastPrefix;
+    CX_HD uint32_t length() const { return flags & 0xFFu; }
+    CX_HD void set_length(uint32_t l) { flags = (flags & ~0xFFu) | (l & 0xFFu); }
+    CX_HD uint32_t recon_idx() const { return (flags >> 8) & 0xFFu; }
+    CX_HD void set_recon_idx(uint32_t r) { flags = (flags & ~0xFF00u) | ((r & 0xFFu) << 8); }
+    CX_HD bool valid() const { return (flags >> 16) & 1u; }
+    CX_HD void set_valid(bool v) { flags = (flags & ~(1u << 16)) | (v ? (1u << 16) : 0u); }
+};
+
+struct SuffixRes {
+    Suffix s;
+    float wSum, conf;
+};
+
+// Chunked SoA: chunk c of pixel p is the uint4 at base[c * P + p].
+//  prefix: c0 rngLo rngHi flags p | c1 f.xyz pathLen | c2 postReconF.xyz wSum |
+//          c3 reconOutDir.xyz conf | c4 primary prim bu bv dir.x | c5 primary dir.y dir.z,
+//          recon prim bu | c6 recon bv dir.xyz | c7 last prim bu bv dir.x | c8 last dir.y dir.z - -
+//  suffix: c0 rngLo rngHi flags p | c1 f.xyz wSum | c2 postReconF.xyz conf |
+//          c3 reconOutDir.xyz recon prim | c4 recon bu bv dir.x dir.y | c5 recon dir.z,
+//          last prim bu bv | c6 last dir.xyz -
+//  gbuffer: c0 rngLo rngHi primaryIntValid canonFlags
+static constexpr int PREFIX_CHUNKS = 9;
+static constexpr int SUFFIX_CHUNKS = 7;
+
+// canonFlags bits: the validity the later passes read from the canonical prefix / suffix
+// buffers (prefix_spatial_reuse.cu:217-218, suffix_spatial_reuse.cu:171-172, final_gather.cu:293-294).
+static constexpr uint32_t CANON_PREFIX_VALID = 1u;
+static constexpr uint32_t CANON_PRIMARY_VALID = 2u;
+static constexpr uint32_t CANON_SUFFIX_VALID = 4u;
+
+CX_HD uint32_t fb(float f) { return __builtin_bit_cast(uint32_t, f); }
+CX_HD float bf(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+__device__ __forceinline__ uint4 ld4(const uint4* base, uint32_t P, int c, uint32_t p) {
+    return base[(size_t)c * P + p];
+}
+__device__ __forceinline__ void st4(uint4* base, uint32_t P, int c, uint32_t p, uint4 v) {
+    base[(size_t)c * P + p] = v;
+}
+
+__device__ __forceinline__ uint32_t load_prefix_flags(const uint4* b, uint32_t P, uint32_t p) {
+    return b[p].z;
+}
+
+__device__ __forceinline__ PrefixRes load_prefix(const uint4* b, uint32_t P, uint32_t p) {
+    const uint4 c0 = ld4(b, P, 0, p), c1 = ld4(b, P, 1, p), c2 = ld4(b, P, 2, p), c3 = ld4(b, P, 3, p);
+    const uint4 c4 = ld4(b, P, 4, p), c5 = ld4(b, P, 5, p), c6 = ld4(b, P, 6, p), c7 = ld4(b, P, 7, p);
+    const uint4 c8 = ld4(b, P, 8, p);
+    PrefixRes r;
+    r.s.rng = (uint64_t)c0.x | ((uint64_t)c0.y << 32);
+    r.s.flags = c0.z;
+    r.s.p = bf(c0.w);
+    r.s.f = V3{bf(c1.x), bf(c1.y), bf(c1.z)};
+    r.s.pathLen = bf(c1.w);
+    r.s.postReconF = V3{bf(c2.x), bf(c2.y), bf(c2.z)};
+    r.wSum = bf(c2.w);
+    r.s.reconOutDir = V3{bf(c3.x), bf(c3.y), bf(c3.z)};
+    r.conf = bf(c3.w);
+    r.s.primary = Packed{c4.x, bf(c4.y), bf(c4.z), V3{bf(c4.w), bf(c5.x), bf(c5.y)}};
+    r.s.recon = Packed{c5.z, bf(c5.w), bf(c6.x), V3{bf(c6.y), bf(c6.z), bf(c6.w)}};
+    r.s.last = Packed{c7.x, bf(c7.y), bf(c7.z), V3{bf(c7.w), bf(c8.x), bf(c8.y)}};
+    return r;
+}
+
+__device__ __forceinline__ void store_prefix(uint4* b, uint32_t P, uint32_t p, const PrefixRes& r) {
+    st4(b, P, 0, p, uint4{(uint32_t)r.s.rng, (uint32_t)(r.s.rng >> 32), r.s.flags, fb(r.s.p)});
+    st4(b, P, 1, p, uint4{fb(r.s.f.x), fb(r.s.f.y), fb(r.s.f.z), fb(r.s.pathLen)});
+    st4(b, P, 2, p, uint4{fb(r.s.postReconF.x), fb(r.s.postReconF.y), fb(r.s.postReconF.z), fb(r.wSum)});
+    st4(b, P, 3, p, uint4{fb(r.s.reconOutDir.x), fb(r.s.reconOutDir.y), fb(r.s.reconOutDir.z), fb(r.conf)});
+    st4(b, P, 4, p, uint4{r.s.primary.prim, fb(r.s.primary.bu), fb(r.s.primary.bv), fb(r.s.primary.dir.x)});
+    st4(b, P, 5, p, uint4{fb(r.s.primary.dir.y), fb(r.s.primary.dir.z), r.s.recon.prim, fb(r.s.recon.bu)});
+    st4(b, P, 6, p, uint4{fb(r.s.recon.bv), fb(r.s.recon.dir.x), fb(r.s.recon.dir.y), fb(r.s.recon.dir.z)});
+    st4(b, P, 7, p, uint4{r.s.last.prim, fb(r.s.last.bu), fb(r.s.last.bv), fb(r.s.last.dir.x)});
+    st4(b, P, 8, p, uint4{fb(r.s.last.dir.y), fb(r.s.last.dir.z), 0u, 0u});
+}
+
+__device__ __forceinline__ uint32_t load_suffix_flags(const uint4* b, uint32_t P, uint32_t p) { return b[p].z; }
+
+__device__ __forceinline__ SuffixRes load_suffix(const uint4* b, uint32_t P, uint32_t p) {
+    const uint4 c0 = ld4(b, P, 0, p), c1 = ld4(b, P, 1, p), c2 = ld4(b, P, 2, p), c3 = ld4(b, P, 3, p);
+    const uint4 c4 = ld4(b, P, 4, p), c5 = ld4(b, P, 5, p), c6 = ld4(b, P, 6, p);
+    SuffixRes r;
+    r.s.rng = (uint64_t)c0.x | ((uint64_t)c0.y << 32);
+    r.s.flags = c0.z;
+    r.s.p = bf(c0.w);
+    r.s.f = V3{bf(c1.x), bf(c1.y), bf(c1.z)};
+    r.wSum = bf(c1.w);
+    r.s.postReconF = V3{bf(c2.x), bf(c2.y), bf(c2.z)};
+    r.conf = bf(c2.w);
+    r.s.reconOutDir = V3{bf(c3.x), bf(c3.y), bf(c3.z)};
+    r.s.recon = Packed{c3.w, bf(c4.x), bf(c4.y), V3{bf(c4.z), bf(c4.w), bf(c5.x)}};
+    r.s.lastPrefix = Packed{c5.y, bf(c5.z), bf(c5.w), V3{bf(c6.x), bf(c6.y), bf(c6.z)}};
+    return r;
+}
+
+__device__ __forceinline__ void store_suffix(uint4* b, uint32_t P, uint32_t p, const SuffixRes& r) {
+    st4(b, P, 0, p, uint4{(uint32_t)r.s.rng, (uint32_t)(r.s.rng >> 32), r.s.flags, fb(r.s.p)});
+    st4(b, P, 1, p, uint4{fb(r.s.f.x), fb(r.s.f.y), fb(r.s.f.z), fb(r.wSum)});
+    st4(b, P, 2, p, uint4{fb(r.s.postReconF.x), fb(r.s.postReconF.y), fb(r.s.postReconF.z), fb(r.conf)});
+    st4(b, P, 3, p, uint4{fb(r.s.reconOutDir.x), fb(r.s.reconOutDir.y), fb(r.s.reconOutDir.z), r.s.recon.prim});
+    st4(b, P, 4, p, uint4{fb(r.s.recon.bu), fb(r.s.recon.bv), fb(r.s.recon.dir.x), fb(r.s.recon.dir.y)});
+    st4(b, P, 5, p, uint4{fb(r.s.recon.dir.z), r.s.lastPrefix.prim, fb(r.s.lastPrefix.bu), fb(r.s.lastPrefix.bv)});
+    st4(b, P, 6, p, uint4{fb(r.s.lastPrefix.dir.x), fb(r.s.lastPrefix.dir.y), fb(r.s.lastPrefix.dir.z), 0u});
+}
+
+// ---- neighbour-search structures ----
+
+// Per-frame entry records before sorting (one slot per pixel / input entry).
+struct EntryBuild {
+    float4* pos;          // [n] pos.xyz, radius
+    uint8_t* valid;       // [n]
+    uint32_t* keys;       // [n] Morton key, 0xFFFFFFFF for invalid entries
+    uint32_t* vals;       // [n] entry index
+    uint32_t* keysAlt;    // sort outputs
+    uint32_t* valsAlt;
+    uint32_t* numValid;   // device scalar
+    V3 lo, scale;         // Morton quantisation: (q - lo) * scale in [0, 1023]
+};
+
+// Implicit complete binary tree over Morton-sorted valid entries: leaf j holds sorted
+// entries [j * leafSize, (j + 1) * leafSize); node i has children 2i+1, 2i+2; the leaf level
+// starts at node 2^levels - 1.  Boxes are padded unions of the entries' radius boxes.
+struct EntryTree {
+    float4* entrySorted;  // [capacity] pos.xyz, radius
+    uint32_t* entryPixel; // [capacity]
+    float4* lo;           // [2^(levels+1) - 1]
+    float4* hi;
+    const uint32_t* numValid;
+    uint32_t levels;
+    uint32_t leafSize;
+};
+
+// ---- frame parameters passed by value to every pass kernel ----
+
+struct FrameArgs {
+    DScene S;
+    cx_launch_params lp;
+    V3 camPos, camU, camV, camW;
+    uint32_t W, H, P;
+    uint32_t front, back;
+    uint32_t copyPrefixToBack;   // prefix_gen_temp_reuse.cu:164-169
+    uint4* prefixRes[2];
+    uint4* suffixRes[2];
+    uint4* gbuf;
+    float* out;
+    EntryTree tree;
+    uint32_t* neighIdx;          // [P][k]
+    float* neighDist;            // [P][k]
+    uint32_t* neighCount;        // [P]
+    uint32_t k;
+    unsigned long long* counters;   // [0] closest rays, [1] shadow rays, [2] total neigh, [3] min, [4] max
+};
+
+CX_HD bool is_pt(const cx_launch_params& lp) { return lp.rendererType == CX_RESTIR_PT; }
+CX_HD bool is_cond(const cx_launch_params& lp) { return lp.rendererType == CX_CONDITIONAL_RESTIR; }
+
+// ---- reservoir (Reservoir.h:8-51) ----
+
+static constexpr float CONFIDENCE_MAX = 50.0f;
+
+template <typename R, typename T>
+__device__ __forceinline__ bool res_update(R& res, const T& sample, float w, Rng& rng, float inConf) {
+    if (isnan(w) || isinf(w)) w = 0.0f;
+    res.wSum += w;
+    res.conf += inConf;
+    res.conf = gmin(res.conf, CONFIDENCE_MAX);
+    if (rng.f() < w / res.wSum) {
+        res.s = sample;
+        return true;
+    }
+    return false;
+}
+
+template <typename R>
+__device__ __forceinline__ void res_finalize(R& res) {
+    const float pHat = luminance(res.s.f);
+    if (pHat == 0.0f) res.wSum = 0.0f;
+    else res.wSum /= pHat;
+}
+
+// ---- pairwise MIS, Jacobian, neighbour selection (path_gen.h:12-72) ----
+
+CX_HD float canonical_mis(V3 atBasis, V3 atNeighbor, float J, float K, float canonM, float neighM) {
+    float w = 1.0f;
+    if (luminance(atBasis) > 0.0f) {
+        const float b = luminance(atBasis) * canonM;
+        w = b / (b + luminance(atNeighbor) * J * neighM * K);
+    }
+    return w;
+}
+
+CX_HD float neighbor_mis(V3 atBasis, V3 atNeighbor, float J, float K, float canonM, float neighM) {
+    float w = 0.0f;
+    if (luminance(atNeighbor) > 0.0f) {
+        w = luminance(atNeighbor) * neighM / (luminance(atNeighbor) * neighM + luminance(atBasis) * J * canonM / K);
+    }
+    return w;
+}
+
+CX_HD void select_neighbor(uint32_t px, uint32_t py, uint32_t R, Rng& rng, uint32_t& nx, uint32_t& ny) {
+    const uint32_t W = 2 * R + 1;
+    const uint32_t x = (px + R) - (rng.u32() % W);
+    const uint32_t y = (py + R) - (rng.u32() % W);
+    if (x == px && y == py) {
+        nx = px + 1;
+        ny = py + 1;
+        return;
+    }
+    nx = x;
+    ny = y;
+}
+
+CX_HD float recon_jacobian(V3 oldXi, V3 newXi, V3 tp, V3 tn) {
+    const float term1 = fabsf(dot(normalize(newXi - tp), tn) / dot(normalize(oldXi - tp), tn));
+    const float term2 = dot(tp - oldXi, tp - oldXi);
+    const float term3 = dot(tp - newXi, tp - newXi);
+    const float result = term1 * term2 / term3;
+    if (isinf(result) || isnan(result)) return 0.0f;
+    return gmax(0.0f, result);
+}
+
+// ---- path generation ----
+
+__device__ __forceinline__ void trace_prefix(const FrameArgs& A, TraceCtx& tc, Prefix& pre, V3 origin, V3 dir,
+                                             Rng& rng) {   // path_gen.h:74-250
+    const DScene& S = A.S;
+    const cx_launch_params& lp = A.lp;
+    V3 currentPos = origin;
+    V3 currentDir = dir;
+    pre.flags = 0u;
+    pre.pathLen = 0.0f;
+    pre.rng = rng.s;
+    pre.p = 1.0f;
+    pre.f = v3(1.0f);
+    pre.postReconF = v3(1.0f);
+    pre.set_valid(true);
+    Inter it;
+    it.valid = false;
+    it.prim = 0;
+    it.bu = it.bv = 0.0f;
+    it.pos = it.dir = it.nrm = v3(0.0f);
+    const uint32_t maxLen = is_pt(lp) ? (uint32_t)lp.maxPathLen : (uint32_t)lp.prefixLen;
+    bool postRecon = false;
+    for (uint32_t traceIdx = 0; traceIdx < maxLen; ++traceIdx) {
+        trace_surface(S, tc, currentPos, currentDir, 1e-3f, 1e16f, it);
+        if (!it.valid) {
+            pre.set_valid(false);
+            return;
+        }
+        pre.pathLen += distance(currentPos, it.pos);
+        pre.set_length(pre.length() + 1);
+        if (pre.length() == 1) pre.primary = it.pack();
+        if (is_cond(lp) && traceIdx == maxLen - 1) {
+            if (!postRecon) {
+                pre.recon = it.pack();
+                pre.set_recon_idx(pre.length());
+                pre.reconOutDir = v3(0.0f);
+            }
+            pre.set_valid(true);
+            pre.last = it.pack();
+            return;
+        }
+        const bool forceNee = is_pt(lp) && traceIdx == maxLen - 1;
+        if (rng.f() < lp.neeProb || forceNee) {
+            pre.p *= lp.neeProb;
+            const EmitterSample es = sample_emitter(S, rng);
+            const V3 lightDir = normalize(es.pos - it.pos);
+            const float dist = length(es.pos - it.pos);
+            if (trace_occluded(S, tc, it.pos, lightDir, 1e-3f, dist)) {
+                pre.set_valid(false);
+                return;
+            }
+            if (!postRecon) {   // path_gen.h:165-187 re-traces the same ray
+                trace_surface(S, tc, currentPos, currentDir, 1e-3f, 1e16f, it);
+                if (it.valid) {
+                    pre.recon = it.pack();
+                    pre.set_recon_idx(pre.length());
+                    pre.reconOutDir = currentDir;
+                    postRecon = true;
+                } else {
+                    pre.set_valid(false);
+                    return;
+                }
+            }
+            const BrdfEval be = ggx_eval(S, it, lightDir);
+            if (be.pdf <= 0.0f) {
+                pre.set_valid(false);
+                return;
+            }
+            const float distFactor = 1.0f / (dist * dist);
+            pre.f = pre.f * (be.brdf * es.color * distFactor);
+            if (postRecon) pre.postReconF = pre.postReconF * (be.brdf * es.color * distFactor);
+            if (pre.length() == 1) pre.f = pre.f + be.emission * distFactor;
+            pre.set_nee(true);
+            pre.set_valid(true);
+            pre.last = it.pack();
+            return;
+        }
+        const BrdfSample bs = ggx_sample(S, it, rng);
+        if (bs.pdf <= 0.0f) {
+            pre.set_valid(false);
+            return;
+        }
+        const float reconDistance = distance(currentPos, it.pos);
+        const bool canRecon =
+            reconDistance > lp.reconMinDistance && bs.roughness > lp.reconMinRoughness && pre.length() >= 2;
+        if (!postRecon && canRecon) {
+            pre.recon = it.pack();
+            pre.set_recon_idx(pre.length());
+            pre.reconOutDir = bs.outDir;
+            postRecon = true;
+        }
+        currentPos = it.pos;
+        currentDir = bs.outDir;
+        pre.f = pre.f * bs.brdf;
+        pre.p *= bs.pdf * (1.0f - lp.neeProb);
+        if (postRecon) pre.postReconF = pre.postReconF * bs.brdf;
+    }
+    pre.last = it.pack();
+}
+
+__device__ __forceinline__ void trace_suffix(const FrameArgs& A, TraceCtx& tc, Suffix& suf, const Prefix& pre,
+                                             Rng& rng) {   // path_gen.h:252-387
+    const DScene& S = A.S;
+    const cx_launch_params& lp = A.lp;
+    suf.f = v3(1.0f);
+    suf.p = 1.0f;
+    suf.postReconF = v3(1.0f);
+    suf.rng = rng.s;
+    suf.lastPrefix = pre.last;
+    suf.flags = 0u;
+    if (!pre.valid() || pre.nee()) return;
+    const uint32_t maxLen = (uint32_t)lp.maxPathLen - pre.length();
+    Inter it = rehydrate(S, pre.last);
+    if (!it.valid) return;
+    bool postRecon = false;
+    V3 lastDir = v3(0.0f);
+    for (uint32_t vertIdx = 0; vertIdx < maxLen; ++vertIdx) {
+        const V3 lastPos = it.pos;
+        if (suf.length() > 0) {
+            trace_surface(S, tc, lastPos, lastDir, 1e-3f, 1e16f, it);
+            if (!it.valid) return;
+        }
+        const bool forceNee = vertIdx == maxLen - 1;
+        if (rng.f() < lp.neeProb || forceNee) {
+            if (!forceNee) suf.p *= lp.neeProb;
+            const EmitterSample em = sample_emitter(S, rng);
+            const V3 lightVec = em.pos - it.pos;
+            const float lightLen = length(lightVec);
+            const V3 lightDir = normalize(lightVec);
+            if (any_bad(lightDir)) return;
+            if (trace_occluded(S, tc, it.pos, lightDir, 1e-3f, lightLen)) return;
+            const BrdfEval be = ggx_eval(S, it, lightDir);
+            if (be.pdf <= 0.0f) return;
+            suf.set_length(suf.length() + 1);
+            if (!postRecon) {
+                Inter ri;
+                ri.valid = false;
+                trace_surface(S, tc, it.pos, lightDir, 1e-3f, 1e16f, ri);
+                if (!ri.valid) return;
+                suf.recon = ri.pack();
+                suf.set_recon_idx(suf.length());
+                suf.reconOutDir = v3(0.0f);
+                postRecon = true;
+            }
+            const float distFactor = 1.0f / (lightLen * lightLen);
+            suf.f = suf.f * (be.brdf * em.color * distFactor);
+            suf.postReconF = suf.postReconF * (em.color * distFactor);
+            if (suf.recon_idx() < suf.length()) suf.postReconF = suf.postReconF * be.brdf;
+            suf.set_valid(true);
+            return;
+        }
+        const BrdfSample bs = ggx_sample(S, it, rng);
+        if (bs.pdf <= 0.0f) return;
+        suf.p *= bs.pdf * (1.0f - lp.neeProb);
+        suf.f = suf.f * bs.brdf;
+        if (postRecon) suf.postReconF = suf.postReconF * bs.brdf;
+        const float reconDistance = distance(lastPos, it.pos);
+        const bool canRecon =
+            reconDistance > lp.reconMinDistance && bs.roughness > lp.reconMinRoughness && pre.length() >= 1;
+        if (!postRecon && canRecon) {
+            suf.recon = it.pack();
+            suf.set_recon_idx(pre.length());   // path_gen.h:378
+            suf.reconOutDir = bs.outDir;
+            postRecon = true;
+        }
+        lastDir = bs.outDir;
+        suf.set_length(suf.length() + 1);
+    }
+}
+
+__device__ __forceinline__ V3 trace_complete_path(const FrameArgs& A, TraceCtx& tc, V3 origin, V3 dir,
+                                                  Rng& rng) {   // path_gen.h:389-472
+    const DScene& S = A.S;
+    const cx_launch_params& lp = A.lp;
+    V3 currentPos = origin;
+    V3 currentDir = dir;
+    uint32_t depth = 0;
+    V3 radiance = v3(0.0f);
+    V3 thr = v3(1.0f);
+    for (uint32_t traceIdx = 0; traceIdx < (uint32_t)lp.maxPathLen; ++traceIdx) {
+        Inter it;
+        it.valid = false;
+        trace_surface(S, tc, currentPos, currentDir, 1e-3f, 1e16f, it);
+        if (!it.valid) break;
+        ++depth;
+        if (rng.f() < lp.neeProb) {
+            thr = thr / lp.neeProb;
+            const EmitterSample es = sample_emitter(S, rng);
+            const V3 lightDir = normalize(es.pos - it.pos);
+            const float dist = length(es.pos - it.pos);
+            if (trace_occluded(S, tc, it.pos, lightDir, 1e-3f, dist)) return v3(0.0f);
+            const BrdfEval be = ggx_eval(S, it, lightDir);
+            const float distFactor = 1.0f / (dist * dist);
+            radiance = thr * be.brdf * es.color * distFactor;
+            if (depth == 1) radiance = radiance + be.emission;
+            break;
+        }
+        const BrdfSample bs = ggx_sample(S, it, rng);
+        if (bs.pdf <= 0.0f) break;
+        currentPos = it.pos;
+        currentDir = bs.outDir;
+        thr = thr * (bs.brdf / bs.pdf);
+        thr = thr / (1.0f - lp.neeProb);
+    }
+    return radiance;
+}
+
+__device__ __forceinline__ bool trace_prefix_fg(const FrameArgs& A, TraceCtx& tc, V3& thr, float& p, Inter& last,
+                                                V3 origin, V3 dir, uint32_t maxLen,
+                                                Rng& rng) {   // path_gen.h:474-531
+    const DScene& S = A.S;
+    V3 currentPos = origin;
+    V3 currentDir = dir;
+    thr = v3(1.0f);
+    p = 1.0f;
+    last.valid = false;
+    for (uint32_t i = 0; i < maxLen; ++i) {
+        trace_surface(S, tc, currentPos, currentDir, 1e-3f, 1e16f, last);
+        if (!last.valid) return false;
+        const BrdfSample bs = ggx_sample(S, last, rng);
+        if (bs.pdf <= 0.0f) return false;
+        currentPos = last.pos;
+        currentDir = bs.outDir;
+        thr = thr * bs.brdf;
+        p *= bs.pdf;
+    }
+    return true;
+}
+
+// ---- hybrid shifts ----
+
+// prefix_reuse.h:10-99: shift src onto dst's primary hit; returns f in dst's domain.
+__device__ __forceinline__ V3 shift_prefix(const FrameArgs& A, TraceCtx& tc, const Prefix& src, const Prefix& dst,
+                                           float& J) {
+    const DScene& S = A.S;
+    J = 0.0f;
+    if (!src.valid() || src.recon_idx() < 2) return v3(0.0f);
+    const Inter srcPrimary = rehydrate(S, src.primary);
+    if (!srcPrimary.valid) return v3(0.0f);
+    Inter curr = rehydrate(S, dst.primary);
+    if (!curr.valid) return v3(0.0f);
+    Inter srcRecon = rehydrate(S, src.recon);
+    if (!srcRecon.valid) return v3(0.0f);
+    const int rvc = (int)(src.recon_idx() - 2u);
+    const uint32_t reconVertCount = rvc > 0 ? (uint32_t)rvc : 0u;
+    Rng srng;
+    srng.s = src.rng;
+    V3 thr = v3(1.0f);
+    for (uint32_t i = 0; i < reconVertCount; ++i) {
+        srng.f();
+        const BrdfSample bs = ggx_sample(S, curr, srng);
+        if (bs.pdf <= 0.0f) return v3(0.0f);
+        thr = thr * bs.brdf;
+        trace_surface(S, tc, curr.pos, bs.outDir, 1e-3f, 1e16f, curr);
+        if (!curr.valid) return v3(0.0f);
+    }
+    const V3 reconVec = srcRecon.pos - curr.pos;
+    const float reconLen = length(reconVec);
+    const V3 reconDir = normalize(reconVec);
+    if (isinf(reconLen) || any_bad(reconDir)) return v3(0.0f);
+    if (trace_occluded(S, tc, curr.pos, reconDir, 1e-3f, reconLen)) return v3(0.0f);
+    const BrdfEval b1 = ggx_eval(S, curr, reconDir);
+    if (b1.pdf <= 0.0f) return v3(0.0f);
+    thr = thr * b1.brdf;
+    srcRecon.dir = reconDir;
+    const BrdfEval b2 = ggx_eval(S, srcRecon, src.reconOutDir);
+    if (b2.pdf <= 0.0f) return v3(0.0f);
+    thr = thr * b2.brdf;
+    J = recon_jacobian(srcPrimary.pos, curr.pos, srcRecon.pos, srcRecon.nrm);
+    const V3 f = thr * src.postReconF;
+    return V3{gmax(0.0f, f.x), gmax(0.0f, f.y), gmax(0.0f, f.z)};
+}
+
+// suffix_reuse.h:10-108: shift src onto the destination's last prefix vertex.
+__device__ __forceinline__ V3 shift_suffix(const FrameArgs& A, TraceCtx& tc, const Suffix& src, const Inter& dstLast,
+                                           float& J) {
+    const DScene& S = A.S;
+    J = 0.0f;
+    if (!src.valid()) return v3(0.0f);
+    if (!dstLast.valid) return v3(0.0f);
+    const Inter srcLast = rehydrate(S, src.lastPrefix);
+    if (!srcLast.valid) return v3(0.0f);
+    Inter srcRecon = rehydrate(S, src.recon);
+    if (!srcRecon.valid) return v3(0.0f);
+    const int rvc = (int)(src.recon_idx() - 1u);
+    const uint32_t reconVertCount = rvc > 0 ? (uint32_t)rvc : 0u;
+    Inter curr = dstLast;
+    Rng srng;
+    srng.s = src.rng;
+    V3 thr = v3(1.0f);
+    for (uint32_t i = 0; i < reconVertCount; ++i) {
+        srng.f();
+        const BrdfSample bs = ggx_sample(S, curr, srng);
+        if (bs.pdf <= 0.0f) return v3(0.0f);
+        thr = thr * bs.brdf;
+        const V3 oldPos = curr.pos;
+        trace_surface(S, tc, oldPos, bs.outDir, 1e-3f, 1e16f, curr);
+        if (!curr.valid) return v3(0.0f);
+    }
+    const V3 reconVec = srcRecon.pos - curr.pos;
+    const float reconLen = length(reconVec);
+    const V3 reconDir = normalize(reconVec);
+    if (any_bad(reconDir)) return v3(0.0f);
+    if (trace_occluded(S, tc, curr.pos, reconDir, 1e-3f, reconLen)) return v3(0.0f);
+    const BrdfEval b1 = ggx_eval(S, curr, reconDir);
+    if (b1.pdf <= 0.0f) return v3(0.0f);
+    thr = thr * b1.brdf;
+    if (src.recon_idx() < src.length()) {
+        srcRecon.dir = reconDir;
+        if (is_zero(src.reconOutDir)) return v3(0.0f);
+        const BrdfEval b2 = ggx_eval(S, srcRecon, src.reconOutDir);
+        if (b2.pdf <= 0.0f) return v3(0.0f);
+        thr = thr * b2.brdf;
+    }
+    J = recon_jacobian(srcLast.pos, curr.pos, srcRecon.pos, srcRecon.nrm);
+    return thr * src.postReconF;
+}
+
+// suffix_reuse.h:111-123
+__device__ __forceinline__ V3 shift_suffix(const FrameArgs& A, TraceCtx& tc, const Suffix& src, const Suffix& dst,
+                                           float& J) {
+    const Inter dl = rehydrate(A.S, dst.lastPrefix);
+    if (!dl.valid) return v3(0.0f);
+    return shift_suffix(A, tc, src, dl, J);
+}
+
+}  // namespace cx

@@ -1,0 +1,451 @@
+// Scene access on the device: BVH traversal (closest hit and occlusion), hit records,
+// rehydration of packed interactions, the GGX BSDF and emitter sampling.
+//
+// Traversal replaces optixTrace over the triangle GAS/IAS (include/graph/trace.h:25-88).
+// The result of a query is a pure function of the ray and the triangle set:
+//   closest hit  = the minimum (t, primitive) over all triangles whose watertight test
+//                  (Woop, Benthin, Wald, JCGT 2013) reports t in [tmin, tmax];
+//   occlusion    = whether any such triangle exists.
+// Node boxes are padded outward at build time and the box test carries a relative slack,
+// so traversal only ever visits more nodes than needed; which BVH is used, and in which
+// order children are visited, cannot change a result.
+#pragma once
+
+#include "cx_dev.h"
+
+namespace cx {
+
+static constexpr uint32_t INVALID_PRIM = 0xFFFFFFFFu;
+static constexpr float PI_F = 3.14159265358979323846264f;   // shader/brdf.cu:12
+
+// Binary BVH node, 64 bytes:
+//   a = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)   b = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//   c = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)   d = (ref0, ref1, -, -)
+// ref >= 0: inner node index; ref < 0: leaf, ~ref = (first triangle slot << 4) | count.
+// Triangle slots hold 3 float4: (v0.xyz, primitive index bits), (v1.xyz, 0), (v2.xyz, 0).
+struct DMaterial {
+    float4 diffRough;   // diffuse rgb, roughness
+    float4 spec;        // specF0 rgb, -
+    float4 emissive;    // emissive rgb, -
+};
+
+struct DScene {
+    const float4* __restrict__ nodes;
+    const float4* __restrict__ tris;
+    const float* __restrict__ pos;        // [V][3]
+    const float* __restrict__ nrm;        // [V][3]
+    const uint32_t* __restrict__ idx;     // [T][3]
+    const uint32_t* __restrict__ triMat;  // [T]
+    const DMaterial* __restrict__ mats;
+    const float* __restrict__ emitFaces;  // [F][9]
+    const uint32_t* __restrict__ emitOff; // [E+1]
+    const float* __restrict__ emitColor;  // [E][3]
+    uint32_t numTris;
+    uint32_t numEmitters;
+    int32_t root;
+    uint32_t pad;
+};
+
+static constexpr int STACK_DEPTH = 48;
+
+struct RayPre {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ RayPre ray_pre(V3 d) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    RayPre r;
+    r.kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
+    r.kx = r.kz == 2 ? 0 : r.kz + 1;
+    r.ky = r.kx == 2 ? 0 : r.kx + 1;
+    if (comp(d, r.kz) < 0.0f) {
+        const int t = r.kx;
+        r.kx = r.ky;
+        r.ky = t;
+    }
+    const float dz = comp(d, r.kz);
+    r.Sx = comp(d, r.kx) / dz;
+    r.Sy = comp(d, r.ky) / dz;
+    r.Sz = 1.0f / dz;
+    return r;
+}
+
+// Watertight ray/triangle test.  u weights v1 and v weights v2 (optixGetTriangleBarycentrics).
+__device__ __forceinline__ bool tri_hit(V3 o, const RayPre& r, V3 v0, V3 v1, V3 v2, float tmin, float tmax,
+                                        float& t, float& u, float& v) {
+    const V3 A = v0 - o, B = v1 - o, C = v2 - o;
+    const float Akx = comp(A, r.kx), Aky = comp(A, r.ky), Akz = comp(A, r.kz);
+    const float Bkx = comp(B, r.kx), Bky = comp(B, r.ky), Bkz = comp(B, r.kz);
+    const float Ckx = comp(C, r.kx), Cky = comp(C, r.ky), Ckz = comp(C, r.kz);
+    const float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
+    const float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
+    const float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    const float tt = T / det;
+    if (!(tt >= tmin && tt <= tmax)) return false;
+    t = tt;
+    u = V / det;
+    v = W / det;
+    return true;
+}
+
+struct Hit {
+    uint32_t prim;
+    float t, u, v;
+};
+
+// Slab test of both children of a node; the far plane carries a relative slack so that a
+// triangle whose computed t equals the current best is never pruned.
+__device__ __forceinline__ void box2(const float4 a, const float4 b, const float4 c, V3 o, V3 inv, float tmin,
+                                     float tcap, bool& hit0, bool& hit1, float& tn0, float& tn1) {
+    const float l0x = (a.x - o.x) * inv.x, h0x = (a.y - o.x) * inv.x;
+    const float l0y = (a.z - o.y) * inv.y, h0y = (a.w - o.y) * inv.y;
+    const float l0z = (c.x - o.z) * inv.z, h0z = (c.y - o.z) * inv.z;
+    const float l1x = (b.x - o.x) * inv.x, h1x = (b.y - o.x) * inv.x;
+    const float l1y = (b.z - o.y) * inv.y, h1y = (b.w - o.y) * inv.y;
+    const float l1z = (c.z - o.z) * inv.z, h1z = (c.w - o.z) * inv.z;
+    const float n0 = fmaxf(fmaxf(fminf(l0x, h0x), fminf(l0y, h0y)), fmaxf(fminf(l0z, h0z), tmin));
+    const float f0 = fminf(fminf(fmaxf(l0x, h0x), fmaxf(l0y, h0y)), fminf(fmaxf(l0z, h0z), tcap));
+    const float n1 = fmaxf(fmaxf(fminf(l1x, h1x), fminf(l1y, h1y)), fmaxf(fminf(l1z, h1z), tmin));
+    const float f1 = fminf(fminf(fmaxf(l1x, h1x), fmaxf(l1y, h1y)), fminf(fmaxf(l1z, h1z), tcap));
+    hit0 = n0 <= f0 * 1.00002f + 1e-30f;
+    hit1 = n1 <= f1 * 1.00002f + 1e-30f;
+    tn0 = n0;
+    tn1 = n1;
+}
+
+__device__ __forceinline__ V3 load_v3(const float* p, uint32_t i) { return V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+
+// Traversal with a per-thread stack column in LDS (stack[i * stride]).
+template <bool ANY>
+__device__ __forceinline__ bool traverse(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit& best,
+                                         int* stack, int stride) {
+    best.prim = INVALID_PRIM;
+    best.t = tmax;
+    best.u = 0.0f;
+    best.v = 0.0f;
+    if (S.numTris == 0) return false;
+    const RayPre rp = ray_pre(d);
+    const V3 inv = V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    int sp = 0;
+    int ref = S.root;
+    while (true) {
+        if (ref >= 0) {
+            const float4* n = S.nodes + 4 * (size_t)ref;
+            const float4 a = n[0], b = n[1], c = n[2];
+            const int4 dd = __builtin_bit_cast(int4, n[3]);
+            bool h0, h1;
+            float t0, t1;
+            box2(a, b, c, o, inv, tmin, best.t, h0, h1, t0, t1);
+            if (h0 && h1) {
+                const bool first0 = t0 <= t1;
+                stack[sp * stride] = first0 ? dd.y : dd.x;
+                ++sp;
+                ref = first0 ? dd.x : dd.y;
+                continue;
+            }
+            if (h0) { ref = dd.x; continue; }
+            if (h1) { ref = dd.y; continue; }
+        } else {
+            const uint32_t code = ~(uint32_t)ref;
+            const uint32_t first = code >> 4, cnt = code & 15u;
+            for (uint32_t i = 0; i < cnt; i++) {
+                const float4* tv = S.tris + 3 * (size_t)(first + i);
+                const float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+                float t, u, v;
+                if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, tmax,
+                             t, u, v))
+                    continue;
+                const uint32_t prim = __builtin_bit_cast(uint32_t, p0.w);
+                if (ANY) {
+                    best.prim = prim;
+                    best.t = t;
+                    return true;
+                }
+                if (t < best.t || (t == best.t && prim < best.prim)) {
+                    best.prim = prim;
+                    best.t = t;
+                    best.u = u;
+                    best.v = v;
+                }
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        ref = stack[sp * stride];
+    }
+    return best.prim != INVALID_PRIM;
+}
+
+// ---- interactions (include/graph/Interaction.h) ----
+
+// PackedInteraction, one flattened mesh (instance 0): prim == INVALID_PRIM means !valid.
+struct Packed {
+    uint32_t prim;
+    float bu, bv;
+    V3 dir;
+};
+
+CX_HD Packed packed_invalid() { return Packed{INVALID_PRIM, 0.0f, 0.0f, V3{0.0f, 0.0f, 0.0f}}; }
+
+struct Inter {
+    uint32_t prim;
+    float bu, bv;
+    V3 pos, dir, nrm;
+    bool valid;
+    CX_HD Packed pack() const {
+        Packed p;
+        p.prim = valid ? prim : INVALID_PRIM;
+        p.bu = bu;
+        p.bv = bv;
+        p.dir = dir;
+        return p;
+    }
+};
+
+// InterpolateBary (Interaction.h:18-25)
+CX_HD V3 interp(float u, float v, V3 a, V3 b, V3 c) { return ((1.0f - u - v) * a + u * b) + v * c; }
+
+// Rehydration (Interaction.h:115-172): normals are interpolated and not normalised.
+__device__ __forceinline__ Inter rehydrate(const DScene& S, const Packed& p) {
+    Inter r;
+    r.prim = p.prim;
+    r.bu = p.bu;
+    r.bv = p.bv;
+    r.dir = p.dir;
+    r.valid = p.prim != INVALID_PRIM && p.prim < S.numTris;
+    r.pos = v3(0.0f);
+    r.nrm = v3(0.0f);
+    if (!r.valid) return r;
+    const uint32_t i0 = S.idx[3 * p.prim], i1 = S.idx[3 * p.prim + 1], i2 = S.idx[3 * p.prim + 2];
+    r.pos = interp(p.bu, p.bv, load_v3(S.pos, i0), load_v3(S.pos, i1), load_v3(S.pos, i2));
+    r.nrm = interp(p.bu, p.bv, load_v3(S.nrm, i0), load_v3(S.nrm, i1), load_v3(S.nrm, i2));
+    return r;
+}
+
+// Per-thread traversal context and ray counters.
+struct TraceCtx {
+    int* stack;
+    int stride;
+    uint32_t closest;
+    uint32_t shadow;
+};
+
+// TraceWithDataPointer<Interaction> + __closesthit__mesh / __miss__main
+// (trace.h:25-45, mesh_hit.cu:17-69, miss.cu:11-19): a miss keeps every other field.
+__device__ __forceinline__ void trace_surface(const DScene& S, TraceCtx& tc, V3 o, V3 d, float tmin, float tmax,
+                                              Inter& io) {
+    Hit h;
+    tc.closest++;
+    traverse<false>(S, o, d, tmin, tmax, h, tc.stack, tc.stride);
+    io.dir = d;
+    if (h.prim == INVALID_PRIM) {
+        io.valid = false;
+        return;
+    }
+    io.valid = true;
+    io.prim = h.prim;
+    io.bu = h.u;
+    io.bv = h.v;
+    const uint32_t i0 = S.idx[3 * h.prim], i1 = S.idx[3 * h.prim + 1], i2 = S.idx[3 * h.prim + 2];
+    io.pos = interp(h.u, h.v, load_v3(S.pos, i0), load_v3(S.pos, i1), load_v3(S.pos, i2));
+    io.nrm = normalize(interp(h.u, h.v, load_v3(S.nrm, i0), load_v3(S.nrm, i1), load_v3(S.nrm, i2)));
+}
+
+// TraceOcclusion (trace.h:63-88)
+__device__ __forceinline__ bool trace_occluded(const DScene& S, TraceCtx& tc, V3 o, V3 d, float tmin, float tmax) {
+    Hit h;
+    tc.shadow++;
+    return traverse<true>(S, o, d, tmin, tmax, h, tc.stack, tc.stride);
+}
+
+// ---- BSDF (shader/brdf.cu) ----
+
+struct BrdfEval {
+    V3 brdf;
+    float pdf;
+    V3 emission;
+    float roughness;
+};
+
+struct BrdfSample {
+    V3 outDir, brdf;
+    float pdf;
+    float roughness;
+};
+
+CX_HD float d_ggx(float NdotH, float roughness) {   // brdf.cu:99-104
+    const float a2 = roughness * roughness;
+    const float d = (NdotH * a2 - NdotH) * NdotH + 1.0f;
+    return a2 / (PI_F * d * d);
+}
+
+CX_HD float v_smith(float NdotL, float NdotV, float roughness) {   // brdf.cu:106-112
+    const float a2 = roughness * roughness;
+    const float lv = NdotL * sqrtf(NdotV * NdotV * (1 - a2) + a2);
+    const float ll = NdotV * sqrtf(NdotL * NdotL * (1 - a2) + a2);
+    return 0.5f / (lv + ll);
+}
+
+struct Mat {
+    V3 diff, spec, emis;
+    float rough;
+};
+
+// Material fetch including specF0 = mix(diffColor, specF0, metal = 0) (brdf.cu:123-138).
+__device__ __forceinline__ Mat load_mat(const DScene& S, uint32_t prim) {
+    const DMaterial& m = S.mats[S.triMat[prim]];
+    Mat r;
+    r.diff = V3{m.diffRough.x, m.diffRough.y, m.diffRough.z};
+    r.rough = m.diffRough.w;
+    const V3 spec = V3{m.spec.x, m.spec.y, m.spec.z};
+    const float metal = 0.0f;
+    r.spec = r.diff * (1.0f - metal) + spec * metal;
+    r.emis = V3{m.emissive.x, m.emissive.y, m.emissive.z};
+    return r;
+}
+
+__device__ __forceinline__ V3 spec_term(const Mat& m, float nDotH, float nDotL, float nDotV, float lDotH) {
+    V3 specBrdf = v3(0.0f);
+    if ((double)dot(m.spec, m.spec) > 1e-6) {
+        const float d = d_ggx(nDotH, m.rough);
+        const float v = v_smith(nDotL, nDotV, m.rough);
+        const float p = pow5(gmax(0.0f, 1.0f - lDotH));
+        const V3 f = m.spec + (v3(1.0f) - m.spec) * p;
+        specBrdf = d * v * f;
+    }
+    return specBrdf;
+}
+
+// __direct_callable__ggx_eval (brdf.cu:115-190)
+__device__ __forceinline__ BrdfEval ggx_eval(const DScene& S, const Inter& it, V3 outDir) {
+    const Mat m = load_mat(S, it.prim);
+    BrdfEval r;
+    const V3 lightDir = outDir;
+    const V3 viewDir = -it.dir;
+    const V3 normal = it.nrm;
+    const float nDotV = dot(normal, viewDir);
+    const float nDotL = dot(normal, lightDir);
+    if (nDotV <= 0.0f || nDotL < 0.0f) {
+        r.brdf = v3(0.0f);
+        r.emission = v3(0.0f);
+        r.pdf = 0.0f;
+        r.roughness = 0.0f;
+        return r;
+    }
+    const V3 diffBrdf = m.diff / PI_F;
+    const float diffProb = dot(m.diff, v3(1.0f)) / (dot(m.diff, v3(1.0f)) + dot(m.spec, v3(1.0f)));
+    const V3 halfway = normalize(lightDir + viewDir);
+    const float nDotH = dot(halfway, normal);
+    const float lDotH = dot(halfway, lightDir);
+    const V3 specBrdf = spec_term(m, nDotH, nDotL, nDotV, lDotH);
+    r.brdf = (diffBrdf + specBrdf) * nDotL;
+    r.pdf = diffProb * nDotL / PI_F + (1.0f - diffProb) * d_ggx(nDotH, m.rough) * nDotH / (4.0f * lDotH);
+    r.emission = m.emis;
+    r.roughness = m.rough;
+    return r;
+}
+
+// ComputeLocalFrame (brdf.cu:192-212) applied to a local vector: mat3 columns (X, Y, Z).
+CX_HD V3 frame_mul(V3 n, V3 l) {
+    const float x = n.x, y = n.y, z = n.z;
+    const float sz = (z >= 0) ? 1.0f : -1.0f;
+    const float a = 1 / (sz + z);
+    const float ya = y * a;
+    const float b = x * ya;
+    const float c = x * sz;
+    const V3 X = V3{c * x * a - 1, sz * b, c};
+    const V3 Y = V3{b, y * ya - sz, y};
+    return V3{X.x * l.x + Y.x * l.y + n.x * l.z, X.y * l.x + Y.y * l.y + n.y * l.z, X.z * l.x + Y.z * l.y + n.z * l.z};
+}
+
+// __direct_callable__ggx_sample (brdf.cu:214-338)
+__device__ __forceinline__ BrdfSample ggx_sample(const DScene& S, const Inter& it, Rng& rng) {
+    const Mat m = load_mat(S, it.prim);
+    const V3 normal = it.nrm;
+    const V3 viewDir = -it.dir;
+    BrdfSample r;
+    r.roughness = m.rough;
+    const float nDotV = dot(normal, viewDir);
+    if (nDotV <= 0) {
+        r.outDir = v3(0.0f);
+        r.brdf = v3(0.0f);
+        r.pdf = 0.0f;
+        return r;
+    }
+    const float diffProb = dot(m.diff, v3(1.0f)) / (dot(m.diff, v3(1.0f)) + dot(m.spec, v3(1.0f)));
+    if (rng.f() < diffProb) {
+        const float phi = rng.f() * 2.0f * PI_F;
+        const float rr = sqrtf(rng.f());
+        float sp, cp;
+        sincos_cw(phi, &sp, &cp);
+        const float x = rr * cp;
+        const float y = rr * sp;
+        const float z = sqrtf(gclamp(1.0f - x * x - y * y, 0.0f, 1.0f));
+        r.outDir = frame_mul(normal, V3{x, y, z});
+    } else {
+        const float u = rng.f();
+        const float mz = sqrtf(gclamp((1 - u) / (1 + (m.rough * m.rough - 1) * u), 0.0f, 1.0f));
+        const float mr = sqrtf(gclamp(1 - mz * mz, 0.0f, 1.0f));
+        const float phi = rng.f() * 2.0f * PI_F;
+        float sp, cp;
+        sincos_cw(phi, &sp, &cp);
+        r.outDir = reflect(it.dir, frame_mul(normal, V3{mr * cp, mr * sp, mz}));
+        if (dot(r.outDir, normal) < 0.0f) {
+            r.outDir = v3(0.0f);
+            r.brdf = v3(0.0f);
+            r.pdf = 0.0f;
+            return r;
+        }
+    }
+    const V3 diffBrdf = m.diff / PI_F;
+    const V3 lightDir = r.outDir;
+    const float nDotL = dot(normal, lightDir);
+    const V3 halfway = normalize(lightDir + viewDir);
+    const float nDotH = dot(halfway, normal);
+    const float lDotH = dot(halfway, lightDir);
+    const V3 specBrdf = spec_term(m, nDotH, nDotL, nDotV, lDotH);
+    r.pdf = diffProb * nDotL / PI_F + (1.0f - diffProb) * d_ggx(nDotH, m.rough) * nDotH / (4.0f * lDotH);
+    r.brdf = (diffBrdf + specBrdf) * nDotL;
+    return r;
+}
+
+// ---- emitters (include/graph/sample_emitter.h:3-12, include/model/Emitter.h:21-54) ----
+
+struct EmitterSample {
+    V3 pos, color;
+};
+
+__device__ __forceinline__ EmitterSample sample_emitter(const DScene& S, Rng& rng) {
+    const uint64_t ei = rng.u64() % (uint64_t)S.numEmitters;
+    const uint32_t f0 = S.emitOff[ei], f1 = S.emitOff[ei + 1];
+    const uint64_t face = f0 + rng.u64() % (uint64_t)(f1 - f0);
+    const float* fv = S.emitFaces + 9 * face;
+    const V3 v0 = V3{fv[0], fv[1], fv[2]}, v1 = V3{fv[3], fv[4], fv[5]}, v2 = V3{fv[6], fv[7], fv[8]};
+    float r0 = rng.f();
+    float r1 = rng.f();
+    float r2 = rng.f();
+    const float rSum = r0 + r1 + r2;
+    r0 /= rSum;
+    r1 /= rSum;
+    r2 /= rSum;
+    EmitterSample e;
+    e.pos = r0 * v0 + r1 * v1 + r2 * v2;
+    e.color = V3{S.emitColor[3 * ei], S.emitColor[3 * ei + 1], S.emitColor[3 * ei + 2]};
+    return e;
+}
+
+}  // namespace cx

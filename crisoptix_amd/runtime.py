@@ -1,0 +1,395 @@
+"""Host-side mirror of the reference's Renderer / Scene / Camera API over the C ABI.
+
+``Renderer(width, height, ctx, camera, scene).launch_frame(output)`` stands in for
+``Renderer(w, h, OptixDeviceContext, Camera&, const Scene&)`` and ``LaunchFrame(glm::vec3*)``
+(include/graph/Renderer.h:11-21); ``Scene(ctx, scene_arrays)`` for
+``Scene(ctx, instances, emitters)`` (include/model/Scene.h:12-24).  Device memory is handled
+through the HIP runtime directly (ctypes), so no framework import is needed.
+
+The HIP library must have been built (``python -m crisoptix_amd.build``); loading raises
+:class:`LibraryNotBuilt` otherwise.  Errors from the C ABI raise :class:`CxError` with the
+message of ``cx_last_error`` (the reference's debug-build ASSERT_CUDA/ASSERT_OPTIX behaviour,
+include/util/custom_assert.h:26-66).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcrisoptix_amd.so")
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+class CxError(RuntimeError):
+    pass
+
+
+_hip = None
+_lib = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise LibraryNotBuilt("HIP runtime libamdhip64.so not found")
+        vp, sz, i32 = C.c_void_p, C.c_size_t, C.c_int
+        _hip.hipMalloc.argtypes = [C.POINTER(vp), sz]
+        _hip.hipFree.argtypes = [vp]
+        _hip.hipMemcpy.argtypes = [vp, vp, sz, i32]
+        _hip.hipMemset.argtypes = [vp, i32, sz]
+        _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipGetDeviceCount.argtypes = [C.POINTER(i32)]
+        _hip.hipEventCreate.argtypes = [C.POINTER(vp)]
+        _hip.hipEventRecord.argtypes = [vp, vp]
+        _hip.hipEventSynchronize.argtypes = [vp]
+        _hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
+        _hip.hipEventDestroy.argtypes = [vp]
+        _hip.hipStreamCreate.argtypes = [C.POINTER(vp)]
+        _hip.hipStreamDestroy.argtypes = [vp]
+        _hip.hipStreamSynchronize.argtypes = [vp]
+        _hip.hipGetErrorString.restype = C.c_char_p
+        _hip.hipGetErrorString.argtypes = [i32]
+    return _hip
+
+
+def _hcheck(rc, what):
+    if rc != 0:
+        raise CxError(f"{what}: {hip().hipGetErrorString(rc).decode()}")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = hip().hipGetDeviceCount(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def synchronize():
+    _hcheck(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class DeviceArray:
+    """A device allocation with a numpy dtype and shape."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _hcheck(hip().hipMalloc(C.byref(p), max(self.nbytes, 16)), "hipMalloc")
+        self.ptr = p.value
+
+    @staticmethod
+    def from_numpy(a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes
+        if self.nbytes:
+            _hcheck(hip().hipMemcpy(self.ptr, a.ctypes.data, self.nbytes, 1), "hipMemcpy H2D")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            _hcheck(hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2), "hipMemcpy D2H")
+        return out
+
+    def zero(self):
+        _hcheck(hip().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def download(ptr: int, nbytes: int, dtype) -> np.ndarray:
+    out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
+    if nbytes:
+        _hcheck(hip().hipMemcpy(out.ctypes.data, ptr, nbytes, 2), "hipMemcpy D2H")
+    return out
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        _hcheck(hip().hipEventCreate(C.byref(e)), "hipEventCreate")
+        self.e = e.value
+
+    def record(self, stream=None):
+        _hcheck(hip().hipEventRecord(self.e, stream), "hipEventRecord")
+
+    def synchronize(self):
+        _hcheck(hip().hipEventSynchronize(self.e), "hipEventSynchronize")
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        _hcheck(hip().hipEventElapsedTime(C.byref(ms), self.e, end.e), "hipEventElapsedTime")
+        return ms.value
+
+    def __del__(self):
+        try:
+            hip().hipEventDestroy(self.e)
+        except Exception:
+            pass
+
+
+def lib():
+    """Loads libcrisoptix_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryNotBuilt(f"{LIB_PATH} is missing: build it with `python -m crisoptix_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+        L.cx_abi_version.restype = i32
+        L.cx_ctx_create.argtypes = [i32, C.POINTER(vp)]
+        L.cx_ctx_destroy.argtypes = [vp]
+        L.cx_last_error.restype = C.c_char_p
+        L.cx_last_error.argtypes = [vp]
+        L.cx_scene_build.argtypes = [vp, C.POINTER(abi.SceneDesc), C.POINTER(vp)]
+        L.cx_scene_destroy.argtypes = [vp]
+        L.cx_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
+        L.cx_trace_closest.argtypes = [vp, vp, vp, u32, vp, vp, vp, vp, vp]
+        L.cx_trace_occluded.argtypes = [vp, vp, vp, u32, vp, vp]
+        L.cx_entries_create.argtypes = [vp, u32, C.POINTER(vp)]
+        L.cx_entries_destroy.argtypes = [vp]
+        L.cx_entries_build.argtypes = [vp, vp, vp, vp, vp, u32, vp, vp, vp]
+        L.cx_entries_query.argtypes = [vp, vp, vp, u32, u32, vp, vp, vp, vp]
+        L.cx_default_params.argtypes = [vp, u32, u32, C.POINTER(abi.LaunchParams)]
+        L.cx_renderer_create.argtypes = [vp, vp, u32, u32, C.POINTER(vp)]
+        L.cx_renderer_destroy.argtypes = [vp]
+        L.cx_renderer_set_params.argtypes = [vp, C.POINTER(abi.LaunchParams)]
+        L.cx_renderer_get_params.argtypes = [vp, C.POINTER(abi.LaunchParams)]
+        L.cx_renderer_seed.argtypes = [vp, u64]
+        L.cx_renderer_reset.argtypes = [vp, vp]
+        L.cx_renderer_frame.argtypes = [vp, C.POINTER(abi.CameraData), i32, vp, vp]
+        L.cx_renderer_begin_frame.argtypes = [vp, C.POINTER(abi.CameraData), i32, vp]
+        L.cx_pass_launch.argtypes = [vp, i32, C.POINTER(abi.LaunchParams), vp]
+        L.cx_renderer_enable_timing.argtypes = [vp, i32]
+        L.cx_renderer_get_stats.argtypes = [vp, C.POINTER(abi.FrameStats)]
+        L.cx_renderer_state.argtypes = [vp, C.POINTER(abi.StateView)]
+        _lib = L
+    return _lib
+
+
+class Context:
+    """Replaces InitCuda/InitOptix (src/main.cpp:35-65)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().cx_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise CxError(f"cx_ctx_create failed ({rc}): no usable HIP device {device}")
+        self.h = h.value
+
+    def check(self, rc: int, what: str):
+        if rc != 0:
+            raise CxError(f"{what} failed ({rc}): {lib().cx_last_error(self.h).decode()}")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cx_ctx_destroy(self.h)
+            self.h = None
+
+
+class Scene:
+    """Device scene: flattened triangles, BVH, materials and explicit emitters."""
+
+    def __init__(self, ctx: Context, scene, max_leaf: int = 0):
+        self.ctx = ctx
+        self._keep: list = []
+        desc = abi.scene_desc(scene, self._keep)
+        desc.bvhMaxLeafSize = max_leaf
+        h = C.c_void_p()
+        ctx.check(lib().cx_scene_build(ctx.h, C.byref(desc), C.byref(h)), "cx_scene_build")
+        self.h = h.value
+        self.host = scene
+        info = abi.SceneInfo()
+        ctx.check(lib().cx_scene_get_info(self.h, C.byref(info)), "cx_scene_get_info")
+        self.info = info
+
+    @property
+    def size(self) -> float:
+        return self.info.sceneSize
+
+    def trace_closest(self, rays: np.ndarray, stream=None):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        d_rays = DeviceArray.from_numpy(rays)
+        prim, t, u, v = (DeviceArray((n,), dt) for dt in (np.uint32, np.float32, np.float32, np.float32))
+        self.ctx.check(lib().cx_trace_closest(self.ctx.h, self.h, d_rays.ptr, n, prim.ptr, t.ptr, u.ptr, v.ptr, stream),
+                       "cx_trace_closest")
+        synchronize()
+        return prim.numpy(), t.numpy(), u.numpy(), v.numpy()
+
+    def trace_occluded(self, rays: np.ndarray, stream=None):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        d_rays = DeviceArray.from_numpy(rays)
+        occ = DeviceArray((n,), np.uint32)
+        self.ctx.check(lib().cx_trace_occluded(self.ctx.h, self.h, d_rays.ptr, n, occ.ptr, stream), "cx_trace_occluded")
+        synchronize()
+        return occ.numpy()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cx_scene_destroy(self.h)
+            self.h = None
+
+
+class Entries:
+    """Prefix-entry search structure (PrefixAccelStruct, src/PrefixAccelStruct.cpp)."""
+
+    def __init__(self, ctx: Context, capacity: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(lib().cx_entries_create(ctx.h, capacity, C.byref(h)), "cx_entries_create")
+        self.h = h.value
+        self.capacity = capacity
+
+    def build(self, pos, radius, valid, scene_min, scene_max):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        self._d = [DeviceArray.from_numpy(pos), DeviceArray.from_numpy(np.ascontiguousarray(radius, np.float32)),
+                   DeviceArray.from_numpy(np.ascontiguousarray(valid, np.uint8))]
+        mn = np.ascontiguousarray(scene_min, np.float32)
+        mx = np.ascontiguousarray(scene_max, np.float32)
+        self.ctx.check(lib().cx_entries_build(self.ctx.h, self.h, self._d[0].ptr, self._d[1].ptr, self._d[2].ptr, n,
+                                              mn.ctypes.data, mx.ctypes.data, None), "cx_entries_build")
+
+    def query(self, q, k: int):
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, 3)
+        nq = q.shape[0]
+        dq = DeviceArray.from_numpy(q)
+        idx = DeviceArray((nq, k), np.uint32)
+        dist = DeviceArray((nq, k), np.float32)
+        cnt = DeviceArray((nq,), np.uint32)
+        self.ctx.check(lib().cx_entries_query(self.ctx.h, self.h, dq.ptr, nq, k, idx.ptr, dist.ptr, cnt.ptr, None),
+                       "cx_entries_query")
+        synchronize()
+        return idx.numpy(), dist.numpy(), cnt.numpy()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cx_entries_destroy(self.h)
+            self.h = None
+
+
+class Renderer:
+    """Renderer(w, h, ctx, Camera&, const Scene&) and LaunchFrame (src/Renderer.cpp:13-406)."""
+
+    def __init__(self, width: int, height: int, ctx: Context, camera, scene: Scene,
+                 params: abi.LaunchParams | None = None, seed: int = 0):
+        self.ctx = ctx
+        self.scene = scene
+        self.camera = camera
+        self.W, self.H = width, height
+        h = C.c_void_p()
+        ctx.check(lib().cx_renderer_create(ctx.h, scene.h, width, height, C.byref(h)), "cx_renderer_create")
+        self.h = h.value
+        if params is not None:
+            self.set_params(params)
+        lib().cx_renderer_seed(self.h, seed)
+        self.output = DeviceArray((height, width, 3), np.float32)
+        self.output.zero()
+
+    def set_params(self, params: abi.LaunchParams):
+        self.ctx.check(lib().cx_renderer_set_params(self.h, C.byref(params)), "cx_renderer_set_params")
+
+    def params(self) -> abi.LaunchParams:
+        p = abi.LaunchParams()
+        self.ctx.check(lib().cx_renderer_get_params(self.h, C.byref(p)), "cx_renderer_get_params")
+        return p
+
+    def seed(self, seed: int):
+        lib().cx_renderer_seed(self.h, seed)
+
+    def reset(self):
+        self.ctx.check(lib().cx_renderer_reset(self.h, None), "cx_renderer_reset")
+
+    def enable_timing(self, on: bool = True):
+        lib().cx_renderer_enable_timing(self.h, 1 if on else 0)
+
+    def launch_frame(self, cam: abi.CameraData | None = None, changed: bool | None = None, output=None,
+                     stream=None):
+        if cam is None:
+            cam = abi.CameraData.from_dict(self.camera.data())
+        if changed is None:
+            changed = bool(getattr(self.camera, "changed", False))
+        out = self.output if output is None else output
+        self.ctx.check(lib().cx_renderer_frame(self.h, C.byref(cam), 1 if changed else 0, out.ptr, stream),
+                       "cx_renderer_frame")
+        return out
+
+    def stats(self) -> abi.FrameStats:
+        st = abi.FrameStats()
+        self.ctx.check(lib().cx_renderer_get_stats(self.h, C.byref(st)), "cx_renderer_get_stats")
+        return st
+
+    def state(self) -> abi.StateView:
+        v = abi.StateView()
+        self.ctx.check(lib().cx_renderer_state(self.h, C.byref(v)), "cx_renderer_state")
+        return v
+
+    def prefix(self, which: int) -> dict:
+        v = self.state()
+        P = self.W * self.H
+        raw = download(v.prefixRes[which], abi.PREFIX_CHUNKS * P * 16, np.uint32)
+        return abi.decode_prefix(raw, P)
+
+    def suffix(self, which: int) -> dict:
+        v = self.state()
+        P = self.W * self.H
+        raw = download(v.suffixRes[which], abi.SUFFIX_CHUNKS * P * 16, np.uint32)
+        return abi.decode_suffix(raw, P)
+
+    def gbuffer(self) -> dict:
+        v = self.state()
+        P = self.W * self.H
+        return abi.decode_gbuffer(download(v.gbuffer, P * 16, np.uint32), P)
+
+    def entries(self):
+        v = self.state()
+        P = self.W * self.H
+        e = download(v.entryPos, P * 16, np.float32).reshape(P, 4)
+        val = download(v.entryValid, P, np.uint8)
+        return e[:, :3], e[:, 3], val
+
+    def neighbors(self):
+        v = self.state()
+        P = self.W * self.H
+        k = max(1, v.k)
+        idx = download(v.neighIdx, P * k * 4, np.uint32).reshape(P, k)
+        dist = download(v.neighDist, P * k * 4, np.float32).reshape(P, k)
+        cnt = download(v.neighCount, P * 4, np.uint32)
+        return idx[:, :v.k], dist[:, :v.k], cnt
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cx_renderer_destroy(self.h)
+            self.h = None

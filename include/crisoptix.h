@@ -1,0 +1,272 @@
+/*
+ * crisoptix.h -- C ABI of the MI355X-native ReSTIR PT / Conditional ReSTIR frame loop.
+ *
+ * This is the drop-in boundary that replaces the OptiX launch path of CrisOptix:
+ * every entry point below names the reference interface it stands in for.
+ * All functions return 0 on success and a negative cx_status on failure; they never
+ * throw across the ABI.  cx_last_error() returns a message for the last failure on a
+ * context.  Device pointers are plain `void*`/typed pointers into HIP device memory;
+ * host pointers are marked `host`.  No torch or glm types appear in any signature.
+ *
+ * Threading: one cx_ctx per host thread (the reference is single-threaded on stream 0,
+ * src/Renderer.cpp:259-371).  Work is enqueued on the caller's hipStream_t (passed as
+ * `void*`, NULL = the default stream); nothing inside a frame synchronises the device.
+ */
+#ifndef CRISOPTIX_H
+#define CRISOPTIX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CX_ABI_VERSION 1
+
+typedef enum cx_status {
+    CX_OK = 0,
+    CX_ERR_INVALID_ARG = -1,
+    CX_ERR_HIP = -2,
+    CX_ERR_OUT_OF_MEMORY = -3,
+    CX_ERR_UNSUPPORTED = -4,
+    CX_ERR_NO_DEVICE = -5,
+    CX_ERR_BUILD = -6
+} cx_status;
+
+/* RendererType (include/graph/restir/settings.h:5-10). */
+typedef enum cx_renderer_type {
+    CX_PATH_TRACER = 0,
+    CX_RESTIR_PT = 1,
+    CX_CONDITIONAL_RESTIR = 2
+} cx_renderer_type;
+
+/* PrefixRadiusType (include/graph/restir/settings.h:15-19). */
+typedef enum cx_radius_type {
+    CX_RADIUS_CONSTANT = 0,
+    CX_RADIUS_PATH_LENGTH = 1
+} cx_radius_type;
+
+/* The six raygen programs of a frame (src/Renderer.cpp:91-136). */
+typedef enum cx_pass {
+    CX_PASS_PREFIX_GEN_TEMP = 0,    /* shader/prefix_gen_temp_reuse.cu:100 */
+    CX_PASS_PREFIX_SPATIAL = 1,     /* shader/prefix_spatial_reuse.cu:136 */
+    CX_PASS_SUFFIX_GEN_TEMP = 2,    /* shader/suffix_gen_temp_reuse.cu:114 */
+    CX_PASS_SUFFIX_SPATIAL = 3,     /* shader/suffix_spatial_reuse.cu:147 */
+    CX_PASS_PREFIX_STORE_ENTRIES = 4, /* shader/prefix_store_entries.cu:9 */
+    CX_PASS_ENTRIES_BUILD = 5,      /* PrefixAccelStruct::Rebuild, src/PrefixAccelStruct.cpp:14-26 */
+    CX_PASS_FINAL_GATHER = 6,       /* shader/final_gather.cu:323 */
+    CX_PASS_COUNT = 7
+} cx_pass;
+
+/*
+ * Per-frame parameter block.  Mirrors LaunchParams (include/graph/LaunchParams.h:11-36)
+ * and RestirParams (include/graph/restir/RestirParams.h:11-59) with every buffer view and
+ * traversable handle removed: buffers are owned by the renderer, and the block is passed
+ * by value as a kernel argument (no __constant__ upload, so it is graph-capturable).
+ * `windowRadius` is WINDOW_RADIUS (include/graph/restir/path_gen.h:12), a parameter here.
+ * `random` and `frameIdx` are maintained by cx_renderer_frame() exactly like
+ * Renderer::LaunchFrame (src/Renderer.cpp:212-215); the values passed in are ignored there.
+ */
+typedef struct cx_launch_params {
+    uint32_t random;
+    uint32_t width;
+    uint32_t height;
+    uint32_t windowRadius;
+    uint64_t frameIdx;
+    float neeProb;
+    int32_t enableAccum;
+    int32_t maxPathLen;
+    int32_t rendererType;          /* cx_renderer_type */
+    /* RestirParams */
+    float reconMinRoughness;
+    float reconMinDistance;
+    int32_t spatialRoundIdx;
+    int32_t prefixLen;
+    int32_t prefixEnableTemporal;
+    int32_t prefixEnableSpatial;
+    int32_t prefixSpatialCount;
+    int32_t prefixSpatialRounds;
+    int32_t suffixEnableTemporal;
+    int32_t suffixEnableSpatial;
+    int32_t suffixSpatialCount;
+    int32_t suffixSpatialRounds;
+    int32_t gatherN;
+    int32_t gatherM;
+    float gatherRadius;
+    int32_t gatherRadiusType;      /* cx_radius_type */
+    int32_t trackPrefixStats;
+    int32_t pad0;
+} cx_launch_params;
+
+/* CameraData (include/graph/Camera.h:6-12) without the unused view matrices. */
+typedef struct cx_camera {
+    float pos[3];
+    float U[3];
+    float V[3];
+    float W[3];
+} cx_camera;
+
+/* Material (include/model/Material.h:16-33, src/Material.cpp:3-24), untextured. */
+typedef struct cx_material {
+    float diffColor[3];
+    float specF0[3];
+    float emissiveColor[3];
+    float roughness;
+} cx_material;
+
+/*
+ * Scene description, host memory.  The reference's Model/Mesh/ModelInstance hierarchy
+ * (src/Scene.cpp:73-174) is flattened into one instance with an identity transform,
+ * because rehydration always reads transforms[0] (include/graph/Interaction.h:143).
+ * Each triangle carries the dense id of its material (the MeshSbtData callable indices of
+ * include/model/Mesh.h:10-16 become material ids).  Emitters are the explicit emitters of
+ * Scene::BuildEmitterTable (src/Scene.cpp:176-204): world-space triangles and one colour
+ * each; they are not part of the traced geometry.
+ */
+typedef struct cx_scene_desc {
+    const float* positions;        /* host, [numVertices][3] */
+    const float* normals;          /* host, [numVertices][3] */
+    uint32_t numVertices;
+    const uint32_t* indices;       /* host, [numTriangles][3] */
+    const uint32_t* triMaterial;   /* host, [numTriangles] */
+    uint32_t numTriangles;
+    const cx_material* materials;  /* host, [numMaterials] */
+    uint32_t numMaterials;
+    const float* emitterFaces;     /* host, [numEmitterFaces][3 vertices][3] world space */
+    const uint32_t* emitterFaceOffset; /* host, [numEmitters + 1] prefix offsets into faces */
+    const float* emitterColor;     /* host, [numEmitters][3] */
+    uint32_t numEmitters;
+    uint32_t bvhMaxLeafSize;       /* 0 = default (4) */
+    uint32_t pad0;
+} cx_scene_desc;
+
+typedef struct cx_scene_info {
+    uint32_t numTriangles;
+    uint32_t numVertices;
+    uint32_t numNodes;
+    uint32_t maxDepth;
+    uint32_t numEmitters;
+    uint32_t numEmitterFaces;
+    float sceneSize;               /* Scene::GetSceneSize, src/Scene.cpp:48-71 */
+    float boundsMin[3];
+    float boundsMax[3];
+    uint64_t deviceBytes;
+} cx_scene_info;
+
+/* Per-pass timing of the last frame, ms (src/Renderer.cpp:396-402), plus ray counts. */
+typedef struct cx_frame_stats {
+    float passMs[CX_PASS_COUNT];
+    float totalMs;
+    uint64_t closestRays;
+    uint64_t shadowRays;
+    /* PrefixAccelStruct::Stats (include/graph/restir/PrefixAccelStruct.h:32-44) */
+    uint32_t minNeighCount;
+    uint32_t maxNeighCount;
+    uint64_t totalNeighCount;
+    uint32_t numEntries;
+    uint32_t pad0;
+} cx_frame_stats;
+
+typedef struct cx_ctx cx_ctx;
+typedef struct cx_scene cx_scene;
+typedef struct cx_renderer cx_renderer;
+typedef struct cx_entries cx_entries;
+
+/* ---- context (replaces InitCuda/InitOptix, src/main.cpp:35-65) ---- */
+int cx_ctx_create(int device, cx_ctx** out);
+int cx_ctx_destroy(cx_ctx* ctx);
+const char* cx_last_error(const cx_ctx* ctx);
+int cx_abi_version(void);
+
+/* ---- scene: replaces Mesh::BuildAccel, Model::BuildAccel, Scene::BuildAccel and
+ *      Scene::BuildEmitterTable (src/Mesh.cpp:54-135, src/Model.cpp:376-457,
+ *      src/Scene.cpp:73-204).  Builds a binary SAH BVH and uploads it. ---- */
+int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* desc, cx_scene** out);
+int cx_scene_destroy(cx_scene* scene);
+int cx_scene_get_info(const cx_scene* scene, cx_scene_info* info);
+
+/* ---- traversal: stands in for optixTrace with the surface / occlusion ray flags
+ *      (include/graph/trace.h:25-88, shader/mesh_hit.cu:17-69, shader/miss.cu:8-26).
+ *      rays: device, [n][8] = origin xyz, direction xyz, tmin, tmax.
+ *      Closest hit writes primitive index (0xFFFFFFFF on a miss), t and the OptiX
+ *      barycentrics (u weights vertex 1, v weights vertex 2).  Occlusion writes 1 if any
+ *      triangle is hit in [tmin, tmax], else 0. ---- */
+int cx_trace_closest(cx_ctx* ctx, const cx_scene* scene, const float* d_rays, uint32_t n,
+                     uint32_t* d_prim, float* d_t, float* d_u, float* d_v, void* stream);
+int cx_trace_occluded(cx_ctx* ctx, const cx_scene* scene, const float* d_rays, uint32_t n,
+                      uint32_t* d_occluded, void* stream);
+
+/* ---- prefix-entry search structure: replaces PrefixAccelStruct::Rebuild/BuildGas/
+ *      BuildTlas (src/PrefixAccelStruct.cpp:14-221) and the point query of
+ *      __intersection__prefix_entry (shader/prefix_search_hit.cu:9-73).
+ *      Entry i is a candidate for query q iff valid[i] != 0 and |q - pos[i]| <= radius[i],
+ *      where radius[i] is the half-width of the entry's box, ((p+r)-(p-r))/2 in float.
+ *      The k nearest candidates by (distance, index) are returned in ascending order. ---- */
+int cx_entries_create(cx_ctx* ctx, uint32_t capacity, cx_entries** out);
+int cx_entries_destroy(cx_entries* entries);
+/* d_pos: device [n][3]; d_radius: device [n] (the requested radius r, before the box
+ * round trip); d_valid: device [n] bytes.  sceneMin/sceneMax: host, bounds used for the
+ * Morton quantisation (entries outside are clamped, which only affects speed). */
+int cx_entries_build(cx_ctx* ctx, cx_entries* entries, const float* d_pos, const float* d_radius,
+                     const uint8_t* d_valid, uint32_t n, const float* sceneMin,
+                     const float* sceneMax, void* stream);
+/* d_query: device [nq][3]; outputs: d_idx/d_dist device [nq][k], d_count device [nq]. */
+int cx_entries_query(cx_ctx* ctx, const cx_entries* entries, const float* d_query, uint32_t nq,
+                     uint32_t k, uint32_t* d_idx, float* d_dist, uint32_t* d_count, void* stream);
+
+/* ---- renderer: Renderer(w, h, ctx, Camera&, const Scene&) and LaunchFrame
+ *      (include/graph/Renderer.h:11-21, src/Renderer.cpp:13-406). ---- */
+/* Defaults of src/Renderer.cpp:34-58 (reconMinDistance = 0.01 * sceneSize). */
+int cx_default_params(const cx_scene* scene, uint32_t width, uint32_t height,
+                      cx_launch_params* out);
+int cx_renderer_create(cx_ctx* ctx, const cx_scene* scene, uint32_t width, uint32_t height,
+                       cx_renderer** out);
+int cx_renderer_destroy(cx_renderer* r);
+int cx_renderer_set_params(cx_renderer* r, const cx_launch_params* params);
+int cx_renderer_get_params(const cx_renderer* r, cx_launch_params* params);
+/* Seeds the per-frame random of src/Renderer.cpp:215 (std::random_device there). */
+int cx_renderer_seed(cx_renderer* r, uint64_t seed);
+/* Resets all per-pixel state to the values uploaded by the constructor
+ * (src/Renderer.cpp:155-199) and the front/back indices to 0/0. */
+int cx_renderer_reset(cx_renderer* r, void* stream);
+/* One LaunchFrame: d_output is caller-owned device memory, [height][width][3] floats.
+ * cameraChanged mirrors Camera::HasChanged (src/Renderer.cpp:213). */
+int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, float* d_output,
+                      void* stream);
+/* One optixLaunch of a single raygen program with the renderer's current state and the
+ * parameter block `params` (frame fields included), for callers that drive the phase
+ * order themselves.  Buffer indices are those of the last cx_renderer_frame or
+ * cx_renderer_begin_frame. */
+int cx_renderer_begin_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged,
+                            float* d_output);
+int cx_pass_launch(cx_renderer* r, cx_pass pass, const cx_launch_params* params, void* stream);
+/* Timings are gathered with hipEvents when enabled (off by default). */
+int cx_renderer_enable_timing(cx_renderer* r, int enable);
+int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* stats);
+/* Device pointers of per-pixel state, for tests and checkpointing.  Records are chunked
+ * structure-of-arrays: chunk c of pixel p is the 16-byte word at base + (c * P + p) * 16
+ * (field order in crisoptix_amd/csrc/cx_restir_dev.h). */
+typedef struct cx_state_view {
+    void* prefixRes[2];      /* 9 chunks per pixel: PrefixPath + reservoir weight/confidence */
+    void* suffixRes[2];      /* 7 chunks per pixel: SuffixPath + reservoir weight/confidence */
+    void* gbuffer;           /* [P] {rng lo, rng hi, primaryIntValid, canonical-buffer flags} */
+    void* neighIdx;          /* [P][k] uint32 neighbour pixel index (PrefixNeighbor::pixelIdx) */
+    void* neighDist;         /* [P][k] float distance (PrefixNeighbor::distance) */
+    void* neighCount;        /* [P] uint32 */
+    void* entryPos;          /* [P] float4 {pos.xyz, radius} of the prefix entries */
+    void* entryValid;        /* [P] uint8 */
+    uint32_t frontBufferIdx;
+    uint32_t backBufferIdx;
+    uint32_t prefixChunks;
+    uint32_t suffixChunks;
+    uint32_t k;
+    uint32_t pad0;
+} cx_state_view;
+int cx_renderer_state(cx_renderer* r, cx_state_view* view);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRISOPTIX_H */

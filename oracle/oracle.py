@@ -1,0 +1,311 @@
+"""ORACLE -- test infrastructure only.
+
+ctypes wrapper of the host C++ restatement of the reference shader logic
+(oracle/crisoptix_oracle.cpp).  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg
+of bench.py import this module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from crisoptix_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liborc.so")
+
+PACKED_DT = np.dtype([("prim", "<u4"), ("bu", "<f4"), ("bv", "<f4"), ("inDir", "<f4", 3), ("valid", "<u4")])
+PREFIX_DT = np.dtype([
+    ("rngState", "<u8"), ("flags", "<u4"), ("f", "<f4", 3), ("postReconF", "<f4", 3), ("p", "<f4"),
+    ("pathLen", "<f4"), ("reconOutDir", "<f4", 3), ("wSum", "<f4"), ("confidence", "<f4"),
+    ("primaryInt", PACKED_DT), ("reconInt", PACKED_DT), ("lastInt", PACKED_DT)], align=True)
+SUFFIX_DT = np.dtype([
+    ("rngState", "<u8"), ("flags", "<u4"), ("f", "<f4", 3), ("postReconF", "<f4", 3), ("p", "<f4"),
+    ("reconOutDir", "<f4", 3), ("wSum", "<f4"), ("confidence", "<f4"),
+    ("reconInt", PACKED_DT), ("lastPrefixInt", PACKED_DT)], align=True)
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, u32, u64, f32, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_int
+        L.orc_scene_create.restype = vp
+        L.orc_scene_create.argtypes = [C.POINTER(abi.SceneDesc), i32]
+        L.orc_scene_destroy.argtypes = [vp]
+        L.orc_scene_size.restype = f32
+        L.orc_scene_size.argtypes = [vp]
+        L.orc_trace_closest.argtypes = [vp, vp, u32, vp, vp, vp, vp]
+        L.orc_trace_occluded.argtypes = [vp, vp, u32, vp]
+        L.orc_tea32.restype = u32
+        L.orc_tea32.argtypes = [u32, u32, u32]
+        L.orc_tea64.restype = u64
+        L.orc_tea64.argtypes = [u32, u32, u32]
+        L.orc_tea_keyed.argtypes = [vp, vp, u32]
+        L.orc_pcg32_stream.argtypes = [u64, u64, i32, u32, vp]
+        L.orc_pcg32_state.argtypes = [u64, u64, vp]
+        L.orc_sincos.argtypes = [vp, u32, vp, vp]
+        L.orc_ggx_eval.argtypes = [vp, vp, vp, u32, vp]
+        L.orc_ggx_sample.argtypes = [vp, vp, vp, u32, vp]
+        L.orc_sample_emitter.argtypes = [vp, vp, u32, vp]
+        L.orc_jacobian.restype = f32
+        L.orc_jacobian.argtypes = [vp, vp, vp, vp]
+        L.orc_pairwise_mis.argtypes = [vp, vp, f32, f32, f32, f32, vp, vp]
+        L.orc_reservoir_stream.restype = i32
+        L.orc_reservoir_stream.argtypes = [vp, vp, u32, u64, vp, vp]
+        L.orc_identity_shift_prefix.argtypes = [vp, C.POINTER(abi.LaunchParams), vp, vp, u64, vp]
+        L.orc_identity_shift_suffix.argtypes = [vp, C.POINTER(abi.LaunchParams), vp, vp, u64, vp]
+        L.orc_renderer_create.restype = vp
+        L.orc_renderer_create.argtypes = [vp, u32, u32]
+        L.orc_renderer_destroy.argtypes = [vp]
+        L.orc_renderer_set_params.argtypes = [vp, C.POINTER(abi.LaunchParams)]
+        L.orc_renderer_seed.argtypes = [vp, u64]
+        L.orc_renderer_reset.argtypes = [vp]
+        L.orc_renderer_frame.argtypes = [vp, C.POINTER(abi.CameraData), i32, vp]
+        L.orc_renderer_pass_ms.argtypes = [vp, vp]
+        L.orc_renderer_buffers.argtypes = [vp, vp, vp]
+        L.orc_dump_prefix.argtypes = [vp, i32, vp]
+        L.orc_dump_suffix.argtypes = [vp, i32, vp]
+        L.orc_dump_gbuffer.argtypes = [vp, vp]
+        L.orc_dump_entries.argtypes = [vp, vp, vp, vp]
+        L.orc_dump_neighbors.restype = u32
+        L.orc_dump_neighbors.argtypes = [vp, vp, vp, vp]
+        L.orc_entries_query.argtypes = [vp, vp, vp, u32, vp, u32, u32, i32, vp, vp, vp]
+        L.orc_num_threads.restype = i32
+        L.orc_set_num_threads.argtypes = [i32]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()
+
+
+def set_num_threads(n: int):
+    lib().orc_set_num_threads(n)
+
+
+def tea32(a, b, rounds=4):
+    return lib().orc_tea32(a, b, rounds)
+
+
+def tea64(a, b, rounds=4):
+    return lib().orc_tea64(a, b, rounds)
+
+
+def tea_keyed(v0, v1, key, rounds):
+    v = np.asarray([v0, v1], np.uint32)
+    k = np.asarray(key, np.uint32)
+    lib().orc_tea_keyed(_p(v), _p(k), rounds)
+    return int(v[0]), int(v[1])
+
+
+def pcg32_stream(initstate, initseq, n, kind=0):
+    out = np.zeros(n * (2 if kind == 2 else 1), np.uint32)
+    lib().orc_pcg32_stream(initstate, initseq, kind, n, _p(out))
+    if kind == 1:
+        return out.view(np.float32)
+    if kind == 2:
+        return out.view(np.uint64)
+    return out
+
+
+def pcg32_state(initstate, initseq=0xDA3E39CB94B95BDB):
+    o = np.zeros(2, np.uint64)
+    lib().orc_pcg32_state(initstate, initseq, _p(o))
+    return int(o[0]), int(o[1])
+
+
+def sincos(x):
+    x = np.ascontiguousarray(x, np.float32)
+    s = np.zeros_like(x)
+    c = np.zeros_like(x)
+    lib().orc_sincos(_p(x), x.size, _p(s), _p(c))
+    return s, c
+
+
+class OracleScene:
+    def __init__(self, scene, brute_force: bool = False):
+        self._keep = []
+        self.desc = abi.scene_desc(scene, self._keep)
+        self.h = lib().orc_scene_create(C.byref(self.desc), 1 if brute_force else 0)
+        self.scene = scene
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    @property
+    def size(self) -> float:
+        return lib().orc_scene_size(self.h)
+
+    def trace_closest(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        prim = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        lib().orc_trace_closest(self.h, _p(rays), n, _p(prim), _p(t), _p(u), _p(v))
+        return prim, t, u, v
+
+    def trace_occluded(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, np.float32)
+        occ = np.zeros(rays.shape[0], np.uint32)
+        lib().orc_trace_occluded(self.h, _p(rays), rays.shape[0], _p(occ))
+        return occ
+
+    def ggx_eval(self, inter: np.ndarray, out_dir: np.ndarray):
+        inter = np.ascontiguousarray(inter, np.float32)
+        out_dir = np.ascontiguousarray(out_dir, np.float32)
+        n = inter.shape[0]
+        out = np.zeros((n, 8), np.float32)
+        lib().orc_ggx_eval(self.h, _p(inter), _p(out_dir), n, _p(out))
+        return out
+
+    def ggx_sample(self, inter: np.ndarray, rng_state: np.ndarray):
+        inter = np.ascontiguousarray(inter, np.float32)
+        st = np.ascontiguousarray(rng_state, np.uint64).copy()
+        n = inter.shape[0]
+        out = np.zeros((n, 8), np.float32)
+        lib().orc_ggx_sample(self.h, _p(inter), _p(st), n, _p(out))
+        return out, st
+
+    def sample_emitter(self, rng_state: np.ndarray):
+        st = np.ascontiguousarray(rng_state, np.uint64).copy()
+        out = np.zeros((st.size, 4), np.float32)
+        lib().orc_sample_emitter(self.h, _p(st), st.size, _p(out))
+        return out, st
+
+    def identity_shift(self, params: abi.LaunchParams, origin, direction, rng_state: int, which="prefix"):
+        o = np.ascontiguousarray(origin, np.float32)
+        d = np.ascontiguousarray(direction, np.float32)
+        out = np.zeros(8, np.float32)
+        fn = lib().orc_identity_shift_prefix if which == "prefix" else lib().orc_identity_shift_suffix
+        fn(self.h, C.byref(params), _p(o), _p(d), rng_state, _p(out))
+        return out
+
+
+def jacobian(old, new, tp, tn) -> float:
+    a = [np.ascontiguousarray(x, np.float32) for x in (old, new, tp, tn)]
+    return lib().orc_jacobian(*[_p(x) for x in a])
+
+
+def pairwise_mis(f_basis, f_neigh, J, K, canon_m, neigh_m):
+    a = np.ascontiguousarray(f_basis, np.float32)
+    b = np.ascontiguousarray(f_neigh, np.float32)
+    cw = C.c_float()
+    nw = C.c_float()
+    lib().orc_pairwise_mis(_p(a), _p(b), J, K, canon_m, neigh_m, C.byref(cw), C.byref(nw))
+    return cw.value, nw.value
+
+
+def reservoir_stream(weights, confidences, rng_state: int):
+    w = np.ascontiguousarray(weights, np.float32)
+    c = np.ascontiguousarray(confidences, np.float32)
+    ws = C.c_float()
+    cf = C.c_float()
+    idx = lib().orc_reservoir_stream(_p(w), _p(c), w.size, rng_state, C.byref(ws), C.byref(cf))
+    return idx, ws.value, cf.value
+
+
+def entries_query(pos, radius, valid, queries, k, brute=False):
+    pos = np.ascontiguousarray(pos, np.float32)
+    radius = np.ascontiguousarray(radius, np.float32)
+    valid = np.ascontiguousarray(valid, np.uint8)
+    q = np.ascontiguousarray(queries, np.float32)
+    nq = q.shape[0]
+    idx = np.zeros((nq, max(k, 1)), np.uint32)
+    dist = np.zeros((nq, max(k, 1)), np.float32)
+    cnt = np.zeros(nq, np.uint32)
+    lib().orc_entries_query(_p(pos), _p(radius), _p(valid), pos.shape[0], _p(q), nq, k, 1 if brute else 0,
+                            _p(idx), _p(dist), _p(cnt))
+    return idx[:, :k], dist[:, :k], cnt
+
+
+class OracleRenderer:
+    """Renderer(w, h, ctx, Camera&, const Scene&) + LaunchFrame (src/Renderer.cpp), on the CPU."""
+
+    def __init__(self, oscene: OracleScene, width: int, height: int, params: abi.LaunchParams, seed: int = 0):
+        self.oscene = oscene
+        self.W, self.H = width, height
+        self.h = lib().orc_renderer_create(oscene.h, width, height)
+        self.params = params.copy()
+        lib().orc_renderer_set_params(self.h, C.byref(self.params))
+        lib().orc_renderer_seed(self.h, seed)
+        self.output = np.zeros((height, width, 3), np.float32)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_renderer_destroy(self.h)
+            self.h = None
+
+    def set_params(self, params: abi.LaunchParams):
+        self.params = params.copy()
+        lib().orc_renderer_set_params(self.h, C.byref(self.params))
+
+    def frame(self, cam: abi.CameraData, changed: bool = False) -> np.ndarray:
+        lib().orc_renderer_frame(self.h, C.byref(cam), 1 if changed else 0, _p(self.output))
+        return self.output
+
+    def pass_ms(self) -> np.ndarray:
+        ms = np.zeros(abi.CX_PASS_COUNT, np.float64)
+        lib().orc_renderer_pass_ms(self.h, _p(ms))
+        return ms
+
+    def buffers(self):
+        fb = np.zeros(2, np.uint32)
+        fr = np.zeros(2, np.uint64)
+        lib().orc_renderer_buffers(self.h, _p(fb), _p(fr))
+        return int(fb[0]), int(fb[1]), int(fr[0]), int(fr[1])
+
+    def prefix(self, which: int) -> np.ndarray:
+        out = np.zeros(self.W * self.H, PREFIX_DT)
+        lib().orc_dump_prefix(self.h, which, _p(out))
+        return out
+
+    def suffix(self, which: int) -> np.ndarray:
+        out = np.zeros(self.W * self.H, SUFFIX_DT)
+        lib().orc_dump_suffix(self.h, which, _p(out))
+        return out
+
+    def gbuffer(self) -> np.ndarray:
+        out = np.zeros((self.W * self.H, 4), np.uint32)
+        lib().orc_dump_gbuffer(self.h, _p(out))
+        return out
+
+    def entries(self):
+        P = self.W * self.H
+        pos = np.zeros((P, 3), np.float32)
+        rad = np.zeros(P, np.float32)
+        val = np.zeros(P, np.uint8)
+        lib().orc_dump_entries(self.h, _p(pos), _p(rad), _p(val))
+        return pos, rad, val
+
+    def neighbors(self):
+        P = self.W * self.H
+        k = max(1, self.params.gatherM - 1)
+        idx = np.zeros((P, k), np.uint32)
+        dist = np.zeros((P, k), np.float32)
+        cnt = np.zeros(P, np.uint32)
+        kk = lib().orc_dump_neighbors(self.h, _p(idx), _p(dist), _p(cnt))
+        return idx[:, :kk], dist[:, :kk], cnt
