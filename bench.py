@@ -1,0 +1,214 @@
+"""Benchmark of the Conditional ReSTIR frame loop on one MI355X.
+
+Workload: 1920x1080 Conditional ReSTIR (prefix/suffix reservoirs, entry-tree neighbour search,
+final gather with 8 suffixes per pixel) on the ~1.06 M-triangle object grid (16 x 16 seeded
+icospheres of subdivision 4 and boxes, scale U[0.3, 1.5], random rotation, on a 200 x 200-quad
+ground plane, GGX roughness U[0.05, 1], albedo U[0.1, 0.9], 16 emitter triangles of radiance
+lognormal(2, 0.5)), max path length 8, camera orbiting by 2*pi/64 per frame.  Synthetic,
+seeded scene data; every frame runs the full pass sequence of Renderer::LaunchFrame.
+
+One step = one frame.  value = counted rays (closest-hit + shadow) per second over the timed
+frames; ms_per_step = GPU time per frame from hipEvents around the timed frames.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms/frame + Mrays/s at 1080p Conditional ReSTIR; rel-MSE vs FP64 truth"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per pixel of the final gather (primary interaction 40, RNG 32, neighbour
+# list write 64 + read 128, k = 8 neighbour suffix reservoirs read twice 2 x 8 x 152, output
+# 24) and per ray (48 B closest hit, 36 B shadow), from the reference's record sizes.
+FINAL_GATHER_STATE_BYTES_PER_PX = 2720
+FRAME_STATE_BYTES_PER_PX = 6704
+CLOSEST_RAY_BYTES = 48
+SHADOW_RAY_BYTES = 36
+WORKLOAD_SEED = 2
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--grid", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-scale", type=int, default=8,
+                    help="CPU baseline renders (W/scale) x (H/scale) pixels of the same workload")
+    return ap.parse_args()
+
+
+def traffic_from_profiles():
+    """HBM bytes per final-gather launch from the committed rocprofv3 counter summary, if any."""
+    path = os.path.join(ROOT, "profiles", "final_gather_1080p_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split()[-1])
+    return None
+
+
+def cpu_baseline(scene, params_for, W, H, scale, frames=2):
+    from crisoptix_amd import abi, scenes
+    from oracle import oracle as O
+
+    w, h = max(1, W // scale), max(1, H // scale)
+    os_ = O.OracleScene(scene)
+    p = params_for(w, h, os_.size)
+    cam = scenes.orbit_camera(scene, w, h)
+    ren = O.OracleRenderer(os_, w, h, p, seed=1)
+    rays = 0
+    secs = 0.0
+    for f in range(frames + 1):
+        cam.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
+        t0 = time.perf_counter()
+        ren.frame(abi.CameraData.from_dict(cam.data()), True)
+        dt = time.perf_counter() - t0
+        if f > 0:   # first frame warms the history
+            secs += dt
+            rays += int(ren.rays().sum())
+    return {
+        "value": rays / secs / 1e6,
+        "unit": "Mrays/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": f"{frames} frames of the same workload at {w}x{h} (1/{scale * scale} of the pixels), "
+                  f"oracle host C++ with OpenMP, {secs * 1000 / frames:.0f} ms/frame",
+        "ms_per_frame_at_sample": secs * 1000 / frames,
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {a.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    from crisoptix_amd import abi, runtime, scenes
+
+    W, H = a.width, a.height
+    t0 = time.perf_counter()
+    scene = scenes.object_grid(a.grid, WORKLOAD_SEED)
+    ctx = runtime.Context(0)
+    gscene = runtime.Scene(ctx, scene)
+    log(f"scene: {scene.num_triangles} triangles, BVH {gscene.info.numNodes} nodes, depth {gscene.info.maxDepth}, "
+        f"built in {time.perf_counter() - t0:.1f}s")
+
+    def params_for(w, h, size):
+        return abi.workload_params("conditional_restir", w, h, size)
+
+    params = params_for(W, H, gscene.size)
+    cam = scenes.orbit_camera(scene, W, H)
+    ren = runtime.Renderer(W, H, ctx, cam, gscene, params=params, seed=1)
+    ren.enable_timing(True)
+
+    def frame():
+        cam.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
+        ren.launch_frame(abi.CameraData.from_dict(cam.data()), True)
+
+    for _ in range(a.warmup):
+        frame()
+    runtime.synchronize()
+    log(f"warmup done ({a.warmup} frames)")
+    start, end = runtime.Event(), runtime.Event()
+    pass_ms = np.zeros(abi.CX_PASS_COUNT)
+    pass_rays = np.zeros((abi.CX_PASS_COUNT, 2))
+    total_rays = 0
+    neigh = 0
+    entries = 0
+    gpu_ms = 0.0
+    for i in range(a.steps):
+        start.record()
+        frame()
+        end.record()
+        end.synchronize()
+        gpu_ms += start.elapsed_ms(end)
+        st = ren.stats()
+        pass_ms += np.asarray(st.passMs)
+        pass_rays[:, 0] += np.asarray(st.passClosestRays)
+        pass_rays[:, 1] += np.asarray(st.passShadowRays)
+        total_rays += st.closestRays + st.shadowRays
+        entries += st.numEntries
+    ms_per_frame = gpu_ms / a.steps
+    mrays = total_rays / (gpu_ms / 1000.0) / 1e6
+    P = W * H
+    fg = abi.PASS_NAMES.index("final_gather")
+    fg_ms = pass_ms[fg] / a.steps
+    fg_bytes = FINAL_GATHER_STATE_BYTES_PER_PX * P + (pass_rays[fg, 0] * CLOSEST_RAY_BYTES +
+                                                      pass_rays[fg, 1] * SHADOW_RAY_BYTES) / a.steps
+    achieved = fg_bytes / (fg_ms / 1000.0) / 1e9
+    frame_bytes = FRAME_STATE_BYTES_PER_PX * P + (pass_rays[:, 0].sum() * CLOSEST_RAY_BYTES +
+                                                  pass_rays[:, 1].sum() * SHADOW_RAY_BYTES) / a.steps
+    out = runtime.DeviceArray.numpy(ren.output)
+    log("per-pass ms: " + ", ".join(f"{n} {m / a.steps:.3f}" for n, m in zip(abi.PASS_NAMES, pass_ms)))
+    log(f"rays/frame {total_rays / a.steps:.0f} ({total_rays / a.steps / P:.2f}/px), entries/frame "
+        f"{entries / a.steps:.0f}, mean radiance {float(out.mean()):.5f}, finite {bool(np.isfinite(out).all())}")
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(scene, params_for, W, H, a.cpu_sample_scale)
+        log(f"cpu baseline: {cpu['value']:.2f} Mrays/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": METRIC,
+        "value": round(mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_frame, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene, orbiting camera)",
+        "config": {
+            "workload": f"conditional_restir_{W}x{H}_objgrid{a.grid}",
+            "resolution": [W, H],
+            "triangles": scene.num_triangles,
+            "max_path_len": int(params.maxPathLen),
+            "gather_m": int(params.gatherM),
+            "rays_per_frame": int(total_rays / a.steps),
+            "rays_per_pixel": round(total_rays / a.steps / P, 3),
+            "pass_ms": {n: round(m / a.steps, 4) for n, m in zip(abi.PASS_NAMES, pass_ms)},
+            "frame_algorithmic_GBps": round(frame_bytes / (ms_per_frame / 1000.0) / 1e9, 2),
+        },
+        "roofline": {
+            "kernel": "k_final_gather",
+            "bound": "hbm",
+            "achieved": round(achieved, 3),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6),
+            "traffic": traffic_from_profiles(),
+            "avg_launch_ms": round(fg_ms, 4),
+            "algorithmic_bytes_per_launch": int(fg_bytes),
+        },
+        "cpu_baseline": None if cpu is None else {k: (round(v, 4) if isinstance(v, float) else v)
+                                                  for k, v in cpu.items() if k != "ms_per_frame_at_sample"},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

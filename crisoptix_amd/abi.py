@@ -73,7 +73,8 @@ class SceneInfo(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [
         ("passMs", C.c_float * CX_PASS_COUNT), ("totalMs", C.c_float), ("closestRays", C.c_uint64),
-        ("shadowRays", C.c_uint64), ("minNeighCount", C.c_uint32), ("maxNeighCount", C.c_uint32),
+        ("shadowRays", C.c_uint64), ("passClosestRays", C.c_uint64 * CX_PASS_COUNT),
+        ("passShadowRays", C.c_uint64 * CX_PASS_COUNT), ("minNeighCount", C.c_uint32), ("maxNeighCount", C.c_uint32),
         ("totalNeighCount", C.c_uint64), ("numEntries", C.c_uint32), ("pad0", C.c_uint32),
     ]
 

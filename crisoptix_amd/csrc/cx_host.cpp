@@ -459,7 +459,7 @@ int cx_renderer_create(cx_ctx* ctx, const cx_scene* scene, uint32_t w, uint32_t 
     }
     A(dalloc(r->allocs, &r->gbuf, r->P));
     A(dalloc(r->allocs, &r->neighCount, r->P));
-    A(dalloc(r->allocs, &r->counters, 8));
+    A(dalloc(r->allocs, &r->counters, NUM_COUNTERS));
     if (e != hipSuccess) {
         free_all(r->allocs);
         delete r;
@@ -613,7 +613,7 @@ int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, f
         r->evMade = true;
     }
     for (bool& b : r->passRan) b = false;
-    CX_HIP(r->ctx, launch_zero_counters(r->counters, 8, s));
+    CX_HIP(r->ctx, launch_zero_counters(r->counters, NUM_COUNTERS, s));
     rec(r, 0, s);
     // src/Renderer.cpp:259-371
     auto run = [&](cx_pass p) -> int {
@@ -667,14 +667,18 @@ int cx_renderer_enable_timing(cx_renderer* r, int enable) {
 int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* st) {
     if (!r || !st) return CX_ERR_INVALID_ARG;
     std::memset(st, 0, sizeof(*st));
-    unsigned long long c[8] = {0};
+    unsigned long long c[NUM_COUNTERS] = {0};
     CX_HIP(r->ctx, hipDeviceSynchronize());
     CX_HIP(r->ctx, hipMemcpy(c, r->counters, sizeof(c), hipMemcpyDeviceToHost));
-    st->closestRays = c[0];
-    st->shadowRays = c[1];
-    st->totalNeighCount = c[2];
-    st->minNeighCount = (uint32_t)std::min<unsigned long long>(c[3], 0xFFFFFFFFull);
-    st->maxNeighCount = (uint32_t)c[4];
+    for (int i = 0; i < CX_PASS_COUNT; i++) {
+        st->passClosestRays[i] = c[2 * i];
+        st->passShadowRays[i] = c[2 * i + 1];
+        st->closestRays += c[2 * i];
+        st->shadowRays += c[2 * i + 1];
+    }
+    st->totalNeighCount = c[STAT_TOTAL];
+    st->minNeighCount = (uint32_t)std::min<unsigned long long>(c[STAT_MIN], 0xFFFFFFFFull);
+    st->maxNeighCount = (uint32_t)c[STAT_MAX];
     uint32_t nv = 0;
     CX_HIP(r->ctx, hipMemcpy(&nv, r->entries->E.numValid, 4, hipMemcpyDeviceToHost));
     st->numEntries = nv;

@@ -34,7 +34,8 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
-// One atomic per wavefront for the ray counters.
+// One atomic per wavefront for the ray counters; counters[2 * pass + {0, 1}] hold the
+// closest-hit and shadow rays of each pass.
 __device__ __forceinline__ void flush_counters(const TraceCtx& tc, unsigned long long* counters) {
     const unsigned long long c = wave_sum((unsigned long long)tc.closest);
     const unsigned long long s = wave_sum((unsigned long long)tc.shadow);
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(BLOCK) void k_prefix_gen_temp(FrameArgs A) {
             write_output(A, p, rad);
         }
     }
-    flush_counters(tc, A.counters);
+    flush_counters(tc, A.counters + 2 * CX_PASS_PREFIX_GEN_TEMP);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(BLOCK) void k_prefix_spatial(FrameArgs A) {
         }
         if (is_pt(lp) && lp.spatialRoundIdx == lp.prefixSpatialRounds - 1) write_output(A, p, ok ? rad : v3(0.0f));
     }
-    flush_counters(tc, A.counters);
+    flush_counters(tc, A.counters + 2 * CX_PASS_PREFIX_SPATIAL);
 }
 
 __device__ __forceinline__ void invalidate(SuffixRes& r) {   // suffix_gen_temp_reuse.cu:12-17
@@ -308,7 +309,7 @@ __global__ __launch_bounds__(BLOCK) void k_suffix_gen_temp(FrameArgs A) {
         store_suffix(back, A.P, p, curr);
         A.gbuf[p] = uint4{(uint32_t)rng.s, (uint32_t)(rng.s >> 32), g.z, flags};
     }
-    flush_counters(tc, A.counters);
+    flush_counters(tc, A.counters + 2 * CX_PASS_SUFFIX_GEN_TEMP);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(BLOCK) void k_suffix_spatial(FrameArgs A) {
         (void)stored;
         store_suffix(front, A.P, p, curr);
     }
-    flush_counters(tc, A.counters);
+    flush_counters(tc, A.counters + 2 * CX_PASS_SUFFIX_SPATIAL);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -458,7 +459,7 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n, unsigned long long minInit) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = (i == 3) ? minInit : 0ull;
+    if (i < n) p[i] = (i == STAT_MIN) ? minInit : 0ull;
 }
 
 // Leaves of the implicit entry tree: leaf j holds sorted entries [j*L, (j+1)*L) ∩ [0, nValid).
@@ -629,9 +630,9 @@ __device__ __forceinline__ V3 gather_canon(const FrameArgs& A, TraceCtx& tc, uin
     const uint32_t neighCount = entries_query<KR>(A.tree, last.pos, k, tc.stack, tc.stride, nIdx, nDist);
     A.neighCount[p] = neighCount;
     if (A.lp.trackPrefixStats) {
-        atomicAdd(&A.counters[2], (unsigned long long)neighCount);
-        atomicMin(&A.counters[3], (unsigned long long)neighCount);
-        atomicMax(&A.counters[4], (unsigned long long)neighCount);
+        atomicAdd(&A.counters[STAT_TOTAL], (unsigned long long)neighCount);
+        atomicMin(&A.counters[STAT_MIN], (unsigned long long)neighCount);
+        atomicMax(&A.counters[STAT_MAX], (unsigned long long)neighCount);
     }
     const uint4* sfront = A.suffixRes[A.front];
     uint32_t validCount = 0, validFlags = 0;
@@ -678,9 +679,9 @@ __device__ __forceinline__ V3 gather_not_canon(const FrameArgs& A, TraceCtx& tc,
     const uint32_t neighCount = entries_query<KR>(A.tree, last.pos, k, tc.stack, tc.stride, nIdx, nDist);
     A.neighCount[p] = neighCount;
     if (A.lp.trackPrefixStats) {
-        atomicAdd(&A.counters[2], (unsigned long long)neighCount);
-        atomicMin(&A.counters[3], (unsigned long long)neighCount);
-        atomicMax(&A.counters[4], (unsigned long long)neighCount);
+        atomicAdd(&A.counters[STAT_TOTAL], (unsigned long long)neighCount);
+        atomicMin(&A.counters[STAT_MIN], (unsigned long long)neighCount);
+        atomicMax(&A.counters[STAT_MAX], (unsigned long long)neighCount);
     }
     const uint4* sfront = A.suffixRes[A.front];
     uint32_t validCount = 0, validFlags = 0;
@@ -728,7 +729,7 @@ __global__ __launch_bounds__(BLOCK) void k_final_gather(FrameArgs A) {
         }
         write_output(A, p, rad);
     }
-    flush_counters(tc, A.counters);
+    flush_counters(tc, A.counters + 2 * CX_PASS_FINAL_GATHER);
 }
 
 // ------------------------------------------------------------------------------------------

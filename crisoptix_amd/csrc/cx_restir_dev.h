@@ -195,8 +195,14 @@ struct FrameArgs {
     float* neighDist;            // [P][k]
     uint32_t* neighCount;        // [P]
     uint32_t k;
-    unsigned long long* counters;   // [0] closest rays, [1] shadow rays, [2] total neigh, [3] min, [4] max
+    unsigned long long* counters;   // [2 * pass + {0, 1}] closest / shadow rays, then STAT_*
 };
+
+// Counter slots after the per-pass ray counters.
+static constexpr int STAT_TOTAL = 2 * CX_PASS_COUNT;
+static constexpr int STAT_MIN = STAT_TOTAL + 1;
+static constexpr int STAT_MAX = STAT_TOTAL + 2;
+static constexpr int NUM_COUNTERS = 32;
 
 CX_HD bool is_pt(const cx_launch_params& lp) { return lp.rendererType == CX_RESTIR_PT; }
 CX_HD bool is_cond(const cx_launch_params& lp) { return lp.rendererType == CX_CONDITIONAL_RESTIR; }

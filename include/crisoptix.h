@@ -160,6 +160,8 @@ typedef struct cx_frame_stats {
     float totalMs;
     uint64_t closestRays;
     uint64_t shadowRays;
+    uint64_t passClosestRays[CX_PASS_COUNT];
+    uint64_t passShadowRays[CX_PASS_COUNT];
     /* PrefixAccelStruct::Stats (include/graph/restir/PrefixAccelStruct.h:32-44) */
     uint32_t minNeighCount;
     uint32_t maxNeighCount;

@@ -387,9 +387,31 @@ static Inter rehydrate(const Scene& s, const Packed& p) {
 }
 
 
+// Ray counters, one cache line per OpenMP thread, summed after each pass.
+struct alignas(64) RayCount {
+    uint64_t closest = 0, shadow = 0;
+};
+static RayCount g_rays[256];
+static inline RayCount& ray_count() {
+#ifdef _OPENMP
+    return g_rays[omp_get_thread_num() & 255];
+#else
+    return g_rays[0];
+#endif
+}
+static void take_rays(uint64_t& closest, uint64_t& shadow) {
+    closest = shadow = 0;
+    for (RayCount& r : g_rays) {
+        closest += r.closest;
+        shadow += r.shadow;
+        r.closest = r.shadow = 0;
+    }
+}
+
 // TraceWithDataPointer<Interaction> + __closesthit__mesh / __miss__main
 // (include/graph/trace.h:25-45, shader/mesh_hit.cu:17-69, shader/miss.cu:11-19).
 static void trace_surface(const Scene& s, const V3& o, const V3& d, float tmin, float tmax, Inter& io) {
+    ray_count().closest++;
     const Hit h = trace_closest(s, o, d, tmin, tmax);
     if (h.prim == INVALID_PRIM) {   // miss.cu:11-19
         io.valid = false;
@@ -408,6 +430,7 @@ static void trace_surface(const Scene& s, const V3& o, const V3& d, float tmin, 
 
 // TraceOcclusion (include/graph/trace.h:63-88); occluded = any hit in [tmin, tmax].
 static bool trace_occ(const Scene& s, const V3& o, const V3& d, float tmin, float tmax) {
+    ray_count().shadow++;
     return trace_occluded(s, o, d, tmin, tmax);
 }
 
@@ -1038,6 +1061,7 @@ struct Renderer {
     uint32_t statMin = 0xFFFFFFFFu, statMax = 0;
     uint64_t statTotal = 0;
     double passMs[CX_PASS_COUNT] = {0};
+    uint64_t passRays[2 * CX_PASS_COUNT] = {0};
 
     uint32_t pix() const { return W * H; }
 };
@@ -1632,6 +1656,14 @@ static double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+static void end_pass(Renderer& R, int pass, std::chrono::steady_clock::time_point t0) {
+    R.passMs[pass] += ms_since(t0);
+    uint64_t c, s;
+    take_rays(c, s);
+    R.passRays[2 * pass] += c;
+    R.passRays[2 * pass + 1] += s;
+}
+
 // Renderer::LaunchFrame (src/Renderer.cpp:209-406)
 static void launch_frame(Renderer& R, const cx_camera& cam, bool camChanged, float* out) {
     Params& P = R.P;
@@ -1648,9 +1680,12 @@ static void launch_frame(Renderer& R, const cx_camera& cam, bool camChanged, flo
     if (P.front == 0) { P.front = 1; P.back = 0; } else { P.front = 0; P.back = 1; }
     P.lp.spatialRoundIdx = 0;
     for (double& m : R.passMs) m = 0.0;
+    for (uint64_t& c : R.passRays) c = 0;
+    uint64_t dc, ds;
+    take_rays(dc, ds);
     auto t0 = std::chrono::steady_clock::now();
     pass_prefix_gen_temp(R, out);
-    R.passMs[CX_PASS_PREFIX_GEN_TEMP] = ms_since(t0);
+    end_pass(R, CX_PASS_PREFIX_GEN_TEMP, t0);
     if (P.lp.rendererType == CX_CONDITIONAL_RESTIR || P.lp.rendererType == CX_RESTIR_PT) {
         if (P.lp.prefixEnableSpatial) {
             t0 = std::chrono::steady_clock::now();
@@ -1658,33 +1693,33 @@ static void launch_frame(Renderer& R, const cx_camera& cam, bool camChanged, flo
                 P.lp.spatialRoundIdx = r;
                 pass_prefix_spatial(R, out);
             }
-            R.passMs[CX_PASS_PREFIX_SPATIAL] = ms_since(t0);
+            end_pass(R, CX_PASS_PREFIX_SPATIAL, t0);
         }
         if (is_cond(P)) {
             t0 = std::chrono::steady_clock::now();
             pass_suffix_gen_temp(R);
-            R.passMs[CX_PASS_SUFFIX_GEN_TEMP] = ms_since(t0);
+            end_pass(R, CX_PASS_SUFFIX_GEN_TEMP, t0);
             if (P.lp.suffixEnableSpatial && P.lp.suffixSpatialCount > 0) {
                 t0 = std::chrono::steady_clock::now();
                 for (int32_t r = 0; r < P.lp.suffixSpatialRounds; ++r) {
                     P.lp.spatialRoundIdx = r;
                     pass_suffix_spatial(R);
                 }
-                R.passMs[CX_PASS_SUFFIX_SPATIAL] = ms_since(t0);
+                end_pass(R, CX_PASS_SUFFIX_SPATIAL, t0);
             }
             t0 = std::chrono::steady_clock::now();
             pass_store_entries(R);
-            R.passMs[CX_PASS_PREFIX_STORE_ENTRIES] = ms_since(t0);
+            end_pass(R, CX_PASS_PREFIX_STORE_ENTRIES, t0);
             t0 = std::chrono::steady_clock::now();
             EntrySet E;
             renderer_entries(R, E);
             const uint32_t k = (uint32_t)std::max(0, P.lp.gatherM - 1);
             R.neighbors.assign((size_t)R.pix() * std::max(1u, k), Neighbor());
             R.neighCount.assign(R.pix(), 0);
-            R.passMs[CX_PASS_ENTRIES_BUILD] = ms_since(t0);
+            end_pass(R, CX_PASS_ENTRIES_BUILD, t0);
             t0 = std::chrono::steady_clock::now();
             pass_final_gather(R, E, out);
-            R.passMs[CX_PASS_FINAL_GATHER] = ms_since(t0);
+            end_pass(R, CX_PASS_FINAL_GATHER, t0);
         }
     }
 }
@@ -2002,6 +2037,11 @@ void orc_renderer_frame(void* r, const cx_camera* cam, int camChanged, float* ou
 
 void orc_renderer_pass_ms(void* r, double* ms) {
     for (int i = 0; i < CX_PASS_COUNT; i++) ms[i] = ((Renderer*)r)->passMs[i];
+}
+
+// Closest-hit and shadow rays of each pass of the last frame: [2 * pass + {0, 1}].
+void orc_renderer_rays(void* r, uint64_t* out) {
+    for (int i = 0; i < 2 * CX_PASS_COUNT; i++) out[i] = ((Renderer*)r)->passRays[i];
 }
 
 void orc_renderer_buffers(void* r, uint32_t* frontBack, uint64_t* frameIdxRandom) {

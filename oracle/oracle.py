@@ -77,6 +77,7 @@ def lib():
         L.orc_renderer_reset.argtypes = [vp]
         L.orc_renderer_frame.argtypes = [vp, C.POINTER(abi.CameraData), i32, vp]
         L.orc_renderer_pass_ms.argtypes = [vp, vp]
+        L.orc_renderer_rays.argtypes = [vp, vp]
         L.orc_renderer_buffers.argtypes = [vp, vp, vp]
         L.orc_dump_prefix.argtypes = [vp, i32, vp]
         L.orc_dump_suffix.argtypes = [vp, i32, vp]
@@ -271,6 +272,12 @@ class OracleRenderer:
         ms = np.zeros(abi.CX_PASS_COUNT, np.float64)
         lib().orc_renderer_pass_ms(self.h, _p(ms))
         return ms
+
+    def rays(self) -> np.ndarray:
+        """[pass][closest, shadow] ray counts of the last frame."""
+        out = np.zeros(2 * abi.CX_PASS_COUNT, np.uint64)
+        lib().orc_renderer_rays(self.h, _p(out))
+        return out.reshape(abi.CX_PASS_COUNT, 2)
 
     def buffers(self):
         fb = np.zeros(2, np.uint32)
