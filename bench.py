@@ -51,8 +51,9 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--grid", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-scale", type=int, default=8,
+    ap.add_argument("--cpu-sample-scale", type=int, default=1,
                     help="CPU baseline renders (W/scale) x (H/scale) pixels of the same workload")
+    ap.add_argument("--cpu-frames", type=int, default=6)
     return ap.parse_args()
 
 
@@ -91,7 +92,7 @@ def cpu_baseline(scene, params_for, W, H, scale, frames=2):
         "unit": "Mrays/s",
         "cores": O.num_threads(),
         "kind": "port",
-        "sample": f"{frames} frames of the same workload at {w}x{h} (1/{scale * scale} of the pixels), "
+        "sample": f"{frames} frames (after 1 warm-up frame) of the same workload at {w}x{h}, "
                   f"oracle host C++ with OpenMP, {secs * 1000 / frames:.0f} ms/frame",
         "ms_per_frame_at_sample": secs * 1000 / frames,
     }
@@ -160,13 +161,15 @@ def main():
                                                   pass_rays[:, 1].sum() * SHADOW_RAY_BYTES) / a.steps
     out = runtime.DeviceArray.numpy(ren.output)
     log("per-pass ms: " + ", ".join(f"{n} {m / a.steps:.3f}" for n, m in zip(abi.PASS_NAMES, pass_ms)))
+    log("per-pass rays (closest, shadow): " + ", ".join(
+        f"{n} {int(c / a.steps)}/{int(s / a.steps)}" for n, (c, s) in zip(abi.PASS_NAMES, pass_rays)))
     log(f"rays/frame {total_rays / a.steps:.0f} ({total_rays / a.steps / P:.2f}/px), entries/frame "
         f"{entries / a.steps:.0f}, mean radiance {float(out.mean()):.5f}, finite {bool(np.isfinite(out).all())}")
 
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
-        cpu = cpu_baseline(scene, params_for, W, H, a.cpu_sample_scale)
+        cpu = cpu_baseline(scene, params_for, W, H, a.cpu_sample_scale, a.cpu_frames)
         log(f"cpu baseline: {cpu['value']:.2f} Mrays/s on {cpu['cores']} threads")
 
     line = {

@@ -129,10 +129,26 @@ __device__ __forceinline__ void box2(const float4 a, const float4 b, const float
 
 __device__ __forceinline__ V3 load_v3(const float* p, uint32_t i) { return V3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 
-// Traversal with a per-thread stack column in LDS (stack[i * stride]).
-template <bool ANY>
-__device__ __forceinline__ bool traverse(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit& best,
-                                         int* stack, int stride) {
+// Traversal stack: the first SHORT entries in a per-thread LDS column (lds[i * stride]), deeper
+// entries in a per-lane private array.
+template <int SHORT>
+struct TravStack {
+    int* lds;
+    int stride;
+    int ovf[STACK_DEPTH - SHORT > 0 ? STACK_DEPTH - SHORT : 1];
+    __device__ __forceinline__ void put(int i, int v) {
+        if (SHORT >= STACK_DEPTH || i < SHORT) lds[i * stride] = v;
+        else ovf[i - SHORT] = v;
+    }
+    __device__ __forceinline__ int get(int i) const {
+        if (SHORT >= STACK_DEPTH || i < SHORT) return lds[i * stride];
+        return ovf[i - SHORT];
+    }
+};
+
+template <bool ANY, typename Stack>
+__device__ __forceinline__ bool traverse_s(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit& best,
+                                           Stack& st) {
     best.prim = INVALID_PRIM;
     best.t = tmax;
     best.u = 0.0f;
@@ -152,7 +168,7 @@ __device__ __forceinline__ bool traverse(const DScene& S, V3 o, V3 d, float tmin
             box2(a, b, c, o, inv, tmin, best.t, h0, h1, t0, t1);
             if (h0 && h1) {
                 const bool first0 = t0 <= t1;
-                stack[sp * stride] = first0 ? dd.y : dd.x;
+                st.put(sp, first0 ? dd.y : dd.x);
                 ++sp;
                 ref = first0 ? dd.x : dd.y;
                 continue;
@@ -185,9 +201,19 @@ __device__ __forceinline__ bool traverse(const DScene& S, V3 o, V3 d, float tmin
         }
         if (sp == 0) break;
         --sp;
-        ref = stack[sp * stride];
+        ref = st.get(sp);
     }
     return best.prim != INVALID_PRIM;
+}
+
+// Traversal with the whole stack in a per-thread LDS column (stack[i * stride]).
+template <bool ANY>
+__device__ __forceinline__ bool traverse(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit& best,
+                                         int* stack, int stride) {
+    TravStack<STACK_DEPTH> st;
+    st.lds = stack;
+    st.stride = stride;
+    return traverse_s<ANY>(S, o, d, tmin, tmax, best, st);
 }
 
 // ---- interactions (include/graph/Interaction.h) ----

@@ -1,0 +1,812 @@
+// Wavefront execution of the frame loop: path generation and hybrid shifts as explicit state
+// machines whose contexts live in chunked SoA queues in HBM, and ray queries answered by
+// separate traversal kernels that run at high occupancy.
+//
+// A machine step consumes the answer to the job's previous ray, runs the reference's logic up
+// to the next trace call, and either appends a new ray to the next queue or finishes.  Closest-
+// hit rays are appended from the front of a queue and shadow rays from the back, so each
+// traversal kernel walks one contiguous range.  Appends are wave-aggregated (one atomic per
+// wavefront, ballot + prefix count), so a wavefront's surviving jobs land in consecutive slots
+// and both the context stores and the next loads are coalesced.
+//
+// Every machine computes exactly what the corresponding function of cx_restir_dev.h computes;
+// only the points where control leaves for a traversal differ.  Traversal is a pure function
+// of the ray, so the re-trace of path_gen.h:165-187 (same ray, same hit) is not issued again.
+#pragma once
+
+#include "cx_restir_dev.h"
+
+namespace cx {
+
+// ---- queues ----
+
+static constexpr uint32_t EMIT_NONE = 0, EMIT_CLOSEST = 1, EMIT_SHADOW = 2;
+
+struct Queue {
+    uint4* ctx;         // [chunks][cap]
+    uint32_t* job;      // [cap]
+    float4* ray;        // [2][cap] origin.xyz tmin | dir.xyz tmax
+    uint4* res;         // [cap] closest: prim t u v; shadow: occluded
+    uint32_t* count;    // [2] closest count (front), shadow count (back)
+    uint32_t cap;
+};
+
+struct Emit {
+    uint32_t kind;
+    V3 o, d;
+    float tmin, tmax;
+};
+
+CX_HD Emit emit_none() { return Emit{EMIT_NONE, v3(0.0f), v3(0.0f), 0.0f, 0.0f}; }
+CX_HD Emit emit_closest(V3 o, V3 d) { return Emit{EMIT_CLOSEST, o, d, 1e-3f, 1e16f}; }
+CX_HD Emit emit_shadow(V3 o, V3 d, float tmax) { return Emit{EMIT_SHADOW, o, d, 1e-3f, tmax}; }
+
+// Slot of the input queue handled by thread i: the closest range [0, nc) first, then the
+// shadow range [cap - ns, cap).
+__device__ __forceinline__ uint32_t in_slot(const Queue& q, uint32_t i, uint32_t nc, uint32_t ns) {
+    return i < nc ? i : q.cap - ns + (i - nc);
+}
+
+// Wave-aggregated append; every lane of the wavefront must call it.  Returns the slot.
+__device__ __forceinline__ uint32_t append(const Queue& q, uint32_t kind) {
+    const uint64_t mc = __ballot(kind == EMIT_CLOSEST);
+    const uint64_t ms = __ballot(kind == EMIT_SHADOW);
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t bc = 0, bs = 0;
+    if (lane == 0) {
+        if (mc) bc = atomicAdd(&q.count[0], (uint32_t)__popcll(mc));
+        if (ms) bs = atomicAdd(&q.count[1], (uint32_t)__popcll(ms));
+    }
+    bc = __shfl(bc, 0, 64);
+    bs = __shfl(bs, 0, 64);
+    if (kind == EMIT_CLOSEST) return bc + (uint32_t)__popcll(mc & lt);
+    if (kind == EMIT_SHADOW) return q.cap - 1u - (bs + (uint32_t)__popcll(ms & lt));
+    return 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ void put_ray(const Queue& q, uint32_t slot, const Emit& e) {
+    q.ray[slot] = float4{e.o.x, e.o.y, e.o.z, e.tmin};
+    q.ray[(size_t)q.cap + slot] = float4{e.d.x, e.d.y, e.d.z, e.tmax};
+}
+
+__device__ __forceinline__ void get_ray(const Queue& q, uint32_t slot, V3& o, V3& d) {
+    const float4 a = q.ray[slot], b = q.ray[(size_t)q.cap + slot];
+    o = V3{a.x, a.y, a.z};
+    d = V3{b.x, b.y, b.z};
+}
+
+template <int N>
+struct Ctx {
+    uint4 w[N];
+};
+
+template <int N>
+__device__ __forceinline__ Ctx<N> load_ctx(const Queue& q, uint32_t slot) {
+    Ctx<N> c;
+#pragma unroll
+    for (int i = 0; i < N; i++) c.w[i] = q.ctx[(size_t)i * q.cap + slot];
+    return c;
+}
+
+template <int N>
+__device__ __forceinline__ void store_ctx(const Queue& q, uint32_t slot, const Ctx<N>& c) {
+#pragma unroll
+    for (int i = 0; i < N; i++) q.ctx[(size_t)i * q.cap + slot] = c.w[i];
+}
+
+// Interaction from a closest-hit answer (__closesthit__mesh / __miss__main): the normal is
+// normalised; a miss only clears `valid` and sets the direction.
+__device__ __forceinline__ Inter inter_from_hit(const DScene& S, uint4 h, V3 d) {
+    Inter it;
+    it.dir = d;
+    it.prim = h.x;
+    it.bu = bf(h.z);
+    it.bv = bf(h.w);
+    it.valid = h.x != INVALID_PRIM;
+    it.pos = v3(0.0f);
+    it.nrm = v3(0.0f);
+    if (!it.valid) return it;
+    const uint32_t i0 = S.idx[3 * h.x], i1 = S.idx[3 * h.x + 1], i2 = S.idx[3 * h.x + 2];
+    it.pos = interp(it.bu, it.bv, load_v3(S.pos, i0), load_v3(S.pos, i1), load_v3(S.pos, i2));
+    it.nrm = normalize(interp(it.bu, it.bv, load_v3(S.nrm, i0), load_v3(S.nrm, i1), load_v3(S.nrm, i2)));
+    return it;
+}
+
+// Reconstructs a stored interaction: rehydration, plus normalisation when it came from a trace.
+__device__ __forceinline__ Inter inter_restore(const DScene& S, const Packed& p, bool traced) {
+    Inter it = rehydrate(S, p);
+    if (traced && it.valid) it.nrm = normalize(it.nrm);
+    return it;
+}
+
+// ---- packing helpers: contexts are written and read as a sequence of 32-bit words ----
+
+template <int N>
+struct Wr {
+    uint32_t a[4 * N];
+    int n;
+    __device__ __forceinline__ Wr() : n(0) {
+#pragma unroll
+        for (int i = 0; i < 4 * N; i++) a[i] = 0u;
+    }
+    __device__ __forceinline__ void u(uint32_t x) { a[n++] = x; }
+    __device__ __forceinline__ void f(float x) { a[n++] = fb(x); }
+    __device__ __forceinline__ void v(V3 x) { f(x.x); f(x.y); f(x.z); }
+    __device__ __forceinline__ void u64(uint64_t x) { u((uint32_t)x); u((uint32_t)(x >> 32)); }
+    __device__ __forceinline__ void pk(const Packed& p) { u(p.prim); f(p.bu); f(p.bv); v(p.dir); }
+    __device__ __forceinline__ Ctx<N> ctx() const {
+        Ctx<N> c;
+#pragma unroll
+        for (int i = 0; i < N; i++) c.w[i] = uint4{a[4 * i], a[4 * i + 1], a[4 * i + 2], a[4 * i + 3]};
+        return c;
+    }
+};
+
+template <int N>
+struct Rd {
+    uint32_t a[4 * N];
+    int n;
+    __device__ __forceinline__ explicit Rd(const Ctx<N>& c) : n(0) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            a[4 * i] = c.w[i].x;
+            a[4 * i + 1] = c.w[i].y;
+            a[4 * i + 2] = c.w[i].z;
+            a[4 * i + 3] = c.w[i].w;
+        }
+    }
+    __device__ __forceinline__ uint32_t u() { return a[n++]; }
+    __device__ __forceinline__ float f() { return bf(a[n++]); }
+    __device__ __forceinline__ V3 v() {
+        const float x = f(), y = f(), z = f();
+        return V3{x, y, z};
+    }
+    __device__ __forceinline__ uint64_t u64() {
+        const uint64_t lo = u(), hi = u();
+        return lo | (hi << 32);
+    }
+    __device__ __forceinline__ Packed pk() {
+        Packed p;
+        p.prim = u();
+        p.bu = f();
+        p.bv = f();
+        p.dir = v();
+        return p;
+    }
+};
+
+// ======================================================================================
+// PrefixGen: TracePrefix (path_gen.h:74-250)
+// ======================================================================================
+
+static constexpr uint32_t PG_TRACE = 1, PG_NEE = 2;
+static constexpr int PG_CHUNKS = 15;
+
+struct PgState {
+    Prefix pre;
+    uint64_t rng;
+    V3 cur, dir;
+    uint32_t traceIdx, postRecon, state;
+    Packed it;            // traced interaction (normalised normal)
+    V3 lightDir;
+    float dist;
+    V3 color;
+};
+
+__device__ __forceinline__ Ctx<PG_CHUNKS> pg_pack(const PgState& s) {
+    Wr<PG_CHUNKS> w;
+    const Prefix& p = s.pre;
+    w.u64(p.rng); w.u(p.flags); w.f(p.p); w.v(p.f); w.f(p.pathLen); w.v(p.postReconF); w.v(p.reconOutDir);
+    w.pk(p.primary); w.pk(p.recon); w.pk(p.last);
+    w.u64(s.rng); w.u(s.traceIdx | (s.postRecon << 8) | (s.state << 16));
+    w.v(s.cur); w.v(s.dir); w.pk(s.it); w.v(s.lightDir); w.f(s.dist); w.v(s.color);
+    return w.ctx();
+}
+
+__device__ __forceinline__ PgState pg_unpack(const Ctx<PG_CHUNKS>& c) {
+    Rd<PG_CHUNKS> r(c);
+    PgState s;
+    Prefix& p = s.pre;
+    p.rng = r.u64(); p.flags = r.u(); p.p = r.f(); p.f = r.v(); p.pathLen = r.f(); p.postReconF = r.v();
+    p.reconOutDir = r.v(); p.primary = r.pk(); p.recon = r.pk(); p.last = r.pk();
+    s.rng = r.u64();
+    const uint32_t m = r.u();
+    s.traceIdx = m & 0xFFu;
+    s.postRecon = (m >> 8) & 1u;
+    s.state = m >> 16;
+    s.cur = r.v(); s.dir = r.v(); s.it = r.pk(); s.lightDir = r.v(); s.dist = r.f(); s.color = r.v();
+    return s;
+}
+
+__device__ __forceinline__ uint32_t pg_max_len(const cx_launch_params& lp) {
+    return is_pt(lp) ? (uint32_t)lp.maxPathLen : (uint32_t)lp.prefixLen;
+}
+
+// Start of TracePrefix with the camera ray: returns the first emission.
+__device__ __forceinline__ Emit pg_start(const cx_launch_params& lp, PgState& s, V3 origin, V3 dir) {
+    s.pre.flags = 0u;
+    s.pre.pathLen = 0.0f;
+    s.pre.rng = s.rng;
+    s.pre.p = 1.0f;
+    s.pre.f = v3(1.0f);
+    s.pre.postReconF = v3(1.0f);
+    s.pre.set_valid(true);
+    s.cur = origin;
+    s.dir = dir;
+    s.traceIdx = 0;
+    s.postRecon = 0;
+    s.it = packed_invalid();
+    if (pg_max_len(lp) == 0) {   // loop body never runs: lastInt stays the default (invalid)
+        s.pre.last = packed_invalid();
+        return emit_none();
+    }
+    s.state = PG_TRACE;
+    return emit_closest(origin, dir);
+}
+
+// After the closest hit of the current vertex (path_gen.h:100-247).
+__device__ __forceinline__ Emit pg_on_hit(const DScene& S, const cx_launch_params& lp, PgState& s, const Inter& it) {
+    Prefix& pre = s.pre;
+    Rng rng;
+    rng.s = s.rng;
+    const uint32_t maxLen = pg_max_len(lp);
+    if (!it.valid) {
+        pre.set_valid(false);
+        return emit_none();
+    }
+    pre.pathLen += distance(s.cur, it.pos);
+    pre.set_length(pre.length() + 1);
+    if (pre.length() == 1) pre.primary = it.pack();
+    if (is_cond(lp) && s.traceIdx == maxLen - 1) {
+        if (!s.postRecon) {
+            pre.recon = it.pack();
+            pre.set_recon_idx(pre.length());
+            pre.reconOutDir = v3(0.0f);
+        }
+        pre.set_valid(true);
+        pre.last = it.pack();
+        return emit_none();
+    }
+    const bool forceNee = is_pt(lp) && s.traceIdx == maxLen - 1;
+    if (rng.f() < lp.neeProb || forceNee) {
+        pre.p *= lp.neeProb;
+        const EmitterSample es = sample_emitter(S, rng);
+        s.lightDir = normalize(es.pos - it.pos);
+        s.dist = length(es.pos - it.pos);
+        s.color = es.color;
+        s.it = it.pack();
+        s.rng = rng.s;
+        s.state = PG_NEE;
+        return emit_shadow(it.pos, s.lightDir, s.dist);
+    }
+    const BrdfSample bs = ggx_sample(S, it, rng);
+    s.rng = rng.s;
+    if (bs.pdf <= 0.0f) {
+        pre.set_valid(false);
+        return emit_none();
+    }
+    const float reconDistance = distance(s.cur, it.pos);
+    const bool canRecon =
+        reconDistance > lp.reconMinDistance && bs.roughness > lp.reconMinRoughness && pre.length() >= 2;
+    if (!s.postRecon && canRecon) {
+        pre.recon = it.pack();
+        pre.set_recon_idx(pre.length());
+        pre.reconOutDir = bs.outDir;
+        s.postRecon = 1;
+    }
+    s.cur = it.pos;
+    s.dir = bs.outDir;
+    pre.f = pre.f * bs.brdf;
+    pre.p *= bs.pdf * (1.0f - lp.neeProb);
+    if (s.postRecon) pre.postReconF = pre.postReconF * bs.brdf;
+    s.traceIdx++;
+    if (s.traceIdx < maxLen) {
+        s.state = PG_TRACE;
+        return emit_closest(s.cur, s.dir);
+    }
+    pre.last = it.pack();
+    return emit_none();
+}
+
+// After the NEE shadow ray (path_gen.h:158-210).
+__device__ __forceinline__ Emit pg_on_shadow(const DScene& S, const cx_launch_params& lp, PgState& s, bool occluded) {
+    Prefix& pre = s.pre;
+    if (occluded) {
+        pre.set_valid(false);
+        return emit_none();
+    }
+    const Inter it = inter_restore(S, s.it, true);
+    if (!s.postRecon) {   // the re-trace of the same ray returns this same hit
+        pre.recon = s.it;
+        pre.set_recon_idx(pre.length());
+        pre.reconOutDir = s.dir;
+        s.postRecon = 1;
+    }
+    const BrdfEval be = ggx_eval(S, it, s.lightDir);
+    if (be.pdf <= 0.0f) {
+        pre.set_valid(false);
+        return emit_none();
+    }
+    const float distFactor = 1.0f / (s.dist * s.dist);
+    pre.f = pre.f * (be.brdf * s.color * distFactor);
+    if (s.postRecon) pre.postReconF = pre.postReconF * (be.brdf * s.color * distFactor);
+    if (pre.length() == 1) pre.f = pre.f + be.emission * distFactor;
+    pre.set_nee(true);
+    pre.set_valid(true);
+    pre.last = s.it;
+    return emit_none();
+}
+
+// ======================================================================================
+// SuffixGen: TraceSuffix (path_gen.h:252-387)
+// ======================================================================================
+
+static constexpr uint32_t SG_TRACE = 1, SG_NEE = 2, SG_RECON = 3;
+static constexpr int SG_CHUNKS = 13;
+
+struct SgState {
+    Suffix suf;
+    uint64_t rng;
+    uint32_t vertIdx, maxLen, prefixLen, postRecon, state, itTraced;
+    Packed it;
+    V3 lastDir;
+    V3 lightDir;
+    float lightLen;
+    V3 color;
+    V3 brdf;
+};
+
+__device__ __forceinline__ Ctx<SG_CHUNKS> sg_pack(const SgState& s) {
+    Wr<SG_CHUNKS> w;
+    const Suffix& u = s.suf;
+    w.u64(u.rng); w.u(u.flags); w.f(u.p); w.v(u.f); w.v(u.postReconF); w.v(u.reconOutDir);
+    w.pk(u.recon); w.pk(u.lastPrefix);
+    w.u64(s.rng);
+    w.u(s.vertIdx | (s.maxLen << 8) | (s.prefixLen << 16) | (s.postRecon << 24) | (s.itTraced << 25));
+    w.u(s.state);
+    w.pk(s.it); w.v(s.lastDir); w.v(s.lightDir); w.f(s.lightLen); w.v(s.color); w.v(s.brdf);
+    return w.ctx();
+}
+
+__device__ __forceinline__ SgState sg_unpack(const Ctx<SG_CHUNKS>& c) {
+    Rd<SG_CHUNKS> r(c);
+    SgState s;
+    Suffix& u = s.suf;
+    u.rng = r.u64(); u.flags = r.u(); u.p = r.f(); u.f = r.v(); u.postReconF = r.v(); u.reconOutDir = r.v();
+    u.recon = r.pk(); u.lastPrefix = r.pk();
+    s.rng = r.u64();
+    const uint32_t m = r.u();
+    s.vertIdx = m & 0xFFu;
+    s.maxLen = (m >> 8) & 0xFFu;
+    s.prefixLen = (m >> 16) & 0xFFu;
+    s.postRecon = (m >> 24) & 1u;
+    s.itTraced = (m >> 25) & 1u;
+    s.state = r.u();
+    s.it = r.pk(); s.lastDir = r.v(); s.lightDir = r.v(); s.lightLen = r.f(); s.color = r.v(); s.brdf = r.v();
+    return s;
+}
+
+// One vertex of the suffix loop once its interaction is known (path_gen.h:296-385).
+__device__ __forceinline__ Emit sg_vertex(const DScene& S, const cx_launch_params& lp, SgState& s, const Inter& it,
+                                          V3 lastPos) {
+    Suffix& suf = s.suf;
+    Rng rng;
+    rng.s = s.rng;
+    const bool forceNee = s.vertIdx == s.maxLen - 1;
+    if (rng.f() < lp.neeProb || forceNee) {
+        if (!forceNee) suf.p *= lp.neeProb;
+        const EmitterSample em = sample_emitter(S, rng);
+        s.rng = rng.s;
+        const V3 lightVec = em.pos - it.pos;
+        s.lightLen = length(lightVec);
+        s.lightDir = normalize(lightVec);
+        if (any_bad(s.lightDir)) return emit_none();
+        s.color = em.color;
+        s.it = it.pack();
+        s.state = SG_NEE;
+        return emit_shadow(it.pos, s.lightDir, s.lightLen);
+    }
+    const BrdfSample bs = ggx_sample(S, it, rng);
+    s.rng = rng.s;
+    if (bs.pdf <= 0.0f) return emit_none();
+    suf.p *= bs.pdf * (1.0f - lp.neeProb);
+    suf.f = suf.f * bs.brdf;
+    if (s.postRecon) suf.postReconF = suf.postReconF * bs.brdf;
+    const float reconDistance = distance(lastPos, it.pos);
+    const bool canRecon =
+        reconDistance > lp.reconMinDistance && bs.roughness > lp.reconMinRoughness && s.prefixLen >= 1;
+    if (!s.postRecon && canRecon) {
+        suf.recon = it.pack();
+        suf.set_recon_idx(s.prefixLen);   // path_gen.h:378
+        suf.reconOutDir = bs.outDir;
+        s.postRecon = 1;
+    }
+    s.lastDir = bs.outDir;
+    suf.set_length(suf.length() + 1);
+    s.vertIdx++;
+    if (s.vertIdx < s.maxLen) {
+        s.state = SG_TRACE;
+        return emit_closest(it.pos, s.lastDir);
+    }
+    return emit_none();
+}
+
+// Start of TraceSuffix from a prefix (path_gen.h:258-294).
+__device__ __forceinline__ Emit sg_start(const DScene& S, const cx_launch_params& lp, SgState& s, const Prefix& pre) {
+    Suffix& suf = s.suf;
+    suf.f = v3(1.0f);
+    suf.p = 1.0f;
+    suf.postReconF = v3(1.0f);
+    suf.rng = s.rng;
+    suf.lastPrefix = pre.last;
+    suf.flags = 0u;
+    s.postRecon = 0;
+    s.vertIdx = 0;
+    s.lastDir = v3(0.0f);
+    s.state = 0;
+    s.itTraced = 0;
+    s.prefixLen = pre.length();
+    if (!pre.valid() || pre.nee()) return emit_none();
+    const uint32_t maxLen = (uint32_t)lp.maxPathLen - pre.length();
+    s.maxLen = maxLen & 0xFFu;
+    const Inter it = rehydrate(S, pre.last);
+    if (!it.valid) return emit_none();
+    if (maxLen == 0) return emit_none();
+    return sg_vertex(S, lp, s, it, it.pos);
+}
+
+__device__ __forceinline__ Emit sg_on_hit(const DScene& S, const cx_launch_params& lp, SgState& s, const Inter& it,
+                                          V3 rayOrigin) {
+    s.itTraced = 1;
+    if (!it.valid) return emit_none();
+    return sg_vertex(S, lp, s, it, rayOrigin);
+}
+
+__device__ __forceinline__ Emit sg_finish(SgState& s) {
+    Suffix& suf = s.suf;
+    const float distFactor = 1.0f / (s.lightLen * s.lightLen);
+    suf.f = suf.f * (s.brdf * s.color * distFactor);
+    suf.postReconF = suf.postReconF * (s.color * distFactor);
+    if (suf.recon_idx() < suf.length()) suf.postReconF = suf.postReconF * s.brdf;
+    suf.set_valid(true);
+    return emit_none();
+}
+
+__device__ __forceinline__ Emit sg_on_shadow(const DScene& S, SgState& s, bool occluded) {
+    if (occluded) return emit_none();
+    const Inter it = inter_restore(S, s.it, s.itTraced != 0);
+    const BrdfEval be = ggx_eval(S, it, s.lightDir);
+    if (be.pdf <= 0.0f) return emit_none();
+    s.suf.set_length(s.suf.length() + 1);
+    s.brdf = be.brdf;
+    if (!s.postRecon) {
+        s.state = SG_RECON;
+        return emit_closest(it.pos, s.lightDir);
+    }
+    return sg_finish(s);
+}
+
+__device__ __forceinline__ Emit sg_on_recon(SgState& s, const Inter& ri) {
+    if (!ri.valid) return emit_none();
+    s.suf.recon = ri.pack();
+    s.suf.set_recon_idx(s.suf.length());
+    s.suf.reconOutDir = v3(0.0f);
+    s.postRecon = 1;
+    return sg_finish(s);
+}
+
+// ======================================================================================
+// CompletePath: TraceCompletePath (path_gen.h:389-472)
+// ======================================================================================
+
+static constexpr uint32_t CP_TRACE = 1, CP_NEE = 2;
+static constexpr int CP_CHUNKS = 7;
+
+struct CpState {
+    uint64_t rng;
+    uint32_t traceIdx, depth, state;
+    V3 cur, dir, thr, radiance;
+    Packed it;
+    V3 lightDir;
+    float dist;
+    V3 color;
+};
+
+__device__ __forceinline__ Ctx<CP_CHUNKS> cp_pack(const CpState& s) {
+    Wr<CP_CHUNKS> w;
+    w.u64(s.rng); w.u(s.traceIdx | (s.depth << 8) | (s.state << 16));
+    w.v(s.cur); w.v(s.dir); w.v(s.thr); w.pk(s.it); w.v(s.lightDir); w.f(s.dist); w.v(s.color);
+    return w.ctx();
+}
+
+__device__ __forceinline__ CpState cp_unpack(const Ctx<CP_CHUNKS>& c) {
+    Rd<CP_CHUNKS> r(c);
+    CpState s;
+    s.rng = r.u64();
+    const uint32_t m = r.u();
+    s.traceIdx = m & 0xFFu;
+    s.depth = (m >> 8) & 0xFFu;
+    s.state = m >> 16;
+    s.cur = r.v(); s.dir = r.v(); s.thr = r.v(); s.it = r.pk(); s.lightDir = r.v(); s.dist = r.f(); s.color = r.v();
+    s.radiance = v3(0.0f);
+    return s;
+}
+
+// Returns an emission; when none, `done` holds the radiance in s.radiance.
+__device__ __forceinline__ Emit cp_on_hit(const DScene& S, const cx_launch_params& lp, CpState& s, const Inter& it) {
+    if (!it.valid) {
+        s.radiance = v3(0.0f);
+        return emit_none();
+    }
+    ++s.depth;
+    Rng rng;
+    rng.s = s.rng;
+    if (rng.f() < lp.neeProb) {
+        s.thr = s.thr / lp.neeProb;
+        const EmitterSample es = sample_emitter(S, rng);
+        s.rng = rng.s;
+        s.lightDir = normalize(es.pos - it.pos);
+        s.dist = length(es.pos - it.pos);
+        s.color = es.color;
+        s.it = it.pack();
+        s.state = CP_NEE;
+        return emit_shadow(it.pos, s.lightDir, s.dist);
+    }
+    const BrdfSample bs = ggx_sample(S, it, rng);
+    s.rng = rng.s;
+    if (bs.pdf <= 0.0f) {
+        s.radiance = v3(0.0f);
+        return emit_none();
+    }
+    s.cur = it.pos;
+    s.dir = bs.outDir;
+    s.thr = s.thr * (bs.brdf / bs.pdf);
+    s.thr = s.thr / (1.0f - lp.neeProb);
+    s.traceIdx++;
+    if (s.traceIdx < (uint32_t)lp.maxPathLen) {
+        s.state = CP_TRACE;
+        return emit_closest(s.cur, s.dir);
+    }
+    s.radiance = v3(0.0f);
+    return emit_none();
+}
+
+__device__ __forceinline__ Emit cp_on_shadow(const DScene& S, CpState& s, bool occluded) {
+    if (occluded) {
+        s.radiance = v3(0.0f);
+        return emit_none();
+    }
+    const Inter it = inter_restore(S, s.it, true);
+    const BrdfEval be = ggx_eval(S, it, s.lightDir);
+    const float distFactor = 1.0f / (s.dist * s.dist);
+    s.radiance = s.thr * be.brdf * s.color * distFactor;
+    if (s.depth == 1) s.radiance = s.radiance + be.emission;
+    return emit_none();
+}
+
+// ======================================================================================
+// PrefixFG: TracePrefixForFinalGather (path_gen.h:474-531)
+// ======================================================================================
+
+static constexpr int FG_CHUNKS = 3;
+
+struct FgState {
+    uint64_t rng;
+    uint32_t idx;
+    V3 thr;
+    float p;
+    V3 dir;
+};
+
+__device__ __forceinline__ Ctx<FG_CHUNKS> fg_pack(const FgState& s) {
+    Wr<FG_CHUNKS> w;
+    w.u64(s.rng); w.u(s.idx); w.f(s.p); w.v(s.thr); w.v(s.dir);
+    return w.ctx();
+}
+
+__device__ __forceinline__ FgState fg_unpack(const Ctx<FG_CHUNKS>& c) {
+    Rd<FG_CHUNKS> r(c);
+    FgState s;
+    s.rng = r.u64(); s.idx = r.u(); s.p = r.f(); s.thr = r.v(); s.dir = r.v();
+    return s;
+}
+
+// ======================================================================================
+// Shift: hybrid shift of a prefix (prefix_reuse.h:10-99) or a suffix (suffix_reuse.h:10-123)
+// ======================================================================================
+
+// Where a shift reads its source / destination record.
+static constexpr uint32_t REF_PREFIX_FRONT = 0, REF_PREFIX_BACK = 1, REF_PREFIX_SCRATCH = 2, REF_SUFFIX_FRONT = 3,
+                          REF_SUFFIX_BACK = 4, REF_SUFFIX_SCRATCH = 5, REF_PACKED_TRACED = 6;
+CX_HD uint32_t make_ref(uint32_t buf, uint32_t pixel) { return (buf << 28) | pixel; }
+
+struct RecordSet {
+    const uint4* prefixRes[2];
+    const uint4* suffixRes[2];
+    const uint4* prefixScratch;   // canonical prefix records (PrefixRes layout)
+    const uint4* suffixScratch;   // canonical suffix records (SuffixRes layout)
+    const uint4* packedScratch;   // [P] x 2 chunks: Packed of a traced interaction
+    uint32_t front, back, P;
+};
+
+__device__ __forceinline__ Prefix load_prefix_ref(const RecordSet& R, uint32_t ref) {
+    const uint32_t buf = ref >> 28, p = ref & 0x0FFFFFFFu;
+    const uint4* b = buf == REF_PREFIX_FRONT ? R.prefixRes[R.front]
+                                             : (buf == REF_PREFIX_BACK ? R.prefixRes[R.back] : R.prefixScratch);
+    return load_prefix(b, R.P, p).s;
+}
+
+__device__ __forceinline__ Suffix load_suffix_ref(const RecordSet& R, uint32_t ref) {
+    const uint32_t buf = ref >> 28, p = ref & 0x0FFFFFFFu;
+    const uint4* b = buf == REF_SUFFIX_FRONT ? R.suffixRes[R.front]
+                                             : (buf == REF_SUFFIX_BACK ? R.suffixRes[R.back] : R.suffixScratch);
+    return load_suffix(b, R.P, p).s;
+}
+
+// Destination interaction of a suffix shift: the destination suffix's last prefix vertex
+// (rehydrated, suffix_reuse.h:217) or a traced interaction (final_gather.cu:265).
+__device__ __forceinline__ Inter load_suffix_dst(const DScene& S, const RecordSet& R, uint32_t ref) {
+    const uint32_t buf = ref >> 28, p = ref & 0x0FFFFFFFu;
+    if (buf == REF_PACKED_TRACED) {
+        const uint4 a = R.packedScratch[2 * (size_t)p], b = R.packedScratch[2 * (size_t)p + 1];
+        const Packed pk = Packed{a.x, bf(a.y), bf(a.z), V3{bf(a.w), bf(b.x), bf(b.y)}};
+        return inter_restore(S, pk, true);
+    }
+    return rehydrate(S, load_suffix_ref(R, ref).lastPrefix);
+}
+
+static constexpr uint32_t SH_TRACE = 1, SH_SHADOW = 2;
+static constexpr int SH_CHUNKS = 4;
+
+struct ShState {
+    uint32_t srcRef, isSuffix, idx, count, state, currTraced;
+    uint64_t srng;
+    V3 thr;
+    Packed curr;
+};
+
+__device__ __forceinline__ Ctx<SH_CHUNKS> sh_pack(const ShState& s) {
+    Wr<SH_CHUNKS> w;
+    w.u(s.srcRef);
+    w.u(s.idx | (s.count << 8) | (s.state << 16) | (s.isSuffix << 24) | (s.currTraced << 25));
+    w.u64(s.srng); w.v(s.thr); w.pk(s.curr);
+    return w.ctx();
+}
+
+__device__ __forceinline__ ShState sh_unpack(const Ctx<SH_CHUNKS>& c) {
+    Rd<SH_CHUNKS> r(c);
+    ShState s;
+    s.srcRef = r.u();
+    const uint32_t m = r.u();
+    s.idx = m & 0xFFu;
+    s.count = (m >> 8) & 0xFFu;
+    s.state = (m >> 16) & 0xFFu;
+    s.isSuffix = (m >> 24) & 1u;
+    s.currTraced = (m >> 25) & 1u;
+    s.srng = r.u64(); s.thr = r.v(); s.curr = r.pk();
+    return s;
+}
+
+// Replay step or reconnection ray, from the current interaction.
+__device__ __forceinline__ Emit sh_continue(const DScene& S, const RecordSet& R, ShState& s, const Inter& curr) {
+    if (s.idx < s.count) {
+        Rng srng;
+        srng.s = s.srng;
+        srng.f();   // dummy draw for the NEE decision (prefix_reuse.h:44, suffix_reuse.h:43)
+        const BrdfSample bs = ggx_sample(S, curr, srng);
+        s.srng = srng.s;
+        if (bs.pdf <= 0.0f) return emit_none();
+        s.thr = s.thr * bs.brdf;
+        s.curr = curr.pack();
+        s.state = SH_TRACE;
+        return emit_closest(curr.pos, bs.outDir);
+    }
+    const Packed reconP = s.isSuffix ? load_suffix_ref(R, s.srcRef).recon : load_prefix_ref(R, s.srcRef).recon;
+    const Inter srcRecon = rehydrate(S, reconP);
+    const V3 reconVec = srcRecon.pos - curr.pos;
+    const float reconLen = length(reconVec);
+    const V3 reconDir = normalize(reconVec);
+    if (!s.isSuffix && isinf(reconLen)) return emit_none();
+    if (any_bad(reconDir)) return emit_none();
+    s.curr = curr.pack();
+    s.state = SH_SHADOW;
+    return emit_shadow(curr.pos, reconDir, reconLen);
+}
+
+// Start of a shift: returns the first emission (none = finished with f = 0, J = 0).
+__device__ __forceinline__ Emit sh_start_prefix(const DScene& S, const RecordSet& R, ShState& s, uint32_t srcRef,
+                                                uint32_t dstRef) {
+    s.srcRef = srcRef;
+    s.isSuffix = 0;
+    s.idx = 0;
+    s.count = 0;
+    s.state = 0;
+    s.currTraced = 0;
+    s.thr = v3(1.0f);
+    s.curr = packed_invalid();
+    const Prefix src = load_prefix_ref(R, srcRef);
+    s.srng = src.rng;
+    if (!src.valid() || src.recon_idx() < 2) return emit_none();
+    if (!rehydrate(S, src.primary).valid) return emit_none();
+    const Inter curr = rehydrate(S, load_prefix_ref(R, dstRef).primary);
+    if (!curr.valid) return emit_none();
+    if (!rehydrate(S, src.recon).valid) return emit_none();
+    const int rvc = (int)(src.recon_idx() - 2u);
+    s.count = rvc > 0 ? (uint32_t)rvc : 0u;
+    return sh_continue(S, R, s, curr);
+}
+
+__device__ __forceinline__ Emit sh_start_suffix(const DScene& S, const RecordSet& R, ShState& s, uint32_t srcRef,
+                                                uint32_t dstRef) {
+    s.srcRef = srcRef;
+    s.isSuffix = 1;
+    s.idx = 0;
+    s.count = 0;
+    s.state = 0;
+    s.currTraced = (dstRef >> 28) == REF_PACKED_TRACED ? 1u : 0u;
+    s.thr = v3(1.0f);
+    s.curr = packed_invalid();
+    const Inter dst = load_suffix_dst(S, R, dstRef);
+    const Suffix src = load_suffix_ref(R, srcRef);
+    s.srng = src.rng;
+    if (!dst.valid) return emit_none();
+    if (!src.valid()) return emit_none();
+    if (!rehydrate(S, src.lastPrefix).valid) return emit_none();
+    if (!rehydrate(S, src.recon).valid) return emit_none();
+    const int rvc = (int)(src.recon_idx() - 1u);
+    s.count = rvc > 0 ? (uint32_t)rvc : 0u;
+    return sh_continue(S, R, s, dst);
+}
+
+__device__ __forceinline__ Emit sh_on_hit(const DScene& S, const RecordSet& R, ShState& s, const Inter& it) {
+    if (!it.valid) return emit_none();
+    s.currTraced = 1;
+    s.idx++;
+    return sh_continue(S, R, s, it);
+}
+
+// Final step: BSDFs at both ends of the reconnection and the Jacobian.  Writes f and J.
+__device__ __forceinline__ void sh_on_shadow(const DScene& S, const RecordSet& R, const ShState& s, bool occluded,
+                                             V3& f, float& J) {
+    f = v3(0.0f);
+    J = 0.0f;
+    if (occluded) return;
+    const Inter curr = inter_restore(S, s.curr, s.currTraced != 0);
+    V3 thr = s.thr;
+    if (!s.isSuffix) {
+        const Prefix src = load_prefix_ref(R, s.srcRef);
+        const Inter srcPrimary = rehydrate(S, src.primary);
+        Inter srcRecon = rehydrate(S, src.recon);
+        const V3 reconDir = normalize(srcRecon.pos - curr.pos);
+        const BrdfEval b1 = ggx_eval(S, curr, reconDir);
+        if (b1.pdf <= 0.0f) return;
+        thr = thr * b1.brdf;
+        srcRecon.dir = reconDir;
+        const BrdfEval b2 = ggx_eval(S, srcRecon, src.reconOutDir);
+        if (b2.pdf <= 0.0f) return;
+        thr = thr * b2.brdf;
+        J = recon_jacobian(srcPrimary.pos, curr.pos, srcRecon.pos, srcRecon.nrm);
+        const V3 ff = thr * src.postReconF;
+        f = V3{gmax(0.0f, ff.x), gmax(0.0f, ff.y), gmax(0.0f, ff.z)};
+        return;
+    }
+    const Suffix src = load_suffix_ref(R, s.srcRef);
+    const Inter srcLast = rehydrate(S, src.lastPrefix);
+    Inter srcRecon = rehydrate(S, src.recon);
+    const V3 reconDir = normalize(srcRecon.pos - curr.pos);
+    const BrdfEval b1 = ggx_eval(S, curr, reconDir);
+    if (b1.pdf <= 0.0f) return;
+    thr = thr * b1.brdf;
+    if (src.recon_idx() < src.length()) {
+        srcRecon.dir = reconDir;
+        if (is_zero(src.reconOutDir)) return;
+        const BrdfEval b2 = ggx_eval(S, srcRecon, src.reconOutDir);
+        if (b2.pdf <= 0.0f) return;
+        thr = thr * b2.brdf;
+    }
+    J = recon_jacobian(srcLast.pos, curr.pos, srcRecon.pos, srcRecon.nrm);
+    f = thr * src.postReconF;
+}
+
+}  // namespace cx

@@ -1,0 +1,1120 @@
+// Wavefront kernels of the frame loop.
+//
+// Per pass: an init/setup kernel creates one job per pixel (or per pixel x neighbour x shift
+// direction) and appends the first rays; then, until no job is left, the closest-hit and
+// shadow traversal kernels answer the queued rays and a step kernel advances every job to its
+// next ray or to its end; finally a resample kernel runs the reference's reservoir updates per
+// pixel in the reference's order.  The traversal kernels use a 16-entry LDS stack per thread
+// (deeper entries spill to private memory), so their occupancy is set by registers.
+#include "cx_internal.h"
+#include "cx_wave.h"
+
+namespace cx {
+
+static constexpr int WBLOCK = 128;
+static constexpr int SHORT_STACK = 16;
+
+__device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ---- traversal kernels over a queue ----
+
+__global__ __launch_bounds__(WBLOCK) void k_q_closest(DScene S, Queue q, uint32_t nc) {
+    __shared__ int s_st[SHORT_STACK * WBLOCK];
+    const uint32_t i = gid();
+    if (i >= nc) return;
+    const float4 a = q.ray[i], b = q.ray[(size_t)q.cap + i];
+    TravStack<SHORT_STACK> st;
+    st.lds = s_st + threadIdx.x;
+    st.stride = WBLOCK;
+    Hit h;
+    traverse_s<false>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
+    const bool hit = h.prim != INVALID_PRIM;
+    q.res[i] = uint4{h.prim, fb(hit ? h.t : 0.0f), fb(hit ? h.u : 0.0f), fb(hit ? h.v : 0.0f)};
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_q_shadow(DScene S, Queue q, uint32_t ns) {
+    __shared__ int s_st[SHORT_STACK * WBLOCK];
+    const uint32_t i = gid();
+    if (i >= ns) return;
+    const uint32_t slot = q.cap - ns + i;
+    const float4 a = q.ray[slot], b = q.ray[(size_t)q.cap + slot];
+    TravStack<SHORT_STACK> st;
+    st.lds = s_st + threadIdx.x;
+    st.stride = WBLOCK;
+    Hit h;
+    const bool occ = traverse_s<true>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
+    q.res[slot] = uint4{occ ? 1u : 0u, 0u, 0u, 0u};
+}
+
+// ---- shared state of the wavefront kernels ----
+
+struct WaveArgs {
+    FrameArgs A;
+    RecordSet R;
+    Queue qin, qout;
+    uint32_t nc, ns;
+    uint4* prefixScratch;   // [PREFIX_CHUNKS][P]
+    uint4* suffixScratch;   // [SUFFIX_CHUNKS][P]
+    uint4* packedScratch;   // [2 * P]
+    uint64_t* rngScratch;   // [P]
+    float4* shiftRes;       // [jobs] f.xyz, J
+    uint32_t* pixFlags;     // [P]
+    uint32_t* nbr;          // [P * S] neighbour pixel of each spatial slot
+    float4* radScratch;     // [P] radiance accumulated over the gather prefixes
+    float4* fgScratch;      // [P] gather-prefix throughput xyz, p
+    uint32_t S;             // spatial neighbour slots of the pass
+    uint32_t gIdx;          // gather prefix index (0 = canonical)
+};
+
+__device__ __forceinline__ void emit_job(const Queue& q, uint32_t slot, uint32_t job, const Emit& e) {
+    q.job[slot] = job;
+    put_ray(q, slot, e);
+}
+
+__device__ __forceinline__ uint64_t gbuf_rng(const uint4& g) { return (uint64_t)g.x | ((uint64_t)g.y << 32); }
+
+__device__ __forceinline__ void tile_xy(const FrameArgs& A, uint32_t p, uint32_t& x, uint32_t& y) {
+    x = p % A.W;
+    y = p / A.W;
+}
+
+// ======================================================================================
+// Pass A: prefix_gen_temp_reuse (shader/prefix_gen_temp_reuse.cu)
+// ======================================================================================
+
+__device__ __forceinline__ void pg_finish(const WaveArgs& W, uint32_t p, const PgState& s) {
+    PrefixRes r;
+    r.s = s.pre;
+    r.wSum = 0.0f;
+    r.conf = 0.0f;
+    store_prefix(W.prefixScratch, W.A.P, p, r);
+    W.rngScratch[p] = s.rng;
+}
+
+__device__ __forceinline__ void cp_finish(const WaveArgs& W, uint32_t p, const CpState& s) {
+    V3 rad = s.radiance;
+    if (any_bad(rad)) rad = v3(0.0f);
+    float* o = W.A.out + 3 * (size_t)p;
+    V3 r = rad;
+    if (W.A.lp.enableAccum) {
+        const V3 old = V3{o[0], o[1], o[2]};
+        const float blend = 1.0f / (float)(W.A.lp.frameIdx + 1);
+        r = blend * rad + (1.0f - blend) * old;
+    }
+    o[0] = r.x;
+    o[1] = r.y;
+    o[2] = r.z;
+}
+
+// Camera ray, then the first ray of TracePrefix (ReSTIR modes) or TraceCompletePath.
+__global__ __launch_bounds__(WBLOCK) void k_w_prefix_init(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    const bool active = p < A.P;
+    Emit e = emit_none();
+    PgState pg;
+    CpState cp;
+    const bool restir = A.lp.rendererType != CX_PATH_TRACER;
+    if (active) {
+        uint32_t x, y;
+        tile_xy(A, p, x, y);
+        Rng rng = rng_seed(tea64(p, A.lp.random));
+        const float jx = rng.f();
+        const float jy = rng.f();
+        float ux = ((float)x + jx) / (float)A.W;
+        float uy = ((float)y + jy) / (float)A.H;
+        ux = 2.0f * ux - 1.0f;
+        uy = 2.0f * uy - 1.0f;
+        const V3 origin = A.camPos;
+        const V3 dir = normalize(A.camU * ux + A.camV * uy + A.camW);
+        if (restir) {
+            pg.rng = rng.s;
+            pg.pre.primary = packed_invalid();
+            pg.pre.recon = packed_invalid();
+            pg.pre.last = packed_invalid();
+            pg.pre.reconOutDir = v3(0.0f);
+            pg.lightDir = v3(0.0f);
+            pg.dist = 0.0f;
+            pg.color = v3(0.0f);
+            e = pg_start(A.lp, pg, origin, dir);
+            if (e.kind == EMIT_NONE) pg_finish(W, p, pg);
+        } else {
+            cp.rng = rng.s;
+            cp.traceIdx = 0;
+            cp.depth = 0;
+            cp.state = CP_TRACE;
+            cp.cur = origin;
+            cp.dir = dir;
+            cp.thr = v3(1.0f);
+            cp.radiance = v3(0.0f);
+            cp.it = packed_invalid();
+            cp.lightDir = v3(0.0f);
+            cp.dist = 0.0f;
+            cp.color = v3(0.0f);
+            if (A.lp.maxPathLen > 0) e = emit_closest(origin, dir);
+            else cp_finish(W, p, cp);
+        }
+    }
+    const uint32_t slot = append(W.qout, e.kind);
+    if (active && e.kind != EMIT_NONE) {
+        if (restir) store_ctx(W.qout, slot, pg_pack(pg));
+        else store_ctx(W.qout, slot, cp_pack(cp));
+        emit_job(W.qout, slot, p, e);
+    }
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_pg_step(WaveArgs W) {
+    const uint32_t i = gid();
+    const bool active = i < W.nc + W.ns;
+    Emit e = emit_none();
+    PgState s;
+    uint32_t job = 0;
+    if (active) {
+        const uint32_t slot = in_slot(W.qin, i, W.nc, W.ns);
+        job = W.qin.job[slot];
+        s = pg_unpack(load_ctx<PG_CHUNKS>(W.qin, slot));
+        const uint4 r = W.qin.res[slot];
+        if (s.state == PG_TRACE) {
+            V3 o, d;
+            get_ray(W.qin, slot, o, d);
+            e = pg_on_hit(W.A.S, W.A.lp, s, inter_from_hit(W.A.S, r, d));
+        } else {
+            e = pg_on_shadow(W.A.S, W.A.lp, s, r.x != 0);
+        }
+        if (e.kind == EMIT_NONE) pg_finish(W, job, s);
+    }
+    const uint32_t slot = append(W.qout, e.kind);
+    if (active && e.kind != EMIT_NONE) {
+        store_ctx(W.qout, slot, pg_pack(s));
+        emit_job(W.qout, slot, job, e);
+    }
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_cp_step(WaveArgs W) {
+    const uint32_t i = gid();
+    const bool active = i < W.nc + W.ns;
+    Emit e = emit_none();
+    CpState s;
+    uint32_t job = 0;
+    if (active) {
+        const uint32_t slot = in_slot(W.qin, i, W.nc, W.ns);
+        job = W.qin.job[slot];
+        s = cp_unpack(load_ctx<CP_CHUNKS>(W.qin, slot));
+        const uint4 r = W.qin.res[slot];
+        if (s.state == CP_TRACE) {
+            V3 o, d;
+            get_ray(W.qin, slot, o, d);
+            e = cp_on_hit(W.A.S, W.A.lp, s, inter_from_hit(W.A.S, r, d));
+        } else {
+            e = cp_on_shadow(W.A.S, s, r.x != 0);
+        }
+        if (e.kind == EMIT_NONE) cp_finish(W, job, s);
+    }
+    const uint32_t slot = append(W.qout, e.kind);
+    if (active && e.kind != EMIT_NONE) {
+        store_ctx(W.qout, slot, cp_pack(s));
+        emit_job(W.qout, slot, job, e);
+    }
+}
+
+// Temporal decision of PrefixGenTempReuse (prefix_gen_temp_reuse.cu:31-54).
+__device__ __forceinline__ bool prefix_temporal(const WaveArgs& W, uint32_t p, const PrefixRes& prev) {
+    const cx_launch_params& lp = W.A.lp;
+    if (!lp.prefixEnableTemporal) return false;
+    const bool skipNee = is_cond(lp) && prev.s.nee();
+    return !(prev.wSum <= 0.0f || !prev.s.valid() || skipNee);
+}
+
+__device__ __forceinline__ void shift_finish(const WaveArgs& W, uint32_t job, V3 f, float J) {
+    W.shiftRes[job] = float4{f.x, f.y, f.z, J};
+}
+
+// Both temporal shifts of a pixel (prefix_gen_temp_reuse.cu:58-61).
+__global__ __launch_bounds__(WBLOCK) void k_w_prefix_temporal_setup(WaveArgs W) {
+    const uint32_t p = gid();
+    const bool px = p < W.A.P;
+    bool temporal = false;
+    if (px) {
+        const uint32_t cf = load_prefix_flags(W.prefixScratch, W.A.P, p);
+        if ((cf >> 16) & 1u) temporal = prefix_temporal(W, p, load_prefix(W.A.prefixRes[W.A.back], W.A.P, p));
+        W.pixFlags[p] = temporal ? 1u : 0u;
+    }
+    for (uint32_t d = 0; d < 2; d++) {
+        Emit e = emit_none();
+        ShState s;
+        const uint32_t job = 2 * p + d;
+        if (temporal) {
+            const uint32_t back = make_ref(REF_PREFIX_BACK, p), canon = make_ref(REF_PREFIX_SCRATCH, p);
+            e = d == 0 ? sh_start_prefix(W.A.S, W.R, s, back, canon) : sh_start_prefix(W.A.S, W.R, s, canon, back);
+            if (e.kind == EMIT_NONE) shift_finish(W, job, v3(0.0f), 0.0f);
+        }
+        const uint32_t slot = append(W.qout, e.kind);
+        if (temporal && e.kind != EMIT_NONE) {
+            store_ctx(W.qout, slot, sh_pack(s));
+            emit_job(W.qout, slot, job, e);
+        }
+    }
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_sh_step(WaveArgs W) {
+    const uint32_t i = gid();
+    const bool active = i < W.nc + W.ns;
+    Emit e = emit_none();
+    ShState s;
+    uint32_t job = 0;
+    if (active) {
+        const uint32_t slot = in_slot(W.qin, i, W.nc, W.ns);
+        job = W.qin.job[slot];
+        s = sh_unpack(load_ctx<SH_CHUNKS>(W.qin, slot));
+        const uint4 r = W.qin.res[slot];
+        if (s.state == SH_TRACE) {
+            V3 o, d;
+            get_ray(W.qin, slot, o, d);
+            e = sh_on_hit(W.A.S, W.R, s, inter_from_hit(W.A.S, r, d));
+            if (e.kind == EMIT_NONE) shift_finish(W, job, v3(0.0f), 0.0f);
+        } else {
+            V3 f;
+            float J;
+            sh_on_shadow(W.A.S, W.R, s, r.x != 0, f, J);
+            shift_finish(W, job, f, J);
+        }
+    }
+    const uint32_t slot = append(W.qout, e.kind);
+    if (active && e.kind != EMIT_NONE) {
+        store_ctx(W.qout, slot, sh_pack(s));
+        emit_job(W.qout, slot, job, e);
+    }
+}
+
+__device__ __forceinline__ void write_out(const FrameArgs& A, uint32_t p, V3 rad) {
+    float* o = A.out + 3 * (size_t)p;
+    V3 r = rad;
+    if (A.lp.enableAccum) {
+        const V3 old = V3{o[0], o[1], o[2]};
+        const float blend = 1.0f / (float)(A.lp.frameIdx + 1);
+        r = blend * rad + (1.0f - blend) * old;
+    }
+    o[0] = r.x;
+    o[1] = r.y;
+    o[2] = r.z;
+}
+
+// Reservoir updates of PrefixGenTempReuse and the raygen epilogue (prefix_gen_temp_reuse.cu:24-172).
+__global__ __launch_bounds__(WBLOCK) void k_w_prefix_resample(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    if (p >= A.P) return;
+    const cx_launch_params& lp = A.lp;
+    const PrefixRes cres = load_prefix(W.prefixScratch, A.P, p);
+    const Prefix& canon = cres.s;
+    Rng rng;
+    rng.s = W.rngScratch[p];
+    const uint32_t oldFlags = A.gbuf[p].w;
+    uint32_t flags = oldFlags & CANON_SUFFIX_VALID;
+    if (canon.valid()) flags |= CANON_PREFIX_VALID;
+    if ((oldFlags & CANON_PRIMARY_VALID) || canon.length() >= 1) flags |= CANON_PRIMARY_VALID;
+    uint4* front = A.prefixRes[A.front];
+    uint4* back = A.prefixRes[A.back];
+    bool validPrimary = false;
+    V3 rad = v3(0.0f);
+    if (canon.valid()) {
+        const float canonPHat = luminance(canon.f);
+        PrefixRes curr = load_prefix(front, A.P, p);
+        curr.wSum = 0.0f;
+        curr.conf = 0.0f;
+        if (!W.pixFlags[p]) {
+            res_update(curr, canon, canonPHat / canon.p, rng, 1.0f);
+            res_finalize(curr);
+        } else {
+            const PrefixRes prev = load_prefix(back, A.P, p);
+            const float4 a = W.shiftRes[2 * p], b = W.shiftRes[2 * p + 1];
+            const V3 fFromCanonOfPrev = V3{a.x, a.y, a.z};
+            const float jPrevToCanon = a.w;
+            const V3 fFromPrevOfCanon = V3{b.x, b.y, b.z};
+            const float jCanonToPrev = b.w;
+            const float pFromCanonOfPrev = luminance(fFromCanonOfPrev) * jPrevToCanon;
+            const float K = 1.0f;
+            const float prevMis = neighbor_mis(fFromCanonOfPrev, prev.s.f, jPrevToCanon, K, 1.0f, prev.conf);
+            const float canonMis = 1.0f + canonical_mis(canon.f, fFromPrevOfCanon, jCanonToPrev, K, 1.0f, prev.conf);
+            const float canonRis = canonMis * canonPHat / canon.p;
+            res_update(curr, canon, canonRis, rng, 1.0f);
+            const float prevRis = prevMis * pFromCanonOfPrev * prev.wSum;
+            Prefix shifted = prev.s;
+            shifted.f = fFromCanonOfPrev;
+            shifted.primary = canon.primary;
+            res_update(curr, shifted, prevRis, rng, prev.conf);
+            if (curr.wSum > 0.0f) {
+                curr.wSum /= K + 1.0f;
+                res_finalize(curr);
+            }
+        }
+        store_prefix(front, A.P, p, curr);
+        if (A.copyPrefixToBack) store_prefix(back, A.P, p, curr);
+        validPrimary = true;
+        rad = curr.wSum * curr.s.f;
+    } else if (A.copyPrefixToBack) {
+        store_prefix(back, A.P, p, load_prefix(front, A.P, p));
+    }
+    if (is_pt(lp) && !lp.prefixEnableSpatial) write_out(A, p, validPrimary ? rad : v3(0.0f));
+    A.gbuf[p] = uint4{(uint32_t)rng.s, (uint32_t)(rng.s >> 32), validPrimary ? 1u : 0u, flags};
+}
+
+// ======================================================================================
+// Pass B / D: spatial reuse (prefix_spatial_reuse.cu, suffix_spatial_reuse.cu)
+// ======================================================================================
+
+// Neighbour selection and validity loop; one shift job pair per valid neighbour.
+template <bool SUFFIX>
+__global__ __launch_bounds__(WBLOCK) void k_w_spatial_setup(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const cx_launch_params& lp = A.lp;
+    const uint32_t p = gid();
+    const bool px = p < A.P;
+    bool ok = false;
+    uint32_t validFlags = 0;
+    const uint32_t S = W.S;
+    if (px) {
+        const uint4 g = A.gbuf[p];
+        if (!SUFFIX) {
+            ok = ((load_prefix_flags(A.prefixRes[A.front], A.P, p) >> 16) & 1u) && (g.w & CANON_PREFIX_VALID) &&
+                 (g.w & CANON_PRIMARY_VALID);
+        } else {
+            const uint32_t pf = load_prefix_flags(A.prefixRes[A.front], A.P, p);
+            ok = ((pf >> 16) & 1u) && !((pf >> 17) & 1u) &&
+                 ((load_suffix_flags(A.suffixRes[A.front], A.P, p) >> 16) & 1u) && (g.w & CANON_SUFFIX_VALID);
+        }
+        if (ok) {
+            Rng rng;
+            rng.s = gbuf_rng(g);
+            uint32_t x, y;
+            tile_xy(A, p, x, y);
+            for (uint32_t i = 0; i < S; ++i) {
+                uint32_t nx, ny;
+                select_neighbor(x, y, lp.windowRadius, rng, nx, ny);
+                uint32_t np = INVALID_PRIM;
+                if (nx < A.W && ny < A.H) {
+                    const uint32_t q = ny * A.W + nx;
+                    if (A.gbuf[q].z) {
+                        if (!SUFFIX) {
+                            const uint32_t nf = load_prefix_flags(A.prefixRes[A.back], A.P, q);
+                            if (((nf >> 16) & 1u) && !(is_cond(lp) && ((nf >> 17) & 1u))) np = q;
+                        } else {
+                            if ((load_suffix_flags(A.suffixRes[A.back], A.P, q) >> 16) & 1u) np = q;
+                        }
+                    }
+                }
+                if (np != INVALID_PRIM) validFlags |= 1u << i;
+                W.nbr[(size_t)p * S + i] = np;
+            }
+            W.rngScratch[p] = rng.s;
+        }
+        W.pixFlags[p] = ok ? (1u | (validFlags << 1)) : 0u;
+    }
+    for (uint32_t i = 0; i < S; ++i) {
+        for (uint32_t d = 0; d < 2; d++) {
+            Emit e = emit_none();
+            ShState s;
+            const uint32_t job = ((uint32_t)p * S + i) * 2 + d;
+            const bool want = ok && ((validFlags >> i) & 1u);
+            if (want) {
+                const uint32_t np = W.nbr[(size_t)p * S + i];
+                if (!SUFFIX) {
+                    const uint32_t neigh = make_ref(REF_PREFIX_BACK, np), curr = make_ref(REF_PREFIX_FRONT, p);
+                    e = d == 0 ? sh_start_prefix(A.S, W.R, s, neigh, curr) : sh_start_prefix(A.S, W.R, s, curr, neigh);
+                } else {
+                    const uint32_t neigh = make_ref(REF_SUFFIX_BACK, np), curr = make_ref(REF_SUFFIX_FRONT, p);
+                    e = d == 0 ? sh_start_suffix(A.S, W.R, s, neigh, curr) : sh_start_suffix(A.S, W.R, s, curr, neigh);
+                }
+                if (e.kind == EMIT_NONE) shift_finish(W, job, v3(0.0f), 0.0f);
+            }
+            const uint32_t slot = append(W.qout, e.kind);
+            if (want && e.kind != EMIT_NONE) {
+                store_ctx(W.qout, slot, sh_pack(s));
+                emit_job(W.qout, slot, job, e);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_prefix_spatial_resample(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const cx_launch_params& lp = A.lp;
+    const uint32_t p = gid();
+    if (p >= A.P) return;
+    const uint32_t pf = W.pixFlags[p];
+    V3 rad = v3(0.0f);
+    bool ok = false;
+    if (pf & 1u) {
+        uint4* front = A.prefixRes[A.front];
+        const uint4* back = A.prefixRes[A.back];
+        PrefixRes curr = load_prefix(front, A.P, p);
+        const Prefix currPrefix = curr.s;
+        Rng rng;
+        rng.s = W.rngScratch[p];
+        const float currWSum = curr.wSum;
+        const float currConf = curr.conf;
+        curr.wSum = 0.0f;
+        curr.conf = 0.0f;
+        const uint32_t validFlags = pf >> 1;
+        const float K = (float)__popc(validFlags);
+        float canonMis = 1.0f;
+        for (uint32_t i = 0; i < W.S; ++i) {
+            if (!((validFlags >> i) & 1u)) continue;
+            const uint32_t np = W.nbr[(size_t)p * W.S + i];
+            const PrefixRes nres = load_prefix(back, A.P, np);
+            const float4 a = W.shiftRes[((size_t)p * W.S + i) * 2], b = W.shiftRes[((size_t)p * W.S + i) * 2 + 1];
+            const V3 fFromCanonOfNeigh = V3{a.x, a.y, a.z};
+            const float jNC = a.w;
+            const V3 fFromNeighOfCanon = V3{b.x, b.y, b.z};
+            const float jCN = b.w;
+            const float pFromCanonOfNeigh = luminance(fFromCanonOfNeigh) * jNC;
+            float nMis = neighbor_mis(fFromCanonOfNeigh, nres.s.f, jNC, K, currConf, nres.conf);
+            if (bad(nMis)) nMis = 0.0f;
+            const float nRis = nMis * pFromCanonOfNeigh * nres.wSum;
+            Prefix shifted = nres.s;
+            shifted.f = fFromCanonOfNeigh;
+            shifted.primary = currPrefix.primary;
+            res_update(curr, shifted, nRis, rng, nres.conf);
+            canonMis += canonical_mis(currPrefix.f, fFromNeighOfCanon, jCN, K, currConf, nres.conf);
+        }
+        if (bad(canonMis)) canonMis = 0.0f;
+        const float canonRis = canonMis * currWSum * luminance(currPrefix.f);
+        res_update(curr, currPrefix, canonRis, rng, currConf);
+        if (curr.wSum > 0.0f) {
+            curr.wSum /= K + 1.0f;
+            res_finalize(curr);
+        }
+        store_prefix(front, A.P, p, curr);
+        const uint4 g = A.gbuf[p];
+        A.gbuf[p] = uint4{(uint32_t)rng.s, (uint32_t)(rng.s >> 32), g.z, g.w};
+        ok = true;
+        rad = curr.wSum * curr.s.f;
+    }
+    if (is_pt(lp) && lp.spatialRoundIdx == lp.prefixSpatialRounds - 1) write_out(A, p, ok ? rad : v3(0.0f));
+}
+
+__device__ __forceinline__ void invalidate_res(SuffixRes& r) {   // suffix_gen_temp_reuse.cu:12-17
+    r.wSum = 0.0f;
+    r.conf = 0.0f;
+    r.s.set_valid(false);
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_suffix_spatial_resample(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    if (p >= A.P) return;
+    const uint32_t pf = W.pixFlags[p];
+    uint4* front = A.suffixRes[A.front];
+    const uint4* back = A.suffixRes[A.back];
+    SuffixRes curr = load_suffix(front, A.P, p);
+    if (!(pf & 1u)) {
+        invalidate_res(curr);
+        store_suffix(front, A.P, p, curr);
+        return;
+    }
+    const Suffix currSuffix = curr.s;
+    Rng rng;
+    rng.s = W.rngScratch[p];
+    const float currWSum = curr.wSum;
+    const float currConf = curr.conf;
+    curr.wSum = 0.0f;
+    curr.conf = 0.0f;
+    const uint32_t validFlags = pf >> 1;
+    const float K = (float)__popc(validFlags);
+    float canonMis = 1.0f;
+    for (uint32_t i = 0; i < W.S; ++i) {
+        if (!((validFlags >> i) & 1u)) continue;
+        const uint32_t np = W.nbr[(size_t)p * W.S + i];
+        const SuffixRes nres = load_suffix(back, A.P, np);
+        const float4 a = W.shiftRes[((size_t)p * W.S + i) * 2], b = W.shiftRes[((size_t)p * W.S + i) * 2 + 1];
+        const V3 fFromCanonOfNeigh = V3{a.x, a.y, a.z};
+        const float jNC = a.w;
+        const V3 fFromNeighOfCanon = V3{b.x, b.y, b.z};
+        const float jCN = b.w;
+        const float pFromCanonOfNeigh = luminance(fFromCanonOfNeigh) * jNC;
+        float nMis = neighbor_mis(fFromCanonOfNeigh, nres.s.f, jNC, K, currConf, nres.conf);
+        if (bad(nMis)) nMis = 0.0f;
+        const float nRis = nMis * pFromCanonOfNeigh * nres.wSum;
+        Suffix shifted = nres.s;
+        shifted.lastPrefix = currSuffix.lastPrefix;
+        shifted.f = fFromCanonOfNeigh;
+        res_update(curr, shifted, nRis, rng, nres.conf);
+        canonMis += canonical_mis(currSuffix.f, fFromNeighOfCanon, jCN, K, currConf, nres.conf);
+    }
+    if (bad(canonMis)) canonMis = 0.0f;
+    const float canonRis = canonMis * currWSum * luminance(currSuffix.f);
+    res_update(curr, currSuffix, canonRis, rng, currConf);
+    if (curr.wSum > 0.0f) {
+        curr.wSum /= K + 1.0f;
+        res_finalize(curr);
+    }
+    store_suffix(front, A.P, p, curr);
+    const uint4 g = A.gbuf[p];
+    A.gbuf[p] = uint4{(uint32_t)rng.s, (uint32_t)(rng.s >> 32), g.z, g.w};
+}
+
+// ======================================================================================
+// Pass C: suffix_gen_temp_reuse (shader/suffix_gen_temp_reuse.cu)
+// ======================================================================================
+
+__device__ __forceinline__ void sg_finish_store(const WaveArgs& W, uint32_t p, const SgState& s) {
+    SuffixRes r;
+    r.s = s.suf;
+    r.wSum = 0.0f;
+    r.conf = 0.0f;
+    store_suffix(W.suffixScratch, W.A.P, p, r);
+    W.rngScratch[p] = s.rng;
+}
+
+__device__ __forceinline__ void sg_init_state(SgState& s, uint64_t rng) {
+    s.rng = rng;
+    s.suf.recon = packed_invalid();
+    s.suf.reconOutDir = v3(0.0f);
+    s.it = packed_invalid();
+    s.lightDir = v3(0.0f);
+    s.lightLen = 0.0f;
+    s.color = v3(0.0f);
+    s.brdf = v3(0.0f);
+    s.maxLen = 0;
+}
+
+// TraceSuffix from the front prefix of every pixel whose prefix is valid and not NEE-terminated.
+// gather != 0: from the canonical gather prefix in the prefix scratch instead.
+__global__ __launch_bounds__(WBLOCK) void k_w_suffix_init(WaveArgs W, int gather) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    const bool px = p < A.P;
+    Emit e = emit_none();
+    SgState s;
+    bool run = false;
+    if (px) {
+        if (!gather) {
+            const uint32_t pf = load_prefix_flags(A.prefixRes[A.front], A.P, p);
+            run = ((pf >> 16) & 1u) && !((pf >> 17) & 1u);
+            W.pixFlags[p] = run ? 1u : 0u;
+            if (run) {
+                sg_init_state(s, gbuf_rng(A.gbuf[p]));
+                e = sg_start(A.S, A.lp, s, load_prefix(A.prefixRes[A.front], A.P, p).s);
+            }
+        } else {
+            const uint32_t pf = load_prefix_flags(W.prefixScratch, A.P, p);
+            run = (W.pixFlags[p] & 1u) && ((pf >> 16) & 1u) && !((pf >> 17) & 1u);
+            if (run) {
+                sg_init_state(s, W.rngScratch[p]);
+                e = sg_start(A.S, A.lp, s, load_prefix(W.prefixScratch, A.P, p).s);
+            }
+        }
+        if (run && e.kind == EMIT_NONE) sg_finish_store(W, p, s);
+    }
+    const uint32_t slot = append(W.qout, e.kind);
+    if (run && e.kind != EMIT_NONE) {
+        store_ctx(W.qout, slot, sg_pack(s));
+        emit_job(W.qout, slot, p, e);
+    }
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_sg_step(WaveArgs W) {
+    const uint32_t i = gid();
+    const bool active = i < W.nc + W.ns;
+    Emit e = emit_none();
+    SgState s;
+    uint32_t job = 0;
+    if (active) {
+        const uint32_t slot = in_slot(W.qin, i, W.nc, W.ns);
+        job = W.qin.job[slot];
+        s = sg_unpack(load_ctx<SG_CHUNKS>(W.qin, slot));
+        const uint4 r = W.qin.res[slot];
+        V3 o, d;
+        get_ray(W.qin, slot, o, d);
+        if (s.state == SG_TRACE) e = sg_on_hit(W.A.S, W.A.lp, s, inter_from_hit(W.A.S, r, d), o);
+        else if (s.state == SG_NEE) e = sg_on_shadow(W.A.S, s, r.x != 0);
+        else e = sg_on_recon(s, inter_from_hit(W.A.S, r, d));
+        if (e.kind == EMIT_NONE) sg_finish_store(W, job, s);
+    }
+    const uint32_t slot = append(W.qout, e.kind);
+    if (active && e.kind != EMIT_NONE) {
+        store_ctx(W.qout, slot, sg_pack(s));
+        emit_job(W.qout, slot, job, e);
+    }
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_suffix_temporal_setup(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    const bool px = p < A.P;
+    bool temporal = false;
+    if (px) {
+        const bool ran = W.pixFlags[p] & 1u;
+        if (ran && ((load_suffix_flags(W.suffixScratch, A.P, p) >> 16) & 1u) && A.lp.suffixEnableTemporal) {
+            const SuffixRes prev = load_suffix(A.suffixRes[A.back], A.P, p);
+            temporal = !(prev.wSum <= 0.0f || !prev.s.valid());
+        }
+        W.pixFlags[p] = (ran ? 1u : 0u) | (temporal ? 2u : 0u);
+    }
+    for (uint32_t d = 0; d < 2; d++) {
+        Emit e = emit_none();
+        ShState s;
+        const uint32_t job = 2 * p + d;
+        if (temporal) {
+            const uint32_t back = make_ref(REF_SUFFIX_BACK, p), canon = make_ref(REF_SUFFIX_SCRATCH, p);
+            e = d == 0 ? sh_start_suffix(A.S, W.R, s, back, canon) : sh_start_suffix(A.S, W.R, s, canon, back);
+            if (e.kind == EMIT_NONE) shift_finish(W, job, v3(0.0f), 0.0f);
+        }
+        const uint32_t slot = append(W.qout, e.kind);
+        if (temporal && e.kind != EMIT_NONE) {
+            store_ctx(W.qout, slot, sh_pack(s));
+            emit_job(W.qout, slot, job, e);
+        }
+    }
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_suffix_resample(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    if (p >= A.P) return;
+    const uint4 g = A.gbuf[p];
+    uint32_t flags = g.w;
+    Rng rng;
+    rng.s = gbuf_rng(g);
+    uint4* front = A.suffixRes[A.front];
+    uint4* back = A.suffixRes[A.back];
+    SuffixRes curr = load_suffix(front, A.P, p);
+    const uint32_t pf = W.pixFlags[p];
+    if (!(pf & 1u)) {
+        invalidate_res(curr);
+    } else {
+        const SuffixRes cres = load_suffix(W.suffixScratch, A.P, p);
+        const Suffix& canon = cres.s;
+        rng.s = W.rngScratch[p];
+        flags = (flags & ~CANON_SUFFIX_VALID) | (canon.valid() ? CANON_SUFFIX_VALID : 0u);
+        if (!canon.valid()) {
+            invalidate_res(curr);
+        } else {
+            const float canonPHat = luminance(canon.f);
+            curr.wSum = 0.0f;
+            curr.conf = 0.0f;
+            if (!(pf & 2u)) {
+                res_update(curr, canon, canonPHat / canon.p, rng, 1.0f);
+                res_finalize(curr);
+            } else {
+                const SuffixRes prev = load_suffix(back, A.P, p);
+                const float4 a = W.shiftRes[2 * p], b = W.shiftRes[2 * p + 1];
+                const V3 fFromCanonOfPrev = V3{a.x, a.y, a.z};
+                const float jPrevToCanon = a.w;
+                const V3 fFromPrevOfCanon = V3{b.x, b.y, b.z};
+                const float jCanonToPrev = b.w;
+                const float pFromCanonOfPrev = luminance(fFromCanonOfPrev) * jPrevToCanon;
+                const float K = 1.0f;
+                const float prevMis = neighbor_mis(fFromCanonOfPrev, prev.s.f, jPrevToCanon, K, 1.0f, prev.conf);
+                const float canonMis =
+                    1.0f + canonical_mis(canon.f, fFromPrevOfCanon, jCanonToPrev, K, 1.0f, prev.conf);
+                const float canonRis = canonMis * canonPHat / canon.p;
+                res_update(curr, canon, canonRis, rng, 1.0f);
+                const float prevRis = prevMis * pFromCanonOfPrev * prev.wSum;
+                Suffix shifted = prev.s;
+                shifted.lastPrefix = canon.lastPrefix;
+                shifted.f = fFromCanonOfPrev;
+                res_update(curr, shifted, prevRis, rng, prev.conf);
+                if (curr.wSum > 0.0f) {
+                    curr.wSum /= K + 1.0f;
+                    res_finalize(curr);
+                }
+            }
+        }
+    }
+    store_suffix(front, A.P, p, curr);
+    store_suffix(back, A.P, p, curr);
+    A.gbuf[p] = uint4{(uint32_t)rng.s, (uint32_t)(rng.s >> 32), g.z, flags};
+}
+
+// ======================================================================================
+// Pass F: final_gather (shader/final_gather.cu)
+// ======================================================================================
+
+// GetRadiance prologue (final_gather.cu:290-308) and the first ray of the canonical prefix.
+__global__ __launch_bounds__(WBLOCK) void k_w_gather_init(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    const bool px = p < A.P;
+    Emit e = emit_none();
+    PgState pg;
+    bool run = false;
+    if (px) {
+        const uint4 g = A.gbuf[p];
+        run = (g.w & CANON_PRIMARY_VALID) != 0;
+        W.pixFlags[p] = run ? 1u : 0u;
+        W.radScratch[p] = float4{0.0f, 0.0f, 0.0f, 0.0f};
+        if (run) {
+            uint32_t x, y;
+            tile_xy(A, p, x, y);
+            Rng rng;
+            rng.s = gbuf_rng(g);
+            const float jx = rng.f();
+            const float jy = rng.f();
+            float ux = ((float)x + jx) / (float)A.W;
+            float uy = ((float)y + jy) / (float)A.H;
+            ux = 2.0f * ux - 1.0f;
+            uy = 2.0f * uy - 1.0f;
+            const V3 dir = normalize(A.camU * ux + A.camV * uy + A.camW);
+            W.fgScratch[p] = float4{dir.x, dir.y, dir.z, 0.0f};
+            pg.rng = rng.s;
+            pg.pre.primary = packed_invalid();
+            pg.pre.recon = packed_invalid();
+            pg.pre.last = packed_invalid();
+            pg.pre.reconOutDir = v3(0.0f);
+            pg.lightDir = v3(0.0f);
+            pg.dist = 0.0f;
+            pg.color = v3(0.0f);
+            e = pg_start(A.lp, pg, A.camPos, dir);
+            if (e.kind == EMIT_NONE) pg_finish(W, p, pg);
+        }
+    }
+    const uint32_t slot = append(W.qout, e.kind);
+    if (run && e.kind != EMIT_NONE) {
+        store_ctx(W.qout, slot, pg_pack(pg));
+        emit_job(W.qout, slot, p, e);
+    }
+}
+
+template <int KR>
+__device__ __forceinline__ uint32_t query_neighbors(const WaveArgs& W, uint32_t p, V3 q, int* stack, int stride) {
+    const uint32_t k = W.A.k;
+    uint32_t* nIdx = W.A.neighIdx + (size_t)p * k;
+    float* nDist = W.A.neighDist + (size_t)p * k;
+    TopK<KR> top;
+    top.init();
+    const EntryTree& T = W.A.tree;
+    const uint32_t n = *T.numValid;
+    uint32_t count = 0;
+    if (!(k == 0 || n == 0 || isnan(q.x) || isnan(q.y) || isnan(q.z))) {
+        const uint32_t leafBase = (1u << T.levels) - 1u;
+        int sp = 0;
+        uint32_t node = 0;
+        while (true) {
+            const float4 lo = T.lo[node], hi = T.hi[node];
+            const bool inside = q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y && q.z >= lo.z && q.z <= hi.z;
+            if (inside) {
+                if (node >= leafBase) {
+                    const uint32_t j = node - leafBase;
+                    const uint32_t b = j * T.leafSize, e = min(b + T.leafSize, n);
+                    for (uint32_t s = b; s < e; s++) {
+                        const float4 ep = T.entrySorted[s];
+                        const float dd = distance(q, V3{ep.x, ep.y, ep.z});
+                        if (dd > ep.w) continue;
+                        top.insert(dd, T.entryPixel[s], k);
+                    }
+                } else {
+                    stack[sp * stride] = (int)(2 * node + 2);
+                    ++sp;
+                    node = 2 * node + 1;
+                    continue;
+                }
+            }
+            if (sp == 0) break;
+            --sp;
+            node = (uint32_t)stack[sp * stride];
+        }
+        count = top.count;
+#pragma unroll
+        for (int j = 0; j < KR; j++) {
+            if ((uint32_t)j < top.count) {
+                nIdx[j] = top.i[j];
+                nDist[j] = top.d[j];
+            }
+        }
+    }
+    W.A.neighCount[p] = count;
+    if (W.A.lp.trackPrefixStats) {
+        atomicAdd(&W.A.counters[STAT_TOTAL], (unsigned long long)count);
+        atomicMin(&W.A.counters[STAT_MIN], (unsigned long long)count);
+        atomicMax(&W.A.counters[STAT_MAX], (unsigned long long)count);
+    }
+    return count;
+}
+
+// After the canonical prefix and suffix: neighbour query and the two shifts per valid
+// neighbour (final_gather.cu:90-177).  gIdx > 0: a gather prefix of FinalGatherNotCanon
+// (final_gather.cu:200-279), one shift per neighbour onto its traced last vertex.
+template <int KR>
+__global__ __launch_bounds__(WBLOCK) void k_w_gather_setup(WaveArgs W) {
+    __shared__ int s_st[32 * WBLOCK];
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    const bool px = p < A.P;
+    const uint32_t k = A.k;
+    bool run = false;
+    uint32_t validFlags = 0, count = 0;
+    if (px && (W.pixFlags[p] & 1u)) {
+        V3 q = v3(0.0f);
+        if (W.gIdx == 0) {
+            const uint32_t pf = load_prefix_flags(W.prefixScratch, A.P, p);
+            if (((pf >> 16) & 1u) && !((pf >> 17) & 1u) && ((load_suffix_flags(W.suffixScratch, A.P, p) >> 16) & 1u) &&
+                k > 0) {
+                const Inter last = rehydrate(A.S, load_suffix(W.suffixScratch, A.P, p).s.lastPrefix);
+                q = last.pos;
+                run = true;
+            }
+        } else if ((W.pixFlags[p] & 4u) && k > 0) {
+            const uint4 a = W.packedScratch[2 * (size_t)p], b = W.packedScratch[2 * (size_t)p + 1];
+            const Packed pk = Packed{a.x, bf(a.y), bf(a.z), V3{bf(a.w), bf(b.x), bf(b.y)}};
+            q = inter_restore(A.S, pk, true).pos;
+            run = true;
+        }
+        if (run) {
+            count = query_neighbors<KR>(W, p, q, s_st + threadIdx.x, WBLOCK);
+            for (uint32_t i = 0; i < count; ++i) {
+                if (!((load_suffix_flags(A.suffixRes[A.front], A.P, A.neighIdx[(size_t)p * k + i]) >> 16) & 1u))
+                    continue;
+                validFlags |= 1u << i;
+            }
+        }
+        W.pixFlags[p] = (W.pixFlags[p] & 7u) | (run ? 8u : 0u) | (validFlags << 4);
+    }
+    const uint32_t dirs = W.gIdx == 0 ? 2u : 1u;
+    for (uint32_t i = 0; i < k; ++i) {
+        for (uint32_t d = 0; d < dirs; d++) {
+            Emit e = emit_none();
+            ShState s;
+            const uint32_t job = ((uint32_t)p * k + i) * 2 + d;
+            const bool want = run && ((validFlags >> i) & 1u);
+            if (want) {
+                const uint32_t n = A.neighIdx[(size_t)p * k + i];
+                const uint32_t neigh = make_ref(REF_SUFFIX_FRONT, n);
+                if (W.gIdx == 0) {
+                    const uint32_t canon = make_ref(REF_SUFFIX_SCRATCH, p);
+                    e = d == 0 ? sh_start_suffix(A.S, W.R, s, neigh, canon) : sh_start_suffix(A.S, W.R, s, canon, neigh);
+                } else {
+                    e = sh_start_suffix(A.S, W.R, s, neigh, make_ref(REF_PACKED_TRACED, p));
+                }
+                if (e.kind == EMIT_NONE) shift_finish(W, job, v3(0.0f), 0.0f);
+            }
+            const uint32_t slot = append(W.qout, e.kind);
+            if (want && e.kind != EMIT_NONE) {
+                store_ctx(W.qout, slot, sh_pack(s));
+                emit_job(W.qout, slot, job, e);
+            }
+        }
+    }
+}
+
+// FinalGatherCanon result (final_gather.cu:79-187) added to the pixel's radiance.
+__global__ __launch_bounds__(WBLOCK) void k_w_gather_combine(WaveArgs W, int last) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    if (p >= A.P) return;
+    const uint32_t pf = W.pixFlags[p];
+    const uint32_t k = A.k;
+    V3 rad = V3{W.radScratch[p].x, W.radScratch[p].y, W.radScratch[p].z};
+    if (pf & 1u) {
+        V3 contrib = v3(0.0f);
+        if (W.gIdx == 0) {
+            const Prefix cp = load_prefix(W.prefixScratch, A.P, p).s;
+            if (!cp.valid()) {
+                contrib = v3(0.0f);
+            } else if (cp.nee()) {
+                contrib = cp.f / cp.p;
+            } else {
+                const Suffix cs = load_suffix(W.suffixScratch, A.P, p).s;
+                if (cs.valid()) {
+                    const V3 canonContrib = cp.f * cs.f / (cp.p * cs.p);
+                    if (k == 0) {
+                        contrib = canonContrib;
+                    } else {
+                        const uint32_t validFlags = pf >> 4;
+                        const uint32_t neighCount = A.neighCount[p];
+                        const float K = (float)__popc(validFlags);
+                        float canonMis = 1.0f;
+                        V3 suffixContrib = v3(0.0f);
+                        for (uint32_t i = 0; i < neighCount; ++i) {
+                            if (!((validFlags >> i) & 1u)) continue;
+                            const SuffixRes nres = load_suffix(A.suffixRes[A.front], A.P, A.neighIdx[(size_t)p * k + i]);
+                            const float4 a = W.shiftRes[((size_t)p * k + i) * 2];
+                            const float4 b = W.shiftRes[((size_t)p * k + i) * 2 + 1];
+                            const V3 fFromCanonOfNeigh = V3{a.x, a.y, a.z};
+                            const float jNC = a.w;
+                            const V3 fFromNeighOfCanon = V3{b.x, b.y, b.z};
+                            const float jCN = b.w;
+                            float nMis = neighbor_mis(fFromCanonOfNeigh, nres.s.f, jNC, K, 1.0f, nres.conf);
+                            if (bad(nMis)) nMis = 0.0f;
+                            const V3 nc = nMis * fFromCanonOfNeigh * nres.wSum * jNC;
+                            if (!any_bad(nc)) suffixContrib = suffixContrib + nc;
+                            canonMis += canonical_mis(cs.f, fFromNeighOfCanon, jCN, K, 1.0f, nres.conf);
+                        }
+                        V3 gatherContrib = suffixContrib * cp.f / cp.p;
+                        gatherContrib = gatherContrib / (float)(__popc(validFlags) + 1);
+                        V3 result = gatherContrib + (canonContrib * canonMis);
+                        result = result / (K + 1.0f);
+                        contrib = any_bad(result) ? v3(0.0f) : result;
+                    }
+                }
+            }
+        } else if (k > 0 && (pf & 4u) && (pf & 8u)) {   // FinalGatherNotCanon
+            const uint32_t validFlags = pf >> 4;
+            const uint32_t neighCount = A.neighCount[p];
+            const float K = (float)__popc(validFlags);
+            V3 suffixContrib = v3(0.0f);
+            for (uint32_t i = 0; i < neighCount; ++i) {
+                if (!((validFlags >> i) & 1u)) continue;
+                const SuffixRes nres = load_suffix(A.suffixRes[A.front], A.P, A.neighIdx[(size_t)p * k + i]);
+                const float4 a = W.shiftRes[((size_t)p * k + i) * 2];
+                const V3 f = V3{a.x, a.y, a.z};
+                const float jNC = a.w;
+                float nMis = neighbor_mis(f, nres.s.f, jNC, K, 1.0f, nres.conf);
+                if (bad(nMis)) nMis = 0.0f;
+                const V3 nc = nMis * f * nres.wSum * jNC;
+                if (!any_bad(nc)) suffixContrib = suffixContrib + nc;
+            }
+            const float4 fg = W.fgScratch[p];
+            V3 result = suffixContrib * V3{fg.x, fg.y, fg.z} / fg.w;
+            result = result / (float)A.lp.gatherN;
+            result = result / (K + 1.0f);
+            contrib = any_bad(result) ? v3(0.0f) : result;
+        }
+        rad = rad + contrib;
+    }
+    W.radScratch[p] = float4{rad.x, rad.y, rad.z, 0.0f};
+    if (last) {
+        if (pf & 1u) {
+            if (any_bad(rad)) rad = v3(0.0f);
+            const uint64_t r = W.rngScratch[p];
+            const uint4 g = A.gbuf[p];
+            A.gbuf[p] = uint4{(uint32_t)r, (uint32_t)(r >> 32), g.z, g.w};
+        }
+        write_out(A, p, rad);
+    }
+}
+
+// TracePrefixForFinalGather (path_gen.h:474-531) for gather prefix gIdx > 0.
+__device__ __forceinline__ void fg_finish(const WaveArgs& W, uint32_t p, const FgState& s, bool ok, const Packed& last) {
+    W.rngScratch[p] = s.rng;
+    W.fgScratch[p] = float4{s.thr.x, s.thr.y, s.thr.z, s.p};
+    W.packedScratch[2 * (size_t)p] = uint4{last.prim, fb(last.bu), fb(last.bv), fb(last.dir.x)};
+    W.packedScratch[2 * (size_t)p + 1] = uint4{fb(last.dir.y), fb(last.dir.z), 0u, 0u};
+    W.pixFlags[p] = (W.pixFlags[p] & 1u) | (ok ? 4u : 0u);
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_fg_init(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    const bool px = p < A.P;
+    Emit e = emit_none();
+    FgState s;
+    bool run = false;
+    if (px) {
+        run = (W.pixFlags[p] & 1u) != 0;
+        W.pixFlags[p] &= 1u;
+        if (run) {
+            s.rng = W.rngScratch[p];
+            s.idx = 0;
+            s.thr = v3(1.0f);
+            s.p = 1.0f;
+            const float4 cd = W.fgScratch[p];
+            s.dir = V3{cd.x, cd.y, cd.z};
+            if (A.lp.prefixLen > 0) e = emit_closest(A.camPos, s.dir);
+            else fg_finish(W, p, s, true, packed_invalid());
+        }
+    }
+    const uint32_t slot = append(W.qout, e.kind);
+    if (run && e.kind != EMIT_NONE) {
+        store_ctx(W.qout, slot, fg_pack(s));
+        emit_job(W.qout, slot, p, e);
+    }
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_fg_step(WaveArgs W) {
+    const uint32_t i = gid();
+    const bool active = i < W.nc + W.ns;
+    Emit e = emit_none();
+    FgState s;
+    uint32_t job = 0;
+    if (active) {
+        const uint32_t slot = in_slot(W.qin, i, W.nc, W.ns);
+        job = W.qin.job[slot];
+        s = fg_unpack(load_ctx<FG_CHUNKS>(W.qin, slot));
+        V3 o, d;
+        get_ray(W.qin, slot, o, d);
+        const Inter it = inter_from_hit(W.A.S, W.qin.res[slot], d);
+        if (!it.valid) {
+            fg_finish(W, job, s, false, packed_invalid());
+        } else {
+            Rng rng;
+            rng.s = s.rng;
+            const BrdfSample bs = ggx_sample(W.A.S, it, rng);
+            s.rng = rng.s;
+            if (bs.pdf <= 0.0f) {
+                fg_finish(W, job, s, false, it.pack());
+            } else {
+                s.thr = s.thr * bs.brdf;
+                s.p *= bs.pdf;
+                s.idx++;
+                s.dir = bs.outDir;
+                if (s.idx < (uint32_t)W.A.lp.prefixLen) e = emit_closest(it.pos, bs.outDir);
+                else fg_finish(W, job, s, true, it.pack());
+            }
+        }
+    }
+    const uint32_t slot = append(W.qout, e.kind);
+    if (active && e.kind != EMIT_NONE) {
+        store_ctx(W.qout, slot, fg_pack(s));
+        emit_job(W.qout, slot, job, e);
+    }
+}
+
+__global__ void k_w_zero2(uint32_t* c) {
+    if (threadIdx.x < 2) c[threadIdx.x] = 0u;
+}
+
+// ==========================================================================================
+// Host launchers
+// ==========================================================================================
+
+static dim3 grid_for(uint32_t n) { return dim3((n + WBLOCK - 1) / WBLOCK); }
+
+hipError_t wave_zero_counts(uint32_t* c, hipStream_t s) {
+    hipLaunchKernelGGL(k_w_zero2, dim3(1), dim3(64), 0, s, c);
+    return hipGetLastError();
+}
+
+hipError_t wave_trace(const DScene& S, const Queue& q, uint32_t nc, uint32_t ns, hipStream_t s) {
+    if (nc) hipLaunchKernelGGL(k_q_closest, grid_for(nc), dim3(WBLOCK), 0, s, S, q, nc);
+    if (ns) hipLaunchKernelGGL(k_q_shadow, grid_for(ns), dim3(WBLOCK), 0, s, S, q, ns);
+    return hipGetLastError();
+}
+
+hipError_t wave_launch(int kernel, const WaveArgsHost& H, uint32_t threads, hipStream_t s) {
+    WaveArgs W;
+    static_assert(sizeof(WaveArgs) == sizeof(WaveArgsHost), "WaveArgsHost mirrors WaveArgs");
+    __builtin_memcpy(&W, &H, sizeof(W));
+    if (threads == 0) return hipSuccess;
+    const dim3 g = grid_for(threads), b(WBLOCK);
+    switch (kernel) {
+        case WK_PREFIX_INIT: hipLaunchKernelGGL(k_w_prefix_init, g, b, 0, s, W); break;
+        case WK_PG_STEP: hipLaunchKernelGGL(k_w_pg_step, g, b, 0, s, W); break;
+        case WK_CP_STEP: hipLaunchKernelGGL(k_w_cp_step, g, b, 0, s, W); break;
+        case WK_PREFIX_TEMPORAL_SETUP: hipLaunchKernelGGL(k_w_prefix_temporal_setup, g, b, 0, s, W); break;
+        case WK_SH_STEP: hipLaunchKernelGGL(k_w_sh_step, g, b, 0, s, W); break;
+        case WK_PREFIX_RESAMPLE: hipLaunchKernelGGL(k_w_prefix_resample, g, b, 0, s, W); break;
+        case WK_PREFIX_SPATIAL_SETUP: hipLaunchKernelGGL(k_w_spatial_setup<false>, g, b, 0, s, W); break;
+        case WK_PREFIX_SPATIAL_RESAMPLE: hipLaunchKernelGGL(k_w_prefix_spatial_resample, g, b, 0, s, W); break;
+        case WK_SUFFIX_INIT: hipLaunchKernelGGL(k_w_suffix_init, g, b, 0, s, W, 0); break;
+        case WK_GATHER_SUFFIX_INIT: hipLaunchKernelGGL(k_w_suffix_init, g, b, 0, s, W, 1); break;
+        case WK_SG_STEP: hipLaunchKernelGGL(k_w_sg_step, g, b, 0, s, W); break;
+        case WK_SUFFIX_TEMPORAL_SETUP: hipLaunchKernelGGL(k_w_suffix_temporal_setup, g, b, 0, s, W); break;
+        case WK_SUFFIX_RESAMPLE: hipLaunchKernelGGL(k_w_suffix_resample, g, b, 0, s, W); break;
+        case WK_SUFFIX_SPATIAL_SETUP: hipLaunchKernelGGL(k_w_spatial_setup<true>, g, b, 0, s, W); break;
+        case WK_SUFFIX_SPATIAL_RESAMPLE: hipLaunchKernelGGL(k_w_suffix_spatial_resample, g, b, 0, s, W); break;
+        case WK_GATHER_INIT: hipLaunchKernelGGL(k_w_gather_init, g, b, 0, s, W); break;
+        case WK_GATHER_SETUP:
+            if (W.A.k <= 8) hipLaunchKernelGGL(k_w_gather_setup<8>, g, b, 0, s, W);
+            else hipLaunchKernelGGL(k_w_gather_setup<32>, g, b, 0, s, W);
+            break;
+        case WK_GATHER_COMBINE: hipLaunchKernelGGL(k_w_gather_combine, g, b, 0, s, W, 0); break;
+        case WK_GATHER_COMBINE_LAST: hipLaunchKernelGGL(k_w_gather_combine, g, b, 0, s, W, 1); break;
+        case WK_FG_INIT: hipLaunchKernelGGL(k_w_fg_init, g, b, 0, s, W); break;
+        case WK_FG_STEP: hipLaunchKernelGGL(k_w_fg_step, g, b, 0, s, W); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace cx
