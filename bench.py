@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--cpu-sample-scale", type=int, default=1,
                     help="CPU baseline renders (W/scale) x (H/scale) pixels of the same workload")
     ap.add_argument("--cpu-frames", type=int, default=6)
+    ap.add_argument("--exec", choices=["wavefront", "megakernel"], default="wavefront",
+                    help="execution mode of the HIP passes (identical results)")
     return ap.parse_args()
 
 
@@ -121,6 +123,7 @@ def main():
     cam = scenes.orbit_camera(scene, W, H)
     ren = runtime.Renderer(W, H, ctx, cam, gscene, params=params, seed=1)
     ren.enable_timing(True)
+    ren.set_exec_mode(a.exec)
 
     def frame():
         cam.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
@@ -187,6 +190,7 @@ def main():
         "data": "synthetic (seeded procedural scene, orbiting camera)",
         "config": {
             "workload": f"conditional_restir_{W}x{H}_objgrid{a.grid}",
+            "exec_mode": a.exec,
             "resolution": [W, H],
             "triangles": scene.num_triangles,
             "max_path_len": int(params.maxPathLen),

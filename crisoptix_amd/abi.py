@@ -7,6 +7,8 @@ import numpy as np
 
 CX_PATH_TRACER, CX_RESTIR_PT, CX_CONDITIONAL_RESTIR = 0, 1, 2
 CX_RADIUS_CONSTANT, CX_RADIUS_PATH_LENGTH = 0, 1
+CX_EXEC_WAVEFRONT, CX_EXEC_MEGAKERNEL = 0, 1
+EXEC_MODES = {"wavefront": CX_EXEC_WAVEFRONT, "megakernel": CX_EXEC_MEGAKERNEL}
 PASS_NAMES = ["prefix_gen_temp", "prefix_spatial", "suffix_gen_temp", "suffix_spatial",
               "prefix_store_entries", "entries_build", "final_gather"]
 CX_PASS_COUNT = len(PASS_NAMES)

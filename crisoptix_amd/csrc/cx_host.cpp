@@ -61,6 +61,35 @@ struct cx_renderer {
     bool evValid = false;
     bool passRan[CX_PASS_COUNT] = {};
     std::vector<void*> allocs;
+    int exec = CX_EXEC_WAVEFRONT;
+    // Wavefront execution: two queues sharing one allocation each (viewed with the capacity of
+    // the machine in flight) and per-pixel scratch.
+    struct {
+        uint4* ctx[2] = {nullptr, nullptr};
+        uint32_t* job[2] = {nullptr, nullptr};
+        float4* ray[2] = {nullptr, nullptr};
+        uint4* res[2] = {nullptr, nullptr};
+        uint32_t* count = nullptr;   // [2][2] closest / shadow count of each queue
+        float4* shiftRes = nullptr;
+        size_t jobCap = 0;           // jobs a queue holds
+        size_t ctxWords = 0;         // 16-byte context words a queue holds
+        uint4* prefixScratch = nullptr;
+        uint4* suffixScratch = nullptr;
+        uint4* packedScratch = nullptr;
+        uint64_t* rngScratch = nullptr;
+        uint32_t* pixFlags = nullptr;
+        uint32_t* validMask = nullptr;
+        uint32_t* nbr = nullptr;
+        float4* radScratch = nullptr;
+        float4* fgScratch = nullptr;
+        float4* camDir = nullptr;
+        std::vector<void*> queueAllocs;
+        std::vector<void*> pixelAllocs;
+    } wave;
+    // Longest path and prefix lengths since the last reset: they bound the number of
+    // trace/step rounds a machine can need (records of earlier frames take part in shifts).
+    uint32_t maxLenSeen = 0;
+    uint32_t maxPrefixSeen = 0;
 };
 
 namespace {
@@ -214,6 +243,8 @@ int check_params(cx_renderer* r, const cx_launch_params& p) {
     if (p.gatherN < 1) return set_err(r->ctx, CX_ERR_INVALID_ARG, "gatherN must be >= 1");
     if (p.maxPathLen < 0 || p.prefixLen < 0 || p.maxPathLen > 255)
         return set_err(r->ctx, CX_ERR_INVALID_ARG, "bad path length");
+    if (p.rendererType == CX_CONDITIONAL_RESTIR && p.prefixLen > p.maxPathLen)
+        return set_err(r->ctx, CX_ERR_INVALID_ARG, "prefixLen must not exceed maxPathLen");
     if (p.windowRadius > 1024) return set_err(r->ctx, CX_ERR_INVALID_ARG, "windowRadius too large");
     return CX_OK;
 }
@@ -236,6 +267,194 @@ int ensure_neighbors(cx_renderer* r, uint32_t k) {
 
 void rec(cx_renderer* r, int i, hipStream_t s) {
     if (r->timing) (void)hipEventRecord(r->ev[i], s);
+}
+
+// ---- wavefront execution ----
+
+uint32_t prefix_max_len(const cx_launch_params& lp) {
+    return lp.rendererType == CX_RESTIR_PT ? (uint32_t)lp.maxPathLen : (uint32_t)lp.prefixLen;
+}
+
+// Shift jobs one pass of these parameters can create (two per pixel and neighbour slot).
+size_t shift_jobs(const cx_renderer* r, const cx_launch_params& lp) {
+    const uint32_t k = lp.gatherM > 1 ? (uint32_t)(lp.gatherM - 1) : 0u;
+    const uint32_t slots = std::max({1u, (uint32_t)lp.prefixSpatialCount, (uint32_t)lp.suffixSpatialCount, k});
+    return (size_t)2 * slots * r->P;
+}
+
+int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
+    auto& w = r->wave;
+    hipError_t e = hipSuccess;
+    auto A = [&](hipError_t x) {
+        if (e == hipSuccess) e = x;
+    };
+    if (w.pixelAllocs.empty()) {
+        const size_t P = r->P;
+        A(dalloc(w.pixelAllocs, &w.prefixScratch, (size_t)PREFIX_CHUNKS * P));
+        A(dalloc(w.pixelAllocs, &w.suffixScratch, (size_t)SUFFIX_CHUNKS * P));
+        A(dalloc(w.pixelAllocs, &w.packedScratch, 2 * P));
+        A(dalloc(w.pixelAllocs, &w.rngScratch, P));
+        A(dalloc(w.pixelAllocs, &w.pixFlags, P));
+        A(dalloc(w.pixelAllocs, &w.validMask, P));
+        A(dalloc(w.pixelAllocs, &w.nbr, 16 * P));
+        A(dalloc(w.pixelAllocs, &w.radScratch, P));
+        A(dalloc(w.pixelAllocs, &w.fgScratch, P));
+        A(dalloc(w.pixelAllocs, &w.camDir, P));
+        A(dalloc(w.pixelAllocs, &w.count, 4));
+        if (e != hipSuccess) {
+            free_all(w.pixelAllocs);
+            return hip_err(r->ctx, e, "wavefront scratch allocation");
+        }
+    }
+    const size_t jobs = std::max<size_t>(shift_jobs(r, lp), r->P);
+    const size_t words = std::max<size_t>((size_t)PG_CHUNKS * r->P, (size_t)SH_CHUNKS * jobs);
+    static_assert(PG_CHUNKS >= SG_CHUNKS && PG_CHUNKS >= CP_CHUNKS && PG_CHUNKS >= FG_CHUNKS,
+                  "path machines fit the prefix-generation context");
+    if (jobs <= w.jobCap && words <= w.ctxWords) return CX_OK;
+    if (!w.queueAllocs.empty()) {
+        e = hipDeviceSynchronize();
+        if (e != hipSuccess) return hip_err(r->ctx, e, "wavefront queue reallocation");
+        free_all(w.queueAllocs);
+    }
+    w.jobCap = 0;
+    w.ctxWords = 0;
+    for (int q = 0; q < 2; q++) {
+        A(dalloc(w.queueAllocs, &w.ctx[q], words));
+        A(dalloc(w.queueAllocs, &w.job[q], jobs));
+        A(dalloc(w.queueAllocs, &w.ray[q], 2 * jobs));
+        A(dalloc(w.queueAllocs, &w.res[q], jobs));
+    }
+    A(dalloc(w.queueAllocs, &w.shiftRes, jobs));
+    if (e != hipSuccess) {
+        free_all(w.queueAllocs);
+        return hip_err(r->ctx, e, "wavefront queue allocation");
+    }
+    w.jobCap = jobs;
+    w.ctxWords = words;
+    return CX_OK;
+}
+
+Queue queue_view(const cx_renderer* r, int side, uint32_t cap) {
+    Queue q;
+    q.ctx = r->wave.ctx[side];
+    q.job = r->wave.job[side];
+    q.ray = r->wave.ray[side];
+    q.res = r->wave.res[side];
+    q.count = r->wave.count + 2 * side;
+    q.cap = cap;
+    return q;
+}
+
+WaveArgs make_wave_args(const cx_renderer* r, const FrameArgs& A) {
+    WaveArgs W;
+    std::memset(&W, 0, sizeof(W));
+    W.A = A;
+    const auto& w = r->wave;
+    for (int b = 0; b < 2; b++) {
+        W.R.prefixRes[b] = A.prefixRes[b];
+        W.R.suffixRes[b] = A.suffixRes[b];
+    }
+    W.R.prefixScratch = w.prefixScratch;
+    W.R.suffixScratch = w.suffixScratch;
+    W.R.packedScratch = w.packedScratch;
+    W.R.front = A.front;
+    W.R.back = A.back;
+    W.R.P = A.P;
+    W.prefixScratch = w.prefixScratch;
+    W.suffixScratch = w.suffixScratch;
+    W.packedScratch = w.packedScratch;
+    W.rngScratch = w.rngScratch;
+    W.shiftRes = w.shiftRes;
+    W.pixFlags = w.pixFlags;
+    W.validMask = w.validMask;
+    W.nbr = w.nbr;
+    W.radScratch = w.radScratch;
+    W.fgScratch = w.fgScratch;
+    W.camDir = w.camDir;
+    return W;
+}
+
+// Runs one machine: `setup` (one thread per pixel) appends the first rays to queue 0, then
+// `rounds` rounds of traversal + step.  Every job finishes within `rounds` rounds (the bounds
+// below are the longest chains of trace calls of each machine), and a round whose queue is
+// empty costs two near-empty launches.
+hipError_t run_machine(const cx_renderer* r, WaveArgs W, int setup, int step, uint32_t cap, uint32_t rounds,
+                       int pass, hipStream_t s) {
+    W.qin = queue_view(r, 1, cap);
+    W.qout = queue_view(r, 0, cap);
+    hipError_t e = wave_zero_counts(W.qout.count, s);
+    if (e == hipSuccess) e = wave_launch(setup, W, r->P, s);
+    int cur = 0;
+    for (uint32_t i = 0; i < rounds && e == hipSuccess; i++) {
+        const Queue qin = queue_view(r, cur, cap), qout = queue_view(r, 1 - cur, cap);
+        e = wave_trace(W.A.S, qin, qout, W.A.counters + 2 * pass, s);
+        W.qin = qin;
+        W.qout = qout;
+        if (e == hipSuccess) e = wave_launch(step, W, 0, s);
+        cur = 1 - cur;
+    }
+    return e;
+}
+
+// A pass of the frame loop in wavefront form; same results as launch_pass.
+hipError_t wave_pass(cx_renderer* r, int pass, const FrameArgs& A, hipStream_t s) {
+    const cx_launch_params& lp = A.lp;
+    WaveArgs W = make_wave_args(r, A);
+    const uint32_t P = r->P;
+    const uint32_t shiftCap = (uint32_t)r->wave.jobCap;
+    const uint32_t L = std::max(r->maxLenSeen, (uint32_t)lp.maxPathLen);
+    const uint32_t PL = std::max(r->maxPrefixSeen, prefix_max_len(lp));
+    // Longest trace chains: a prefix walk is up to PL closest hits + 1 shadow ray; a complete
+    // path maxPathLen + 1; a suffix walk up to L - 1 closest hits + 1 shadow + 1 reconnection;
+    // a shift replays up to max(PL, L) - 1 vertices and casts 1 shadow ray.
+    const uint32_t pgRounds = PL + 1, cpRounds = (uint32_t)lp.maxPathLen + 1, sgRounds = L + 1;
+    const uint32_t shRounds = std::max(PL, L) + 1, fgRounds = (uint32_t)lp.prefixLen;
+    const bool restir = lp.rendererType != CX_PATH_TRACER;
+    hipError_t e = hipSuccess;
+    switch (pass) {
+        case CX_PASS_PREFIX_GEN_TEMP:
+            if (!restir) return run_machine(r, W, WK_PREFIX_INIT, WK_CP_STEP, P, cpRounds, pass, s);
+            e = run_machine(r, W, WK_PREFIX_INIT, WK_PG_STEP, P, pgRounds, pass, s);
+            if (e == hipSuccess)
+                e = run_machine(r, W, WK_PREFIX_TEMPORAL_SETUP, WK_SH_STEP, shiftCap,
+                                lp.prefixEnableTemporal ? shRounds : 0u, pass, s);
+            if (e == hipSuccess) e = wave_launch(WK_PREFIX_RESAMPLE, W, P, s);
+            return e;
+        case CX_PASS_PREFIX_SPATIAL:
+            W.S = (uint32_t)lp.prefixSpatialCount;
+            e = run_machine(r, W, WK_PREFIX_SPATIAL_SETUP, WK_SH_STEP, shiftCap, shRounds, pass, s);
+            if (e == hipSuccess) e = wave_launch(WK_PREFIX_SPATIAL_RESAMPLE, W, P, s);
+            return e;
+        case CX_PASS_SUFFIX_GEN_TEMP:
+            e = run_machine(r, W, WK_SUFFIX_INIT, WK_SG_STEP, P, sgRounds, pass, s);
+            if (e == hipSuccess)
+                e = run_machine(r, W, WK_SUFFIX_TEMPORAL_SETUP, WK_SH_STEP, shiftCap,
+                                lp.suffixEnableTemporal ? shRounds : 0u, pass, s);
+            if (e == hipSuccess) e = wave_launch(WK_SUFFIX_RESAMPLE, W, P, s);
+            return e;
+        case CX_PASS_SUFFIX_SPATIAL:
+            W.S = (uint32_t)lp.suffixSpatialCount;
+            e = run_machine(r, W, WK_SUFFIX_SPATIAL_SETUP, WK_SH_STEP, shiftCap, shRounds, pass, s);
+            if (e == hipSuccess) e = wave_launch(WK_SUFFIX_SPATIAL_RESAMPLE, W, P, s);
+            return e;
+        case CX_PASS_FINAL_GATHER: {
+            const bool more = A.k > 0 && lp.gatherN > 1;
+            e = run_machine(r, W, WK_GATHER_INIT, WK_PG_STEP, P, pgRounds, pass, s);
+            if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SUFFIX_INIT, WK_SG_STEP, P, sgRounds, pass, s);
+            W.gIdx = 0;
+            if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SETUP, WK_SH_STEP, shiftCap, shRounds, pass, s);
+            if (e == hipSuccess) e = wave_launch(more ? WK_GATHER_COMBINE : WK_GATHER_COMBINE_LAST, W, P, s);
+            for (int g = 1; more && g < lp.gatherN && e == hipSuccess; g++) {
+                W.gIdx = (uint32_t)g;
+                e = run_machine(r, W, WK_FG_INIT, WK_FG_STEP, P, fgRounds, pass, s);
+                if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SETUP, WK_SH_STEP, shiftCap, shRounds, pass, s);
+                if (e == hipSuccess)
+                    e = wave_launch(g == lp.gatherN - 1 ? WK_GATHER_COMBINE_LAST : WK_GATHER_COMBINE, W, P, s);
+            }
+            return e;
+        }
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -491,6 +710,8 @@ int cx_renderer_destroy(cx_renderer* r) {
     if (r->evMade)
         for (auto& ev : r->ev) (void)hipEventDestroy(ev);
     if (r->entries) cx_entries_destroy(r->entries);
+    free_all(r->wave.queueAllocs);
+    free_all(r->wave.pixelAllocs);
     free_all(r->allocs);
     delete r;
     return CX_OK;
@@ -530,6 +751,8 @@ int cx_renderer_reset(cx_renderer* r, void* stream) {
     r->front = 0;
     r->back = 0;
     r->frameIdx = 0;
+    r->maxLenSeen = 0;
+    r->maxPrefixSeen = 0;
     FrameArgs A = make_args(r);
     CX_HIP(r->ctx, launch_reset(A, (hipStream_t)stream));
     return CX_OK;
@@ -564,6 +787,10 @@ int cx_pass_launch(cx_renderer* r, cx_pass pass, const cx_launch_params* params,
     if (prc != CX_OK) return prc;
     prc = ensure_neighbors(r, std::max(params->gatherM > 1 ? (uint32_t)(params->gatherM - 1) : 0u, 1u));
     if (prc != CX_OK) return prc;
+    if (r->exec == CX_EXEC_WAVEFRONT) {
+        prc = ensure_wave(r, *params);
+        if (prc != CX_OK) return prc;
+    }
     hipStream_t s = (hipStream_t)stream;
     const cx_launch_params saved = r->params;
     r->params = *params;
@@ -577,7 +804,7 @@ int cx_pass_launch(cx_renderer* r, cx_pass pass, const cx_launch_params* params,
         case CX_PASS_PREFIX_SPATIAL:
         case CX_PASS_SUFFIX_GEN_TEMP:
         case CX_PASS_SUFFIX_SPATIAL:
-            e = launch_pass(pass, A, s);
+            e = r->exec == CX_EXEC_WAVEFRONT ? wave_pass(r, pass, A, s) : launch_pass(pass, A, s);
             if (e == hipSuccess && pass == CX_PASS_PREFIX_SPATIAL)
                 e = hipMemcpyAsync(r->prefixRes[r->back], r->prefixRes[r->front], (size_t)PREFIX_CHUNKS * r->P * 16,
                                    hipMemcpyDeviceToDevice, s);
@@ -595,10 +822,14 @@ int cx_pass_launch(cx_renderer* r, cx_pass pass, const cx_launch_params* params,
         case CX_PASS_ENTRIES_BUILD:
             e = launch_tree_build(r->entries->E, r->entries->T, r->P, r->entries->sortTemp, r->entries->sortTempBytes, s);
             break;
-        case CX_PASS_FINAL_GATHER: e = launch_pass(pass, A, s); break;
+        case CX_PASS_FINAL_GATHER:
+            e = r->exec == CX_EXEC_WAVEFRONT ? wave_pass(r, pass, A, s) : launch_pass(pass, A, s);
+            break;
         default: rc = set_err(r->ctx, CX_ERR_INVALID_ARG, "unknown pass");
     }
     r->params = saved;
+    r->maxLenSeen = std::max(r->maxLenSeen, (uint32_t)params->maxPathLen);
+    r->maxPrefixSeen = std::max(r->maxPrefixSeen, prefix_max_len(*params));
     if (e != hipSuccess) return hip_err(r->ctx, e, "pass launch");
     return rc;
 }
@@ -657,6 +888,16 @@ int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, f
     r->evValid = r->timing;
     return CX_OK;
 }
+
+int cx_renderer_set_exec_mode(cx_renderer* r, int mode) {
+    if (!r) return CX_ERR_INVALID_ARG;
+    if (mode != CX_EXEC_WAVEFRONT && mode != CX_EXEC_MEGAKERNEL)
+        return set_err(r->ctx, CX_ERR_INVALID_ARG, "unknown execution mode");
+    r->exec = mode;
+    return CX_OK;
+}
+
+int cx_renderer_get_exec_mode(const cx_renderer* r) { return r ? r->exec : -1; }
 
 int cx_renderer_enable_timing(cx_renderer* r, int enable) {
     if (!r) return CX_ERR_INVALID_ARG;

@@ -8,6 +8,7 @@
 
 #include "../../include/crisoptix.h"
 #include "cx_restir_dev.h"
+#include "cx_wave.h"
 
 namespace cx {
 
@@ -24,6 +25,37 @@ hipError_t launch_entries_query(const EntryTree& T, const float* q, uint32_t nq,
                                 float* dist, uint32_t* count, hipStream_t s);
 hipError_t launch_trace(const DScene& S, const float* rays, uint32_t n, int anyHit, uint32_t* prim, float* t,
                         float* u, float* v, uint32_t* occ, hipStream_t s);
+
+// Wavefront kernels (cx_wave.hip).  Step kernels (*_STEP) run grid-stride over the jobs of
+// W.qin; the others run one thread per pixel (threads = P).
+enum WaveKernel {
+    WK_PREFIX_INIT,
+    WK_PG_STEP,
+    WK_CP_STEP,
+    WK_PREFIX_TEMPORAL_SETUP,
+    WK_SH_STEP,
+    WK_PREFIX_RESAMPLE,
+    WK_PREFIX_SPATIAL_SETUP,
+    WK_PREFIX_SPATIAL_RESAMPLE,
+    WK_SUFFIX_INIT,
+    WK_GATHER_SUFFIX_INIT,
+    WK_SG_STEP,
+    WK_SUFFIX_TEMPORAL_SETUP,
+    WK_SUFFIX_RESAMPLE,
+    WK_SUFFIX_SPATIAL_SETUP,
+    WK_SUFFIX_SPATIAL_RESAMPLE,
+    WK_GATHER_INIT,
+    WK_GATHER_SETUP,
+    WK_GATHER_COMBINE,
+    WK_GATHER_COMBINE_LAST,
+    WK_FG_INIT,
+    WK_FG_STEP,
+};
+
+hipError_t wave_zero_counts(uint32_t* c, hipStream_t s);
+hipError_t wave_trace(const DScene& S, const Queue& qin, const Queue& qout, unsigned long long* counters,
+                      hipStream_t s);
+hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStream_t s);
 
 // Host BVH build output (binary SAH BVH in the node format of cx_scene_dev.h).
 struct BvhHost {

@@ -501,6 +501,89 @@ __device__ __forceinline__ bool trace_prefix_fg(const FrameArgs& A, TraceCtx& tc
     return true;
 }
 
+// ---- neighbour search over the entry tree ----
+
+// Neighbour query under C2: every entry with |q - pos| <= r is a candidate; the k smallest
+// (distance, index) are kept in ascending order.  Up to KR candidates live in registers.
+template <int KR>
+struct TopK {
+    float d[KR];
+    uint32_t i[KR];
+    uint32_t count;
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int j = 0; j < KR; j++) {
+            d[j] = INFINITY;
+            i[j] = 0xFFFFFFFFu;
+        }
+        count = 0;
+    }
+    __device__ __forceinline__ static bool less(float da, uint32_t ia, float db, uint32_t ib) {
+        return da < db || (da == db && ia < ib);
+    }
+    // Sorted insertion; a candidate not smaller than the k-th entry changes nothing.
+    __device__ __forceinline__ void insert(float cd, uint32_t ci, uint32_t k) {
+#pragma unroll
+        for (int j = KR - 1; j >= 0; j--) {
+            if ((uint32_t)j >= k) continue;
+            if (less(cd, ci, d[j], i[j])) {
+                if (j > 0 && less(cd, ci, d[j - 1], i[j - 1])) {
+                    d[j] = d[j - 1];
+                    i[j] = i[j - 1];
+                } else {
+                    d[j] = cd;
+                    i[j] = ci;
+                }
+            }
+        }
+        if (count < k) count++;
+    }
+};
+
+template <int KR>
+__device__ __forceinline__ uint32_t entries_query(const EntryTree& T, V3 q, uint32_t k, int* stack, int stride,
+                                                  uint32_t* outIdx, float* outDist) {
+    TopK<KR> top;
+    top.init();
+    const uint32_t n = *T.numValid;
+    if (k == 0 || n == 0 || isnan(q.x) || isnan(q.y) || isnan(q.z)) return 0;
+    const uint32_t leafBase = (1u << T.levels) - 1u;
+    int sp = 0;
+    uint32_t node = 0;
+    while (true) {
+        const float4 lo = T.lo[node], hi = T.hi[node];
+        const bool inside = q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y && q.z >= lo.z && q.z <= hi.z;
+        if (inside) {
+            if (node >= leafBase) {
+                const uint32_t j = node - leafBase;
+                const uint32_t b = j * T.leafSize, e = min(b + T.leafSize, n);
+                for (uint32_t s = b; s < e; s++) {
+                    const float4 ep = T.entrySorted[s];
+                    const float dd = distance(q, V3{ep.x, ep.y, ep.z});
+                    if (dd > ep.w) continue;
+                    top.insert(dd, T.entryPixel[s], k);
+                }
+            } else {
+                stack[sp * stride] = (int)(2 * node + 2);
+                ++sp;
+                node = 2 * node + 1;
+                continue;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        node = (uint32_t)stack[sp * stride];
+    }
+#pragma unroll
+    for (int j = 0; j < KR; j++) {
+        if ((uint32_t)j < top.count) {
+            outIdx[j] = top.i[j];
+            outDist[j] = top.d[j];
+        }
+    }
+    return top.count;
+}
+
 // ---- hybrid shifts ----
 
 // prefix_reuse.h:10-99: shift src onto dst's primary hit; returns f in dst's domain.

@@ -656,6 +656,28 @@ __device__ __forceinline__ Inter load_suffix_dst(const DScene& S, const RecordSe
     return rehydrate(S, load_suffix_ref(R, ref).lastPrefix);
 }
 
+// Everything a wavefront kernel reads or writes.  qin is the queue whose rays were just
+// answered, qout the queue the kernel appends to; both share one allocation per side and are
+// viewed with the capacity and chunk count of the machine in flight.
+struct WaveArgs {
+    FrameArgs A;
+    RecordSet R;
+    Queue qin, qout;
+    uint4* prefixScratch;   // [PREFIX_CHUNKS][P] canonical prefix of the pass
+    uint4* suffixScratch;   // [SUFFIX_CHUNKS][P] canonical suffix of the pass
+    uint4* packedScratch;   // [P][2] last vertex of a final-gather prefix (Packed)
+    uint64_t* rngScratch;   // [P] pixel generator after path generation / neighbour selection
+    float4* shiftRes;       // [jobs] f.xyz, Jacobian of each finished shift
+    uint32_t* pixFlags;     // [P] per-pass pixel flags
+    uint32_t* validMask;    // [P] valid-neighbour mask of the final gather
+    uint32_t* nbr;          // [P][16] neighbour pixel of each spatial slot
+    float4* radScratch;     // [P] radiance summed over the final-gather prefixes
+    float4* fgScratch;      // [P] final-gather prefix throughput xyz, pdf
+    float4* camDir;         // [P] camera ray direction of the final gather
+    uint32_t S;             // spatial neighbour slots of the pass
+    uint32_t gIdx;          // final-gather prefix index (0 = canonical)
+};
+
 static constexpr uint32_t SH_TRACE = 1, SH_SHADOW = 2;
 static constexpr int SH_CHUNKS = 4;
 

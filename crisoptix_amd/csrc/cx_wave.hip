@@ -6,8 +6,9 @@
 // next ray or to its end; finally a resample kernel runs the reference's reservoir updates per
 // pixel in the reference's order.  The traversal kernels use a 16-entry LDS stack per thread
 // (deeper entries spill to private memory), so their occupancy is set by registers.
+#include <algorithm>
+
 #include "cx_internal.h"
-#include "cx_wave.h"
 
 namespace cx {
 
@@ -16,55 +17,47 @@ static constexpr int SHORT_STACK = 16;
 
 __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
-// ---- traversal kernels over a queue ----
+// ---- traversal of a queue ----
 
-__global__ __launch_bounds__(WBLOCK) void k_q_closest(DScene S, Queue q, uint32_t nc) {
+// Answers every ray of qin: closest-hit rays in [0, nc), shadow rays in [cap - ns, cap).
+// Grid-stride over the device-side counts, so the host never waits for them.  Also clears the
+// counts of qout (the next step kernel's output queue) and adds the rays to the pass counters.
+__global__ __launch_bounds__(WBLOCK) void k_q_trace(DScene S, Queue qin, Queue qout, unsigned long long* counters) {
     __shared__ int s_st[SHORT_STACK * WBLOCK];
-    const uint32_t i = gid();
-    if (i >= nc) return;
-    const float4 a = q.ray[i], b = q.ray[(size_t)q.cap + i];
+    const uint32_t nc = qin.count[0], ns = qin.count[1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        qout.count[0] = 0u;
+        qout.count[1] = 0u;
+        if (counters) {
+            if (nc) atomicAdd(&counters[0], (unsigned long long)nc);
+            if (ns) atomicAdd(&counters[1], (unsigned long long)ns);
+        }
+    }
     TravStack<SHORT_STACK> st;
     st.lds = s_st + threadIdx.x;
     st.stride = WBLOCK;
-    Hit h;
-    traverse_s<false>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
-    const bool hit = h.prim != INVALID_PRIM;
-    q.res[i] = uint4{h.prim, fb(hit ? h.t : 0.0f), fb(hit ? h.u : 0.0f), fb(hit ? h.v : 0.0f)};
+    const uint32_t n = nc + ns;
+    for (uint32_t i = blockIdx.x * WBLOCK + threadIdx.x; i < n; i += gridDim.x * WBLOCK) {
+        const uint32_t slot = in_slot(qin, i, nc, ns);
+        const float4 a = qin.ray[slot], b = qin.ray[(size_t)qin.cap + slot];
+        Hit h;
+        if (i < nc) {
+            traverse_s<false>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
+            const bool hit = h.prim != INVALID_PRIM;
+            qin.res[slot] = uint4{h.prim, fb(hit ? h.t : 0.0f), fb(hit ? h.u : 0.0f), fb(hit ? h.v : 0.0f)};
+        } else {
+            const bool occ = traverse_s<true>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
+            qin.res[slot] = uint4{occ ? 1u : 0u, 0u, 0u, 0u};
+        }
+    }
 }
 
-__global__ __launch_bounds__(WBLOCK) void k_q_shadow(DScene S, Queue q, uint32_t ns) {
-    __shared__ int s_st[SHORT_STACK * WBLOCK];
-    const uint32_t i = gid();
-    if (i >= ns) return;
-    const uint32_t slot = q.cap - ns + i;
-    const float4 a = q.ray[slot], b = q.ray[(size_t)q.cap + slot];
-    TravStack<SHORT_STACK> st;
-    st.lds = s_st + threadIdx.x;
-    st.stride = WBLOCK;
-    Hit h;
-    const bool occ = traverse_s<true>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
-    q.res[slot] = uint4{occ ? 1u : 0u, 0u, 0u, 0u};
-}
-
-// ---- shared state of the wavefront kernels ----
-
-struct WaveArgs {
-    FrameArgs A;
-    RecordSet R;
-    Queue qin, qout;
-    uint32_t nc, ns;
-    uint4* prefixScratch;   // [PREFIX_CHUNKS][P]
-    uint4* suffixScratch;   // [SUFFIX_CHUNKS][P]
-    uint4* packedScratch;   // [2 * P]
-    uint64_t* rngScratch;   // [P]
-    float4* shiftRes;       // [jobs] f.xyz, J
-    uint32_t* pixFlags;     // [P]
-    uint32_t* nbr;          // [P * S] neighbour pixel of each spatial slot
-    float4* radScratch;     // [P] radiance accumulated over the gather prefixes
-    float4* fgScratch;      // [P] gather-prefix throughput xyz, p
-    uint32_t S;             // spatial neighbour slots of the pass
-    uint32_t gIdx;          // gather prefix index (0 = canonical)
-};
+// Grid-stride loop over the jobs of qin whose bound is uniform per block, so every lane of a
+// wavefront reaches each wave-aggregated append.
+#define CX_FOR_JOBS(W)                                                         \
+    const uint32_t _nc = (W).qin.count[0], _ns = (W).qin.count[1];           \
+    const uint32_t _n = _nc + _ns;                                            \
+    for (uint32_t _base = blockIdx.x * WBLOCK; _base < _n; _base += gridDim.x * WBLOCK)
 
 __device__ __forceinline__ void emit_job(const Queue& q, uint32_t slot, uint32_t job, const Emit& e) {
     q.job[slot] = job;
@@ -164,13 +157,14 @@ __global__ __launch_bounds__(WBLOCK) void k_w_prefix_init(WaveArgs W) {
 }
 
 __global__ __launch_bounds__(WBLOCK) void k_w_pg_step(WaveArgs W) {
-    const uint32_t i = gid();
-    const bool active = i < W.nc + W.ns;
+    CX_FOR_JOBS(W) {
+    const uint32_t i = _base + threadIdx.x;
+    const bool active = i < _n;
     Emit e = emit_none();
     PgState s;
     uint32_t job = 0;
     if (active) {
-        const uint32_t slot = in_slot(W.qin, i, W.nc, W.ns);
+        const uint32_t slot = in_slot(W.qin, i, _nc, _ns);
         job = W.qin.job[slot];
         s = pg_unpack(load_ctx<PG_CHUNKS>(W.qin, slot));
         const uint4 r = W.qin.res[slot];
@@ -188,16 +182,18 @@ __global__ __launch_bounds__(WBLOCK) void k_w_pg_step(WaveArgs W) {
         store_ctx(W.qout, slot, pg_pack(s));
         emit_job(W.qout, slot, job, e);
     }
+    }
 }
 
 __global__ __launch_bounds__(WBLOCK) void k_w_cp_step(WaveArgs W) {
-    const uint32_t i = gid();
-    const bool active = i < W.nc + W.ns;
+    CX_FOR_JOBS(W) {
+    const uint32_t i = _base + threadIdx.x;
+    const bool active = i < _n;
     Emit e = emit_none();
     CpState s;
     uint32_t job = 0;
     if (active) {
-        const uint32_t slot = in_slot(W.qin, i, W.nc, W.ns);
+        const uint32_t slot = in_slot(W.qin, i, _nc, _ns);
         job = W.qin.job[slot];
         s = cp_unpack(load_ctx<CP_CHUNKS>(W.qin, slot));
         const uint4 r = W.qin.res[slot];
@@ -214,6 +210,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_cp_step(WaveArgs W) {
     if (active && e.kind != EMIT_NONE) {
         store_ctx(W.qout, slot, cp_pack(s));
         emit_job(W.qout, slot, job, e);
+    }
     }
 }
 
@@ -257,13 +254,14 @@ __global__ __launch_bounds__(WBLOCK) void k_w_prefix_temporal_setup(WaveArgs W) 
 }
 
 __global__ __launch_bounds__(WBLOCK) void k_w_sh_step(WaveArgs W) {
-    const uint32_t i = gid();
-    const bool active = i < W.nc + W.ns;
+    CX_FOR_JOBS(W) {
+    const uint32_t i = _base + threadIdx.x;
+    const bool active = i < _n;
     Emit e = emit_none();
     ShState s;
     uint32_t job = 0;
     if (active) {
-        const uint32_t slot = in_slot(W.qin, i, W.nc, W.ns);
+        const uint32_t slot = in_slot(W.qin, i, _nc, _ns);
         job = W.qin.job[slot];
         s = sh_unpack(load_ctx<SH_CHUNKS>(W.qin, slot));
         const uint4 r = W.qin.res[slot];
@@ -283,6 +281,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_sh_step(WaveArgs W) {
     if (active && e.kind != EMIT_NONE) {
         store_ctx(W.qout, slot, sh_pack(s));
         emit_job(W.qout, slot, job, e);
+    }
     }
 }
 
@@ -614,13 +613,14 @@ __global__ __launch_bounds__(WBLOCK) void k_w_suffix_init(WaveArgs W, int gather
 }
 
 __global__ __launch_bounds__(WBLOCK) void k_w_sg_step(WaveArgs W) {
-    const uint32_t i = gid();
-    const bool active = i < W.nc + W.ns;
+    CX_FOR_JOBS(W) {
+    const uint32_t i = _base + threadIdx.x;
+    const bool active = i < _n;
     Emit e = emit_none();
     SgState s;
     uint32_t job = 0;
     if (active) {
-        const uint32_t slot = in_slot(W.qin, i, W.nc, W.ns);
+        const uint32_t slot = in_slot(W.qin, i, _nc, _ns);
         job = W.qin.job[slot];
         s = sg_unpack(load_ctx<SG_CHUNKS>(W.qin, slot));
         const uint4 r = W.qin.res[slot];
@@ -635,6 +635,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_sg_step(WaveArgs W) {
     if (active && e.kind != EMIT_NONE) {
         store_ctx(W.qout, slot, sg_pack(s));
         emit_job(W.qout, slot, job, e);
+    }
     }
 }
 
@@ -756,7 +757,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_init(WaveArgs W) {
             ux = 2.0f * ux - 1.0f;
             uy = 2.0f * uy - 1.0f;
             const V3 dir = normalize(A.camU * ux + A.camV * uy + A.camW);
-            W.fgScratch[p] = float4{dir.x, dir.y, dir.z, 0.0f};
+            W.camDir[p] = float4{dir.x, dir.y, dir.z, 0.0f};
             pg.rng = rng.s;
             pg.pre.primary = packed_invalid();
             pg.pre.recon = packed_invalid();
@@ -781,48 +782,7 @@ __device__ __forceinline__ uint32_t query_neighbors(const WaveArgs& W, uint32_t 
     const uint32_t k = W.A.k;
     uint32_t* nIdx = W.A.neighIdx + (size_t)p * k;
     float* nDist = W.A.neighDist + (size_t)p * k;
-    TopK<KR> top;
-    top.init();
-    const EntryTree& T = W.A.tree;
-    const uint32_t n = *T.numValid;
-    uint32_t count = 0;
-    if (!(k == 0 || n == 0 || isnan(q.x) || isnan(q.y) || isnan(q.z))) {
-        const uint32_t leafBase = (1u << T.levels) - 1u;
-        int sp = 0;
-        uint32_t node = 0;
-        while (true) {
-            const float4 lo = T.lo[node], hi = T.hi[node];
-            const bool inside = q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y && q.z >= lo.z && q.z <= hi.z;
-            if (inside) {
-                if (node >= leafBase) {
-                    const uint32_t j = node - leafBase;
-                    const uint32_t b = j * T.leafSize, e = min(b + T.leafSize, n);
-                    for (uint32_t s = b; s < e; s++) {
-                        const float4 ep = T.entrySorted[s];
-                        const float dd = distance(q, V3{ep.x, ep.y, ep.z});
-                        if (dd > ep.w) continue;
-                        top.insert(dd, T.entryPixel[s], k);
-                    }
-                } else {
-                    stack[sp * stride] = (int)(2 * node + 2);
-                    ++sp;
-                    node = 2 * node + 1;
-                    continue;
-                }
-            }
-            if (sp == 0) break;
-            --sp;
-            node = (uint32_t)stack[sp * stride];
-        }
-        count = top.count;
-#pragma unroll
-        for (int j = 0; j < KR; j++) {
-            if ((uint32_t)j < top.count) {
-                nIdx[j] = top.i[j];
-                nDist[j] = top.d[j];
-            }
-        }
-    }
+    const uint32_t count = entries_query<KR>(W.A.tree, q, k, stack, stride, nIdx, nDist);
     W.A.neighCount[p] = count;
     if (W.A.lp.trackPrefixStats) {
         atomicAdd(&W.A.counters[STAT_TOTAL], (unsigned long long)count);
@@ -868,7 +828,8 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_setup(WaveArgs W) {
                 validFlags |= 1u << i;
             }
         }
-        W.pixFlags[p] = (W.pixFlags[p] & 7u) | (run ? 8u : 0u) | (validFlags << 4);
+        W.pixFlags[p] = (W.pixFlags[p] & 7u) | (run ? 8u : 0u);
+        W.validMask[p] = validFlags;
     }
     const uint32_t dirs = W.gIdx == 0 ? 2u : 1u;
     for (uint32_t i = 0; i < k; ++i) {
@@ -920,7 +881,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_combine(WaveArgs W, int las
                     if (k == 0) {
                         contrib = canonContrib;
                     } else {
-                        const uint32_t validFlags = pf >> 4;
+                        const uint32_t validFlags = W.validMask[p];
                         const uint32_t neighCount = A.neighCount[p];
                         const float K = (float)__popc(validFlags);
                         float canonMis = 1.0f;
@@ -949,7 +910,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_combine(WaveArgs W, int las
                 }
             }
         } else if (k > 0 && (pf & 4u) && (pf & 8u)) {   // FinalGatherNotCanon
-            const uint32_t validFlags = pf >> 4;
+            const uint32_t validFlags = W.validMask[p];
             const uint32_t neighCount = A.neighCount[p];
             const float K = (float)__popc(validFlags);
             V3 suffixContrib = v3(0.0f);
@@ -1008,7 +969,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_fg_init(WaveArgs W) {
             s.idx = 0;
             s.thr = v3(1.0f);
             s.p = 1.0f;
-            const float4 cd = W.fgScratch[p];
+            const float4 cd = W.camDir[p];
             s.dir = V3{cd.x, cd.y, cd.z};
             if (A.lp.prefixLen > 0) e = emit_closest(A.camPos, s.dir);
             else fg_finish(W, p, s, true, packed_invalid());
@@ -1022,13 +983,14 @@ __global__ __launch_bounds__(WBLOCK) void k_w_fg_init(WaveArgs W) {
 }
 
 __global__ __launch_bounds__(WBLOCK) void k_w_fg_step(WaveArgs W) {
-    const uint32_t i = gid();
-    const bool active = i < W.nc + W.ns;
+    CX_FOR_JOBS(W) {
+    const uint32_t i = _base + threadIdx.x;
+    const bool active = i < _n;
     Emit e = emit_none();
     FgState s;
     uint32_t job = 0;
     if (active) {
-        const uint32_t slot = in_slot(W.qin, i, W.nc, W.ns);
+        const uint32_t slot = in_slot(W.qin, i, _nc, _ns);
         job = W.qin.job[slot];
         s = fg_unpack(load_ctx<FG_CHUNKS>(W.qin, slot));
         V3 o, d;
@@ -1058,6 +1020,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_fg_step(WaveArgs W) {
         store_ctx(W.qout, slot, fg_pack(s));
         emit_job(W.qout, slot, job, e);
     }
+    }
 }
 
 __global__ void k_w_zero2(uint32_t* c) {
@@ -1068,37 +1031,46 @@ __global__ void k_w_zero2(uint32_t* c) {
 // Host launchers
 // ==========================================================================================
 
+// Grid of the grid-stride kernels: enough wavefronts to fill every CU at full occupancy.
+static constexpr uint32_t MAX_STRIDE_BLOCKS = 4096;
+
 static dim3 grid_for(uint32_t n) { return dim3((n + WBLOCK - 1) / WBLOCK); }
+
+static dim3 stride_grid(uint32_t cap) {
+    return dim3(std::max(1u, std::min((cap + WBLOCK - 1) / WBLOCK, MAX_STRIDE_BLOCKS)));
+}
 
 hipError_t wave_zero_counts(uint32_t* c, hipStream_t s) {
     hipLaunchKernelGGL(k_w_zero2, dim3(1), dim3(64), 0, s, c);
     return hipGetLastError();
 }
 
-hipError_t wave_trace(const DScene& S, const Queue& q, uint32_t nc, uint32_t ns, hipStream_t s) {
-    if (nc) hipLaunchKernelGGL(k_q_closest, grid_for(nc), dim3(WBLOCK), 0, s, S, q, nc);
-    if (ns) hipLaunchKernelGGL(k_q_shadow, grid_for(ns), dim3(WBLOCK), 0, s, S, q, ns);
+hipError_t wave_trace(const DScene& S, const Queue& qin, const Queue& qout, unsigned long long* counters,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_q_trace, stride_grid(qin.cap), dim3(WBLOCK), 0, s, S, qin, qout, counters);
     return hipGetLastError();
 }
 
-hipError_t wave_launch(int kernel, const WaveArgsHost& H, uint32_t threads, hipStream_t s) {
-    WaveArgs W;
-    static_assert(sizeof(WaveArgs) == sizeof(WaveArgsHost), "WaveArgsHost mirrors WaveArgs");
-    __builtin_memcpy(&W, &H, sizeof(W));
+hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStream_t s) {
+    const dim3 b(WBLOCK);
+    switch (kernel) {
+        case WK_PG_STEP: hipLaunchKernelGGL(k_w_pg_step, stride_grid(W.qin.cap), b, 0, s, W); return hipGetLastError();
+        case WK_CP_STEP: hipLaunchKernelGGL(k_w_cp_step, stride_grid(W.qin.cap), b, 0, s, W); return hipGetLastError();
+        case WK_SG_STEP: hipLaunchKernelGGL(k_w_sg_step, stride_grid(W.qin.cap), b, 0, s, W); return hipGetLastError();
+        case WK_SH_STEP: hipLaunchKernelGGL(k_w_sh_step, stride_grid(W.qin.cap), b, 0, s, W); return hipGetLastError();
+        case WK_FG_STEP: hipLaunchKernelGGL(k_w_fg_step, stride_grid(W.qin.cap), b, 0, s, W); return hipGetLastError();
+        default: break;
+    }
     if (threads == 0) return hipSuccess;
-    const dim3 g = grid_for(threads), b(WBLOCK);
+    const dim3 g = grid_for(threads);
     switch (kernel) {
         case WK_PREFIX_INIT: hipLaunchKernelGGL(k_w_prefix_init, g, b, 0, s, W); break;
-        case WK_PG_STEP: hipLaunchKernelGGL(k_w_pg_step, g, b, 0, s, W); break;
-        case WK_CP_STEP: hipLaunchKernelGGL(k_w_cp_step, g, b, 0, s, W); break;
         case WK_PREFIX_TEMPORAL_SETUP: hipLaunchKernelGGL(k_w_prefix_temporal_setup, g, b, 0, s, W); break;
-        case WK_SH_STEP: hipLaunchKernelGGL(k_w_sh_step, g, b, 0, s, W); break;
         case WK_PREFIX_RESAMPLE: hipLaunchKernelGGL(k_w_prefix_resample, g, b, 0, s, W); break;
         case WK_PREFIX_SPATIAL_SETUP: hipLaunchKernelGGL(k_w_spatial_setup<false>, g, b, 0, s, W); break;
         case WK_PREFIX_SPATIAL_RESAMPLE: hipLaunchKernelGGL(k_w_prefix_spatial_resample, g, b, 0, s, W); break;
         case WK_SUFFIX_INIT: hipLaunchKernelGGL(k_w_suffix_init, g, b, 0, s, W, 0); break;
         case WK_GATHER_SUFFIX_INIT: hipLaunchKernelGGL(k_w_suffix_init, g, b, 0, s, W, 1); break;
-        case WK_SG_STEP: hipLaunchKernelGGL(k_w_sg_step, g, b, 0, s, W); break;
         case WK_SUFFIX_TEMPORAL_SETUP: hipLaunchKernelGGL(k_w_suffix_temporal_setup, g, b, 0, s, W); break;
         case WK_SUFFIX_RESAMPLE: hipLaunchKernelGGL(k_w_suffix_resample, g, b, 0, s, W); break;
         case WK_SUFFIX_SPATIAL_SETUP: hipLaunchKernelGGL(k_w_spatial_setup<true>, g, b, 0, s, W); break;
@@ -1111,7 +1083,6 @@ hipError_t wave_launch(int kernel, const WaveArgsHost& H, uint32_t threads, hipS
         case WK_GATHER_COMBINE: hipLaunchKernelGGL(k_w_gather_combine, g, b, 0, s, W, 0); break;
         case WK_GATHER_COMBINE_LAST: hipLaunchKernelGGL(k_w_gather_combine, g, b, 0, s, W, 1); break;
         case WK_FG_INIT: hipLaunchKernelGGL(k_w_fg_init, g, b, 0, s, W); break;
-        case WK_FG_STEP: hipLaunchKernelGGL(k_w_fg_step, g, b, 0, s, W); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

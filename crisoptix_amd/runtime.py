@@ -191,6 +191,8 @@ def lib():
         L.cx_renderer_begin_frame.argtypes = [vp, C.POINTER(abi.CameraData), i32, vp]
         L.cx_pass_launch.argtypes = [vp, i32, C.POINTER(abi.LaunchParams), vp]
         L.cx_renderer_enable_timing.argtypes = [vp, i32]
+        L.cx_renderer_set_exec_mode.argtypes = [vp, i32]
+        L.cx_renderer_get_exec_mode.argtypes = [vp]
         L.cx_renderer_get_stats.argtypes = [vp, C.POINTER(abi.FrameStats)]
         L.cx_renderer_state.argtypes = [vp, C.POINTER(abi.StateView)]
         _lib = L
@@ -334,6 +336,14 @@ class Renderer:
 
     def enable_timing(self, on: bool = True):
         lib().cx_renderer_enable_timing(self.h, 1 if on else 0)
+
+    def set_exec_mode(self, mode: str):
+        """'wavefront' (default) or 'megakernel'; both give identical results."""
+        self.ctx.check(lib().cx_renderer_set_exec_mode(self.h, abi.EXEC_MODES[mode]), "cx_renderer_set_exec_mode")
+
+    def exec_mode(self) -> str:
+        m = lib().cx_renderer_get_exec_mode(self.h)
+        return {v: k for k, v in abi.EXEC_MODES.items()}[m]
 
     def launch_frame(self, cam: abi.CameraData | None = None, changed: bool | None = None, output=None,
                      stream=None):

@@ -243,6 +243,20 @@ int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, f
 int cx_renderer_begin_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged,
                             float* d_output);
 int cx_pass_launch(cx_renderer* r, cx_pass pass, const cx_launch_params* params, void* stream);
+/*
+ * Execution strategy of the passes; both give identical results.  No counterpart in the
+ * reference, whose passes are single OptiX raygen launches.
+ *   CX_EXEC_WAVEFRONT (default): path generation and shifts run as state machines whose
+ *     rays are answered by separate traversal kernels, several launches per pass, with no
+ *     host synchronisation inside a frame.
+ *   CX_EXEC_MEGAKERNEL: one kernel per pass, traversal inline.
+ */
+typedef enum cx_exec_mode {
+    CX_EXEC_WAVEFRONT = 0,
+    CX_EXEC_MEGAKERNEL = 1
+} cx_exec_mode;
+int cx_renderer_set_exec_mode(cx_renderer* r, int mode);
+int cx_renderer_get_exec_mode(const cx_renderer* r);
 /* Timings are gathered with hipEvents when enabled (off by default). */
 int cx_renderer_enable_timing(cx_renderer* r, int enable);
 int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* stats);

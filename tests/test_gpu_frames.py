@@ -2,7 +2,8 @@
 shader logic (shader/*.cu, include/graph/restir/*.h) on identical seeded inputs.
 
 Per-pixel radiance must agree within 1e-4 relative after every frame; reservoir records,
-RNG states and neighbour sets are compared bit for bit."""
+RNG states and neighbour sets are compared bit for bit.  Both execution modes of the HIP
+library (wavefront state machines and one megakernel per pass) are checked."""
 import numpy as np
 import pytest
 
@@ -25,7 +26,7 @@ CASES = {
 }
 
 
-def _setup(case, orc, rt, ctx):
+def _setup(case, orc, rt, ctx, exec_mode="wavefront"):
     c = CASES[case]
     sc = scenes.cornell_box(1, 8) if c["scene"] == "cornell" else small_grid_scene()
     W, H = c["W"], c["H"]
@@ -37,6 +38,7 @@ def _setup(case, orc, rt, ctx):
         setattr(p, k, v)
     cam = scenes.cornell_camera(W, H) if c["scene"] == "cornell" else scenes.orbit_camera(sc, W, H)
     ren = rt.Renderer(W, H, ctx, cam, gscene, params=p, seed=1234)
+    ren.set_exec_mode(exec_mode)
     oren = orc.OracleRenderer(oscene, W, H, p, seed=1234)
     return c, cam, ren, oren
 
@@ -54,9 +56,11 @@ def _compare_prefix(g, o, which_valid):
     assert np.array_equal(g["last_prim"][v], o["lastInt"]["prim"][v])
 
 
+@pytest.mark.parametrize("exec_mode", ["wavefront", "megakernel"])
 @pytest.mark.parametrize("case", list(CASES))
-def test_frames_match_oracle(case, orc, cx_runtime, cx_ctx):
-    c, cam, ren, oren = _setup(case, orc, cx_runtime, cx_ctx)
+def test_frames_match_oracle(case, exec_mode, orc, cx_runtime, cx_ctx):
+    print(f"case {case} exec {exec_mode}", flush=True)
+    c, cam, ren, oren = _setup(case, orc, cx_runtime, cx_ctx, exec_mode)
     P = c["W"] * c["H"]
     worst = 0
     for f in range(c["frames"]):
