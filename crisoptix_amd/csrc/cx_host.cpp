@@ -404,11 +404,20 @@ hipError_t wave_pass(cx_renderer* r, int pass, const FrameArgs& A, hipStream_t s
     const uint32_t shiftCap = (uint32_t)r->wave.jobCap;
     const uint32_t L = std::max(r->maxLenSeen, (uint32_t)lp.maxPathLen);
     const uint32_t PL = std::max(r->maxPrefixSeen, prefix_max_len(lp));
-    // Longest trace chains: a prefix walk is up to PL closest hits + 1 shadow ray; a complete
-    // path maxPathLen + 1; a suffix walk up to L - 1 closest hits + 1 shadow + 1 reconnection;
-    // a shift replays up to max(PL, L) - 1 vertices and casts 1 shadow ray.
-    const uint32_t pgRounds = PL + 1, cpRounds = (uint32_t)lp.maxPathLen + 1, sgRounds = L + 1;
-    const uint32_t shRounds = std::max(PL, L) + 1, fgRounds = (uint32_t)lp.prefixLen;
+    // Longest trace chains, each round answering one ray per job:
+    //  * prefix walk: up to PL closest hits + 1 shadow ray;
+    //  * complete path: maxPathLen closest hits + 1 shadow ray;
+    //  * suffix walk: a prefix of length >= 1 leaves at most L - 1 vertices, the first one
+    //    rehydrated, so up to L - 2 closest hits + 1 shadow + 1 reconnection ray;
+    //  * prefix shift: reconIdx <= PL, replays reconIdx - 2 vertices + 1 shadow ray;
+    //  * suffix shift: reconIdx <= max(PL, L - 1) (path_gen.h:338 and :378), replays
+    //    reconIdx - 1 vertices + 1 shadow ray;
+    //  * final-gather prefix: prefixLen closest hits.
+    const uint32_t pgRounds = PL + 1, cpRounds = (uint32_t)lp.maxPathLen + 1;
+    const uint32_t sgRounds = std::max(L, 2u);
+    const uint32_t shPrefixRounds = std::max(PL, 2u) - 1;
+    const uint32_t shSuffixRounds = std::max({PL, L > 0 ? L - 1 : 0u, 1u});
+    const uint32_t fgRounds = (uint32_t)lp.prefixLen;
     const bool restir = lp.rendererType != CX_PATH_TRACER;
     hipError_t e = hipSuccess;
     switch (pass) {
@@ -417,24 +426,24 @@ hipError_t wave_pass(cx_renderer* r, int pass, const FrameArgs& A, hipStream_t s
             e = run_machine(r, W, WK_PREFIX_INIT, WK_PG_STEP, P, pgRounds, pass, s);
             if (e == hipSuccess)
                 e = run_machine(r, W, WK_PREFIX_TEMPORAL_SETUP, WK_SH_STEP, shiftCap,
-                                lp.prefixEnableTemporal ? shRounds : 0u, pass, s);
+                                lp.prefixEnableTemporal ? shPrefixRounds : 0u, pass, s);
             if (e == hipSuccess) e = wave_launch(WK_PREFIX_RESAMPLE, W, P, s);
             return e;
         case CX_PASS_PREFIX_SPATIAL:
             W.S = (uint32_t)lp.prefixSpatialCount;
-            e = run_machine(r, W, WK_PREFIX_SPATIAL_SETUP, WK_SH_STEP, shiftCap, shRounds, pass, s);
+            e = run_machine(r, W, WK_PREFIX_SPATIAL_SETUP, WK_SH_STEP, shiftCap, shPrefixRounds, pass, s);
             if (e == hipSuccess) e = wave_launch(WK_PREFIX_SPATIAL_RESAMPLE, W, P, s);
             return e;
         case CX_PASS_SUFFIX_GEN_TEMP:
             e = run_machine(r, W, WK_SUFFIX_INIT, WK_SG_STEP, P, sgRounds, pass, s);
             if (e == hipSuccess)
                 e = run_machine(r, W, WK_SUFFIX_TEMPORAL_SETUP, WK_SH_STEP, shiftCap,
-                                lp.suffixEnableTemporal ? shRounds : 0u, pass, s);
+                                lp.suffixEnableTemporal ? shSuffixRounds : 0u, pass, s);
             if (e == hipSuccess) e = wave_launch(WK_SUFFIX_RESAMPLE, W, P, s);
             return e;
         case CX_PASS_SUFFIX_SPATIAL:
             W.S = (uint32_t)lp.suffixSpatialCount;
-            e = run_machine(r, W, WK_SUFFIX_SPATIAL_SETUP, WK_SH_STEP, shiftCap, shRounds, pass, s);
+            e = run_machine(r, W, WK_SUFFIX_SPATIAL_SETUP, WK_SH_STEP, shiftCap, shSuffixRounds, pass, s);
             if (e == hipSuccess) e = wave_launch(WK_SUFFIX_SPATIAL_RESAMPLE, W, P, s);
             return e;
         case CX_PASS_FINAL_GATHER: {
@@ -442,12 +451,12 @@ hipError_t wave_pass(cx_renderer* r, int pass, const FrameArgs& A, hipStream_t s
             e = run_machine(r, W, WK_GATHER_INIT, WK_PG_STEP, P, pgRounds, pass, s);
             if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SUFFIX_INIT, WK_SG_STEP, P, sgRounds, pass, s);
             W.gIdx = 0;
-            if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SETUP, WK_SH_STEP, shiftCap, shRounds, pass, s);
+            if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SETUP, WK_SH_STEP, shiftCap, shSuffixRounds, pass, s);
             if (e == hipSuccess) e = wave_launch(more ? WK_GATHER_COMBINE : WK_GATHER_COMBINE_LAST, W, P, s);
             for (int g = 1; more && g < lp.gatherN && e == hipSuccess; g++) {
                 W.gIdx = (uint32_t)g;
                 e = run_machine(r, W, WK_FG_INIT, WK_FG_STEP, P, fgRounds, pass, s);
-                if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SETUP, WK_SH_STEP, shiftCap, shRounds, pass, s);
+                if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SETUP, WK_SH_STEP, shiftCap, shSuffixRounds, pass, s);
                 if (e == hipSuccess)
                     e = wave_launch(g == lp.gatherN - 1 ? WK_GATHER_COMBINE_LAST : WK_GATHER_COMBINE, W, P, s);
             }

@@ -1782,6 +1782,170 @@ static void dump_packed(const Packed& p, orc_packed_dump& d) {
 
 static void v3out(const V3& v, float* o) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
 
+// ---------------------------------------------------------------------------------------
+// FP64 truth: the expectation of the PathTracer estimator (path_gen.h:389-472, the NEE model
+// of include/graph/sample_emitter.h and include/model/Emitter.h:34-54), estimated with every
+// shading quantity in double precision and independent 53-bit uniforms.  Only the traversal
+// runs in float (geometry is float data); hit positions and normals are recomputed in double
+// from the float vertices and the hit's barycentrics.
+// ---------------------------------------------------------------------------------------
+
+struct D3 {
+    double x = 0, y = 0, z = 0;
+    D3() = default;
+    D3(double a, double b, double c) : x(a), y(b), z(c) {}
+    explicit D3(const V3& v) : x(v.x), y(v.y), z(v.z) {}
+    explicit D3(double s) : x(s), y(s), z(s) {}
+    D3 operator+(const D3& o) const { return {x + o.x, y + o.y, z + o.z}; }
+    D3 operator-(const D3& o) const { return {x - o.x, y - o.y, z - o.z}; }
+    D3 operator*(const D3& o) const { return {x * o.x, y * o.y, z * o.z}; }
+    D3 operator*(double s) const { return {x * s, y * s, z * s}; }
+    D3 operator/(double s) const { return {x / s, y / s, z / s}; }
+    D3 operator-() const { return {-x, -y, -z}; }
+};
+static inline D3 operator*(double s, const D3& v) { return v * s; }
+static inline double ddot(const D3& a, const D3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline double dlen(const D3& a) { return std::sqrt(ddot(a, a)); }
+static inline D3 dnorm(const D3& a) { return a / dlen(a); }
+static inline V3 tof(const D3& a) { return V3((float)a.x, (float)a.y, (float)a.z); }
+
+// splitmix64 (Steele, Lea, Flood, "Fast splittable pseudorandom number generators", 2014).
+struct SplitMix {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+struct DInter {
+    bool valid = false;
+    uint32_t prim = 0;
+    D3 pos, nrm, inDir;
+};
+
+static DInter d_trace(const Scene& s, const D3& o, const D3& d) {
+    DInter it;
+    it.inDir = d;
+    const Hit h = trace_closest(s, tof(o), tof(d), 1e-3f, 1e16f);
+    if (h.prim == INVALID_PRIM) return it;
+    it.valid = true;
+    it.prim = h.prim;
+    const double u = h.u, v = h.v, w = 1.0 - u - v;
+    const D3 p0(s.vtx(h.prim, 0)), p1(s.vtx(h.prim, 1)), p2(s.vtx(h.prim, 2));
+    const D3 n0(s.nrm[s.idx[h.prim * 3]]), n1(s.nrm[s.idx[h.prim * 3 + 1]]), n2(s.nrm[s.idx[h.prim * 3 + 2]]);
+    it.pos = w * p0 + u * p1 + v * p2;
+    it.nrm = dnorm(w * n0 + u * n1 + v * n2);
+    return it;
+}
+
+struct DMat {
+    D3 diff, spec, emissive;
+    double rough;
+};
+
+static DMat d_material(const Scene& s, uint32_t prim) {   // brdf.cu:123-138 with metal = 0
+    const Material& m = s.mats[s.triMat[prim]];
+    return DMat{D3(m.diff), D3(m.diff), D3(m.emissive), (double)m.roughness};
+}
+
+static inline double d_ggx_D(double nh, double r) {
+    const double a2 = r * r;
+    const double d = (nh * a2 - nh) * nh + 1.0;
+    return a2 / (M_PI * d * d);
+}
+static inline double d_smith(double nl, double nv, double r) {
+    const double a2 = r * r;
+    return 0.5 / (nl * std::sqrt(nv * nv * (1 - a2) + a2) + nv * std::sqrt(nl * nl * (1 - a2) + a2));
+}
+
+// BSDF value (times cos) and sampling pdf of brdf.cu:115-190 for light direction l.
+static void d_eval(const DMat& m, const DInter& it, const D3& l, D3& f, double& pdf) {
+    const D3 v = -it.inDir;
+    const double nv = ddot(it.nrm, v), nl = ddot(it.nrm, l);
+    f = D3(0.0);
+    pdf = 0.0;
+    if (nv <= 0.0 || nl < 0.0) return;
+    const double sd = m.diff.x + m.diff.y + m.diff.z, ss = m.spec.x + m.spec.y + m.spec.z;
+    const double diffProb = sd / (sd + ss);
+    const D3 h = dnorm(l + v);
+    const double nh = ddot(h, it.nrm), lh = ddot(h, l);
+    D3 spec(0.0);
+    if (ddot(m.spec, m.spec) > 1e-6) {
+        const double q = std::pow(std::max(0.0, 1.0 - lh), 5.0);
+        const D3 F = m.spec + (D3(1.0) - m.spec) * q;
+        spec = F * (d_ggx_D(nh, m.rough) * d_smith(nl, nv, m.rough));
+    }
+    f = (m.diff / M_PI + spec) * nl;
+    pdf = diffProb * nl / M_PI + (1.0 - diffProb) * d_ggx_D(nh, m.rough) * nh / (4.0 * lh);
+}
+
+// Direction sampling of brdf.cu:214-338 (cosine lobe or GGX normal distribution).
+static bool d_sample(const DMat& m, const DInter& it, SplitMix& rng, D3& l) {
+    const D3 n = it.nrm;
+    if (ddot(n, -it.inDir) <= 0.0) return false;
+    const double sz = n.z >= 0 ? 1.0 : -1.0, a = 1.0 / (sz + n.z), ya = n.y * a, b = n.x * ya, c = n.x * sz;
+    const D3 t0(c * n.x * a - 1, sz * b, c), t1(b, n.y * ya - sz, n.y);
+    const double sd = m.diff.x + m.diff.y + m.diff.z, ss = m.spec.x + m.spec.y + m.spec.z;
+    if (rng.uniform() < sd / (sd + ss)) {
+        const double phi = 2.0 * M_PI * rng.uniform(), r = std::sqrt(rng.uniform());
+        const double x = r * std::cos(phi), y = r * std::sin(phi), z = std::sqrt(std::max(0.0, 1.0 - x * x - y * y));
+        l = t0 * x + t1 * y + n * z;
+        return true;
+    }
+    const double u = rng.uniform();
+    const double mz = std::sqrt(std::min(1.0, std::max(0.0, (1 - u) / (1 + (m.rough * m.rough - 1) * u))));
+    const double mr = std::sqrt(std::max(0.0, 1 - mz * mz));
+    const double phi = 2.0 * M_PI * rng.uniform();
+    const D3 h = t0 * (mr * std::cos(phi)) + t1 * (mr * std::sin(phi)) + n * mz;
+    l = it.inDir - h * (2.0 * ddot(h, it.inDir));
+    return ddot(l, n) >= 0.0;
+}
+
+// One sample of the PathTracer estimator from the pixel's camera ray.
+static D3 d_path(const Scene& s, const D3& origin, const D3& dir, uint32_t maxLen, double neeProb, SplitMix& rng) {
+    D3 pos = origin, d = dir, thr(1.0);
+    for (uint32_t depth = 1; depth <= maxLen; ++depth) {
+        const DInter it = d_trace(s, pos, d);
+        if (!it.valid) break;
+        const DMat m = d_material(s, it.prim);
+        if (rng.uniform() < neeProb) {
+            thr = thr / neeProb;
+            const EmitterData& e = s.emitters[std::min<size_t>((size_t)(rng.uniform() * s.emitters.size()),
+                                                               s.emitters.size() - 1)];
+            const size_t nf = e.faces.size() / 3;
+            const size_t fi = std::min<size_t>((size_t)(rng.uniform() * nf), nf - 1);
+            double r0 = rng.uniform(), r1 = rng.uniform(), r2 = rng.uniform();
+            const double rs = r0 + r1 + r2;
+            r0 /= rs; r1 /= rs; r2 /= rs;
+            const D3 lp = r0 * D3(e.faces[3 * fi]) + r1 * D3(e.faces[3 * fi + 1]) + r2 * D3(e.faces[3 * fi + 2]);
+            const D3 lv = lp - it.pos;
+            const double dist = dlen(lv);
+            const D3 ldir = lv / dist;
+            if (trace_occluded(s, tof(it.pos), tof(ldir), 1e-3f, (float)dist)) return D3(0.0);
+            D3 f;
+            double pdf;
+            d_eval(m, it, ldir, f, pdf);
+            D3 rad = thr * f * D3(e.color) / (dist * dist);
+            if (depth == 1) rad = rad + m.emissive;
+            return rad;
+        }
+        D3 l;
+        if (!d_sample(m, it, rng, l)) break;
+        D3 f;
+        double pdf;
+        d_eval(m, it, l, f, pdf);
+        if (pdf <= 0.0) break;
+        thr = thr * f / pdf / (1.0 - neeProb);
+        pos = it.pos;
+        d = l;
+    }
+    return D3(0.0);
+}
+
 extern "C" {
 
 int orc_abi_version(void) { return 1; }
@@ -1800,6 +1964,36 @@ void orc_set_num_threads(int n) {
 #else
     (void)n;
 #endif
+}
+
+
+// FP64 truth image: spp samples per pixel of the PathTracer estimator through the pinhole
+// camera (Camera.h:14-22 with a uniform jitter in the pixel).  out: [h][w][3] mean,
+// var: [h][w][3] variance of the mean (may be null).
+void orc_truth_render(const void* sp, const cx_camera* cam, uint32_t w, uint32_t h, uint32_t maxPathLen,
+                      double neeProb, uint32_t spp, uint64_t seed, double* out, double* var) {
+    const Scene& s = *(const Scene*)sp;
+    const D3 cp(cam->pos[0], cam->pos[1], cam->pos[2]), U(cam->U[0], cam->U[1], cam->U[2]),
+        V(cam->V[0], cam->V[1], cam->V[2]), W(cam->W[0], cam->W[1], cam->W[2]);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < (int64_t)w * h; p++) {
+        const uint32_t x = (uint32_t)(p % w), y = (uint32_t)(p / w);
+        SplitMix rng{seed * 0x9E3779B97F4A7C15ULL + (uint64_t)p * 0xD1B54A32D192ED03ULL};
+        rng.next();
+        D3 sum(0.0), sq(0.0);
+        for (uint32_t i = 0; i < spp; i++) {
+            const double ux = 2.0 * ((x + rng.uniform()) / w) - 1.0, uy = 2.0 * ((y + rng.uniform()) / h) - 1.0;
+            const D3 c = d_path(s, cp, dnorm(U * ux + V * uy + W), maxPathLen, neeProb, rng);
+            sum = sum + c;
+            sq = sq + c * c;
+        }
+        const D3 mean = sum / (double)spp;
+        out[3 * p] = mean.x; out[3 * p + 1] = mean.y; out[3 * p + 2] = mean.z;
+        if (var) {
+            const D3 v = (sq / (double)spp - mean * mean) / (double)std::max(1u, spp - 1);
+            var[3 * p] = v.x; var[3 * p + 1] = v.y; var[3 * p + 2] = v.z;
+        }
+    }
 }
 
 uint32_t orc_tea32(uint32_t a, uint32_t b, uint32_t rounds) { return sample_tea32(a, b, rounds); }

@@ -86,6 +86,7 @@ def lib():
         L.orc_dump_neighbors.restype = u32
         L.orc_dump_neighbors.argtypes = [vp, vp, vp, vp]
         L.orc_entries_query.argtypes = [vp, vp, vp, u32, vp, u32, u32, i32, vp, vp, vp]
+        L.orc_truth_render.argtypes = [vp, C.POINTER(abi.CameraData), u32, u32, u32, C.c_double, u32, u64, vp, vp]
         L.orc_num_threads.restype = i32
         L.orc_set_num_threads.argtypes = [i32]
         _lib = L
@@ -204,6 +205,24 @@ class OracleScene:
         fn = lib().orc_identity_shift_prefix if which == "prefix" else lib().orc_identity_shift_suffix
         fn(self.h, C.byref(params), _p(o), _p(d), rng_state, _p(out))
         return out
+
+
+def truth_render(oscene: "OracleScene", cam: abi.CameraData, width: int, height: int, max_path_len: int,
+                 nee_prob: float, spp: int, seed: int = 0):
+    """FP64 truth of the PathTracer estimator (path_gen.h:389-472): (mean, variance of the
+    mean), both [height][width][3] float64."""
+    out = np.zeros((height, width, 3), np.float64)
+    var = np.zeros((height, width, 3), np.float64)
+    lib().orc_truth_render(oscene.h, C.byref(cam), width, height, max_path_len, float(nee_prob), spp, seed,
+                           _p(out), _p(var))
+    return out, var
+
+
+def rel_mse(img, ref) -> float:
+    """mean over pixels and channels of (I - I_ref)^2 / (I_ref^2 + 1e-2)."""
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
 
 
 def jacobian(old, new, tp, tn) -> float:

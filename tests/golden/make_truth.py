@@ -1,0 +1,68 @@
+"""Writes the FP64 truth images of this directory: the expectation of the reference's
+PathTracer estimator (path_gen.h:389-472), estimated by the oracle's double-precision renderer
+(orc_truth_render) with many samples per pixel.  The GPU tests and bench.py measure
+rel-MSE = mean((I - I_ref)^2 / (I_ref^2 + 1e-2)) of accumulated HIP images against them.
+
+    python tests/golden/make_truth.py [name ...]
+
+Each truth_<name>.npz holds the recipe (JSON), the camera, the mean image, the variance of the
+mean, and the samples per pixel.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from crisoptix_amd import abi, scenes  # noqa: E402
+
+# Scenes and views of the rel-MSE measurements: the Cornell box of the ReSTIR PT workload, and
+# the 1.06 M-triangle object grid of the Conditional ReSTIR workload seen from the orbit
+# camera's first position, at a resolution the CPU can converge.
+TRUTHS = {
+    "cornell_64x64": dict(scene=("cornell_box", dict(seed=1, tess=8)), camera="cornell", W=64, H=64,
+                          maxPathLen=4, neeProb=0.3, spp=65536),
+    "objgrid16_96x54": dict(scene=("object_grid", dict(grid=16, seed=2)), camera="orbit", W=96, H=54,
+                            maxPathLen=8, neeProb=0.3, spp=16384),
+}
+
+
+def build(recipe):
+    name, kw = recipe
+    return getattr(scenes, name)(**kw)
+
+
+def camera(t, sc):
+    if t["camera"] == "cornell":
+        return scenes.cornell_camera(t["W"], t["H"])
+    return scenes.orbit_camera(sc, t["W"], t["H"])
+
+
+def main(names):
+    from oracle import oracle as O
+    for name in names:
+        t = TRUTHS[name]
+        sc = build(t["scene"])
+        os_ = O.OracleScene(sc)
+        cam = abi.CameraData.from_dict(camera(t, sc).data())
+        t0 = time.time()
+        mean, var = O.truth_render(os_, cam, t["W"], t["H"], t["maxPathLen"], t["neeProb"], t["spp"], seed=7)
+        dt = time.time() - t0
+        cd = np.concatenate([np.asarray(getattr(cam, k), np.float32) for k in ("pos", "U", "V", "W")])
+        path = os.path.join(HERE, f"truth_{name}.npz")
+        np.savez_compressed(path, recipe=np.asarray(json.dumps(t)), camera=cd, mean=mean, var=var,
+                            spp=np.asarray(t["spp"]))
+        print(f"{name}: {dt:.0f} s on {O.num_threads()} threads, mean {mean.mean():.5f}, "
+              f"relative std of the mean {np.sqrt(var.mean()) / max(mean.mean(), 1e-12):.4f} -> {path}",
+              flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or list(TRUTHS))
