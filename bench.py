@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=6)
     ap.add_argument("--exec", choices=["wavefront", "megakernel"], default="wavefront",
                     help="execution mode of the HIP passes (identical results)")
+    ap.add_argument("--relmse-frames", type=int, default=4096,
+                    help="accumulated frames of the rel-MSE measurement (0 = skip)")
     return ap.parse_args()
 
 
@@ -68,6 +70,41 @@ def traffic_from_profiles():
         if line.startswith("traffic_bytes_per_launch"):
             return float(line.split()[-1])
     return None
+
+
+def rel_mse_vs_truth(runtime, ctx, gscene, scene, frames, exec_mode):
+    """rel-MSE = mean((I - I_ref)^2 / (I_ref^2 + 1e-2)) of HIP images accumulated over `frames`
+    frames of a static view against the committed FP64 truth of the PathTracer estimator
+    (tests/golden/truth_objgrid16_96x54.npz, written by tests/golden/make_truth.py).  The
+    reference's ReSTIR estimators can leave a pixel at +inf (unclamped reconnection Jacobian,
+    include/graph/restir/path_gen.h:70); such pixels are counted and left out of the mean."""
+    from crisoptix_amd import abi, scenes
+    path = os.path.join(ROOT, "tests", "golden", "truth_objgrid16_96x54.npz")
+    with np.load(path, allow_pickle=False) as z:
+        truth = z["mean"]
+        tcam = z["camera"]
+        spp = int(z["spp"])
+    h, w = truth.shape[:2]
+    cam = scenes.orbit_camera(scene, w, h)
+    cd = abi.CameraData.from_dict(cam.data())
+    if not np.array_equal(np.concatenate([np.asarray(getattr(cd, k), np.float32) for k in ("pos", "U", "V", "W")]),
+                          tcam):
+        raise RuntimeError("truth fixture was rendered from another view")
+    out = {}
+    for mode in ("conditional_restir", "path_tracer"):
+        p = abi.workload_params(mode, w, h, gscene.size)
+        p.enableAccum = 1
+        ren = runtime.Renderer(w, h, ctx, cam, gscene, params=p, seed=3)
+        ren.set_exec_mode(exec_mode)
+        for f in range(frames):
+            ren.launch_frame(cd, f == 0)
+        img = ren.output.numpy().astype(np.float64)
+        finite = np.isfinite(img).all(-1)
+        out[mode] = float(np.mean((img[finite] - truth[finite]) ** 2 / (truth[finite] ** 2 + 1e-2)))
+        out[mode + "_nonfinite_px"] = int((~finite).sum())
+    out.update({"frames": frames, "view": f"{w}x{h} static, orbit camera frame 0", "truth_spp": spp,
+                "definition": "mean over finite pixels and channels of (I - I_ref)^2 / (I_ref^2 + 1e-2)"})
+    return out
 
 
 def cpu_baseline(scene, params_for, W, H, scale, frames=2):
@@ -169,6 +206,13 @@ def main():
     log(f"rays/frame {total_rays / a.steps:.0f} ({total_rays / a.steps / P:.2f}/px), entries/frame "
         f"{entries / a.steps:.0f}, mean radiance {float(out.mean()):.5f}, finite {bool(np.isfinite(out).all())}")
 
+    relmse = None
+    if a.relmse_frames > 0 and a.grid == 16:
+        log(f"rel-MSE vs FP64 truth ({a.relmse_frames} accumulated frames) ...")
+        t1 = time.perf_counter()
+        relmse = rel_mse_vs_truth(runtime, ctx, gscene, scene, a.relmse_frames, a.exec)
+        log(f"rel-MSE: {relmse} ({time.perf_counter() - t1:.1f}s)")
+
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
@@ -199,6 +243,8 @@ def main():
             "rays_per_pixel": round(total_rays / a.steps / P, 3),
             "pass_ms": {n: round(m / a.steps, 4) for n, m in zip(abi.PASS_NAMES, pass_ms)},
             "frame_algorithmic_GBps": round(frame_bytes / (ms_per_frame / 1000.0) / 1e9, 2),
+            "entries_per_frame": int(entries / a.steps),
+            "rel_mse_vs_fp64_truth": relmse,
         },
         "roofline": {
             "kernel": "k_final_gather",

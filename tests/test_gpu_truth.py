@@ -6,9 +6,18 @@ rel-MSE = mean over pixels and channels of (I - I_ref)^2 / (I_ref^2 + 1e-2).
 * PathTracer, 4096 accumulated frames: an unbiased estimate of the truth's expectation, so the
   rel-MSE must be what the per-sample variance predicts (no bias term), and the image mean must
   agree within 5 standard errors.
-* ReSTIR PT and Conditional ReSTIR: the reference's estimators deviate from the PathTracer
-  expectation in known ways (emission scaling, self re-trace, reconnection index, prefix
-  Jacobian at the primary hit), so their rel-MSE is measured and reported, with a loose bound.
+* Conditional ReSTIR: the reference's estimator deviates from the PathTracer expectation in
+  known ways (emission scaling, self re-trace, reconnection index, prefix Jacobian at the
+  primary hit), so its rel-MSE is measured and reported, with a loose bound.
+* ReSTIR PT on the Cornell box does not converge, in the reference as here: the prefix shift
+  evaluates the reconnection Jacobian with the source's primary-hit position instead of the
+  vertex before the reconnection (include/graph/restir/prefix_reuse.h:88-92), and the Jacobian
+  is not clamped (path_gen.h:70 is commented out).  A primary hit and a reconnection vertex on
+  the same tessellated floor give cos_old at rounding-noise level, J up to ~1e8, RIS weights
+  that overflow in FinalizeGRIS (Reservoir.h:44-49) and an unguarded output wSum * f of +inf;
+  accumulation keeps such a pixel at +inf.  So this mode is checked for reproducing that
+  behaviour exactly: the accumulated image after 256 frames equals the oracle's, non-finite
+  pixels included.
 """
 import math
 import os
@@ -65,10 +74,31 @@ def test_path_tracer_converges_to_truth(cx_runtime, cx_ctx):
     assert got < 1.5 * expect
 
 
-@pytest.mark.parametrize("mode", ["restir_pt_cornell", "conditional_restir"])
-def test_restir_rel_mse_reported(mode, cx_runtime, cx_ctx):
-    img, t = accumulate(cx_runtime, cx_ctx, mode, FRAMES)
+def test_conditional_restir_rel_mse_reported(cx_runtime, cx_ctx):
+    img, t = accumulate(cx_runtime, cx_ctx, "conditional_restir", FRAMES)
     got = rel_mse(img, t["mean"])
-    print(f"{mode} {FRAMES} frames: rel-MSE vs FP64 truth {got:.3e}, mean {img.mean():.5f} vs {t['mean'].mean():.5f}")
+    print(f"conditional_restir {FRAMES} frames: rel-MSE vs FP64 truth {got:.3e}, "
+          f"mean {img.mean():.5f} vs {t['mean'].mean():.5f}")
     assert np.isfinite(img).all()
     assert got < 0.5
+
+
+def test_restir_pt_divergence_matches_oracle(cx_runtime, cx_ctx, orc):
+    frames = 256
+    img, t = accumulate(cx_runtime, cx_ctx, "restir_pt_cornell", frames)
+    sc = scenes.cornell_box(1, 8)
+    os_ = orc.OracleScene(sc)
+    p = abi.workload_params("restir_pt_cornell", 64, 64, os_.size)
+    p.maxPathLen = 4
+    p.enableAccum = 1
+    cd = abi.CameraData.from_dict(scenes.cornell_camera(64, 64).data())
+    ren = orc.OracleRenderer(os_, 64, 64, p, seed=99)
+    for f in range(frames):
+        ref = ren.frame(cd, f == 0)
+    fin_g, fin_o = np.isfinite(img).all(-1), np.isfinite(ref).all(-1)
+    print(f"restir_pt_cornell {frames} frames: {int((~fin_g).sum())} non-finite pixels on the GPU, "
+          f"{int((~fin_o).sum())} in the oracle")
+    assert np.array_equal(fin_g, fin_o)
+    assert (~fin_g).any(), "expected the reference's unclamped prefix Jacobian to overflow some pixels"
+    a, b = img[fin_g], ref[fin_o].astype(np.float64)
+    assert np.all(np.abs(a - b) <= 1e-4 * np.maximum(np.abs(a), np.abs(b)) + 1e-6)

@@ -116,3 +116,33 @@ def test_restir_canonical_estimator_relation(orc):
         means.append(np.mean([r.frame(cam, i == 0).mean() for i in range(48)]))
     ratio = means[1] / means[0]
     assert abs(ratio - 1 / 0.3) < 0.15 * (1 / 0.3), means
+
+
+def test_unclamped_jacobian_can_make_reservoirs_infinite(orc):
+    """The prefix shift evaluates the reconnection Jacobian with the source's primary-hit
+    position in place of the vertex before the reconnection (prefix_reuse.h:88-92), the Jacobian
+    is not clamped (path_gen.h:70 is commented out), and FinalizeGRIS divides wSum by the
+    sample's target luminance (Reservoir.h:44-49) with no bound.  A primary hit and a
+    reconnection vertex on the same tessellated floor make cos_old rounding noise, the RIS weight
+    finite but huge, and the final division overflows to +inf: the reservoir then stores
+    wSum = +inf beside a finite, tiny f, and the unguarded ReSTIR PT output wSum * f is +inf.
+    This is the reference's estimator, so the GPU frames reproduce it bit for bit."""
+    sc = scenes.cornell_box(1, 8)
+    os_ = orc.OracleScene(sc)
+    W = H = 64
+    cd = abi.CameraData.from_dict(scenes.cornell_camera(W, H).data())
+    p = abi.workload_params("restir_pt_cornell", W, H, os_.size)
+    p.maxPathLen = 4
+    r = orc.OracleRenderer(os_, W, H, p, seed=99)
+    for i in range(64):
+        out = r.frame(cd, i == 0)
+        bad = ~np.isfinite(out).all(-1)
+        if bad.any():
+            break
+    assert bad.any(), "expected the unclamped Jacobian to overflow a reservoir within 64 frames"
+    assert bad.sum() <= 4
+    front = r.prefix(r.buffers()[0])
+    for px in np.flatnonzero(bad.ravel()):
+        rec = front[px]
+        assert np.isposinf(rec["wSum"])
+        assert np.isfinite(rec["f"]).all() and 0 < float(np.max(rec["f"])) < 1e-3
