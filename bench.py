@@ -8,7 +8,10 @@ lognormal(2, 0.5)), max path length 8, camera orbiting by 2*pi/64 per frame.  Sy
 seeded scene data; every frame runs the full pass sequence of Renderer::LaunchFrame.
 
 One step = one frame.  value = counted rays (closest-hit + shadow) per second over the timed
-frames; ms_per_step = GPU time per frame from hipEvents around the timed frames.
+frames; ms_per_step = GPU time per frame from hipEvents around the timed frames.  roofline: the
+dominant kernel is the ray traversal (k_q_trace, about half of the frame), timed by HIP events
+around each of its launches on the frame's stream; its algorithmic bytes are 48 B per
+closest-hit ray (32 B ray + 16 B hit) and 36 B per shadow ray (32 B ray + 4 B flag).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -58,12 +61,14 @@ def parse():
                     help="execution mode of the HIP passes (identical results)")
     ap.add_argument("--relmse-frames", type=int, default=4096,
                     help="accumulated frames of the rel-MSE measurement (0 = skip)")
+    ap.add_argument("--bvh-width", type=int, choices=[2, 4, 8], default=4, help="children per BVH node")
     return ap.parse_args()
 
 
-def traffic_from_profiles():
-    """HBM bytes per final-gather launch from the committed rocprofv3 counter summary, if any."""
-    path = os.path.join(ROOT, "profiles", "final_gather_1080p_traffic.txt")
+def traffic_from_profiles(kernel, width):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this
+    workload (FETCH_SIZE and WRITE_SIZE, each scaled by a copy kernel of known size), if any."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_bvh{width}_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -142,8 +147,9 @@ def main():
     t0 = time.perf_counter()
     scene = scenes.object_grid(a.grid, WORKLOAD_SEED)
     ctx = runtime.Context(0)
-    gscene = runtime.Scene(ctx, scene)
-    log(f"scene: {scene.num_triangles} triangles, BVH {gscene.info.numNodes} nodes, depth {gscene.info.maxDepth}, "
+    gscene = runtime.Scene(ctx, scene, width=a.bvh_width)
+    log(f"scene: {scene.num_triangles} triangles, BVH {gscene.info.numNodes} binary nodes, depth {gscene.info.maxDepth}, "
+        f"width {gscene.info.bvhWidth} ({gscene.info.numWideNodes} wide nodes, stack bound {gscene.info.stackBound}), "
         f"built in {time.perf_counter() - t0:.1f}s")
 
     def params_for(w, h, size):
@@ -167,6 +173,8 @@ def main():
     pass_ms = np.zeros(abi.CX_PASS_COUNT)
     pass_rays = np.zeros((abi.CX_PASS_COUNT, 2))
     total_rays = 0
+    trace_ms = 0.0
+    trace_launches = 0
     neigh = 0
     entries = 0
     gpu_ms = 0.0
@@ -182,6 +190,8 @@ def main():
         pass_rays[:, 1] += np.asarray(st.passShadowRays)
         total_rays += st.closestRays + st.shadowRays
         entries += st.numEntries
+        trace_ms += st.traceMs
+        trace_launches += st.traceLaunches
     ms_per_frame = gpu_ms / a.steps
     mrays = total_rays / (gpu_ms / 1000.0) / 1e6
     P = W * H
@@ -189,7 +199,20 @@ def main():
     fg_ms = pass_ms[fg] / a.steps
     fg_bytes = FINAL_GATHER_STATE_BYTES_PER_PX * P + (pass_rays[fg, 0] * CLOSEST_RAY_BYTES +
                                                       pass_rays[fg, 1] * SHADOW_RAY_BYTES) / a.steps
-    achieved = fg_bytes / (fg_ms / 1000.0) / 1e9
+    fg_gbs = fg_bytes / (fg_ms / 1000.0) / 1e9
+    ray_bytes = (pass_rays[:, 0].sum() * CLOSEST_RAY_BYTES + pass_rays[:, 1].sum() * SHADOW_RAY_BYTES) / a.steps
+    if trace_launches:
+        # Dominant kernel of the wavefront mode: the traversal launches, algorithmic bytes = the
+        # rays they answer x (32 B ray + 16 B hit, or 32 B ray + 4 B flag).
+        roof_kernel = "k_q_trace"
+        launches_per_frame = trace_launches / a.steps
+        avg_launch_ms = trace_ms / trace_launches
+        bytes_per_launch = ray_bytes / launches_per_frame
+    else:
+        roof_kernel = "k_final_gather"
+        avg_launch_ms = fg_ms
+        bytes_per_launch = fg_bytes
+    achieved = bytes_per_launch / (avg_launch_ms / 1000.0) / 1e9
     frame_bytes = FRAME_STATE_BYTES_PER_PX * P + (pass_rays[:, 0].sum() * CLOSEST_RAY_BYTES +
                                                   pass_rays[:, 1].sum() * SHADOW_RAY_BYTES) / a.steps
     out = runtime.DeviceArray.numpy(ren.output)
@@ -237,18 +260,22 @@ def main():
             "pass_ms": {n: round(m / a.steps, 4) for n, m in zip(abi.PASS_NAMES, pass_ms)},
             "frame_algorithmic_GBps": round(frame_bytes / (ms_per_frame / 1000.0) / 1e9, 2),
             "entries_per_frame": int(entries / a.steps),
+            "bvh_width": int(gscene.info.bvhWidth),
+            "trace_ms_per_frame": round(trace_ms / a.steps, 4),
+            "final_gather_pass": {"ms": round(fg_ms, 4), "algorithmic_bytes": int(fg_bytes),
+                                  "algorithmic_GBps": round(fg_gbs, 2)},
             "rel_mse_vs_fp64_truth": relmse,
         },
         "roofline": {
-            "kernel": "k_final_gather",
+            "kernel": roof_kernel,
             "bound": "hbm",
             "achieved": round(achieved, 3),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
-            "traffic": traffic_from_profiles(),
-            "avg_launch_ms": round(fg_ms, 4),
-            "algorithmic_bytes_per_launch": int(fg_bytes),
+            "traffic": traffic_from_profiles(roof_kernel, int(gscene.info.bvhWidth)),
+            "avg_launch_ms": round(avg_launch_ms, 5),
+            "algorithmic_bytes_per_launch": int(bytes_per_launch),
         },
         "cpu_baseline": None if cpu is None else {k: (round(v, 4) if isinstance(v, float) else v)
                                                   for k, v in cpu.items() if k != "ms_per_frame_at_sample"},

@@ -53,13 +53,16 @@ class Material(C.Structure):
                 ("roughness", C.c_float)]
 
 
+ABI_VERSION = 3   # CX_ABI_VERSION of include/crisoptix.h
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("positions", C.c_void_p), ("normals", C.c_void_p), ("numVertices", C.c_uint32),
         ("indices", C.c_void_p), ("triMaterial", C.c_void_p), ("numTriangles", C.c_uint32),
         ("materials", C.c_void_p), ("numMaterials", C.c_uint32),
         ("emitterFaces", C.c_void_p), ("emitterFaceOffset", C.c_void_p), ("emitterColor", C.c_void_p),
-        ("numEmitters", C.c_uint32), ("bvhMaxLeafSize", C.c_uint32), ("pad0", C.c_uint32),
+        ("numEmitters", C.c_uint32), ("bvhMaxLeafSize", C.c_uint32), ("bvhWidth", C.c_uint32),
     ]
 
 
@@ -68,7 +71,8 @@ class SceneInfo(C.Structure):
         ("numTriangles", C.c_uint32), ("numVertices", C.c_uint32), ("numNodes", C.c_uint32),
         ("maxDepth", C.c_uint32), ("numEmitters", C.c_uint32), ("numEmitterFaces", C.c_uint32),
         ("sceneSize", C.c_float), ("boundsMin", C.c_float * 3), ("boundsMax", C.c_float * 3),
-        ("deviceBytes", C.c_uint64),
+        ("deviceBytes", C.c_uint64), ("bvhWidth", C.c_uint32), ("numWideNodes", C.c_uint32),
+        ("stackBound", C.c_uint32), ("pad0", C.c_uint32),
     ]
 
 
@@ -77,7 +81,8 @@ class FrameStats(C.Structure):
         ("passMs", C.c_float * CX_PASS_COUNT), ("totalMs", C.c_float), ("closestRays", C.c_uint64),
         ("shadowRays", C.c_uint64), ("passClosestRays", C.c_uint64 * CX_PASS_COUNT),
         ("passShadowRays", C.c_uint64 * CX_PASS_COUNT), ("minNeighCount", C.c_uint32), ("maxNeighCount", C.c_uint32),
-        ("totalNeighCount", C.c_uint64), ("numEntries", C.c_uint32), ("pad0", C.c_uint32),
+        ("totalNeighCount", C.c_uint64), ("numEntries", C.c_uint32), ("traceLaunches", C.c_uint32),
+        ("traceMs", C.c_float), ("pad1", C.c_uint32),
     ]
 
 
@@ -112,6 +117,7 @@ def scene_desc(scene, keep: list) -> SceneDesc:
     d.numMaterials = arrs["materials"].shape[0]
     d.numEmitters = arrs["emitterColor"].shape[0]
     d.bvhMaxLeafSize = 0
+    d.bvhWidth = 0
     return d
 
 

@@ -226,4 +226,128 @@ bool build_bvh(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t
     return true;
 }
 
+
+// Collapses the binary BVH into one of width N (4 or 8): each wide node takes the two children
+// of a binary node and keeps replacing the inner child of largest surface area by its two
+// children until it holds N children or only leaves remain.  Child boxes are the binary ones
+// (already padded), so the traversal stays conservative.  Node layout: see cx_scene_dev.h.
+// Returns false (and leaves `out` empty) when the worst-case traversal stack, the largest sum
+// of (children - 1) along a root-to-leaf path, exceeds `stackLimit`.
+namespace {
+
+struct WideBuilder {
+    const BvhHost& bin;
+    int width;
+    int groups;
+    std::vector<float4>& out;
+    uint32_t maxStack = 0;
+
+    struct Child {
+        float lo[3], hi[3];
+        int32_t ref;
+    };
+
+    static float area(const Child& c) {
+        const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+
+    void children_of(int32_t node, Child& c0, Child& c1) const {
+        const float4* n = &bin.nodes[4 * (size_t)node];
+        c0 = Child{{n[0].x, n[0].z, n[2].x}, {n[0].y, n[0].w, n[2].y}, 0};
+        c1 = Child{{n[1].x, n[1].z, n[2].z}, {n[1].y, n[1].w, n[2].w}, 0};
+        int32_t refs[4];
+        std::memcpy(refs, &n[3], 16);
+        c0.ref = refs[0];
+        c1.ref = refs[1];
+    }
+
+    // Emits the wide node for binary inner node `node`; returns its index.  `depthStack` is the
+    // stack bound accumulated above it.
+    int32_t emit(int32_t node, uint32_t depthStack) {
+        Child ch[8];
+        int n = 2;
+        children_of(node, ch[0], ch[1]);
+        while (n < width) {
+            int best = -1;
+            float bestA = -1.0f;
+            for (int i = 0; i < n; i++)
+                if (ch[i].ref >= 0 && area(ch[i]) > bestA) {
+                    bestA = area(ch[i]);
+                    best = i;
+                }
+            if (best < 0) break;
+            Child a, b;
+            children_of(ch[best].ref, a, b);
+            ch[best] = a;
+            ch[n++] = b;
+        }
+        const uint32_t stack = depthStack + (uint32_t)(n - 1);
+        maxStack = std::max(maxStack, stack);
+        const uint32_t nodeWords = 8u * (uint32_t)groups;
+        const int32_t idx = (int32_t)(out.size() / nodeWords);
+        out.resize(out.size() + nodeWords);
+        int32_t refs[8];
+        for (int i = 0; i < width; i++) refs[i] = i < n ? ch[i].ref : WIDE_EMPTY;
+        for (int i = 0; i < n; i++)
+            if (ch[i].ref >= 0) refs[i] = emit(ch[i].ref, stack);
+        float4* w = &out[(size_t)idx * nodeWords];
+        for (int g = 0; g < groups; g++) {
+            float v[6][4];
+            int32_t r4[4];
+            for (int j = 0; j < 4; j++) {
+                const int i = 4 * g + j;
+                for (int a = 0; a < 3; a++) {
+                    v[2 * a][j] = i < n ? ch[i].lo[a] : WIDE_EMPTY_COORD;
+                    v[2 * a + 1][j] = i < n ? ch[i].hi[a] : WIDE_EMPTY_COORD;
+                }
+                r4[j] = refs[i];
+            }
+            for (int k = 0; k < 6; k++) w[8 * g + k] = make_float4(v[k][0], v[k][1], v[k][2], v[k][3]);
+            std::memcpy(&w[8 * g + 6], r4, 16);
+            w[8 * g + 7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        return idx;
+    }
+};
+
+}  // namespace
+
+bool collapse_bvh(const BvhHost& bin, int width, uint32_t stackLimit, std::vector<float4>& out, int32_t& root,
+                  uint32_t& stackBound) {
+    out.clear();
+    root = bin.root;
+    stackBound = 0;
+    if (width != 4 && width != 8) return false;
+    if (bin.root < 0) return true;   // a single leaf: no inner node
+    WideBuilder B{bin, width, width / 4, out};
+    root = B.emit(bin.root, 0);
+    stackBound = B.maxStack;
+    // Every reference must be empty, an inner node of `out` or a leaf inside the triangle slots.
+    const size_t nodeWords = 2 * (size_t)width, numNodes = out.size() / nodeWords;
+    const uint64_t slots = bin.tris.size() / 3;
+    bool ok = B.maxStack + 1 <= stackLimit;
+    for (size_t i = 0; ok && i < numNodes; i++)
+        for (int g = 0; g < width / 4; g++) {
+            int32_t r4[4];
+            std::memcpy(r4, &out[i * nodeWords + 8 * g + 6], 16);
+            for (int j = 0; j < 4; j++) {
+                const int32_t r = r4[j];
+                if (r == WIDE_EMPTY) continue;
+                if (r >= 0) {
+                    ok = ok && (size_t)r < numNodes;
+                } else {
+                    const uint32_t code = ~(uint32_t)r;
+                    ok = ok && (code & 15u) != 0 && (uint64_t)(code >> 4) + (code & 15u) <= slots;
+                }
+            }
+        }
+    if (!ok) {
+        out.clear();
+        root = bin.root;
+        return false;
+    }
+    return true;
+}
+
 }  // namespace cx

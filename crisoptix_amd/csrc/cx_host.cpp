@@ -60,6 +60,10 @@ struct cx_renderer {
     bool evMade = false;
     bool evValid = false;
     bool passRan[CX_PASS_COUNT] = {};
+    // Event pairs around each traversal launch of the frame (timing enabled, wavefront mode).
+    std::vector<hipEvent_t> traceEv;
+    uint32_t traceEvUsed = 0;
+    bool traceEvOk = true;
     std::vector<void*> allocs;
     int exec = CX_EXEC_WAVEFRONT;
     // Wavefront execution: two queues sharing one allocation each (viewed with the capacity of
@@ -174,6 +178,8 @@ int entries_alloc(cx_ctx* ctx, uint32_t capacity, cx_entries** out) {
     A(dalloc(en->allocs, &en->T.entryPixel, n));
     A(dalloc(en->allocs, &en->T.lo, nodes));
     A(dalloc(en->allocs, &en->T.hi, nodes));
+    A(dalloc(en->allocs, &en->T.plo, nodes));
+    A(dalloc(en->allocs, &en->T.phi, nodes));
     en->T.numValid = en->E.numValid;
     en->sortTempBytes = sort_temp_bytes(n);
     if (e == hipSuccess) {
@@ -378,7 +384,21 @@ WaveArgs make_wave_args(const cx_renderer* r, const FrameArgs& A) {
 // `rounds` rounds of traversal + step.  Every job finishes within `rounds` rounds (the bounds
 // below are the longest chains of trace calls of each machine), and a round whose queue is
 // empty costs two near-empty launches.
-hipError_t run_machine(const cx_renderer* r, WaveArgs W, int setup, int step, uint32_t cap, uint32_t rounds,
+// Records one event of the traversal timing pairs (grows the pool outside any capture).
+void trace_rec(cx_renderer* r, hipStream_t s) {
+    if (!r->timing) return;
+    if (r->traceEvUsed == r->traceEv.size()) {
+        hipEvent_t ev = nullptr;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableSystemFence) != hipSuccess) {
+            r->traceEvOk = false;
+            return;
+        }
+        r->traceEv.push_back(ev);
+    }
+    (void)hipEventRecord(r->traceEv[r->traceEvUsed++], s);
+}
+
+hipError_t run_machine(cx_renderer* r, WaveArgs W, int setup, int step, uint32_t cap, uint32_t rounds,
                        int pass, hipStream_t s) {
     W.qin = queue_view(r, 1, cap);
     W.qout = queue_view(r, 0, cap);
@@ -387,7 +407,9 @@ hipError_t run_machine(const cx_renderer* r, WaveArgs W, int setup, int step, ui
     int cur = 0;
     for (uint32_t i = 0; i < rounds && e == hipSuccess; i++) {
         const Queue qin = queue_view(r, cur, cap), qout = queue_view(r, 1 - cur, cap);
+        trace_rec(r, s);
         e = wave_trace(W.A.S, qin, qout, W.A.counters + 2 * pass, s);
+        trace_rec(r, s);
         W.qin = qin;
         W.qout = qout;
         if (e == hipSuccess) e = wave_launch(step, W, 0, s);
@@ -513,6 +535,16 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     std::string err;
     if (!build_bvh(d->positions, d->indices, d->numTriangles, d->bvhMaxLeafSize ? d->bvhMaxLeafSize : 4, bvh, err))
         return set_err(ctx, CX_ERR_BUILD, err);
+    const uint32_t wantWidth = d->bvhWidth ? d->bvhWidth : 4u;
+    if (wantWidth != 2 && wantWidth != 4 && wantWidth != 8)
+        return set_err(ctx, CX_ERR_INVALID_ARG, "bvhWidth must be 0, 2, 4 or 8");
+    std::vector<float4> wide;
+    int32_t wroot = bvh.root;
+    uint32_t stackBound = bvh.maxDepth;
+    uint32_t width = 2;
+    if (wantWidth > 2 && collapse_bvh(bvh, (int)wantWidth, (uint32_t)STACK_DEPTH, wide, wroot, stackBound))
+        width = wantWidth;
+    if (width == 2) stackBound = bvh.maxDepth;
     cx_scene* s = new (std::nothrow) cx_scene();
     if (!s) return set_err(ctx, CX_ERR_OUT_OF_MEMORY, "host allocation failed");
     s->ctx = ctx;
@@ -535,6 +567,7 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     }
     const float4* nodesSrc = bvh.nodes.empty() ? nullptr : bvh.nodes.data();
     s->d.nodes = (const float4*)up(nodesSrc, bvh.nodes.size(), sizeof(float4));
+    s->d.wnodes = width > 2 ? (const float4*)up(wide.data(), wide.size(), sizeof(float4)) : nullptr;
     s->d.tris = (const float4*)up(bvh.tris.data(), bvh.tris.size(), sizeof(float4));
     s->d.pos = (const float*)up(d->positions, 3 * (size_t)d->numVertices, sizeof(float));
     s->d.nrm = (const float*)up(d->normals, 3 * (size_t)d->numVertices, sizeof(float));
@@ -552,6 +585,8 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     s->d.numTris = d->numTriangles;
     s->d.numEmitters = d->numEmitters;
     s->d.root = bvh.root;
+    s->d.wroot = wroot;
+    s->d.width = width;
     // Scene::GetSceneSize (src/Scene.cpp:48-71): bbox of all vertices and the origin.
     float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};
     for (uint32_t v = 0; v < d->numVertices; v++)
@@ -572,7 +607,10 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     s->info.maxDepth = bvh.maxDepth;
     s->info.numEmitters = d->numEmitters;
     s->info.numEmitterFaces = numFaces;
-    s->info.deviceBytes = (bvh.nodes.size() + bvh.tris.size()) * sizeof(float4) +
+    s->info.bvhWidth = width;
+    s->info.numWideNodes = width > 2 ? (uint32_t)(wide.size() / (2 * width)) : 0u;
+    s->info.stackBound = stackBound;
+    s->info.deviceBytes = (bvh.nodes.size() + wide.size() + bvh.tris.size()) * sizeof(float4) +
                           (size_t)d->numVertices * 24 + (size_t)d->numTriangles * 16 + mats.size() * sizeof(DMaterial);
     *out = s;
     return CX_OK;
@@ -588,6 +626,53 @@ int cx_scene_destroy(cx_scene* s) {
 int cx_scene_get_info(const cx_scene* s, cx_scene_info* info) {
     if (!s || !info) return CX_ERR_INVALID_ARG;
     *info = s->info;
+    return CX_OK;
+}
+
+int cx_bvh_host_trace(const cx_scene_desc* d, const float* rays, uint32_t n, int anyHit, uint32_t* prim, float* t,
+                      float* u, float* v, uint32_t* widthUsed) {
+    if (!d || (n && (!rays || !prim || (!anyHit && (!t || !u || !v))))) return CX_ERR_INVALID_ARG;
+    const uint32_t wantWidth = d->bvhWidth ? d->bvhWidth : 4u;
+    if (wantWidth != 2 && wantWidth != 4 && wantWidth != 8) return CX_ERR_INVALID_ARG;
+    BvhHost bvh;
+    std::string err;
+    if (!build_bvh(d->positions, d->indices, d->numTriangles, d->bvhMaxLeafSize ? d->bvhMaxLeafSize : 4, bvh, err))
+        return CX_ERR_BUILD;
+    std::vector<float4> wide;
+    int32_t wroot = bvh.root;
+    uint32_t stackBound = 0, width = 2;
+    if (wantWidth > 2 && collapse_bvh(bvh, (int)wantWidth, (uint32_t)STACK_DEPTH, wide, wroot, stackBound))
+        width = wantWidth;
+    if (widthUsed) *widthUsed = width;
+    DScene S{};
+    S.nodes = bvh.nodes.data();
+    S.wnodes = width > 2 ? wide.data() : nullptr;
+    S.tris = bvh.tris.data();
+    S.numTris = d->numTriangles;
+    S.root = bvh.root;
+    S.wroot = wroot;
+    S.width = width;
+    struct HostStack {
+        int e[STACK_DEPTH];
+        void put(int i, int x) { e[i] = x; }
+        int get(int i) const { return e[i]; }
+    };
+    for (uint32_t i = 0; i < n; i++) {
+        const float* r = rays + 8 * (size_t)i;
+        HostStack st;
+        Hit h;
+        const V3 o{r[0], r[1], r[2]}, dir{r[3], r[4], r[5]};
+        if (anyHit) {
+            prim[i] = traverse_s<true>(S, o, dir, r[6], r[7], h, st) ? 1u : 0u;
+        } else {
+            traverse_s<false>(S, o, dir, r[6], r[7], h, st);
+            const bool hit = h.prim != INVALID_PRIM;
+            prim[i] = h.prim;
+            t[i] = hit ? h.t : 0.0f;
+            u[i] = hit ? h.u : 0.0f;
+            v[i] = hit ? h.v : 0.0f;
+        }
+    }
     return CX_OK;
 }
 
@@ -718,6 +803,7 @@ int cx_renderer_destroy(cx_renderer* r) {
     if (!r) return CX_OK;
     if (r->evMade)
         for (auto& ev : r->ev) (void)hipEventDestroy(ev);
+    for (auto& ev : r->traceEv) (void)hipEventDestroy(ev);
     if (r->entries) cx_entries_destroy(r->entries);
     free_all(r->wave.queueAllocs);
     free_all(r->wave.pixelAllocs);
@@ -849,10 +935,14 @@ int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, f
     hipStream_t s = (hipStream_t)stream;
     cx_launch_params& lp = r->params;
     if (r->timing && !r->evMade) {
-        for (auto& ev : r->ev) CX_HIP(r->ctx, hipEventCreate(&ev));
+        // Timing-only events: no system-scope fence (cache writeback and invalidation) at
+        // each record, which would both slow the frame and inflate the measured intervals.
+        for (auto& ev : r->ev) CX_HIP(r->ctx, hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
         r->evMade = true;
     }
     for (bool& b : r->passRan) b = false;
+    r->traceEvUsed = 0;
+    r->traceEvOk = true;
     CX_HIP(r->ctx, launch_zero_counters(r->counters, NUM_COUNTERS, s));
     rec(r, 0, s);
     // src/Renderer.cpp:259-371
@@ -939,6 +1029,14 @@ int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* st) {
             st->passMs[i] = ms;
         }
         (void)hipEventElapsedTime(&st->totalMs, r->ev[0], r->ev[CX_PASS_COUNT]);
+        if (r->traceEvOk) {
+            st->traceLaunches = r->traceEvUsed / 2;
+            for (uint32_t i = 0; i + 1 < r->traceEvUsed; i += 2) {
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, r->traceEv[i], r->traceEv[i + 1]);
+                st->traceMs += ms;
+            }
+        }
     }
     return CX_OK;
 }

@@ -68,4 +68,9 @@ struct BvhHost {
 bool build_bvh(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t maxLeaf, BvhHost& out,
                std::string& err);
 
+// Wide (4- or 8-child) node array from a binary BVH; false if its worst-case traversal stack
+// would not fit in `stackLimit` entries.
+bool collapse_bvh(const BvhHost& bin, int width, uint32_t stackLimit, std::vector<float4>& out, int32_t& root,
+                  uint32_t& stackBound);
+
 }  // namespace cx

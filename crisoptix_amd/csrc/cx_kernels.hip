@@ -473,11 +473,14 @@ __global__ void k_tree_leaves(EntryTree T, const uint32_t* sortedVals, const flo
     const uint32_t b = j * T.leafSize;
     const uint32_t e = min(b + T.leafSize, n);
     float4 lo = float4{INFINITY, INFINITY, INFINITY, 0.0f}, hi = float4{-INFINITY, -INFINITY, -INFINITY, 0.0f};
+    float4 plo = lo, phi = hi;
     for (uint32_t i = b; i < e; i++) {
         const uint32_t pix = sortedVals[i];
         const float4 q = entryPos[pix];
         T.entrySorted[i] = q;
         T.entryPixel[i] = pix;
+        plo = float4{fminf(plo.x, q.x), fminf(plo.y, q.y), fminf(plo.z, q.z), 0.0f};
+        phi = float4{fmaxf(phi.x, q.x), fmaxf(phi.y, q.y), fmaxf(phi.z, q.z), 0.0f};
         const float r = q.w;
         lo.x = fminf(lo.x, q.x - r);
         lo.y = fminf(lo.y, q.y - r);
@@ -495,6 +498,8 @@ __global__ void k_tree_leaves(EntryTree T, const uint32_t* sortedVals, const flo
     }
     T.lo[leafBase + j] = lo;
     T.hi[leafBase + j] = hi;
+    T.plo[leafBase + j] = plo;
+    T.phi[leafBase + j] = phi;
 }
 
 __global__ void k_tree_level(EntryTree T, uint32_t levelBase, uint32_t count) {
@@ -505,6 +510,9 @@ __global__ void k_tree_level(EntryTree T, uint32_t levelBase, uint32_t count) {
     const float4 l0 = T.lo[c0], l1 = T.lo[c1], h0 = T.hi[c0], h1 = T.hi[c1];
     T.lo[node] = float4{fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.0f};
     T.hi[node] = float4{fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.0f};
+    const float4 pl0 = T.plo[c0], pl1 = T.plo[c1], ph0 = T.phi[c0], ph1 = T.phi[c1];
+    T.plo[node] = float4{fminf(pl0.x, pl1.x), fminf(pl0.y, pl1.y), fminf(pl0.z, pl1.z), 0.0f};
+    T.phi[node] = float4{fmaxf(ph0.x, ph1.x), fmaxf(ph0.y, ph1.y), fmaxf(ph0.z, ph1.z), 0.0f};
 }
 
 template <int KR>

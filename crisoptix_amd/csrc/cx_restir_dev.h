@@ -170,8 +170,10 @@ struct EntryBuild {
 struct EntryTree {
     float4* entrySorted;  // [capacity] pos.xyz, radius
     uint32_t* entryPixel; // [capacity]
-    float4* lo;           // [2^(levels+1) - 1]
+    float4* lo;           // [2^(levels+1) - 1] union of the entries' [pos - r, pos + r], padded
     float4* hi;
+    float4* plo;          // [2^(levels+1) - 1] exact bounds of the entries' positions
+    float4* phi;
     const uint32_t* numValid;
     uint32_t levels;
     uint32_t leafSize;
@@ -540,6 +542,27 @@ struct TopK {
     }
 };
 
+// Lower bound of distance(q, p) over the positions p inside [lo, hi]: per axis the gap to the
+// box (0 inside), squared and summed in the order length() uses.  Float subtraction, squaring,
+// addition and sqrt are monotone, and the bounds are exact entry coordinates, so the bound never
+// exceeds the float distance the leaf test computes for any entry of the node.
+__device__ __forceinline__ float box_distance(V3 q, float4 lo, float4 hi) {
+    const float dx = q.x < lo.x ? lo.x - q.x : (q.x > hi.x ? q.x - hi.x : 0.0f);
+    const float dy = q.y < lo.y ? lo.y - q.y : (q.y > hi.y ? q.y - hi.y : 0.0f);
+    const float dz = q.z < lo.z ? lo.z - q.z : (q.z > hi.z ? q.z - hi.z : 0.0f);
+    return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ __forceinline__ bool in_box(V3 q, float4 lo, float4 hi) {
+    return q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y && q.z >= lo.z && q.z <= hi.z;
+}
+
+// Depth-first k-nearest search: a node is entered when q lies in its radius box (only then
+// can one of its entries have |q - p| <= r) and, once k candidates are held, when the lower
+// bound of its entries' distances does not exceed the k-th distance (an entry farther than
+// that cannot enter the set; one at exactly that distance can, through the index tie-break).
+// The nearer child (by that bound) is visited first.  The set found is the same as with an
+// exhaustive visit of all candidates.
 template <int KR>
 __device__ __forceinline__ uint32_t entries_query(const EntryTree& T, V3 q, uint32_t k, int* stack, int stride,
                                                   uint32_t* outIdx, float* outDist) {
@@ -548,31 +571,55 @@ __device__ __forceinline__ uint32_t entries_query(const EntryTree& T, V3 q, uint
     const uint32_t n = *T.numValid;
     if (k == 0 || n == 0 || isnan(q.x) || isnan(q.y) || isnan(q.z)) return 0;
     const uint32_t leafBase = (1u << T.levels) - 1u;
+    float kth = INFINITY;   // k-th smallest distance once k candidates are held
+    auto admits = [&](uint32_t node, float& bound) -> bool {
+        if (!in_box(q, T.lo[node], T.hi[node])) return false;
+        bound = box_distance(q, T.plo[node], T.phi[node]);
+        return bound <= kth;
+    };
+    float b0 = 0.0f;
+    if (!admits(0u, b0)) return 0;
     int sp = 0;
     uint32_t node = 0;
     while (true) {
-        const float4 lo = T.lo[node], hi = T.hi[node];
-        const bool inside = q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y && q.z >= lo.z && q.z <= hi.z;
-        if (inside) {
-            if (node >= leafBase) {
-                const uint32_t j = node - leafBase;
-                const uint32_t b = j * T.leafSize, e = min(b + T.leafSize, n);
-                for (uint32_t s = b; s < e; s++) {
-                    const float4 ep = T.entrySorted[s];
-                    const float dd = distance(q, V3{ep.x, ep.y, ep.z});
-                    if (dd > ep.w) continue;
-                    top.insert(dd, T.entryPixel[s], k);
+        if (node >= leafBase) {
+            const uint32_t j = node - leafBase;
+            const uint32_t b = j * T.leafSize, e = min(b + T.leafSize, n);
+            for (uint32_t s = b; s < e; s++) {
+                const float4 ep = T.entrySorted[s];
+                const float dd = distance(q, V3{ep.x, ep.y, ep.z});
+                if (dd > ep.w || dd > kth) continue;
+                top.insert(dd, T.entryPixel[s], k);
+                if (top.count == k) {
+#pragma unroll
+                    for (int m = 0; m < KR; m++)
+                        if ((uint32_t)m + 1u == k) kth = top.d[m];
                 }
-            } else {
-                stack[sp * stride] = (int)(2 * node + 2);
+            }
+        } else {
+            const uint32_t c0 = 2 * node + 1, c1 = 2 * node + 2;
+            float bd0 = 0.0f, bd1 = 0.0f;
+            const bool ok0 = admits(c0, bd0), ok1 = admits(c1, bd1);
+            if (ok0 && ok1) {
+                const bool near0 = bd0 <= bd1;
+                stack[sp * stride] = (int)(near0 ? c1 : c0);
                 ++sp;
-                node = 2 * node + 1;
+                node = near0 ? c0 : c1;
+                continue;
+            }
+            if (ok0 || ok1) {
+                node = ok0 ? c0 : c1;
                 continue;
             }
         }
-        if (sp == 0) break;
-        --sp;
-        node = (uint32_t)stack[sp * stride];
+        // Pop the next deferred sibling that the k-th distance found since has not ruled out.
+        bool found = false;
+        while (sp > 0 && !found) {
+            --sp;
+            node = (uint32_t)stack[sp * stride];
+            found = box_distance(q, T.plo[node], T.phi[node]) <= kth;
+        }
+        if (!found) break;
     }
 #pragma unroll
     for (int j = 0; j < KR; j++) {

@@ -29,8 +29,17 @@ struct DMaterial {
     float4 emissive;    // emissive rgb, -
 };
 
+// Wide BVH node (width N = 4 or 8), N/4 groups of 7 float4, padded to 8 float4 per group:
+//   lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4], ref[4]
+// with refs as in the binary node; an empty slot has ref WIDE_EMPTY (never a leaf code, since
+// a leaf's first slot plus its count stays below 2^27) and a point box at WIDE_EMPTY_COORD,
+// which no ray with tmax <= 1e30 can reach.
+static constexpr int32_t WIDE_EMPTY = (int32_t)0x80000000;
+static constexpr float WIDE_EMPTY_COORD = 3.0e38f;
+
 struct DScene {
     const float4* __restrict__ nodes;
+    const float4* __restrict__ wnodes;    // wide nodes (width 4 or 8), or null
     const float4* __restrict__ tris;
     const float* __restrict__ pos;        // [V][3]
     const float* __restrict__ nrm;        // [V][3]
@@ -42,18 +51,20 @@ struct DScene {
     const float* __restrict__ emitColor;  // [E][3]
     uint32_t numTris;
     uint32_t numEmitters;
-    int32_t root;
-    uint32_t pad;
+    int32_t root;         // root of `nodes` (binary)
+    int32_t wroot;        // root of `wnodes`
+    uint32_t width;       // 2: traverse `nodes`; 4 or 8: traverse `wnodes`
+    uint32_t pad[3];
 };
 
-static constexpr int STACK_DEPTH = 48;
+static constexpr int STACK_DEPTH = 64;
 
 struct RayPre {
     int kx, ky, kz;
     float Sx, Sy, Sz;
 };
 
-__device__ __forceinline__ RayPre ray_pre(V3 d) {
+CX_HD RayPre ray_pre(V3 d) {
     const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     RayPre r;
     r.kz = (ax >= ay) ? ((ax >= az) ? 0 : 2) : ((ay >= az) ? 1 : 2);
@@ -72,7 +83,7 @@ __device__ __forceinline__ RayPre ray_pre(V3 d) {
 }
 
 // Watertight ray/triangle test.  u weights v1 and v weights v2 (optixGetTriangleBarycentrics).
-__device__ __forceinline__ bool tri_hit(V3 o, const RayPre& r, V3 v0, V3 v1, V3 v2, float tmin, float tmax,
+CX_HD bool tri_hit(V3 o, const RayPre& r, V3 v0, V3 v1, V3 v2, float tmin, float tmax,
                                         float& t, float& u, float& v) {
     const V3 A = v0 - o, B = v1 - o, C = v2 - o;
     const float Akx = comp(A, r.kx), Aky = comp(A, r.ky), Akz = comp(A, r.kz);
@@ -109,7 +120,7 @@ struct Hit {
 
 // Slab test of both children of a node; the far plane carries a relative slack so that a
 // triangle whose computed t equals the current best is never pruned.
-__device__ __forceinline__ void box2(const float4 a, const float4 b, const float4 c, V3 o, V3 inv, float tmin,
+CX_HD void box2(const float4 a, const float4 b, const float4 c, V3 o, V3 inv, float tmin,
                                      float tcap, bool& hit0, bool& hit1, float& tn0, float& tn1) {
     const float l0x = (a.x - o.x) * inv.x, h0x = (a.y - o.x) * inv.x;
     const float l0y = (a.z - o.y) * inv.y, h0y = (a.w - o.y) * inv.y;
@@ -136,19 +147,141 @@ struct TravStack {
     int* lds;
     int stride;
     int ovf[STACK_DEPTH - SHORT > 0 ? STACK_DEPTH - SHORT : 1];
-    __device__ __forceinline__ void put(int i, int v) {
+    CX_HD void put(int i, int v) {
         if (SHORT >= STACK_DEPTH || i < SHORT) lds[i * stride] = v;
         else ovf[i - SHORT] = v;
     }
-    __device__ __forceinline__ int get(int i) const {
+    CX_HD int get(int i) const {
         if (SHORT >= STACK_DEPTH || i < SHORT) return lds[i * stride];
         return ovf[i - SHORT];
     }
 };
 
+// Closest-hit / any-hit leaf loop shared by the binary and wide traversals.  Returns true when
+// an any-hit query is answered.
+template <bool ANY>
+CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float tmin,
+                                          float tmax, Hit& best) {
+    const uint32_t code = ~(uint32_t)ref;
+    const uint32_t first = code >> 4, cnt = code & 15u;
+    for (uint32_t i = 0; i < cnt; i++) {
+        const float4* tv = S.tris + 3 * (size_t)(first + i);
+        const float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+        float t, u, v;
+        if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, tmax, t, u, v))
+            continue;
+        const uint32_t prim = __builtin_bit_cast(uint32_t, p0.w);
+        if (ANY) {
+            best.prim = prim;
+            best.t = t;
+            return true;
+        }
+        if (t < best.t || (t == best.t && prim < best.prim)) {
+            best.prim = prim;
+            best.t = t;
+            best.u = u;
+            best.v = v;
+        }
+    }
+    return false;
+}
+
+// Slab test of one child of a wide node (same arithmetic and slack as box2).
+CX_HD float box1(float lx, float hx, float ly, float hy, float lz, float hz, V3 o, V3 inv,
+                                      float tmin, float tcap, bool& hit) {
+    const float ax = (lx - o.x) * inv.x, bx = (hx - o.x) * inv.x;
+    const float ay = (ly - o.y) * inv.y, by = (hy - o.y) * inv.y;
+    const float az = (lz - o.z) * inv.z, bz = (hz - o.z) * inv.z;
+    const float n = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+    const float f = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tcap));
+    hit = n <= f * 1.00002f + 1e-30f;
+    return n;
+}
+
+CX_HD void cswap(float& ta, int& ra, float& tb, int& rb) {
+    const bool sw = tb < ta;
+    const float t = sw ? tb : ta;
+    const int r = sw ? rb : ra;
+    tb = sw ? ta : tb;
+    rb = sw ? ra : rb;
+    ta = t;
+    ra = r;
+}
+
+// Wide traversal: test the N child boxes, sort the hit ones by entry distance (a fixed
+// compare-exchange network on registers, misses at +inf), continue with the nearest and push
+// the rest farthest first.
+template <bool ANY, int N, typename Stack>
+CX_HD bool traverse_wide(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit& best,
+                                              Stack& st) {
+    const RayPre rp = ray_pre(d);
+    const V3 inv = V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    int sp = 0;
+    int ref = S.wroot;
+    while (true) {
+        if (ref >= 0) {
+            const float4* n = S.wnodes + (size_t)ref * (2 * N);
+            float tn[N];
+            int cr[N];
+#pragma unroll
+            for (int g = 0; g < N / 4; g++) {
+                const float4 lx = n[8 * g + 0], hx = n[8 * g + 1], ly = n[8 * g + 2], hy = n[8 * g + 3];
+                const float4 lz = n[8 * g + 4], hz = n[8 * g + 5];
+                const int4 rr = __builtin_bit_cast(int4, n[8 * g + 6]);
+                const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+                const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+                const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+                const int ra[4] = {rr.x, rr.y, rr.z, rr.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    bool h;
+                    const float t = box1(lxa[j], hxa[j], lya[j], hya[j], lza[j], hza[j], o, inv, tmin, best.t, h);
+                    const bool use = h && ra[j] != WIDE_EMPTY;
+                    tn[4 * g + j] = use ? t : INFINITY;
+                    cr[4 * g + j] = ra[j];
+                }
+            }
+            if constexpr (N == 4) {
+                cswap(tn[0], cr[0], tn[1], cr[1]);
+                cswap(tn[2], cr[2], tn[3], cr[3]);
+                cswap(tn[0], cr[0], tn[2], cr[2]);
+                cswap(tn[1], cr[1], tn[3], cr[3]);
+                cswap(tn[1], cr[1], tn[2], cr[2]);
+            } else {
+                // Batcher's odd-even merge sort of 8 keys (19 compare-exchanges).
+#define CX_CS(a, b) cswap(tn[a], cr[a], tn[b], cr[b])
+                CX_CS(0, 1); CX_CS(2, 3); CX_CS(4, 5); CX_CS(6, 7);
+                CX_CS(0, 2); CX_CS(1, 3); CX_CS(4, 6); CX_CS(5, 7);
+                CX_CS(1, 2); CX_CS(5, 6);
+                CX_CS(0, 4); CX_CS(1, 5); CX_CS(2, 6); CX_CS(3, 7);
+                CX_CS(2, 4); CX_CS(3, 5);
+                CX_CS(1, 2); CX_CS(3, 4); CX_CS(5, 6);
+#undef CX_CS
+            }
+            if (tn[0] != INFINITY) {
+#pragma unroll
+                for (int i = N - 1; i >= 1; i--) {
+                    if (tn[i] != INFINITY) {
+                        st.put(sp, cr[i]);
+                        ++sp;
+                    }
+                }
+                ref = cr[0];
+                continue;
+            }
+        } else {
+            if (leaf_hits<ANY>(S, ref, o, rp, tmin, tmax, best)) return true;
+        }
+        if (sp == 0) break;
+        --sp;
+        ref = st.get(sp);
+    }
+    return best.prim != INVALID_PRIM;
+}
+
 template <bool ANY, typename Stack>
-__device__ __forceinline__ bool traverse_s(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit& best,
-                                           Stack& st) {
+CX_HD bool traverse_bin(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit& best,
+                                             Stack& st) {
     best.prim = INVALID_PRIM;
     best.t = tmax;
     best.u = 0.0f;
@@ -176,34 +309,33 @@ __device__ __forceinline__ bool traverse_s(const DScene& S, V3 o, V3 d, float tm
             if (h0) { ref = dd.x; continue; }
             if (h1) { ref = dd.y; continue; }
         } else {
-            const uint32_t code = ~(uint32_t)ref;
-            const uint32_t first = code >> 4, cnt = code & 15u;
-            for (uint32_t i = 0; i < cnt; i++) {
-                const float4* tv = S.tris + 3 * (size_t)(first + i);
-                const float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
-                float t, u, v;
-                if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, tmax,
-                             t, u, v))
-                    continue;
-                const uint32_t prim = __builtin_bit_cast(uint32_t, p0.w);
-                if (ANY) {
-                    best.prim = prim;
-                    best.t = t;
-                    return true;
-                }
-                if (t < best.t || (t == best.t && prim < best.prim)) {
-                    best.prim = prim;
-                    best.t = t;
-                    best.u = u;
-                    best.v = v;
-                }
-            }
+            if (leaf_hits<ANY>(S, ref, o, rp, tmin, tmax, best)) return true;
         }
         if (sp == 0) break;
         --sp;
         ref = st.get(sp);
     }
     return best.prim != INVALID_PRIM;
+}
+
+// Traversal at a BVH width fixed at compile time (WIDTH = S.width; the caller dispatches).
+template <bool ANY, int WIDTH, typename Stack>
+CX_HD bool traverse_w(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit& best, Stack& st) {
+    best.prim = INVALID_PRIM;
+    best.t = tmax;
+    best.u = 0.0f;
+    best.v = 0.0f;
+    if (S.numTris == 0) return false;
+    if constexpr (WIDTH == 2) return traverse_bin<ANY>(S, o, d, tmin, tmax, best, st);
+    else return traverse_wide<ANY, WIDTH>(S, o, d, tmin, tmax, best, st);
+}
+
+// Dispatch on the scene's BVH width (uniform across a launch).
+template <bool ANY, typename Stack>
+CX_HD bool traverse_s(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit& best, Stack& st) {
+    if (S.width == 4) return traverse_w<ANY, 4>(S, o, d, tmin, tmax, best, st);
+    if (S.width == 8) return traverse_w<ANY, 8>(S, o, d, tmin, tmax, best, st);
+    return traverse_w<ANY, 2>(S, o, d, tmin, tmax, best, st);
 }
 
 // Traversal with the whole stack in a per-thread LDS column (stack[i * stride]).

@@ -22,6 +22,7 @@ __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + thr
 // Answers every ray of qin: closest-hit rays in [0, nc), shadow rays in [cap - ns, cap).
 // Grid-stride over the device-side counts, so the host never waits for them.  Also clears the
 // counts of qout (the next step kernel's output queue) and adds the rays to the pass counters.
+template <int WIDTH>
 __global__ __launch_bounds__(WBLOCK) void k_q_trace(DScene S, Queue qin, Queue qout, unsigned long long* counters) {
     __shared__ int s_st[SHORT_STACK * WBLOCK];
     const uint32_t nc = qin.count[0], ns = qin.count[1];
@@ -42,11 +43,11 @@ __global__ __launch_bounds__(WBLOCK) void k_q_trace(DScene S, Queue qin, Queue q
         const float4 a = qin.ray[slot], b = qin.ray[(size_t)qin.cap + slot];
         Hit h;
         if (i < nc) {
-            traverse_s<false>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
+            traverse_w<false, WIDTH>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
             const bool hit = h.prim != INVALID_PRIM;
             qin.res[slot] = uint4{h.prim, fb(hit ? h.t : 0.0f), fb(hit ? h.u : 0.0f), fb(hit ? h.v : 0.0f)};
         } else {
-            const bool occ = traverse_s<true>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
+            const bool occ = traverse_w<true, WIDTH>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
             qin.res[slot] = uint4{occ ? 1u : 0u, 0u, 0u, 0u};
         }
     }
@@ -1047,7 +1048,10 @@ hipError_t wave_zero_counts(uint32_t* c, hipStream_t s) {
 
 hipError_t wave_trace(const DScene& S, const Queue& qin, const Queue& qout, unsigned long long* counters,
                       hipStream_t s) {
-    hipLaunchKernelGGL(k_q_trace, stride_grid(qin.cap), dim3(WBLOCK), 0, s, S, qin, qout, counters);
+    const dim3 g = stride_grid(qin.cap), b(WBLOCK);
+    if (S.width == 4) hipLaunchKernelGGL(k_q_trace<4>, g, b, 0, s, S, qin, qout, counters);
+    else if (S.width == 8) hipLaunchKernelGGL(k_q_trace<8>, g, b, 0, s, S, qin, qout, counters);
+    else hipLaunchKernelGGL(k_q_trace<2>, g, b, 0, s, S, qin, qout, counters);
     return hipGetLastError();
 }
 

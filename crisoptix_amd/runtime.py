@@ -167,6 +167,9 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
         L.cx_abi_version.restype = i32
+        if L.cx_abi_version() != abi.ABI_VERSION:
+            raise LibraryNotBuilt(f"{LIB_PATH} has ABI {L.cx_abi_version()}, expected {abi.ABI_VERSION}: rebuild it")
+        L.cx_bvh_host_trace.argtypes = [vp, vp, u32, i32, vp, vp, vp, vp, C.POINTER(u32)]
         L.cx_ctx_create.argtypes = [i32, C.POINTER(vp)]
         L.cx_ctx_destroy.argtypes = [vp]
         L.cx_last_error.restype = C.c_char_p
@@ -199,6 +202,27 @@ def lib():
     return _lib
 
 
+def host_trace(scene, rays: np.ndarray, any_hit: bool = False, width: int = 4, max_leaf: int = 0):
+    """cx_bvh_host_trace: the BVH of `scene` at `width` built and traversed on the CPU with the
+    device traversal code.  Returns (prim, t, u, v, width_used), or (occluded, width_used)."""
+    keep: list = []
+    desc = abi.scene_desc(scene, keep)
+    desc.bvhMaxLeafSize = max_leaf
+    desc.bvhWidth = width
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    prim = np.empty(n, np.uint32)
+    t, u, v = (np.empty(n, np.float32) for _ in range(3))
+    used = C.c_uint32(0)
+    rc = lib().cx_bvh_host_trace(C.byref(desc), rays.ctypes.data, n, 1 if any_hit else 0, prim.ctypes.data,
+                                 t.ctypes.data, u.ctypes.data, v.ctypes.data, C.byref(used))
+    if rc != 0:
+        raise CxError(f"cx_bvh_host_trace failed ({rc})")
+    if any_hit:
+        return prim, used.value
+    return prim, t, u, v, used.value
+
+
 class Context:
     """Replaces InitCuda/InitOptix (src/main.cpp:35-65)."""
 
@@ -222,11 +246,12 @@ class Context:
 class Scene:
     """Device scene: flattened triangles, BVH, materials and explicit emitters."""
 
-    def __init__(self, ctx: Context, scene, max_leaf: int = 0):
+    def __init__(self, ctx: Context, scene, max_leaf: int = 0, width: int = 0):
         self.ctx = ctx
         self._keep: list = []
         desc = abi.scene_desc(scene, self._keep)
         desc.bvhMaxLeafSize = max_leaf
+        desc.bvhWidth = width
         h = C.c_void_p()
         ctx.check(lib().cx_scene_build(ctx.h, C.byref(desc), C.byref(h)), "cx_scene_build")
         self.h = h.value

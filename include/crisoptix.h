@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define CX_ABI_VERSION 1
+#define CX_ABI_VERSION 3
 
 typedef enum cx_status {
     CX_OK = 0,
@@ -138,7 +138,9 @@ typedef struct cx_scene_desc {
     const float* emitterColor;     /* host, [numEmitters][3] */
     uint32_t numEmitters;
     uint32_t bvhMaxLeafSize;       /* 0 = default (4) */
-    uint32_t pad0;
+    uint32_t bvhWidth;             /* children per BVH node: 2, 4 or 8; 0 = default (4).  A wide
+                                      BVH whose worst-case traversal stack would not fit falls
+                                      back to 2 (cx_scene_info.bvhWidth reports the width used) */
 } cx_scene_desc;
 
 typedef struct cx_scene_info {
@@ -152,6 +154,10 @@ typedef struct cx_scene_info {
     float boundsMin[3];
     float boundsMax[3];
     uint64_t deviceBytes;
+    uint32_t bvhWidth;             /* width the traversal uses */
+    uint32_t numWideNodes;
+    uint32_t stackBound;           /* worst-case traversal stack entries */
+    uint32_t pad0;
 } cx_scene_info;
 
 /* Per-pass timing of the last frame, ms (src/Renderer.cpp:396-402), plus ray counts. */
@@ -167,7 +173,11 @@ typedef struct cx_frame_stats {
     uint32_t maxNeighCount;
     uint64_t totalNeighCount;
     uint32_t numEntries;
-    uint32_t pad0;
+    /* Ray traversal launches of the wavefront mode (the dominant kernel): their count in the
+       frame and their summed duration from HIP events around each launch (timing enabled). */
+    uint32_t traceLaunches;
+    float traceMs;
+    uint32_t pad1;
 } cx_frame_stats;
 
 typedef struct cx_ctx cx_ctx;
@@ -183,7 +193,8 @@ int cx_abi_version(void);
 
 /* ---- scene: replaces Mesh::BuildAccel, Model::BuildAccel, Scene::BuildAccel and
  *      Scene::BuildEmitterTable (src/Mesh.cpp:54-135, src/Model.cpp:376-457,
- *      src/Scene.cpp:73-204).  Builds a binary SAH BVH and uploads it. ---- */
+ *      src/Scene.cpp:73-204).  Builds a binary SAH BVH on the host, collapses it to
+ *      desc->bvhWidth children per node and uploads it. ---- */
 int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* desc, cx_scene** out);
 int cx_scene_destroy(cx_scene* scene);
 int cx_scene_get_info(const cx_scene* scene, cx_scene_info* info);
@@ -198,6 +209,15 @@ int cx_trace_closest(cx_ctx* ctx, const cx_scene* scene, const float* d_rays, ui
                      uint32_t* d_prim, float* d_t, float* d_u, float* d_v, void* stream);
 int cx_trace_occluded(cx_ctx* ctx, const cx_scene* scene, const float* d_rays, uint32_t n,
                       uint32_t* d_occluded, void* stream);
+
+/* ---- host traversal over the same node formats, for checking a build on a machine without a
+ *      GPU: builds the BVH of `desc` at `bvhWidth` (as cx_scene_build) on the host and answers
+ *      closest-hit (anyHit = 0: prim, t, u, v) or occlusion (anyHit = 1: occluded in prim)
+ *      queries with the device traversal code compiled for the CPU.  All pointers are host
+ *      memory.  *widthUsed receives the width actually traversed (a wide BVH whose stack
+ *      bound does not fit falls back to 2).  No counterpart in the reference. ---- */
+int cx_bvh_host_trace(const cx_scene_desc* desc, const float* rays, uint32_t n, int anyHit, uint32_t* prim,
+                      float* t, float* u, float* v, uint32_t* widthUsed);
 
 /* ---- prefix-entry search structure: replaces PrefixAccelStruct::Rebuild/BuildGas/
  *      BuildTlas (src/PrefixAccelStruct.cpp:14-221) and the point query of

@@ -1,0 +1,29 @@
+"""The wide (4- and 8-child) BVH collapse and its traversal, checked on the CPU: the device
+traversal code compiled for the host (cx_bvh_host_trace) over binary, 4-wide and 8-wide node
+arrays gives the oracle's closest hits (primitive, t, barycentrics) and occlusion bit for bit.
+Results are a function of the ray and the triangle set alone, so every BVH shape must agree."""
+import numpy as np
+import pytest
+
+from crisoptix_amd import runtime, scenes
+from parity_util import bits, random_rays, shadow_rays, small_grid_scene
+
+pytestmark = pytest.mark.skipif(not __import__("os").path.exists(runtime.LIB_PATH), reason="HIP library not built")
+
+
+@pytest.mark.parametrize("name", ["cornell", "grid3"])
+@pytest.mark.parametrize("width,max_leaf", [(2, 4), (4, 4), (8, 4), (4, 1), (8, 8)])
+def test_host_traversal_matches_oracle(name, width, max_leaf, orc):
+    sc = scenes.cornell_box(1, 8) if name == "cornell" else small_grid_scene()
+    rays = random_rays(sc, 6000, seed=21)
+    po, to, uo, vo = orc.OracleScene(sc).trace_closest(rays)
+    pg, tg, ug, vg, used = runtime.host_trace(sc, rays, width=width, max_leaf=max_leaf)
+    assert used == width
+    assert np.array_equal(po, pg)
+    assert np.array_equal(bits(to), bits(tg))
+    assert np.array_equal(bits(uo), bits(ug))
+    assert np.array_equal(bits(vo), bits(vg))
+    srays = shadow_rays(sc, 6000, seed=22)
+    oo = orc.OracleScene(sc).trace_occluded(srays)
+    og, _ = runtime.host_trace(sc, srays, any_hit=True, width=width, max_leaf=max_leaf)
+    assert np.array_equal(oo.astype(np.uint32), og)

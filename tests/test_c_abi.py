@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     lib = C.CDLL(LIB)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.cx_abi_version() == 1
+    assert lib.cx_abi_version() == abi.ABI_VERSION
 
 
 def test_struct_layouts_match_header(tmp_path):

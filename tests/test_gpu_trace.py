@@ -44,3 +44,21 @@ def test_leaf_size_independence(orc, cx_runtime, cx_ctx):
     b = cx_runtime.Scene(cx_ctx, sc, max_leaf=8).trace_closest(rays)
     for x, y in zip(a, b):
         assert np.array_equal(bits(x), bits(y))
+
+
+@pytest.mark.parametrize("name", ["cornell", "grid3"])
+def test_bvh_width_independence(name, orc, cx_runtime, cx_ctx):
+    """Binary, 4-wide and 8-wide node arrays give the oracle's hits and occlusion bit for bit."""
+    sc = scenes.cornell_box(1, 8) if name == "cornell" else small_grid_scene()
+    rays = random_rays(sc, 30000, seed=14)
+    srays = shadow_rays(sc, 30000, seed=15)
+    oscene = orc.OracleScene(sc)
+    ref = oscene.trace_closest(rays)
+    occ = oscene.trace_occluded(srays)
+    for width in (2, 4, 8):
+        gscene = cx_runtime.Scene(cx_ctx, sc, width=width)
+        assert gscene.info.bvhWidth == width
+        got = gscene.trace_closest(rays)
+        for x, y in zip(ref, got):
+            assert np.array_equal(bits(x), bits(y)), width
+        assert np.array_equal(occ, gscene.trace_occluded(srays)), width
