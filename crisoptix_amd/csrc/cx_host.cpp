@@ -22,6 +22,9 @@ struct cx_ctx {
 struct cx_scene {
     cx_ctx* ctx = nullptr;
     DScene d{};
+    DMaterial* evalMats = nullptr;   // evaluation table of the adjoint (device)
+    float* evalEmit = nullptr;
+    uint32_t numMaterials = 0;
     std::vector<void*> allocs;
     cx_scene_info info{};
 };
@@ -574,9 +577,12 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     s->d.idx = (const uint32_t*)up(d->indices, 3 * (size_t)d->numTriangles, sizeof(uint32_t));
     s->d.triMat = (const uint32_t*)up(d->triMaterial, d->numTriangles, sizeof(uint32_t));
     s->d.mats = (const DMaterial*)up(mats.data(), mats.size(), sizeof(DMaterial));
+    s->evalMats = (DMaterial*)up(mats.data(), mats.size(), sizeof(DMaterial));
+    s->numMaterials = d->numMaterials;
     s->d.emitFaces = (const float*)up(d->emitterFaces, 9 * (size_t)numFaces, sizeof(float));
     s->d.emitOff = (const uint32_t*)up(d->emitterFaceOffset, d->numEmitters + 1, sizeof(uint32_t));
     s->d.emitColor = (const float*)up(d->emitterColor, 3 * (size_t)d->numEmitters, sizeof(float));
+    s->evalEmit = (float*)up(d->emitterColor, 3 * (size_t)d->numEmitters, sizeof(float));
     if (e != hipSuccess) {
         free_all(s->allocs);
         delete s;
@@ -626,6 +632,35 @@ int cx_scene_destroy(cx_scene* s) {
 int cx_scene_get_info(const cx_scene* s, cx_scene_info* info) {
     if (!s || !info) return CX_ERR_INVALID_ARG;
     *info = s->info;
+    return CX_OK;
+}
+
+int cx_scene_set_materials(cx_scene* s, const cx_material* mats, uint32_t n, int tables) {
+    if (!s || !mats) return CX_ERR_INVALID_ARG;
+    if (n != s->numMaterials) return set_err(s->ctx, CX_ERR_INVALID_ARG, "material count differs from the scene's");
+    if (tables & ~(CX_TABLE_SAMPLING | CX_TABLE_EVAL)) return set_err(s->ctx, CX_ERR_INVALID_ARG, "unknown table");
+    std::vector<DMaterial> dm(n);
+    for (uint32_t m = 0; m < n; m++) {
+        const cx_material& cm = mats[m];
+        dm[m].diffRough = make_float4(cm.diffColor[0], cm.diffColor[1], cm.diffColor[2], cm.roughness);
+        dm[m].spec = make_float4(cm.specF0[0], cm.specF0[1], cm.specF0[2], 0.0f);
+        dm[m].emissive = make_float4(cm.emissiveColor[0], cm.emissiveColor[1], cm.emissiveColor[2], 0.0f);
+    }
+    if (tables & CX_TABLE_SAMPLING)
+        CX_HIP(s->ctx, hipMemcpy((void*)s->d.mats, dm.data(), n * sizeof(DMaterial), hipMemcpyHostToDevice));
+    if (tables & CX_TABLE_EVAL)
+        CX_HIP(s->ctx, hipMemcpy(s->evalMats, dm.data(), n * sizeof(DMaterial), hipMemcpyHostToDevice));
+    return CX_OK;
+}
+
+int cx_scene_set_emitter_colors(cx_scene* s, const float* colors, uint32_t n, int tables) {
+    if (!s || !colors) return CX_ERR_INVALID_ARG;
+    if (n != s->d.numEmitters) return set_err(s->ctx, CX_ERR_INVALID_ARG, "emitter count differs from the scene's");
+    if (tables & ~(CX_TABLE_SAMPLING | CX_TABLE_EVAL)) return set_err(s->ctx, CX_ERR_INVALID_ARG, "unknown table");
+    if (tables & CX_TABLE_SAMPLING)
+        CX_HIP(s->ctx, hipMemcpy((void*)s->d.emitColor, colors, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    if (tables & CX_TABLE_EVAL)
+        CX_HIP(s->ctx, hipMemcpy(s->evalEmit, colors, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice));
     return CX_OK;
 }
 
@@ -1038,6 +1073,28 @@ int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* st) {
             }
         }
     }
+    return CX_OK;
+}
+
+int cx_renderer_adjoint(cx_renderer* r, const float* dLdI, double* gAlbedo, double* gRough, double* gEmit,
+                        float* replay, uint32_t* status, void* stream) {
+    if (!r) return CX_ERR_INVALID_ARG;
+    const int type = r->params.rendererType;
+    if (type != CX_RESTIR_PT && type != CX_PATH_TRACER)
+        return set_err(r->ctx, CX_ERR_INVALID_ARG, "the adjoint covers ReSTIR PT and the path tracer");
+    if (!r->out) return set_err(r->ctx, CX_ERR_INVALID_ARG, "no frame rendered yet");
+    if ((gAlbedo || gRough || gEmit) && !dLdI) return set_err(r->ctx, CX_ERR_INVALID_ARG, "gradients need dLdI");
+    FrameArgs A = make_args(r);
+    AdjArgs G;
+    G.dLdI = dLdI;
+    G.gAlbedo = gAlbedo;
+    G.gRough = gRough;
+    G.gEmit = gEmit;
+    G.replay = replay;
+    G.status = status;
+    G.evalMats = r->scene->evalMats;
+    G.evalEmit = r->scene->evalEmit;
+    CX_HIP(r->ctx, launch_adjoint(A, G, (hipStream_t)stream));
     return CX_OK;
 }
 

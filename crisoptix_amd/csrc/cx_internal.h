@@ -26,6 +26,19 @@ hipError_t launch_entries_query(const EntryTree& T, const float* q, uint32_t nq,
 hipError_t launch_trace(const DScene& S, const float* rays, uint32_t n, int anyHit, uint32_t* prim, float* t,
                         float* u, float* v, uint32_t* occ, hipStream_t s);
 
+// Path-replay adjoint (cx_adjoint.hip).  Gradients accumulate into the double arrays.
+struct AdjArgs {
+    const float* dLdI;             // [P][3] dL/d(pixel estimate), or null (replay only)
+    double* gAlbedo;               // [materials][3] or null
+    double* gRough;                // [materials] or null
+    double* gEmit;                 // [emitters][3] or null
+    float* replay;                 // [P][3] the replayed estimate, or null
+    uint32_t* status;              // [3] replayed, replay differs from the record, skipped
+    const DMaterial* evalMats;     // values and derivatives use these materials
+    const float* evalEmit;         // and these emitter colours
+};
+hipError_t launch_adjoint(const FrameArgs& A, const AdjArgs& G, hipStream_t s);
+
 // Wavefront kernels (cx_wave.hip).  Step kernels (*_STEP) run grid-stride over the jobs of
 // W.qin; the others run one thread per pixel (threads = P).
 enum WaveKernel {

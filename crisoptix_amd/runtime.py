@@ -170,6 +170,9 @@ def lib():
         if L.cx_abi_version() != abi.ABI_VERSION:
             raise LibraryNotBuilt(f"{LIB_PATH} has ABI {L.cx_abi_version()}, expected {abi.ABI_VERSION}: rebuild it")
         L.cx_bvh_host_trace.argtypes = [vp, vp, u32, i32, vp, vp, vp, vp, C.POINTER(u32)]
+        L.cx_scene_set_materials.argtypes = [vp, vp, u32, i32]
+        L.cx_scene_set_emitter_colors.argtypes = [vp, vp, u32, i32]
+        L.cx_renderer_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.cx_ctx_create.argtypes = [i32, C.POINTER(vp)]
         L.cx_ctx_destroy.argtypes = [vp]
         L.cx_last_error.restype = C.c_char_p
@@ -263,6 +266,20 @@ class Scene:
     @property
     def size(self) -> float:
         return self.info.sceneSize
+
+    TABLE_SAMPLING, TABLE_EVAL, TABLE_BOTH = 1, 2, 3
+
+    def set_materials(self, materials: np.ndarray, tables: int = 3):
+        """materials: [M, 10] float32 rows of cx_material (diffuse rgb, specF0 rgb, emissive rgb,
+        roughness).  tables: 1 sampling, 2 evaluation, 3 both."""
+        m = np.ascontiguousarray(materials, np.float32).reshape(-1, 10)
+        self.ctx.check(lib().cx_scene_set_materials(self.h, m.ctypes.data, m.shape[0], tables),
+                       "cx_scene_set_materials")
+
+    def set_emitter_colors(self, colors: np.ndarray, tables: int = 3):
+        c = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)
+        self.ctx.check(lib().cx_scene_set_emitter_colors(self.h, c.ctypes.data, c.shape[0], tables),
+                       "cx_scene_set_emitter_colors")
 
     def trace_closest(self, rays: np.ndarray, stream=None):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
@@ -380,6 +397,12 @@ class Renderer:
         self.ctx.check(lib().cx_renderer_frame(self.h, C.byref(cam), 1 if changed else 0, out.ptr, stream),
                        "cx_renderer_frame")
         return out
+
+    def adjoint(self, dLdI=None, g_albedo=None, g_rough=None, g_emit=None, replay=None, status=None, stream=None):
+        """cx_renderer_adjoint over device buffers (DeviceArray or raw device pointers)."""
+        ptr = lambda a: None if a is None else (a if isinstance(a, int) else a.ptr)
+        self.ctx.check(lib().cx_renderer_adjoint(self.h, ptr(dLdI), ptr(g_albedo), ptr(g_rough), ptr(g_emit),
+                                                 ptr(replay), ptr(status), stream), "cx_renderer_adjoint")
 
     def stats(self) -> abi.FrameStats:
         st = abi.FrameStats()

@@ -266,6 +266,24 @@ def object_grid(grid: int = 16, seed: int = 2, ground_quads: int = 200, ico_subd
                    meta={"grid": grid, "spacing": spacing, "half": half})
 
 
+def inverse_scene(seed: int = 4) -> Scene:
+    """The differentiable-rendering scene: the object grid at 8 x 8 objects (icospheres of
+    subdivision 3 and boxes) on a 140 x 140-quad ground, about 100 k triangles, with 64 object
+    materials plus the ground's and 16 emitter triangles."""
+    return object_grid(grid=8, seed=seed, ground_quads=140, ico_subdiv=3, num_lights=16)
+
+
+def perturbed_materials(materials: np.ndarray, seed: int, albedo: float = 0.2, roughness: float = 0.3) -> np.ndarray:
+    """Seeded target parameters for inverse rendering: albedo + U[-albedo, albedo] per channel
+    (clamped to [0.02, 0.98]) and roughness + U[-roughness, roughness] (clamped to [0.05, 1])."""
+    rng = np.random.default_rng(seed)
+    m = np.array(materials, np.float32, copy=True)
+    n = m.shape[0]
+    m[:, 0:3] = np.clip(m[:, 0:3] + rng.uniform(-albedo, albedo, (n, 3)), 0.02, 0.98)
+    m[:, 9] = np.clip(m[:, 9] + rng.uniform(-roughness, roughness, n), 0.05, 1.0)
+    return m.astype(np.float32)
+
+
 @dataclass
 class Camera:
     """Camera (include/graph/Camera.h:25-79, src/Camera.cpp): position, view direction, up,

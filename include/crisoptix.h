@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define CX_ABI_VERSION 3
+#define CX_ABI_VERSION 4
 
 typedef enum cx_status {
     CX_OK = 0,
@@ -300,6 +300,36 @@ typedef struct cx_state_view {
     uint32_t pad0;
 } cx_state_view;
 int cx_renderer_state(cx_renderer* r, cx_state_view* view);
+
+/* ---- differentiable rendering (no counterpart in the reference, which has none) ----
+ * Material and emitter parameters live in two device tables.  The sampling table drives every
+ * direction, hit and discrete decision of the frame loop (and is what cx_scene_build
+ * uploads); the evaluation table gives the BSDF values, emitter colours and derivatives of
+ * cx_renderer_adjoint.  A forward pass sets both; a finite-difference check perturbs only the
+ * evaluation table, which replays the same paths at other parameter values. */
+enum {
+    CX_TABLE_SAMPLING = 1,
+    CX_TABLE_EVAL = 2,
+};
+int cx_scene_set_materials(cx_scene* scene, const cx_material* materials, uint32_t n, int tables);
+/* colors: host, [n][3] emitter radiance (Emitter colour, include/model/Emitter.h:13-55) */
+int cx_scene_set_emitter_colors(cx_scene* scene, const float* colors, uint32_t n, int tables);
+
+/* Path-replay adjoint of the last frame's estimate (ReSTIR PT or the path tracer), with no
+ * tape: each pixel re-traces its reservoir-selected path from the reservoir record (the shift
+ * that produced it, prefix_reuse.h:10-99, and the source's post-reconnection vertices from its
+ * PCG32 state) and accumulates W * dL/dI . df/dtheta with W and the path held fixed (the path
+ * tracer: dL/dI . df/dtheta / p).  Call after cx_renderer_frame and before the next one.
+ *   d_dLdI     device [P][3] float, dL/d(pixel estimate); may be null when no gradient is asked
+ *   d_gAlbedo  device [materials][3] double, accumulated (+=), or null
+ *   d_gRough   device [materials] double, accumulated, or null
+ *   d_gEmit    device [emitters][3] double, accumulated, or null
+ *   d_replay   device [P][3] float, the replayed estimate W * f at the evaluation table, or null
+ *   d_status   device [3] uint32, accumulated: pixels replayed, pixels whose replayed f differs
+ *              in any bit from the record's (zero when the two tables are equal), pixels
+ *              skipped (the replay left the recorded path), or null */
+int cx_renderer_adjoint(cx_renderer* r, const float* d_dLdI, double* d_gAlbedo, double* d_gRough, double* d_gEmit,
+                        float* d_replay, uint32_t* d_status, void* stream);
 
 #ifdef __cplusplus
 }

@@ -147,6 +147,10 @@ struct Scene {
     std::vector<uint32_t> triMat;
     std::vector<Material> mats;
     std::vector<EmitterData> emitters;
+    // Evaluation tables of the FP64 adjoint (values and derivatives); `mats` and the emitters'
+    // colours drive sampling and every decision.
+    std::vector<Material> evalMats;
+    std::vector<V3> evalEmit;
     std::vector<Node> nodes;
     std::vector<uint32_t> triOrder;
     float sceneSize = 0.0f;
@@ -1946,6 +1950,349 @@ static D3 d_path(const Scene& s, const D3& origin, const D3& dir, uint32_t maxLe
     return D3(0.0);
 }
 
+
+// ---------------------------------------------------------------------------------------
+// FP64 path-replay adjoint (checks cx_renderer_adjoint).  The estimator and the replay are
+// those of crisoptix_amd/csrc/cx_adjoint.hip: the gradient of I = W * f(x) with the
+// reservoir-selected path x and its weight W held fixed (ReSTIR PT), or of f(x)/p(x) with p
+// held fixed (path tracer).  Geometry, directions and decisions are replayed in float with
+// the sampling tables, exactly as the frame produced them; every BSDF value, emitter colour,
+// product and derivative is computed in double from those float inputs and the evaluation
+// tables.  The BSDF derivatives are written out from the formula of brdf.cu:99-190 with
+// F0 = albedo (metal = 0) and checked against double-precision central differences.
+// ---------------------------------------------------------------------------------------
+
+namespace {
+
+struct AFactor {
+    D3 v, dA, dR, dC;
+    uint32_t mat = 0, emit = 0xFFFFFFFFu;
+};
+
+struct ADouble {
+    D3 val, dA, dR;
+};
+
+ADouble brdf64(const Material& mf, const V3& nF, const V3& vF, const V3& lF) {
+    const D3 a(mf.diff), n(nF), v(vF), l(lF);
+    const double r = mf.roughness;
+    const double nV = ddot(n, v), nL = ddot(n, l);
+    const D3 h = dnorm(l + v);
+    const double nH = ddot(h, n), lH = ddot(h, l);
+    const double pi = 3.14159265358979323846;
+    ADouble o;
+    D3 spec(0.0), dR(0.0);
+    double dAs = 1.0 / pi;
+    if (ddot(a, a) > 1e-6) {   // F0 = albedo
+        const double a2 = r * r;
+        const double q = nH * nH * (a2 - 1.0) + 1.0;
+        const double D = a2 / (pi * q * q);
+        const double sv = std::sqrt(nV * nV * (1.0 - a2) + a2), sl = std::sqrt(nL * nL * (1.0 - a2) + a2);
+        const double lam = nL * sv + nV * sl;
+        const double V = 0.5 / lam;
+        const double x = std::max(0.0, 1.0 - lH);
+        const double P = x * x * x * x * x;
+        const D3 F = a + (D3(1.0) - a) * P;
+        spec = F * (D * V);
+        dAs += D * V * (1.0 - P);
+        const double dD = (q - 2.0 * a2 * nH * nH) / (pi * q * q * q);
+        const double dlam = nL * (1.0 - nV * nV) / (2.0 * sv) + nV * (1.0 - nL * nL) / (2.0 * sl);
+        const double dV = -0.5 * dlam / (lam * lam);
+        dR = F * (2.0 * r * (dD * V + D * dV));
+    }
+    o.val = (a / pi + spec) * nL;
+    o.dA = D3(dAs * nL);
+    o.dR = dR * nL;
+    return o;
+}
+
+struct AdjOut {
+    double* gA;
+    double* gR;
+    double* gE;
+};
+
+struct AReplay {
+    const Scene& s;
+    const Params& P;
+    std::vector<AFactor> F;
+    bool ok = true;
+};
+
+EmitterSample sample_emitter_idx(const Scene& s, PCG32& rng, uint32_t& ei) {
+    const uint64_t count = s.emitters.size();
+    const uint64_t e = rng.next_u64() % count;
+    const EmitterData& ed = s.emitters[e];
+    const uint64_t faceIdx = rng.next_u64() % (uint64_t)(ed.faces.size() / 3);
+    const V3 v0 = ed.faces[faceIdx * 3 + 0], v1 = ed.faces[faceIdx * 3 + 1], v2 = ed.faces[faceIdx * 3 + 2];
+    float r0 = rng.next_float();
+    float r1 = rng.next_float();
+    float r2 = rng.next_float();
+    const float rSum = r0 + r1 + r2;
+    r0 /= rSum;
+    r1 /= rSum;
+    r2 /= rSum;
+    ei = (uint32_t)e;
+    return {r0 * v0 + r1 * v1 + r2 * v2, ed.color, 1.0f};
+}
+
+// NEE at `it`: factor (brdf * colour) * distFactor in double; brdf/colour/distFactor/emission
+// returned for the path tracer's order of terms.
+bool nee64(AReplay& R, const Inter& it, PCG32& rng, bool requireValue, AFactor& fac, D3& brdf, D3& color,
+           double& distFactor, D3& emis) {
+    uint32_t ei = 0;
+    const EmitterSample es = sample_emitter_idx(R.s, rng, ei);
+    const V3 lightDir = normalize(es.pos - it.pos);
+    const float dist = length(es.pos - it.pos);
+    if (trace_occ(R.s, it.pos, lightDir, 1e-3f, dist)) return false;
+    color = D3(R.s.evalEmit[ei]);
+    const Material& m = R.s.evalMats[R.s.triMat[it.prim]];
+    distFactor = 1.0 / ((double)dist * (double)dist);
+    fac.mat = R.s.triMat[it.prim];
+    fac.emit = ei;
+    const float nDotV = dot(it.normal, -it.inDir), nDotL = dot(it.normal, lightDir);
+    if (nDotV <= 0.0f || nDotL < 0.0f) {
+        if (requireValue) return false;
+        brdf = emis = D3(0.0);
+        fac.v = fac.dA = fac.dR = fac.dC = D3(0.0);
+        return true;
+    }
+    const ADouble b = brdf64(m, it.normal, -it.inDir, lightDir);
+    brdf = b.val;
+    emis = D3(m.emissive);
+    fac.v = b.val * color * distFactor;
+    fac.dA = b.dA * color * distFactor;
+    fac.dR = b.dR * color * distFactor;
+    fac.dC = b.val * distFactor;
+    return true;
+}
+
+AFactor bsdf_factor(AReplay& R, const Inter& it, const V3& outDir) {
+    const ADouble b = brdf64(R.s.evalMats[R.s.triMat[it.prim]], it.normal, -it.inDir, outDir);
+    AFactor f;
+    f.v = b.val;
+    f.dA = b.dA;
+    f.dR = b.dR;
+    f.mat = R.s.triMat[it.prim];
+    return f;
+}
+
+// TracePrefix (ReSTIR PT form) from vertex traceIdx0; see cx_adjoint.hip.
+bool replay_tail64(AReplay& R, Inter it, bool haveIt, V3 pos, V3 dir, uint32_t traceIdx0, PCG32& rng, D3& prod,
+                   D3& emis, bool& neeFirst) {
+    const uint32_t maxLen = (uint32_t)R.P.lp.maxPathLen;
+    prod = D3(1.0);
+    emis = D3(0.0);
+    neeFirst = false;
+    for (uint32_t traceIdx = traceIdx0; traceIdx < maxLen; ++traceIdx) {
+        if (!(haveIt && traceIdx == traceIdx0)) {
+            trace_surface(R.s, pos, dir, 1e-3f, 1e16f, it);
+            if (!it.valid) return false;
+        }
+        const bool forceNee = traceIdx == maxLen - 1;
+        if (rng.next_float() < R.P.lp.neeProb || forceNee) {
+            AFactor nf;
+            D3 b, c, e;
+            double df;
+            if (!nee64(R, it, rng, true, nf, b, c, df, e)) return false;
+            prod = prod * nf.v;
+            R.F.push_back(nf);
+            if (traceIdx == 0) {
+                emis = e * df;
+                neeFirst = true;
+            }
+            return true;
+        }
+        const BrdfSample bs = ggx_sample(R.s, it, rng);
+        if (bs.samplingPdf <= 0.0f) return false;
+        const AFactor f = bsdf_factor(R, it, bs.outDir);
+        prod = prod * f.v;
+        R.F.push_back(f);
+        pos = it.pos;
+        dir = bs.outDir;
+    }
+    return false;
+}
+
+Inter traced_inter(const Scene& s, const Packed& pk) {
+    Inter r = rehydrate(s, pk);
+    if (r.valid) r.normal = normalize(r.normal);
+    return r;
+}
+
+bool replay_post_recon64(AReplay& R, const Prefix& src, D3& prc) {
+    const uint32_t r = src.reconIdx(), L = src.length();
+    PCG32 rng = src.rng;
+    for (uint32_t i = 0; i < 4u * (r - 1u); i++) rng.next_u32();
+    const Inter y = traced_inter(R.s, src.reconInt);
+    if (!y.valid) return false;
+    if (L == r) {
+        rng.next_float();
+        AFactor nf;
+        D3 b, c, e;
+        double df;
+        if (!nee64(R, y, rng, true, nf, b, c, df, e)) return false;
+        prc = nf.v;
+        R.F.push_back(nf);
+        return true;
+    }
+    rng.next_float();
+    const BrdfSample bs = ggx_sample(R.s, y, rng);
+    if (bs.samplingPdf <= 0.0f) return false;
+    const AFactor yf = bsdf_factor(R, y, bs.outDir);
+    R.F.push_back(yf);
+    D3 tail, emis;
+    bool first;
+    Inter none;
+    if (!replay_tail64(R, none, false, y.pos, bs.outDir, r, rng, tail, emis, first)) return false;
+    prc = yf.v * tail;
+    return true;
+}
+
+bool replay_shift64(AReplay& R, const Prefix& src, D3& f, D3& mask) {
+    if (!src.valid() || src.reconIdx() < 2) return false;
+    Inter curr = rehydrate(R.s, src.primaryInt);
+    Inter srcRecon = rehydrate(R.s, src.reconInt);
+    if (!curr.valid || !srcRecon.valid) return false;
+    const int rvc = (int)src.reconIdx() - 2;
+    PCG32 srng = src.rng;
+    D3 thr(1.0);
+    for (int i = 0; i < rvc; ++i) {
+        srng.next_float();
+        const BrdfSample bs = ggx_sample(R.s, curr, srng);
+        if (bs.samplingPdf <= 0.0f) return false;
+        const AFactor fb_ = bsdf_factor(R, curr, bs.outDir);
+        thr = thr * fb_.v;
+        R.F.push_back(fb_);
+        trace_surface(R.s, curr.pos, bs.outDir, 1e-3f, 1e16f, curr);
+        if (!curr.valid) return false;
+    }
+    const V3 reconVec = srcRecon.pos - curr.pos;
+    const float reconLen = length(reconVec);
+    const V3 reconDir = normalize(reconVec);
+    if (std::isinf(reconLen) || any_nan_inf(reconDir)) return false;
+    if (trace_occ(R.s, curr.pos, reconDir, 1e-3f, reconLen)) return false;
+    if (dot(curr.normal, -curr.inDir) <= 0.0f || dot(curr.normal, reconDir) < 0.0f) return false;
+    const AFactor b1 = bsdf_factor(R, curr, reconDir);
+    thr = thr * b1.v;
+    R.F.push_back(b1);
+    srcRecon.inDir = reconDir;
+    if (dot(srcRecon.normal, -srcRecon.inDir) <= 0.0f || dot(srcRecon.normal, src.reconOutDir) < 0.0f) return false;
+    const AFactor b2 = bsdf_factor(R, srcRecon, src.reconOutDir);
+    thr = thr * b2.v;
+    R.F.push_back(b2);
+    D3 prc;
+    if (!replay_post_recon64(R, src, prc)) return false;
+    const D3 x = thr * prc;
+    f = D3(std::max(0.0, x.x), std::max(0.0, x.y), std::max(0.0, x.z));
+    mask = D3(x.x > 0 ? 1.0 : 0.0, x.y > 0 ? 1.0 : 0.0, x.z > 0 ? 1.0 : 0.0);
+    return true;
+}
+
+void contract64(const std::vector<AFactor>& F, const D3& u, std::vector<double>& gA, std::vector<double>& gR,
+                std::vector<double>& gE) {
+    const size_t n = F.size();
+    std::vector<D3> suf(n + 1, D3(1.0));
+    for (size_t k = n; k-- > 0;) suf[k] = F[k].v * suf[k + 1];
+    D3 pre(1.0);
+    for (size_t k = 0; k < n; ++k) {
+        const AFactor& x = F[k];
+        const D3 w = u * (pre * suf[k + 1]);
+        const D3 ga = w * x.dA;
+        gA[3 * x.mat + 0] += ga.x;
+        gA[3 * x.mat + 1] += ga.y;
+        gA[3 * x.mat + 2] += ga.z;
+        gR[x.mat] += ddot(w, x.dR);
+        if (x.emit != 0xFFFFFFFFu) {
+            const D3 ge = w * x.dC;
+            gE[3 * x.emit + 0] += ge.x;
+            gE[3 * x.emit + 1] += ge.y;
+            gE[3 * x.emit + 2] += ge.z;
+        }
+        pre = pre * x.v;
+    }
+}
+
+// One pixel; returns the replayed estimate and accumulates into the thread's gradients.
+// status: 0 replayed, 2 skipped, -1 nothing to replay.
+int adjoint_pixel64(const Renderer& R, uint32_t p, const D3& g, D3& out, std::vector<double>& gA,
+                    std::vector<double>& gR, std::vector<double>& gE) {
+    const Scene& s = *R.scene;
+    AReplay A{s, R.P, {}};
+    out = D3(0.0);
+    if (R.P.lp.rendererType == CX_RESTIR_PT) {
+        const Reservoir<Prefix>& res = R.prefixRes[2 * p + R.P.front];
+        if (!R.gbuf[p].primaryIntValid || !res.sample.valid() || res.wSum == 0.0f) return -1;
+        PCG32 canon(sample_tea64(p, R.P.lp.random));
+        canon.next2d();
+        D3 f(0.0), mask(1.0);
+        bool ok;
+        if (res.sample.rng.state == canon.state) {
+            PCG32 rng = res.sample.rng;
+            D3 prod, emis;
+            bool neeFirst = false;
+            const Inter prim = traced_inter(s, res.sample.primaryInt);
+            ok = prim.valid && replay_tail64(A, prim, true, V3(0.0f), V3(0.0f), 0u, rng, prod, emis, neeFirst);
+            f = (neeFirst && res.sample.length() == 1) ? prod + emis : prod;
+        } else {
+            ok = replay_shift64(A, res.sample, f, mask);
+        }
+        if (!ok) return 2;
+        const double W = res.wSum;
+        out = f * W;
+        contract64(A.F, g * mask * W, gA, gR, gE);
+        return 0;
+    }
+    // path tracer
+    const uint32_t x = p % R.W, y = p / R.W;
+    PCG32 rng(sample_tea64(p, R.P.lp.random));
+    V3 pos, dir;
+    camera_ray(R, x, y, rng, pos, dir);
+    const double neeProb = R.P.lp.neeProb;
+    D3 thr(1.0), radiance(0.0);
+    uint32_t depth = 0;
+    bool contributes = false;
+    for (uint32_t traceIdx = 0; traceIdx < (uint32_t)R.P.lp.maxPathLen; ++traceIdx) {
+        Inter it;
+        trace_surface(s, pos, dir, 1e-3f, 1e16f, it);
+        if (!it.valid) break;
+        ++depth;
+        if (rng.next_float() < R.P.lp.neeProb) {
+            thr = thr / neeProb;
+            AFactor nf;
+            D3 b, c, e;
+            double df;
+            if (!nee64(A, it, rng, false, nf, b, c, df, e)) break;
+            radiance = thr * b * c * df;
+            if (depth == 1) radiance = radiance + e;
+            nf.v = nf.v / neeProb;
+            nf.dA = nf.dA / neeProb;
+            nf.dR = nf.dR / neeProb;
+            nf.dC = nf.dC / neeProb;
+            A.F.push_back(nf);
+            contributes = true;
+            break;
+        }
+        const BrdfSample bs = ggx_sample(s, it, rng);
+        if (bs.samplingPdf <= 0.0f) break;
+        AFactor f = bsdf_factor(A, it, bs.outDir);
+        const double k = 1.0 / ((double)bs.samplingPdf * (1.0 - neeProb));
+        f.v = f.v * k;
+        f.dA = f.dA * k;
+        f.dR = f.dR * k;
+        thr = thr * f.v;
+        A.F.push_back(f);
+        pos = it.pos;
+        dir = bs.outDir;
+    }
+    if (!contributes) return -1;
+    out = radiance;
+    contract64(A.F, g, gA, gR, gE);
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 int orc_abi_version(void) { return 1; }
@@ -2056,6 +2403,8 @@ void* orc_scene_create(const cx_scene_desc* d, int bruteForce) {
     }
     s->sceneSize = distance(mn, mx);
     s->bmin = mn; s->bmax = mx;
+    s->evalMats = s->mats;
+    for (const EmitterData& e : s->emitters) s->evalEmit.push_back(e.color);
     s->bruteForce = bruteForce != 0;
     build_bvh(*s);
     return s;
@@ -2336,6 +2685,75 @@ void orc_entries_query(const float* pos, const float* r, const uint8_t* valid, u
             idx[(size_t)j * k + i] = i < c ? nb[i].pixelIdx : 0xFFFFFFFFu;
             dist[(size_t)j * k + i] = i < c ? nb[i].distance : 0.0f;
         }
+    }
+}
+
+
+// FP64 path-replay adjoint of the last frame (see above).  dLdI: [P][3] (may be null: replay
+// only); gA [M][3], gR [M], gE [E][3] are overwritten (null to skip); replay [P][3] (may be
+// null); status[3]: pixels replayed, -, pixels skipped.
+void orc_renderer_adjoint(void* rp, const float* dLdI, double* gA, double* gR, double* gE, double* replay,
+                          uint32_t* status) {
+    const Renderer& R = *(const Renderer*)rp;
+    const Scene& s = *R.scene;
+    const size_t M = s.mats.size(), E = s.emitters.size();
+    const int64_t n = R.pix();
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    std::vector<std::vector<double>> tA(nt, std::vector<double>(3 * M, 0.0)), tR(nt, std::vector<double>(M, 0.0)),
+        tE(nt, std::vector<double>(3 * E, 0.0));
+    std::vector<uint32_t> st(3 * (size_t)nt, 0u);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t pi = 0; pi < n; pi++) {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        const D3 g = dLdI ? D3(dLdI[3 * pi], dLdI[3 * pi + 1], dLdI[3 * pi + 2]) : D3(0.0);
+        D3 out;
+        const int rc = adjoint_pixel64(R, (uint32_t)pi, g, out, tA[t], tR[t], tE[t]);
+        if (rc == 0) st[3 * t + 0]++;
+        if (rc == 2) st[3 * t + 2]++;
+        if (replay) {
+            replay[3 * pi + 0] = out.x;
+            replay[3 * pi + 1] = out.y;
+            replay[3 * pi + 2] = out.z;
+        }
+    }
+    for (int t = 0; t < nt; t++) {
+        for (size_t i = 0; i < 3 * M; i++)
+            if (gA) gA[i] = (t ? gA[i] : 0.0) + tA[t][i];
+        for (size_t i = 0; i < M; i++)
+            if (gR) gR[i] = (t ? gR[i] : 0.0) + tR[t][i];
+        for (size_t i = 0; i < 3 * E; i++)
+            if (gE) gE[i] = (t ? gE[i] : 0.0) + tE[t][i];
+        if (status)
+            for (int k = 0; k < 3; k++) status[k] = (t ? status[k] : 0u) + st[3 * t + k];
+    }
+}
+
+// Evaluation (2) and sampling (1) tables of the oracle scene.
+void orc_scene_set_materials(void* sp, const cx_material* mats, uint32_t n, int tables) {
+    Scene& s = *(Scene*)sp;
+    for (uint32_t m = 0; m < n && m < s.mats.size(); m++) {
+        Material x;
+        x.diff = V3(mats[m].diffColor[0], mats[m].diffColor[1], mats[m].diffColor[2]);
+        x.spec = V3(mats[m].specF0[0], mats[m].specF0[1], mats[m].specF0[2]);
+        x.emissive = V3(mats[m].emissiveColor[0], mats[m].emissiveColor[1], mats[m].emissiveColor[2]);
+        x.roughness = mats[m].roughness;
+        if (tables & 1) s.mats[m] = x;
+        if (tables & 2) s.evalMats[m] = x;
+    }
+}
+
+void orc_scene_set_emitter_colors(void* sp, const float* colors, uint32_t n, int tables) {
+    Scene& s = *(Scene*)sp;
+    for (uint32_t e = 0; e < n && e < s.emitters.size(); e++) {
+        const V3 c(colors[3 * e], colors[3 * e + 1], colors[3 * e + 2]);
+        if (tables & 1) s.emitters[e].color = c;
+        if (tables & 2) s.evalEmit[e] = c;
     }
 }
 

@@ -87,6 +87,9 @@ def lib():
         L.orc_dump_neighbors.argtypes = [vp, vp, vp, vp]
         L.orc_entries_query.argtypes = [vp, vp, vp, u32, vp, u32, u32, i32, vp, vp, vp]
         L.orc_truth_render.argtypes = [vp, C.POINTER(abi.CameraData), u32, u32, u32, C.c_double, u32, u64, vp, vp]
+        L.orc_renderer_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.orc_scene_set_materials.argtypes = [vp, vp, u32, i32]
+        L.orc_scene_set_emitter_colors.argtypes = [vp, vp, u32, i32]
         L.orc_num_threads.restype = i32
         L.orc_set_num_threads.argtypes = [i32]
         _lib = L
@@ -191,6 +194,15 @@ class OracleScene:
         out = np.zeros((n, 8), np.float32)
         lib().orc_ggx_sample(self.h, _p(inter), _p(st), n, _p(out))
         return out, st
+
+    def set_materials(self, materials: np.ndarray, tables: int = 3):
+        """[M, 10] cx_material rows; tables: 1 sampling, 2 evaluation (the FP64 adjoint's), 3 both."""
+        m = np.ascontiguousarray(materials, np.float32).reshape(-1, 10)
+        lib().orc_scene_set_materials(self.h, _p(m), m.shape[0], tables)
+
+    def set_emitter_colors(self, colors: np.ndarray, tables: int = 3):
+        c = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)
+        lib().orc_scene_set_emitter_colors(self.h, _p(c), c.shape[0], tables)
 
     def sample_emitter(self, rng_state: np.ndarray):
         st = np.ascontiguousarray(rng_state, np.uint64).copy()
@@ -326,6 +338,18 @@ class OracleRenderer:
         val = np.zeros(P, np.uint8)
         lib().orc_dump_entries(self.h, _p(pos), _p(rad), _p(val))
         return pos, rad, val
+
+    def adjoint(self, dLdI=None):
+        """FP64 path-replay adjoint of the last frame: (g_albedo [M,3], g_rough [M], g_emit [E,3],
+        replayed estimate [H,W,3] float64, status [replayed, -, skipped])."""
+        sc = self.oscene.scene
+        M, E = sc.materials.shape[0], sc.emitter_color.shape[0]
+        gA, gR, gE = np.zeros((M, 3)), np.zeros(M), np.zeros((E, 3))
+        rep = np.zeros((self.H, self.W, 3))
+        st = np.zeros(3, np.uint32)
+        g = None if dLdI is None else np.ascontiguousarray(dLdI, np.float32)
+        lib().orc_renderer_adjoint(self.h, None if g is None else _p(g), _p(gA), _p(gR), _p(gE), _p(rep), _p(st))
+        return gA, gR, gE, rep, st
 
     def neighbors(self):
         P = self.W * self.H
