@@ -1,0 +1,215 @@
+"""Wavefront OBJ / MTL scenes, the model format the reference loads through assimp
+(src/Model.cpp:9-374, e.g. data/model/Cornell/CornellBox-Original.obj in src/main.cpp:176).
+
+Geometry: `v`, `vn`, `f` (polygons fan-triangulated like aiProcess_Triangulate; `v/vt/vn`,
+`v//vn`, `v/vt` and negative, relative indices), `usemtl`, `mtllib`.  Faces without normals get
+their flat face normal (aiProcess_GenNormals).  `vt` and `map_*` are read and ignored: textures
+are not supported (a material with a texture map is loaded with its constant colours).
+
+Materials (Model::LoadMaterials, src/Model.cpp:199-290): diffuse `Kd`, specular `Ks` (the
+specF0 slot), emissive `Ke`, roughness `Pr` (AI_MATKEY_ROUGHNESS_FACTOR; 1 when absent, as the
+reference's default).  A material the file names but does not define, or faces before any
+`usemtl`, get assimp's OBJ default material (diffuse 0.6).
+
+Emitters (Scene::BuildEmitterTable, src/Scene.cpp:176-204): every group of faces whose material
+has |Ke| > 0.01 becomes an implicit emitter over those faces with colour (1, 1, 1) (the reference
+forces implicit emitters to unit colour, Scene.cpp:194); `emitter_color` overrides that.  The
+reference also traces those faces as geometry; `trace_emitters=False` keeps them out of the
+traced triangles, which the benchmark scenes do with their explicit emitters (a shadow ray
+aimed at a point of a traced emitter would otherwise report its own emitter as the occluder
+whenever t rounds below the distance).
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from .scenes import Scene
+
+DEFAULT_MATERIAL = {"Kd": (0.6, 0.6, 0.6), "Ks": (0.0, 0.0, 0.0), "Ke": (0.0, 0.0, 0.0), "Pr": 1.0}
+
+
+def parse_mtl(text: str) -> dict:
+    mats: dict = {}
+    cur = None
+    for line in text.splitlines():
+        t = line.split("#", 1)[0].split()
+        if not t:
+            continue
+        key, args = t[0], t[1:]
+        if key == "newmtl":
+            cur = " ".join(args)
+            mats[cur] = dict(DEFAULT_MATERIAL)
+        elif cur is None:
+            continue
+        elif key in ("Kd", "Ks", "Ke") and len(args) >= 3:
+            mats[cur][key] = tuple(float(x) for x in args[:3])
+        elif key == "Pr" and args:
+            mats[cur]["Pr"] = float(args[0])
+        elif key.startswith("map_"):
+            mats[cur].setdefault("textures", []).append(args[-1] if args else "")
+    return mats
+
+
+def parse_obj(text: str, mtl_loader=None):
+    """Returns (positions [V,3], normals [V,3], triangles [T,3], material name per triangle,
+    material table).  Vertices are emitted per face corner (no sharing), so flat normals and
+    the file's per-corner normals are both kept."""
+    vs, vns = [], []
+    pos, nrm, tri_mat = [], [], []
+    mats: dict = {}
+    cur = None
+
+    def idx(tok: str, n: int) -> int:
+        i = int(tok)
+        return i - 1 if i > 0 else n + i
+
+    for line in text.splitlines():
+        t = line.split("#", 1)[0].split()
+        if not t:
+            continue
+        key, args = t[0], t[1:]
+        if key == "v":
+            vs.append([float(x) for x in args[:3]])
+        elif key == "vn":
+            vns.append([float(x) for x in args[:3]])
+        elif key == "mtllib" and mtl_loader is not None:
+            for name in args:
+                mats.update(parse_mtl(mtl_loader(name)))
+        elif key == "usemtl":
+            cur = " ".join(args)
+        elif key == "f":
+            corners = []
+            for c in args:
+                parts = c.split("/")
+                vi = idx(parts[0], len(vs))
+                ni = idx(parts[2], len(vns)) if len(parts) > 2 and parts[2] else None
+                corners.append((vi, ni))
+            for k in range(1, len(corners) - 1):   # fan triangulation
+                face = [corners[0], corners[k], corners[k + 1]]
+                p = np.asarray([vs[c[0]] for c in face], np.float64)
+                if any(c[1] is None for c in face):
+                    n = np.cross(p[1] - p[0], p[2] - p[0])
+                    ln = np.linalg.norm(n)
+                    n = n / ln if ln > 0 else np.asarray([0.0, 1.0, 0.0])
+                    ns = [n, n, n]
+                else:
+                    ns = [np.asarray(vns[c[1]], np.float64) for c in face]
+                pos.extend(p)
+                nrm.extend(ns)
+                tri_mat.append(cur)
+    P = np.asarray(pos, np.float32).reshape(-1, 3)
+    N = np.asarray(nrm, np.float32).reshape(-1, 3)
+    T = np.arange(len(P), dtype=np.uint32).reshape(-1, 3)
+    return P, N, T, tri_mat, mats
+
+
+def load_obj(path: str, trace_emitters: bool = True, emitter_color=(1.0, 1.0, 1.0)) -> Scene:
+    """An OBJ file (with its MTL libraries next to it) as a flattened scene (one instance)."""
+    base = os.path.dirname(os.path.abspath(path))
+
+    def loader(name):
+        with open(os.path.join(base, name)) as f:
+            return f.read()
+
+    with open(path) as f:
+        text = f.read()
+    return scene_from_obj(text, loader, trace_emitters, emitter_color, name=os.path.basename(path))
+
+
+def scene_from_obj(text: str, mtl_loader=None, trace_emitters: bool = True, emitter_color=(1.0, 1.0, 1.0),
+                   name: str = "obj") -> Scene:
+    P, N, T, tri_mat, mats = parse_obj(text, mtl_loader)
+    names = []
+    for m in tri_mat:
+        key = m if m in mats else None
+        if key not in names:
+            names.append(key)
+    table = []
+    for key in names:
+        m = mats.get(key, DEFAULT_MATERIAL) if key is not None else DEFAULT_MATERIAL
+        table.append([*m["Kd"], *m["Ks"], *m["Ke"], m["Pr"]])
+    mat_of = np.asarray([names.index(m if m in mats else None) for m in tri_mat], np.uint32)
+    materials = np.asarray(table, np.float32).reshape(-1, 10)
+    emissive = np.linalg.norm(materials[:, 6:9], axis=1) > 0.01
+    faces, offs, cols = [], [0], []
+    for mi in np.flatnonzero(emissive):
+        tris = T[mat_of == mi]
+        faces.append(P[tris])
+        offs.append(offs[-1] + len(tris))
+        cols.append(emitter_color)
+    keep = np.ones(len(T), bool) if trace_emitters else ~emissive[mat_of]
+    if not faces:
+        raise ValueError("the scene has no emissive material (the frame loop needs an emitter)")
+    T2 = T[keep]
+    used = np.unique(T2.reshape(-1))
+    remap = np.full(len(P), 0, np.uint32)
+    remap[used] = np.arange(len(used), dtype=np.uint32)
+    return Scene(
+        positions=P[used], normals=N[used], indices=remap[T2], tri_material=mat_of[keep],
+        materials=materials, emitter_faces=np.concatenate(faces).astype(np.float32),
+        emitter_face_offset=np.asarray(offs, np.uint32),
+        emitter_color=np.asarray(cols, np.float32).reshape(-1, 3), name=name,
+        meta={"half": float(np.abs(P).max()) if len(P) else 1.0})
+
+
+def cornell_box_obj() -> tuple[str, str]:
+    """An OBJ + MTL pair of a Cornell box in the layout of the classic one (white floor,
+    ceiling and back wall, red left wall, green right wall, two white blocks, a ceiling light
+    quad), written here from its published room and block dimensions: the test asset for the
+    loader, which the reference's own CornellBox-Original.obj is not (it is not in its tree)."""
+    mtl = """# Cornell box materials
+newmtl white
+Kd 0.725 0.71 0.68
+Ks 0 0 0
+newmtl red
+Kd 0.63 0.065 0.05
+newmtl green
+Kd 0.14 0.45 0.091
+newmtl light
+Kd 0.78 0.78 0.78
+Ke 17 12 4
+Pr 1.0
+"""
+    v = []
+    f = []
+
+    def quad(a, b, c, d, mat):
+        base = len(v)
+        v.extend([a, b, c, d])
+        f.append((mat, [base + 1, base + 2, base + 3, base + 4]))
+
+    S = 5.56
+    quad((S, 0, 0), (0, 0, 0), (0, 0, S), (S, 0, S), "white")          # floor (normal +y)
+    quad((S, S, 0), (S, S, S), (0, S, S), (0, S, 0), "white")          # ceiling (normal -y)
+    quad((S, 0, S), (0, 0, S), (0, S, S), (S, S, S), "white")          # back wall (normal -z)
+    quad((S, 0, 0), (S, 0, S), (S, S, S), (S, S, 0), "red")            # left wall (normal -x)
+    quad((0, 0, S), (0, 0, 0), (0, S, 0), (0, S, S), "green")          # right wall (normal +x)
+    quad((3.43, S - 0.001, 2.27), (3.43, S - 0.001, 3.32), (2.13, S - 0.001, 3.32), (2.13, S - 0.001, 2.27),
+         "light")
+
+    def block(x0, z0, w, d, h, ang):
+        c, s = np.cos(ang), np.sin(ang)
+        cx, cz = x0 + w / 2, z0 + d / 2
+        pts = []
+        for (dx, dz) in ((-w / 2, -d / 2), (w / 2, -d / 2), (w / 2, d / 2), (-w / 2, d / 2)):
+            pts.append((cx + c * dx - s * dz, cz + s * dx + c * dz))
+        lo = [(p[0], 0.0, p[1]) for p in pts]
+        hi = [(p[0], h, p[1]) for p in pts]
+        quad(hi[0], hi[3], hi[2], hi[1], "white")
+        for i in range(4):
+            j = (i + 1) % 4
+            quad(lo[i], hi[i], hi[j], lo[j], "white")
+
+    block(1.3, 0.65, 1.6, 1.6, 1.65, -0.29)
+    block(2.65, 2.96, 1.6, 1.6, 3.3, 0.29)
+    lines = ["# Cornell box", "mtllib cornell.mtl"]
+    lines += [f"v {a:.4f} {b:.4f} {c:.4f}" for a, b, c in v]
+    cur = None
+    for mat, idx in f:
+        if mat != cur:
+            lines.append(f"usemtl {mat}")
+            cur = mat
+        lines.append("f " + " ".join(str(i) for i in idx))
+    return "\n".join(lines) + "\n", mtl

@@ -635,6 +635,12 @@ int cx_scene_get_info(const cx_scene* s, cx_scene_info* info) {
     return CX_OK;
 }
 
+int cx_tonemap(cx_ctx* ctx, const float* hdr, uint8_t* ldr, uint32_t n, void* stream) {
+    if (!ctx || (n && (!hdr || !ldr))) return set_err(ctx, CX_ERR_INVALID_ARG, "null argument");
+    CX_HIP(ctx, launch_tonemap(hdr, ldr, n, (hipStream_t)stream));
+    return CX_OK;
+}
+
 int cx_scene_set_materials(cx_scene* s, const cx_material* mats, uint32_t n, int tables) {
     if (!s || !mats) return CX_ERR_INVALID_ARG;
     if (n != s->numMaterials) return set_err(s->ctx, CX_ERR_INVALID_ARG, "material count differs from the scene's");

@@ -38,6 +38,7 @@ struct AdjArgs {
     const float* evalEmit;         // and these emitter colours
 };
 hipError_t launch_adjoint(const FrameArgs& A, const AdjArgs& G, hipStream_t s);
+hipError_t launch_tonemap(const float* hdr, uint8_t* ldr, uint32_t n, hipStream_t s);
 
 // Wavefront kernels (cx_wave.hip).  Step kernels (*_STEP) run grid-stride over the jobs of
 // W.qin; the others run one thread per pixel (threads = P).

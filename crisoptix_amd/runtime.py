@@ -173,6 +173,7 @@ def lib():
         L.cx_scene_set_materials.argtypes = [vp, vp, u32, i32]
         L.cx_scene_set_emitter_colors.argtypes = [vp, vp, u32, i32]
         L.cx_renderer_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        L.cx_tonemap.argtypes = [vp, vp, vp, u32, vp]
         L.cx_ctx_create.argtypes = [i32, C.POINTER(vp)]
         L.cx_ctx_destroy.argtypes = [vp]
         L.cx_last_error.restype = C.c_char_p
@@ -224,6 +225,14 @@ def host_trace(scene, rays: np.ndarray, any_hit: bool = False, width: int = 4, m
     if any_hit:
         return prim, used.value
     return prim, t, u, v, used.value
+
+
+def tonemap(ctx: "Context", hdr: "DeviceArray", stream=None) -> np.ndarray:
+    """cx_tonemap of a float image on the device; returns the sRGB bytes (same shape)."""
+    ldr = DeviceArray(hdr.shape, np.uint8)
+    n = int(np.prod(hdr.shape))
+    ctx.check(lib().cx_tonemap(ctx.h, hdr.ptr, ldr.ptr, n, stream), "cx_tonemap")
+    return ldr.numpy()
 
 
 class Context:

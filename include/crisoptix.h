@@ -301,6 +301,11 @@ typedef struct cx_state_view {
 } cx_state_view;
 int cx_renderer_state(cx_renderer* r, cx_state_view* view);
 
+/* ---- image output: ToneMapping (src/kernel/tonemap.cu:7-39).  d_hdr: device [n] float
+ *      linear values (n = 3 * pixels), d_ldr: device [n] bytes, sRGB-encoded, clamped to
+ *      [0, 1] and truncated (LinearToSrgb); NaN encodes as 0. ---- */
+int cx_tonemap(cx_ctx* ctx, const float* d_hdr, uint8_t* d_ldr, uint32_t n, void* stream);
+
 /* ---- differentiable rendering (no counterpart in the reference, which has none) ----
  * Material and emitter parameters live in two device tables.  The sampling table drives every
  * direction, hit and discrete decision of the frame loop (and is what cx_scene_build

@@ -2757,4 +2757,24 @@ void orc_scene_set_emitter_colors(void* sp, const float* colors, uint32_t n, int
     }
 }
 
+
+// LinearToSrgb / ApplySrgbGamma (src/kernel/tonemap.cu:7-24): the sRGB curve selected as
+// glm::lerp does, glm's clamp to [0, 1], x 255, truncation to a byte (NaN -> 0).  The power is
+// evaluated in double and rounded once to float.
+void orc_tonemap(const float* hdr, uint8_t* ldr, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) {
+        const float c = hdr[i];
+        const bool cond = c < 0.0031308f;
+        const float ifTrue = 12.92f * c;
+        const float p = (float)std::pow((double)c, (double)(1.0f / 2.4f));
+        const float ifFalse = 1.055f * p - 0.055f;
+        const float t = cond ? 1.0f : 0.0f;
+        float v = ifFalse * (1.0f - t) + ifTrue * t;
+        v = (v < 0.0f) ? 0.0f : v;
+        v = (1.0f < v) ? 1.0f : v;
+        v = v * 255.0f;
+        ldr[i] = std::isnan(v) ? (uint8_t)0 : (uint8_t)v;
+    }
+}
+
 }  // extern "C"

@@ -90,6 +90,7 @@ def lib():
         L.orc_renderer_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_scene_set_materials.argtypes = [vp, vp, u32, i32]
         L.orc_scene_set_emitter_colors.argtypes = [vp, vp, u32, i32]
+        L.orc_tonemap.argtypes = [vp, vp, u32]
         L.orc_num_threads.restype = i32
         L.orc_set_num_threads.argtypes = [i32]
         _lib = L
@@ -106,6 +107,13 @@ def num_threads() -> int:
 
 def set_num_threads(n: int):
     lib().orc_set_num_threads(n)
+
+
+def tonemap(hdr: np.ndarray) -> np.ndarray:
+    h = np.ascontiguousarray(hdr, np.float32)
+    out = np.zeros(h.shape, np.uint8)
+    lib().orc_tonemap(_p(h), _p(out), h.size)
+    return out
 
 
 def tea32(a, b, rounds=4):
