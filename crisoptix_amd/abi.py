@@ -53,7 +53,7 @@ class Material(C.Structure):
                 ("roughness", C.c_float)]
 
 
-ABI_VERSION = 4   # CX_ABI_VERSION of include/crisoptix.h
+ABI_VERSION = 5   # CX_ABI_VERSION of include/crisoptix.h
 
 
 class SceneDesc(C.Structure):

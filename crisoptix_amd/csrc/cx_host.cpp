@@ -17,6 +17,7 @@ using namespace cx;
 struct cx_ctx {
     int device = 0;
     std::string err;
+    std::vector<hipStream_t> pool;   // streams of cx_batch_frame, created on first use
 };
 
 struct cx_scene {
@@ -97,6 +98,7 @@ struct cx_renderer {
     // trace/step rounds a machine can need (records of earlier frames take part in shifts).
     uint32_t maxLenSeen = 0;
     uint32_t maxPrefixSeen = 0;
+    uint32_t gridLimit = 4096;   // blocks of the grid-stride queue kernels
 };
 
 namespace {
@@ -351,6 +353,7 @@ Queue queue_view(const cx_renderer* r, int side, uint32_t cap) {
     q.res = r->wave.res[side];
     q.count = r->wave.count + 2 * side;
     q.cap = cap;
+    q.maxBlocks = r->gridLimit;
     return q;
 }
 
@@ -511,6 +514,8 @@ int cx_ctx_create(int device, cx_ctx** out) {
 }
 
 int cx_ctx_destroy(cx_ctx* ctx) {
+    if (ctx)
+        for (hipStream_t s : ctx->pool) (void)hipStreamDestroy(s);
     delete ctx;
     return CX_OK;
 }
@@ -1026,6 +1031,43 @@ int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, f
     }
     rec(r, 7, s);
     r->evValid = r->timing;
+    return CX_OK;
+}
+
+int cx_renderer_set_queue_grid(cx_renderer* r, uint32_t maxBlocks) {
+    if (!r) return CX_ERR_INVALID_ARG;
+    if (maxBlocks < 1 || maxBlocks > (1u << 20)) return set_err(r->ctx, CX_ERR_INVALID_ARG, "queue grid out of range");
+    r->gridLimit = maxBlocks;
+    return CX_OK;
+}
+
+int cx_batch_frame(cx_ctx* ctx, cx_renderer* const* rs, const cx_camera* cams, const int* cameraChanged,
+                   float* const* outs, uint32_t n, void* const* streams) {
+    if (!ctx || (n && (!rs || !cams || !cameraChanged || !outs))) return set_err(ctx, CX_ERR_INVALID_ARG, "null argument");
+    for (uint32_t i = 0; i < n; i++) {
+        if (!rs[i] || !outs[i]) return set_err(ctx, CX_ERR_INVALID_ARG, "null renderer or output");
+        for (uint32_t j = 0; j < i; j++)
+            if (rs[j] == rs[i]) return set_err(ctx, CX_ERR_INVALID_ARG, "a renderer appears twice in the batch");
+    }
+    while (!streams && ctx->pool.size() < n) {
+        hipStream_t s = nullptr;
+        CX_HIP(ctx, hipStreamCreate(&s));
+        ctx->pool.push_back(s);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        void* s = streams ? streams[i] : (void*)ctx->pool[i];
+        const int rc = cx_renderer_frame(rs[i], &cams[i], cameraChanged[i], outs[i], s);
+        if (rc != CX_OK) {
+            if (rs[i]->ctx != ctx) set_err(ctx, rc, cx_last_error(rs[i]->ctx));
+            return rc;
+        }
+    }
+    return CX_OK;
+}
+
+int cx_batch_sync(cx_ctx* ctx) {
+    if (!ctx) return CX_ERR_INVALID_ARG;
+    for (hipStream_t s : ctx->pool) CX_HIP(ctx, hipStreamSynchronize(s));
     return CX_OK;
 }
 

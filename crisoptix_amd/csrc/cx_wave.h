@@ -29,6 +29,7 @@ struct Queue {
     uint4* res;         // [cap] closest: prim t u v; shadow: occluded
     uint32_t* count;    // [2] closest count (front), shadow count (back)
     uint32_t cap;
+    uint32_t maxBlocks; // grid-stride kernels over this queue launch at most this many blocks
 };
 
 struct Emit {

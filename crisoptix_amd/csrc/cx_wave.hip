@@ -1033,12 +1033,12 @@ __global__ void k_w_zero2(uint32_t* c) {
 // ==========================================================================================
 
 // Grid of the grid-stride kernels: enough wavefronts to fill every CU at full occupancy.
-static constexpr uint32_t MAX_STRIDE_BLOCKS = 4096;
-
 static dim3 grid_for(uint32_t n) { return dim3((n + WBLOCK - 1) / WBLOCK); }
 
-static dim3 stride_grid(uint32_t cap) {
-    return dim3(std::max(1u, std::min((cap + WBLOCK - 1) / WBLOCK, MAX_STRIDE_BLOCKS)));
+// Blocks of a grid-stride launch over a queue: enough for its capacity, at most q.maxBlocks
+// (the renderer's queue-grid setting, 4096 by default: 16 blocks of 128 threads per CU).
+static dim3 stride_grid(const Queue& q) {
+    return dim3(std::max(1u, std::min((q.cap + WBLOCK - 1) / WBLOCK, std::max(1u, q.maxBlocks))));
 }
 
 hipError_t wave_zero_counts(uint32_t* c, hipStream_t s) {
@@ -1048,7 +1048,7 @@ hipError_t wave_zero_counts(uint32_t* c, hipStream_t s) {
 
 hipError_t wave_trace(const DScene& S, const Queue& qin, const Queue& qout, unsigned long long* counters,
                       hipStream_t s) {
-    const dim3 g = stride_grid(qin.cap), b(WBLOCK);
+    const dim3 g = stride_grid(qin), b(WBLOCK);
     if (S.width == 4) hipLaunchKernelGGL(k_q_trace<4>, g, b, 0, s, S, qin, qout, counters);
     else if (S.width == 8) hipLaunchKernelGGL(k_q_trace<8>, g, b, 0, s, S, qin, qout, counters);
     else hipLaunchKernelGGL(k_q_trace<2>, g, b, 0, s, S, qin, qout, counters);
@@ -1058,11 +1058,11 @@ hipError_t wave_trace(const DScene& S, const Queue& qin, const Queue& qout, unsi
 hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStream_t s) {
     const dim3 b(WBLOCK);
     switch (kernel) {
-        case WK_PG_STEP: hipLaunchKernelGGL(k_w_pg_step, stride_grid(W.qin.cap), b, 0, s, W); return hipGetLastError();
-        case WK_CP_STEP: hipLaunchKernelGGL(k_w_cp_step, stride_grid(W.qin.cap), b, 0, s, W); return hipGetLastError();
-        case WK_SG_STEP: hipLaunchKernelGGL(k_w_sg_step, stride_grid(W.qin.cap), b, 0, s, W); return hipGetLastError();
-        case WK_SH_STEP: hipLaunchKernelGGL(k_w_sh_step, stride_grid(W.qin.cap), b, 0, s, W); return hipGetLastError();
-        case WK_FG_STEP: hipLaunchKernelGGL(k_w_fg_step, stride_grid(W.qin.cap), b, 0, s, W); return hipGetLastError();
+        case WK_PG_STEP: hipLaunchKernelGGL(k_w_pg_step, stride_grid(W.qin), b, 0, s, W); return hipGetLastError();
+        case WK_CP_STEP: hipLaunchKernelGGL(k_w_cp_step, stride_grid(W.qin), b, 0, s, W); return hipGetLastError();
+        case WK_SG_STEP: hipLaunchKernelGGL(k_w_sg_step, stride_grid(W.qin), b, 0, s, W); return hipGetLastError();
+        case WK_SH_STEP: hipLaunchKernelGGL(k_w_sh_step, stride_grid(W.qin), b, 0, s, W); return hipGetLastError();
+        case WK_FG_STEP: hipLaunchKernelGGL(k_w_fg_step, stride_grid(W.qin), b, 0, s, W); return hipGetLastError();
         default: break;
     }
     if (threads == 0) return hipSuccess;

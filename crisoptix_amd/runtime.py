@@ -174,6 +174,9 @@ def lib():
         L.cx_scene_set_emitter_colors.argtypes = [vp, vp, u32, i32]
         L.cx_renderer_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.cx_tonemap.argtypes = [vp, vp, vp, u32, vp]
+        L.cx_batch_frame.argtypes = [vp, vp, vp, vp, vp, u32, vp]
+        L.cx_batch_sync.argtypes = [vp]
+        L.cx_renderer_set_queue_grid.argtypes = [vp, u32]
         L.cx_ctx_create.argtypes = [i32, C.POINTER(vp)]
         L.cx_ctx_destroy.argtypes = [vp]
         L.cx_last_error.restype = C.c_char_p
@@ -233,6 +236,29 @@ def tonemap(ctx: "Context", hdr: "DeviceArray", stream=None) -> np.ndarray:
     n = int(np.prod(hdr.shape))
     ctx.check(lib().cx_tonemap(ctx.h, hdr.ptr, ldr.ptr, n, stream), "cx_tonemap")
     return ldr.numpy()
+
+
+class Batch:
+    """cx_batch_frame over several renderers (views of any scenes and resolutions)."""
+
+    def __init__(self, ctx: "Context", renderers: list):
+        self.ctx = ctx
+        self.rs = list(renderers)
+        n = len(self.rs)
+        self._h = (C.c_void_p * n)(*[r.h for r in self.rs])
+        self._outs = (C.c_void_p * n)(*[r.output.ptr for r in self.rs])
+        self._cams = (abi.CameraData * n)()
+        self._chg = (C.c_int * n)()
+
+    def frame(self, cams: list, changed: list):
+        for i, (c, ch) in enumerate(zip(cams, changed)):
+            self._cams[i] = c
+            self._chg[i] = 1 if ch else 0
+        self.ctx.check(lib().cx_batch_frame(self.ctx.h, self._h, self._cams, self._chg, self._outs, len(self.rs),
+                                            None), "cx_batch_frame")
+
+    def sync(self):
+        self.ctx.check(lib().cx_batch_sync(self.ctx.h), "cx_batch_sync")
 
 
 class Context:
@@ -391,6 +417,9 @@ class Renderer:
     def set_exec_mode(self, mode: str):
         """'wavefront' (default) or 'megakernel'; both give identical results."""
         self.ctx.check(lib().cx_renderer_set_exec_mode(self.h, abi.EXEC_MODES[mode]), "cx_renderer_set_exec_mode")
+
+    def set_queue_grid(self, max_blocks: int):
+        self.ctx.check(lib().cx_renderer_set_queue_grid(self.h, max_blocks), "cx_renderer_set_queue_grid")
 
     def exec_mode(self) -> str:
         m = lib().cx_renderer_get_exec_mode(self.h)

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define CX_ABI_VERSION 4
+#define CX_ABI_VERSION 5
 
 typedef enum cx_status {
     CX_OK = 0,
@@ -277,6 +277,20 @@ typedef enum cx_exec_mode {
 } cx_exec_mode;
 int cx_renderer_set_exec_mode(cx_renderer* r, int mode);
 int cx_renderer_get_exec_mode(const cx_renderer* r);
+/* ---- multi-view batch (no counterpart in the reference, which renders one view) ----
+ * Enqueues one frame of each of n renderers (views of any scenes and resolutions) and
+ * returns: view i runs cx_renderer_frame(renderers[i], &cams[i], cameraChanged[i], outs[i])
+ * on streams[i], or, when streams is NULL, on the i-th stream of the context's pool, so the
+ * views' kernels overlap on the GPU.  Each view's output is its plain frame's, bit for bit.
+ * cx_batch_sync waits for the pool's streams. */
+int cx_batch_frame(cx_ctx* ctx, cx_renderer* const* renderers, const cx_camera* cams, const int* cameraChanged,
+                   float* const* outs, uint32_t n, void* const* streams);
+int cx_batch_sync(cx_ctx* ctx);
+
+/* Wavefront mode: the grid-stride traversal and step kernels launch at most maxBlocks blocks
+ * of 128 threads (default 4096); a tuning knob (crisoptix_amd/tune.py), results unchanged. */
+int cx_renderer_set_queue_grid(cx_renderer* r, uint32_t maxBlocks);
+
 /* Timings are gathered with hipEvents when enabled (off by default). */
 int cx_renderer_enable_timing(cx_renderer* r, int enable);
 int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* stats);
