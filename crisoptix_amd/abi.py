@@ -53,7 +53,9 @@ class Material(C.Structure):
                 ("roughness", C.c_float)]
 
 
-ABI_VERSION = 5   # CX_ABI_VERSION of include/crisoptix.h
+ABI_VERSION = 6   # CX_ABI_VERSION of include/crisoptix.h
+CX_SPEC_TEX_COLOR = 0
+CX_SPEC_TEX_OCC_ROUGH_METAL = 1
 
 
 class SceneDesc(C.Structure):
@@ -63,7 +65,19 @@ class SceneDesc(C.Structure):
         ("materials", C.c_void_p), ("numMaterials", C.c_uint32),
         ("emitterFaces", C.c_void_p), ("emitterFaceOffset", C.c_void_p), ("emitterColor", C.c_void_p),
         ("numEmitters", C.c_uint32), ("bvhMaxLeafSize", C.c_uint32), ("bvhWidth", C.c_uint32),
+        ("uvs", C.c_void_p), ("materialTex", C.c_void_p), ("textures", C.c_void_p), ("numTextures", C.c_uint32),
+        ("pad0", C.c_uint32),
     ]
+
+
+class Texture(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class MaterialTex(C.Structure):
+    """One row of Scene.material_tex (an int32 [M, 4] array is passed as this struct's array)."""
+    _fields_ = [("diffTex", C.c_int32), ("specTex", C.c_int32), ("roughTex", C.c_int32),
+                ("specTexUsage", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):
@@ -72,7 +86,7 @@ class SceneInfo(C.Structure):
         ("maxDepth", C.c_uint32), ("numEmitters", C.c_uint32), ("numEmitterFaces", C.c_uint32),
         ("sceneSize", C.c_float), ("boundsMin", C.c_float * 3), ("boundsMax", C.c_float * 3),
         ("deviceBytes", C.c_uint64), ("bvhWidth", C.c_uint32), ("numWideNodes", C.c_uint32),
-        ("stackBound", C.c_uint32), ("pad0", C.c_uint32),
+        ("stackBound", C.c_uint32), ("numTextures", C.c_uint32),
     ]
 
 
@@ -118,6 +132,32 @@ def scene_desc(scene, keep: list) -> SceneDesc:
     d.numEmitters = arrs["emitterColor"].shape[0]
     d.bvhMaxLeafSize = 0
     d.bvhWidth = 0
+    uvs = getattr(scene, "uvs", None)
+    if uvs is not None:
+        arrs["uvs"] = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+        if arrs["uvs"].shape[0] != d.numVertices:
+            raise ValueError("uvs must have one (u, v) per vertex")
+        d.uvs = arrs["uvs"].ctypes.data
+    mt = getattr(scene, "material_tex", None)
+    if mt is not None:
+        arrs["materialTex"] = np.ascontiguousarray(mt, np.int32).reshape(-1, 4)
+        if arrs["materialTex"].shape[0] != d.numMaterials:
+            raise ValueError("material_tex must have one row per material")
+        d.materialTex = arrs["materialTex"].ctypes.data
+    texs = list(getattr(scene, "textures", None) or [])
+    if texs:
+        imgs = [np.ascontiguousarray(t, np.uint8) for t in texs]
+        for im in imgs:
+            if im.ndim != 3 or im.shape[2] != 4:
+                raise ValueError("textures are [height, width, 4] uint8 RGBA arrays")
+        tab = (Texture * len(imgs))()
+        for i, im in enumerate(imgs):
+            tab[i].rgba = im.ctypes.data
+            tab[i].width = im.shape[1]
+            tab[i].height = im.shape[0]
+        arrs["_textures"] = (imgs, tab)
+        d.textures = C.addressof(tab)
+        d.numTextures = len(imgs)
     return d
 
 

@@ -50,23 +50,17 @@ struct Factors {
     }
 };
 
-__device__ __forceinline__ Mat load_mat_from(const DMaterial* mats, const DScene& S, uint32_t prim) {
-    const DMaterial& m = mats[S.triMat[prim]];
-    Mat r;
-    r.diff = V3{m.diffRough.x, m.diffRough.y, m.diffRough.z};
-    r.rough = m.diffRough.w;
-    const V3 spec = V3{m.spec.x, m.spec.y, m.spec.z};
-    const float metal = 0.0f;
-    r.spec = r.diff * (1.0f - metal) + spec * metal;
-    r.emis = V3{m.emissive.x, m.emissive.y, m.emissive.z};
-    return r;
+__device__ __forceinline__ Mat load_mat_from(const DMaterial* mats, const DScene& S, const Inter& it) {
+    return resolve_mat(S, mats[S.triMat[it.prim]], it.prim, it.bu, it.bv);
 }
 
 // BSDF value (the expression of ggx_eval / ggx_sample, brdf.cu:115-338) and its derivatives.
-// With F0 = albedo (metal = 0): value_c = (a_c / pi + D V (a_c + (1 - a_c) P)) nL, so
-// d/da_c = (1/pi + D V (1 - P)) nL and d/dr = (dD/dr V + D dV/dr) F_c nL, with
+// With F0 = a (1 - m) + s m (metal m; m = 0 unless the material maps OccRoughMetal):
+// value_c = (a_c / pi + D V (F0_c + (1 - F0_c) P)) nL, so
+// d/da_c = (1/pi + D V (1 - P) (1 - m)) nL and d/dr = (dD/dr V + D dV/dr) F_c nL, with
 // D = a2 / (pi q^2), q = nH^2 (a2 - 1) + 1, V = 0.5 / (nL s_v + nV s_l),
-// s_v = sqrt(nV^2 (1 - a2) + a2), s_l = sqrt(nL^2 (1 - a2) + a2), a2 = r^2.
+// s_v = sqrt(nV^2 (1 - a2) + a2), s_l = sqrt(nL^2 (1 - a2) + a2), a2 = r^2.  A diffuse colour
+// or roughness read from a texture is not the material's parameter: its derivative is zero.
 struct BrdfD {
     V3 val, dA, dR;
 };
@@ -88,7 +82,7 @@ __device__ __forceinline__ BrdfD brdf_d(const Mat& m, V3 normal, V3 viewDir, V3 
         const float d = d_ggx(nDotH, m.rough);
         const float v = v_smith(nDotL, nDotV, m.rough);
         const float p = pow5(gmax(0.0f, 1.0f - lDotH));
-        dA += d * v * (1.0f - p);
+        dA += d * v * (1.0f - p) * (1.0f - m.metal);
         const float q = nDotH * nDotH * (a2 - 1.0f) + 1.0f;
         const float dDda2 = (q - 2.0f * a2 * nDotH * nDotH) / (PI_F * q * q * q);
         const float sv = sqrtf(nDotV * nDotV * (1.0f - a2) + a2);
@@ -100,8 +94,8 @@ __device__ __forceinline__ BrdfD brdf_d(const Mat& m, V3 normal, V3 viewDir, V3 
         const V3 F = m.spec + (v3(1.0f) - m.spec) * p;
         dR = F * dDVdr;
     }
-    r.dA = v3(dA * nDotL);
-    r.dR = dR * nDotL;
+    r.dA = m.diffParam ? v3(dA * nDotL) : v3(0.0f);
+    r.dR = m.roughParam ? dR * nDotL : v3(0.0f);
     return r;
 }
 
@@ -148,7 +142,7 @@ __device__ __forceinline__ bool nee_factor(const AdjCtx& C, const Inter& it, Rng
     const float dist = length(lp - it.pos);
     if (trace_occluded(S, C.tc, it.pos, lightDir, 1e-3f, dist)) return false;
     o.color = V3{C.G.evalEmit[3 * ei], C.G.evalEmit[3 * ei + 1], C.G.evalEmit[3 * ei + 2]};
-    const Mat m = load_mat_from(C.G.evalMats, S, it.prim);
+    const Mat m = load_mat_from(C.G.evalMats, S, it);
     o.distFactor = 1.0f / (dist * dist);
     const float nDotV = dot(it.nrm, -it.dir), nDotL = dot(it.nrm, lightDir);
     o.fac.mat = S.triMat[it.prim];
@@ -202,7 +196,7 @@ __device__ __forceinline__ bool replay_prefix_tail(const AdjCtx& C, Inter it, bo
         }
         const BrdfSample bs = ggx_sample(S, it, rng);
         if (bs.pdf <= 0.0f) return false;
-        const BrdfD b = brdf_d(load_mat_from(C.G.evalMats, S, it.prim), it.nrm, -it.dir, bs.outDir);
+        const BrdfD b = brdf_d(load_mat_from(C.G.evalMats, S, it), it.nrm, -it.dir, bs.outDir);
         prod = prod * b.val;
         Factor bf_;
         bf_.v = b.val;
@@ -248,7 +242,7 @@ __device__ __forceinline__ bool replay_post_recon(const AdjCtx& C, const Prefix&
     rng.f();
     const BrdfSample bs = ggx_sample(S, y, rng);
     if (bs.pdf <= 0.0f) return false;
-    const BrdfD b = brdf_d(load_mat_from(C.G.evalMats, S, y.prim), y.nrm, -y.dir, bs.outDir);
+    const BrdfD b = brdf_d(load_mat_from(C.G.evalMats, S, y), y.nrm, -y.dir, bs.outDir);
     Factor yf;
     yf.v = b.val;
     yf.dA = b.dA;
@@ -279,7 +273,7 @@ __device__ __forceinline__ bool replay_shift(const AdjCtx& C, const Prefix& src,
         srng.f();
         const BrdfSample bs = ggx_sample(S, curr, srng);
         if (bs.pdf <= 0.0f) return false;
-        const BrdfD b = brdf_d(load_mat_from(C.G.evalMats, S, curr.prim), curr.nrm, -curr.dir, bs.outDir);
+        const BrdfD b = brdf_d(load_mat_from(C.G.evalMats, S, curr), curr.nrm, -curr.dir, bs.outDir);
         thr = thr * b.val;
         F.push(Factor{b.val, b.dA, b.dR, v3(0.0f), S.triMat[curr.prim], NO_EMITTER});
         trace_surface(S, C.tc, curr.pos, bs.outDir, 1e-3f, 1e16f, curr);
@@ -291,12 +285,12 @@ __device__ __forceinline__ bool replay_shift(const AdjCtx& C, const Prefix& src,
     if (isinf(reconLen) || any_bad(reconDir)) return false;
     if (trace_occluded(S, C.tc, curr.pos, reconDir, 1e-3f, reconLen)) return false;
     if (dot(curr.nrm, -curr.dir) <= 0.0f || dot(curr.nrm, reconDir) < 0.0f) return false;
-    const BrdfD b1 = brdf_d(load_mat_from(C.G.evalMats, S, curr.prim), curr.nrm, -curr.dir, reconDir);
+    const BrdfD b1 = brdf_d(load_mat_from(C.G.evalMats, S, curr), curr.nrm, -curr.dir, reconDir);
     thr = thr * b1.val;
     F.push(Factor{b1.val, b1.dA, b1.dR, v3(0.0f), S.triMat[curr.prim], NO_EMITTER});
     srcRecon.dir = reconDir;
     if (dot(srcRecon.nrm, -srcRecon.dir) <= 0.0f || dot(srcRecon.nrm, src.reconOutDir) < 0.0f) return false;
-    const BrdfD b2 = brdf_d(load_mat_from(C.G.evalMats, S, srcRecon.prim), srcRecon.nrm, -srcRecon.dir, src.reconOutDir);
+    const BrdfD b2 = brdf_d(load_mat_from(C.G.evalMats, S, srcRecon), srcRecon.nrm, -srcRecon.dir, src.reconOutDir);
     thr = thr * b2.val;
     F.push(Factor{b2.val, b2.dA, b2.dR, v3(0.0f), S.triMat[srcRecon.prim], NO_EMITTER});
     V3 prc;
@@ -427,7 +421,7 @@ __device__ __forceinline__ void adjoint_path_tracer(const AdjCtx& C, uint32_t p,
         }
         const BrdfSample bs = ggx_sample(S, it, rng);
         if (bs.pdf <= 0.0f) break;
-        const BrdfD b = brdf_d(load_mat_from(G.evalMats, S, it.prim), it.nrm, -it.dir, bs.outDir);
+        const BrdfD b = brdf_d(load_mat_from(G.evalMats, S, it), it.nrm, -it.dir, bs.outDir);
         pos = it.pos;
         dir = bs.outDir;
         thr = thr * (b.val / bs.pdf);

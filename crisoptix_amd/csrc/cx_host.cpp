@@ -26,6 +26,7 @@ struct cx_scene {
     DMaterial* evalMats = nullptr;   // evaluation table of the adjoint (device)
     float* evalEmit = nullptr;
     uint32_t numMaterials = 0;
+    std::vector<int4> matTex;        // texture bindings per material (kept by set_materials)
     std::vector<void*> allocs;
     cx_scene_info info{};
 };
@@ -539,6 +540,31 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     for (uint32_t e = 0; e < d->numEmitters; e++)
         if (d->emitterFaceOffset[e + 1] <= d->emitterFaceOffset[e])
             return set_err(ctx, CX_ERR_INVALID_ARG, "every emitter needs at least one face");
+    // Textures: the sampler's wrap arithmetic needs 1 <= size <= 65536 per axis, and texel
+    // offsets are 32-bit.
+    if (d->numTextures && !d->textures) return set_err(ctx, CX_ERR_INVALID_ARG, "missing texture array");
+    std::vector<uint4> texInfo(d->numTextures);
+    uint64_t numTexels = 0;
+    for (uint32_t t = 0; t < d->numTextures; t++) {
+        const cx_texture& tx = d->textures[t];
+        if (!tx.rgba || tx.width == 0 || tx.height == 0 || tx.width > 65536u || tx.height > 65536u)
+            return set_err(ctx, CX_ERR_INVALID_ARG, "texture size must be 1..65536 per axis with data");
+        texInfo[t] = make_uint4((uint32_t)numTexels, tx.width, tx.height, 0u);
+        numTexels += (uint64_t)tx.width * tx.height;
+        if (numTexels >= (1ull << 32)) return set_err(ctx, CX_ERR_INVALID_ARG, "more than 2^32 texels");
+    }
+    std::vector<int4> matTex(d->numMaterials, make_int4(-1, -1, -1, 0));
+    if (d->materialTex)
+        for (uint32_t m = 0; m < d->numMaterials; m++) {
+            const cx_material_tex& mt = d->materialTex[m];
+            const int32_t ids[3] = {mt.diffTex, mt.specTex, mt.roughTex};
+            for (int k = 0; k < 3; k++)
+                if (ids[k] < -1 || (ids[k] >= 0 && (uint32_t)ids[k] >= d->numTextures))
+                    return set_err(ctx, CX_ERR_INVALID_ARG, "material texture index out of range");
+            if (mt.specTexUsage > CX_SPEC_TEX_OCC_ROUGH_METAL)
+                return set_err(ctx, CX_ERR_INVALID_ARG, "unknown specTexUsage");
+            matTex[m] = make_int4(mt.diffTex, mt.specTex, mt.roughTex, (int32_t)mt.specTexUsage);
+        }
     BvhHost bvh;
     std::string err;
     if (!build_bvh(d->positions, d->indices, d->numTriangles, d->bvhMaxLeafSize ? d->bvhMaxLeafSize : 4, bvh, err))
@@ -572,7 +598,11 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
         mats[m].diffRough = make_float4(cm.diffColor[0], cm.diffColor[1], cm.diffColor[2], cm.roughness);
         mats[m].spec = make_float4(cm.specF0[0], cm.specF0[1], cm.specF0[2], 0.0f);
         mats[m].emissive = make_float4(cm.emissiveColor[0], cm.emissiveColor[1], cm.emissiveColor[2], 0.0f);
+        mats[m].tex = matTex[m];
     }
+    std::vector<uint32_t> texels((size_t)numTexels);
+    for (uint32_t t = 0; t < d->numTextures; t++)
+        std::memcpy(texels.data() + texInfo[t].x, d->textures[t].rgba, (size_t)texInfo[t].y * texInfo[t].z * 4);
     const float4* nodesSrc = bvh.nodes.empty() ? nullptr : bvh.nodes.data();
     s->d.nodes = (const float4*)up(nodesSrc, bvh.nodes.size(), sizeof(float4));
     s->d.wnodes = width > 2 ? (const float4*)up(wide.data(), wide.size(), sizeof(float4)) : nullptr;
@@ -588,6 +618,11 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     s->d.emitOff = (const uint32_t*)up(d->emitterFaceOffset, d->numEmitters + 1, sizeof(uint32_t));
     s->d.emitColor = (const float*)up(d->emitterColor, 3 * (size_t)d->numEmitters, sizeof(float));
     s->evalEmit = (float*)up(d->emitterColor, 3 * (size_t)d->numEmitters, sizeof(float));
+    s->d.uv = d->uvs ? (const float*)up(d->uvs, 2 * (size_t)d->numVertices, sizeof(float)) : nullptr;
+    s->d.texels = (const uint32_t*)up(texels.data(), texels.size(), sizeof(uint32_t));
+    s->d.texInfo = (const uint4*)up(texInfo.data(), texInfo.size(), sizeof(uint4));
+    s->d.numTextures = d->numTextures;
+    s->matTex = matTex;
     if (e != hipSuccess) {
         free_all(s->allocs);
         delete s;
@@ -621,8 +656,10 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     s->info.bvhWidth = width;
     s->info.numWideNodes = width > 2 ? (uint32_t)(wide.size() / (2 * width)) : 0u;
     s->info.stackBound = stackBound;
+    s->info.numTextures = d->numTextures;
     s->info.deviceBytes = (bvh.nodes.size() + wide.size() + bvh.tris.size()) * sizeof(float4) +
-                          (size_t)d->numVertices * 24 + (size_t)d->numTriangles * 16 + mats.size() * sizeof(DMaterial);
+                          (size_t)d->numVertices * (d->uvs ? 32 : 24) + (size_t)d->numTriangles * 16 +
+                          mats.size() * sizeof(DMaterial) + texels.size() * 4 + texInfo.size() * sizeof(uint4);
     *out = s;
     return CX_OK;
 }
@@ -656,6 +693,7 @@ int cx_scene_set_materials(cx_scene* s, const cx_material* mats, uint32_t n, int
         dm[m].diffRough = make_float4(cm.diffColor[0], cm.diffColor[1], cm.diffColor[2], cm.roughness);
         dm[m].spec = make_float4(cm.specF0[0], cm.specF0[1], cm.specF0[2], 0.0f);
         dm[m].emissive = make_float4(cm.emissiveColor[0], cm.emissiveColor[1], cm.emissiveColor[2], 0.0f);
+        dm[m].tex = s->matTex[m];
     }
     if (tables & CX_TABLE_SAMPLING)
         CX_HIP(s->ctx, hipMemcpy((void*)s->d.mats, dm.data(), n * sizeof(DMaterial), hipMemcpyHostToDevice));

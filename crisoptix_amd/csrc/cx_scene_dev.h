@@ -27,7 +27,10 @@ struct DMaterial {
     float4 diffRough;   // diffuse rgb, roughness
     float4 spec;        // specF0 rgb, -
     float4 emissive;    // emissive rgb, -
+    int4 tex;           // diffuse, specular, roughness texture (-1: none), specular usage
 };
+
+static constexpr int32_t SPEC_USAGE_OCC_ROUGH_METAL = 1;   // SpecTexUsage::OccRoughMetal
 
 // Wide BVH node (width N = 4 or 8), N/4 groups of 7 float4, padded to 8 float4 per group:
 //   lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4], ref[4]
@@ -54,7 +57,11 @@ struct DScene {
     int32_t root;         // root of `nodes` (binary)
     int32_t wroot;        // root of `wnodes`
     uint32_t width;       // 2: traverse `nodes`; 4 or 8: traverse `wnodes`
-    uint32_t pad[3];
+    uint32_t numTextures;
+    uint32_t pad[2];
+    const float* __restrict__ uv;         // [V][2], or null (all zero)
+    const uint32_t* __restrict__ texels;  // RGBA8 texels of every texture, one word each
+    const uint4* __restrict__ texInfo;    // [textures]: first texel, width, height, -
 };
 
 static constexpr int STACK_DEPTH = 64;
@@ -461,19 +468,86 @@ CX_HD float v_smith(float NdotL, float NdotV, float roughness) {   // brdf.cu:10
 struct Mat {
     V3 diff, spec, emis;
     float rough;
+    float metal;
+    bool diffParam;    // diff is the material's constant (not a texture value)
+    bool roughParam;   // rough is the material's constant
 };
 
-// Material fetch including specF0 = mix(diffColor, specF0, metal = 0) (brdf.cu:123-138).
-__device__ __forceinline__ Mat load_mat(const DScene& S, uint32_t prim) {
-    const DMaterial& m = S.mats[S.triMat[prim]];
+// tex2D<float4> of an RGBA8 texture created as in src/Texture.cpp:44-76 (wrap addressing,
+// linear filtering, normalized coordinates, normalized-float reads), channels r, g, b.
+// Canonical filter: x = frac(u) * width - 0.5, texels floor(x) and floor(x) + 1 wrapped,
+// weight alpha = frac(x) rounded to a multiple of 1/256 (likewise y); the exact integer sum of
+// weight * texel is rounded once, sum / (65536 * 255).  frac(u) = u - floor(u) lies in [0, 1],
+// so floor(x) lies in [-1, width - 1] for width <= 65536; a non-finite coordinate reads as 0.
+__device__ __forceinline__ V3 tex_fetch(const DScene& S, int32_t t, float u, float v) {
+    const uint4 ti = S.texInfo[t];
+    if (!(fabsf(u) <= 3.0e38f)) u = 0.0f;
+    if (!(fabsf(v) <= 3.0e38f)) v = 0.0f;
+    const float x = (u - floorf(u)) * (float)ti.y - 0.5f;
+    const float y = (v - floorf(v)) * (float)ti.z - 0.5f;
+    const float xf = floorf(x), yf = floorf(y);
+    const uint32_t a = (uint32_t)rintf((x - xf) * 256.0f), b = (uint32_t)rintf((y - yf) * 256.0f);
+    const uint32_t i0 = xf < 0.0f ? ti.y - 1u : (uint32_t)xf, j0 = yf < 0.0f ? ti.z - 1u : (uint32_t)yf;
+    const uint32_t i1 = i0 + 1u == ti.y ? 0u : i0 + 1u, j1 = j0 + 1u == ti.z ? 0u : j0 + 1u;
+    const uint32_t* row0 = S.texels + ti.x + (size_t)j0 * ti.y;
+    const uint32_t* row1 = S.texels + ti.x + (size_t)j1 * ti.y;
+    const uint32_t t00 = row0[i0], t10 = row0[i1], t01 = row1[i0], t11 = row1[i1];
+    const uint32_t w00 = (256u - a) * (256u - b), w10 = a * (256u - b), w01 = (256u - a) * b, w11 = a * b;
+    float c[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint32_t sh = 8u * (uint32_t)k;
+        const uint32_t sum = w00 * ((t00 >> sh) & 255u) + w10 * ((t10 >> sh) & 255u) + w01 * ((t01 >> sh) & 255u) +
+                             w11 * ((t11 >> sh) & 255u);
+        c[k] = (float)sum / 16711680.0f;
+    }
+    return V3{c[0], c[1], c[2]};
+}
+
+// The material of a hit: GetFromTexIfPossible for diffuse colour, specular value and roughness
+// at the interaction's uv (InterpolateBary of the vertex uvs, Interaction.h:171), the
+// OccRoughMetal channel mapping, then specF0 = mix(diffColor, specF0, metal) (brdf.cu:121-138).
+__device__ __forceinline__ Mat resolve_mat(const DScene& S, const DMaterial& m, uint32_t prim, float bu, float bv) {
     Mat r;
     r.diff = V3{m.diffRough.x, m.diffRough.y, m.diffRough.z};
     r.rough = m.diffRough.w;
-    const V3 spec = V3{m.spec.x, m.spec.y, m.spec.z};
-    const float metal = 0.0f;
-    r.spec = r.diff * (1.0f - metal) + spec * metal;
+    V3 spec = V3{m.spec.x, m.spec.y, m.spec.z};
     r.emis = V3{m.emissive.x, m.emissive.y, m.emissive.z};
+    r.diffParam = true;
+    r.roughParam = true;
+    float metal = 0.0f;
+    const int4 t = m.tex;
+    if ((t.x & t.y & t.z) != -1 || t.w == SPEC_USAGE_OCC_ROUGH_METAL) {
+        float tu = 0.0f, tv = 0.0f;
+        if ((t.x & t.y & t.z) != -1 && S.uv != nullptr) {
+            const uint32_t i0 = S.idx[3 * prim], i1 = S.idx[3 * prim + 1], i2 = S.idx[3 * prim + 2];
+            const float w = 1.0f - bu - bv;
+            tu = (w * S.uv[2 * i0] + bu * S.uv[2 * i1]) + bv * S.uv[2 * i2];
+            tv = (w * S.uv[2 * i0 + 1] + bu * S.uv[2 * i1 + 1]) + bv * S.uv[2 * i2 + 1];
+        }
+        if (t.x >= 0) {
+            r.diff = tex_fetch(S, t.x, tu, tv);
+            r.diffParam = false;
+        }
+        if (t.y >= 0) spec = tex_fetch(S, t.y, tu, tv);
+        if (t.z >= 0) {
+            r.rough = tex_fetch(S, t.z, tu, tv).x;
+            r.roughParam = false;
+        }
+        if (t.w == SPEC_USAGE_OCC_ROUGH_METAL) {
+            r.rough = spec.y;
+            metal = spec.z;
+            spec = v3(1.0f);
+            r.roughParam = false;
+        }
+    }
+    r.spec = r.diff * (1.0f - metal) + spec * metal;
+    r.metal = metal;
     return r;
+}
+
+__device__ __forceinline__ Mat load_mat(const DScene& S, const Inter& it) {
+    return resolve_mat(S, S.mats[S.triMat[it.prim]], it.prim, it.bu, it.bv);
 }
 
 __device__ __forceinline__ V3 spec_term(const Mat& m, float nDotH, float nDotL, float nDotV, float lDotH) {
@@ -490,7 +564,7 @@ __device__ __forceinline__ V3 spec_term(const Mat& m, float nDotH, float nDotL, 
 
 // __direct_callable__ggx_eval (brdf.cu:115-190)
 __device__ __forceinline__ BrdfEval ggx_eval(const DScene& S, const Inter& it, V3 outDir) {
-    const Mat m = load_mat(S, it.prim);
+    const Mat m = load_mat(S, it);
     BrdfEval r;
     const V3 lightDir = outDir;
     const V3 viewDir = -it.dir;
@@ -532,7 +606,7 @@ CX_HD V3 frame_mul(V3 n, V3 l) {
 
 // __direct_callable__ggx_sample (brdf.cu:214-338)
 __device__ __forceinline__ BrdfSample ggx_sample(const DScene& S, const Inter& it, Rng& rng) {
-    const Mat m = load_mat(S, it.prim);
+    const Mat m = load_mat(S, it);
     const V3 normal = it.nrm;
     const V3 viewDir = -it.dir;
     BrdfSample r;

@@ -59,6 +59,10 @@ class Scene:
     emitter_color: np.ndarray      # float32 [E,3]
     name: str = ""
     meta: dict = field(default_factory=dict)
+    uvs: np.ndarray | None = None          # float32 [V,2] texture coordinates, or None (all 0)
+    material_tex: np.ndarray | None = None  # int32 [M,4] diffuse, specular, roughness texture
+    #                                         (-1: none), specular usage (0 Color, 1 OccRoughMetal)
+    textures: list = field(default_factory=list)  # uint8 [H,W,4] RGBA images
 
     @property
     def num_triangles(self) -> int:
@@ -271,6 +275,46 @@ def inverse_scene(seed: int = 4) -> Scene:
     subdivision 3 and boxes) on a 140 x 140-quad ground, about 100 k triangles, with 64 object
     materials plus the ground's and 16 emitter triangles."""
     return object_grid(grid=8, seed=seed, ground_quads=140, ico_subdiv=3, num_lights=16)
+
+
+def with_textures(scene: Scene, seed: int = 7, scale: float = 0.3) -> Scene:
+    """The scene with seeded procedural textures, exercising every texture path of the BSDF
+    (shader/brdf.cu:121-138):
+      * uvs: planar from the vertex positions, u = scale (x + 0.37 y), v = scale (z - 0.21 y),
+        so they run far outside [0, 1) and exercise the wrap addressing;
+      * texture 0, 16 x 16 random colours (channels 20..255): diffuse of material 0 (the
+        ground) and of every material m = 1 mod 4;
+      * texture 1, 8 x 8 random roughness (red 26..230): roughness of materials m = 2 mod 4;
+      * texture 2, 4 x 8 (height x width) OccRoughMetal (green: roughness 26..230, blue: metal
+        0..255): specular of materials m = 3 mod 4, with SpecTexUsage OccRoughMetal;
+      * the last material maps OccRoughMetal from its constant specular value (0, 0.5, 0.7):
+        roughness 0.5, metal 0.7, no texture."""
+    rng = np.random.default_rng(seed)
+    p = scene.positions.astype(np.float64)
+    uv = np.stack([scale * (p[:, 0] + 0.37 * p[:, 1]), scale * (p[:, 2] - 0.21 * p[:, 1])], 1).astype(np.float32)
+    t0 = rng.integers(20, 256, (16, 16, 4)).astype(np.uint8)
+    t1 = rng.integers(26, 231, (8, 8, 4)).astype(np.uint8)
+    t2 = rng.integers(0, 256, (4, 8, 4)).astype(np.uint8)
+    t2[..., 1] = rng.integers(26, 231, (4, 8))
+    M = scene.materials.shape[0]
+    mt = np.full((M, 4), -1, np.int32)
+    mt[:, 3] = 0
+    mats = np.array(scene.materials, np.float32, copy=True)
+    for m in range(M):
+        if m == 0 or m % 4 == 1:
+            mt[m, 0] = 0
+        elif m % 4 == 2:
+            mt[m, 2] = 1
+        else:
+            mt[m, 1] = 2
+            mt[m, 3] = 1
+    mt[M - 1] = (-1, -1, -1, 1)
+    mats[M - 1, 3:6] = (0.0, 0.5, 0.7)
+    return Scene(positions=scene.positions, normals=scene.normals, indices=scene.indices,
+                 tri_material=scene.tri_material, materials=mats, emitter_faces=scene.emitter_faces,
+                 emitter_face_offset=scene.emitter_face_offset, emitter_color=scene.emitter_color,
+                 name=scene.name + "_textured", meta=dict(scene.meta), uvs=uv, material_tex=mt,
+                 textures=[t0, t1, t2])
 
 
 def perturbed_materials(materials: np.ndarray, seed: int, albedo: float = 0.2, roughness: float = 0.3) -> np.ndarray:

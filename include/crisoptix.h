@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define CX_ABI_VERSION 5
+#define CX_ABI_VERSION 6
 
 typedef enum cx_status {
     CX_OK = 0,
@@ -107,13 +107,45 @@ typedef struct cx_camera {
     float W[3];
 } cx_camera;
 
-/* Material (include/model/Material.h:16-33, src/Material.cpp:3-24), untextured. */
+/* Material constants (include/model/Material.h:16-33, src/Material.cpp:3-24); the texture
+ * bindings of a material are in cx_material_tex. */
 typedef struct cx_material {
     float diffColor[3];
     float specF0[3];
     float emissiveColor[3];
     float roughness;
 } cx_material;
+
+/* SpecTexUsage (include/model/Material.h:10-14).  OCC_ROUGH_METAL reads roughness from the
+ * green and metalness from the blue channel of the specular value (texture or constant) and
+ * sets the specular colour to 1 before the metal mix (shader/brdf.cu:127-138). */
+typedef enum cx_spec_tex_usage {
+    CX_SPEC_TEX_COLOR = 0,
+    CX_SPEC_TEX_OCC_ROUGH_METAL = 1
+} cx_spec_tex_usage;
+
+/* Texture bindings of one material (MaterialSbtData::hasDiffTex/diffTex, hasSpecTex/specTex,
+ * hasRoughTex/roughTex, specTexUsage; include/model/Material.h:23-32): an index into
+ * cx_scene_desc.textures, or -1 for none. */
+typedef struct cx_material_tex {
+    int32_t diffTex;
+    int32_t specTex;
+    int32_t roughTex;
+    uint32_t specTexUsage;         /* cx_spec_tex_usage */
+} cx_material_tex;
+
+/* An RGBA8 texture (src/Texture.cpp:7-76: stbi_load(..., STBI_rgb_alpha), rows in file order,
+ * sampled with wrap addressing, bilinear filtering, normalized coordinates and normalized-float
+ * reads).  Sampling at (u, v): x = frac(u) * width - 0.5, y = frac(v) * height - 0.5; texels
+ * floor(x), floor(x) + 1 (and likewise in y) wrapped modulo the size; weights
+ * alpha = frac(x), beta = frac(y), each rounded to a multiple of 1/256 (the 8 fractional bits
+ * the CUDA programming guide gives for linear filtering); the value is the exact weighted sum
+ * of the 8-bit texels rounded once to float: sum(w_i * texel_i) / (65536 * 255). */
+typedef struct cx_texture {
+    const uint8_t* rgba;           /* host, [height][width][4] */
+    uint32_t width;
+    uint32_t height;
+} cx_texture;
 
 /*
  * Scene description, host memory.  The reference's Model/Mesh/ModelInstance hierarchy
@@ -141,6 +173,13 @@ typedef struct cx_scene_desc {
     uint32_t bvhWidth;             /* children per BVH node: 2, 4 or 8; 0 = default (4).  A wide
                                       BVH whose worst-case traversal stack would not fit falls
                                       back to 2 (cx_scene_info.bvhWidth reports the width used) */
+    const float* uvs;              /* host, [numVertices][2] texture coordinates (Vertex::uv,
+                                      src/Model.cpp:79-92), or NULL: all (0, 0) */
+    const cx_material_tex* materialTex; /* host, [numMaterials], or NULL: no textures and
+                                      CX_SPEC_TEX_COLOR for every material */
+    const cx_texture* textures;    /* host, [numTextures] */
+    uint32_t numTextures;
+    uint32_t pad0;
 } cx_scene_desc;
 
 typedef struct cx_scene_info {
@@ -157,7 +196,7 @@ typedef struct cx_scene_info {
     uint32_t bvhWidth;             /* width the traversal uses */
     uint32_t numWideNodes;
     uint32_t stackBound;           /* worst-case traversal stack entries */
-    uint32_t pad0;
+    uint32_t numTextures;
 } cx_scene_info;
 
 /* Per-pass timing of the last frame, ms (src/Renderer.cpp:396-402), plus ray counts. */
@@ -330,6 +369,7 @@ enum {
     CX_TABLE_SAMPLING = 1,
     CX_TABLE_EVAL = 2,
 };
+/* Sets the constants of every material; texture bindings stay as the scene was built. */
 int cx_scene_set_materials(cx_scene* scene, const cx_material* materials, uint32_t n, int tables);
 /* colors: host, [n][3] emitter radiance (Emitter colour, include/model/Emitter.h:13-55) */
 int cx_scene_set_emitter_colors(cx_scene* scene, const float* colors, uint32_t n, int tables);
@@ -340,8 +380,10 @@ int cx_scene_set_emitter_colors(cx_scene* scene, const float* colors, uint32_t n
  * PCG32 state) and accumulates W * dL/dI . df/dtheta with W and the path held fixed (the path
  * tracer: dL/dI . df/dtheta / p).  Call after cx_renderer_frame and before the next one.
  *   d_dLdI     device [P][3] float, dL/d(pixel estimate); may be null when no gradient is asked
- *   d_gAlbedo  device [materials][3] double, accumulated (+=), or null
- *   d_gRough   device [materials] double, accumulated, or null
+ *   d_gAlbedo  device [materials][3] double, accumulated (+=), or null; zero for a material
+ *              whose diffuse colour comes from a texture (texels are not parameters)
+ *   d_gRough   device [materials] double, accumulated, or null; zero for a material whose
+ *              roughness comes from a texture or from CX_SPEC_TEX_OCC_ROUGH_METAL
  *   d_gEmit    device [emitters][3] double, accumulated, or null
  *   d_replay   device [P][3] float, the replayed estimate W * f at the evaluation table, or null
  *   d_status   device [3] uint32, accumulated: pixels replayed, pixels whose replayed f differs

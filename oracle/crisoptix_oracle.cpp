@@ -129,6 +129,13 @@ struct PCG32 {
 struct Material {
     V3 diff, spec, emissive;
     float roughness;
+    int32_t diffTex = -1, specTex = -1, roughTex = -1;   // MaterialSbtData has*Tex / *Tex
+    uint32_t specUsage = 0;                              // SpecTexUsage: 0 Color, 1 OccRoughMetal
+};
+
+struct Texture {   // RGBA8, rows in file order (src/Texture.cpp:7-30)
+    uint32_t w = 0, h = 0;
+    std::vector<uint8_t> rgba;
 };
 
 struct EmitterData {   // include/model/Emitter.h:13-55, faces pre-transformed (src/Emitter.cpp:20-27)
@@ -143,6 +150,8 @@ struct Node {
 
 struct Scene {
     std::vector<V3> pos, nrm;
+    std::vector<float> uv;          // 2 per vertex, or empty (all zero)
+    std::vector<Texture> textures;
     std::vector<uint32_t> idx;      // 3 per triangle
     std::vector<uint32_t> triMat;
     std::vector<Material> mats;
@@ -461,14 +470,65 @@ static inline V3 FresnelSchlick(const V3& F0, float VdotH) {   // brdf.cu:34-37
     return F0 + (V3(1.0f) - F0) * p;
 }
 
-// Material fetch with the metal mix of brdf.cu:123-138 (no textures: metal = 0).
+// tex2D<float4> of a texture made by Texture::LoadToDevice (src/Texture.cpp:44-76): wrap
+// addressing, linear filtering, normalized coordinates, normalized-float reads.  The filter is
+// the CUDA programming guide's (texture fetching, linear filtering): with x = frac(u) * w - 0.5,
+// i = floor(x), alpha = frac(x) held with 8 fractional bits,
+//   T = (1-alpha)(1-beta) T[i,j] + alpha(1-beta) T[i+1,j] + (1-alpha)beta T[i,j+1] + alpha beta T[i+1,j+1],
+// indices wrapped modulo the size.  Canonical here: alpha and beta are rounded to the nearest
+// multiple of 1/256, and the weighted sum, an integer in 1/(65536*255) units, is rounded once.
+// texel_sum returns that integer per channel; a non-finite coordinate reads as 0.
+static void texel_sum(const Texture& t, float u, float v, uint32_t sum[3]) {
+    if (!std::isfinite(u)) u = 0.0f;
+    if (!std::isfinite(v)) v = 0.0f;
+    const float x = (u - std::floor(u)) * (float)t.w - 0.5f;
+    const float y = (v - std::floor(v)) * (float)t.h - 0.5f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const int64_t ia = (int64_t)std::nearbyint((x - fx) * 256.0f), ib = (int64_t)std::nearbyint((y - fy) * 256.0f);
+    const int64_t W = t.w, H = t.h;
+    const int64_t xi[2] = {(((int64_t)fx % W) + W) % W, (((int64_t)fx + 1) % W + W) % W};
+    const int64_t yi[2] = {(((int64_t)fy % H) + H) % H, (((int64_t)fy + 1) % H + H) % H};
+    const int64_t wx[2] = {256 - ia, ia}, wy[2] = {256 - ib, ib};
+    for (int c = 0; c < 3; c++) {
+        int64_t acc = 0;
+        for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) acc += wx[a] * wy[b] * t.rgba[(size_t)(yi[b] * W + xi[a]) * 4 + c];
+        sum[c] = (uint32_t)acc;
+    }
+}
+
+static V3 tex_fetch(const Texture& t, float u, float v) {
+    uint32_t q[3];
+    texel_sum(t, u, v, q);
+    return V3((float)q[0] / 16711680.0f, (float)q[1] / 16711680.0f, (float)q[2] / 16711680.0f);
+}
+
+// The interaction's uv: InterpolateBary of the vertex uvs (Interaction.h:171).
+static void inter_uv(const Scene& s, uint32_t prim, float bu, float bv, float& u, float& v) {
+    u = v = 0.0f;
+    if (s.uv.empty()) return;
+    const uint32_t i0 = s.idx[prim * 3], i1 = s.idx[prim * 3 + 1], i2 = s.idx[prim * 3 + 2];
+    u = ((1.0f - bu - bv) * s.uv[2 * i0] + bu * s.uv[2 * i1]) + bv * s.uv[2 * i2];
+    v = ((1.0f - bu - bv) * s.uv[2 * i0 + 1] + bu * s.uv[2 * i1 + 1]) + bv * s.uv[2 * i2 + 1];
+}
+
+// Material fetch of ggx_eval / ggx_sample (brdf.cu:121-138): GetFromTexIfPossible for the
+// diffuse colour, specular value and roughness, the OccRoughMetal channel mapping, and
+// specF0 = mix(diffColor, specF0, metal).
 static inline void material_params(const Scene& s, const Inter& it, V3& diffColor, V3& specF0, float& roughness,
                                    V3& emissive) {
     const Material& m = s.mats[s.triMat[it.prim]];
-    diffColor = m.diff;
-    specF0 = m.spec;
-    roughness = m.roughness;
-    const float metal = 0.0f;
+    float u = 0.0f, v = 0.0f;
+    if (m.diffTex >= 0 || m.specTex >= 0 || m.roughTex >= 0) inter_uv(s, it.prim, it.bu, it.bv, u, v);
+    diffColor = m.diffTex >= 0 ? tex_fetch(s.textures[m.diffTex], u, v) : m.diff;
+    specF0 = m.specTex >= 0 ? tex_fetch(s.textures[m.specTex], u, v) : m.spec;
+    roughness = m.roughTex >= 0 ? tex_fetch(s.textures[m.roughTex], u, v).x : m.roughness;
+    float metal = 0.0f;
+    if (m.specUsage == 1) {
+        roughness = specF0.y;
+        metal = specF0.z;
+        specF0 = V3(1.0f);
+    }
     specF0 = mix(diffColor, specF0, metal);
     emissive = m.emissive;
 }
@@ -1828,6 +1888,7 @@ struct SplitMix {
 struct DInter {
     bool valid = false;
     uint32_t prim = 0;
+    float bu = 0.0f, bv = 0.0f;
     D3 pos, nrm, inDir;
 };
 
@@ -1838,6 +1899,8 @@ static DInter d_trace(const Scene& s, const D3& o, const D3& d) {
     if (h.prim == INVALID_PRIM) return it;
     it.valid = true;
     it.prim = h.prim;
+    it.bu = h.u;
+    it.bv = h.v;
     const double u = h.u, v = h.v, w = 1.0 - u - v;
     const D3 p0(s.vtx(h.prim, 0)), p1(s.vtx(h.prim, 1)), p2(s.vtx(h.prim, 2));
     const D3 n0(s.nrm[s.idx[h.prim * 3]]), n1(s.nrm[s.idx[h.prim * 3 + 1]]), n2(s.nrm[s.idx[h.prim * 3 + 2]]);
@@ -1846,14 +1909,42 @@ static DInter d_trace(const Scene& s, const D3& o, const D3& d) {
     return it;
 }
 
+// A hit's material in double (brdf.cu:121-138): texture values are the filter's exact sums
+// divided in double; diffParam / roughParam say whether the value is the material's constant.
 struct DMat {
     D3 diff, spec, emissive;
-    double rough;
+    double rough = 0.0, metal = 0.0;
+    bool diffParam = true, roughParam = true;
 };
 
-static DMat d_material(const Scene& s, uint32_t prim) {   // brdf.cu:123-138 with metal = 0
-    const Material& m = s.mats[s.triMat[prim]];
-    return DMat{D3(m.diff), D3(m.diff), D3(m.emissive), (double)m.roughness};
+static D3 tex_fetch64(const Texture& t, float u, float v) {
+    uint32_t q[3];
+    texel_sum(t, u, v, q);
+    return D3(q[0] / 16711680.0, q[1] / 16711680.0, q[2] / 16711680.0);
+}
+
+static DMat resolve64(const Scene& s, const Material& m, uint32_t prim, float bu, float bv) {
+    DMat r;
+    float u = 0.0f, v = 0.0f;
+    if (m.diffTex >= 0 || m.specTex >= 0 || m.roughTex >= 0) inter_uv(s, prim, bu, bv, u, v);
+    r.diff = m.diffTex >= 0 ? tex_fetch64(s.textures[m.diffTex], u, v) : D3(m.diff);
+    r.diffParam = m.diffTex < 0;
+    D3 spec = m.specTex >= 0 ? tex_fetch64(s.textures[m.specTex], u, v) : D3(m.spec);
+    r.rough = m.roughTex >= 0 ? tex_fetch64(s.textures[m.roughTex], u, v).x : (double)m.roughness;
+    r.roughParam = m.roughTex < 0;
+    if (m.specUsage == 1) {
+        r.rough = spec.y;
+        r.metal = spec.z;
+        spec = D3(1.0);
+        r.roughParam = false;
+    }
+    r.spec = r.diff * (1.0 - r.metal) + spec * r.metal;
+    r.emissive = D3(m.emissive);
+    return r;
+}
+
+static DMat d_material(const Scene& s, const DInter& it) {
+    return resolve64(s, s.mats[s.triMat[it.prim]], it.prim, it.bu, it.bv);
 }
 
 static inline double d_ggx_D(double nh, double r) {
@@ -1915,7 +2006,7 @@ static D3 d_path(const Scene& s, const D3& origin, const D3& dir, uint32_t maxLe
     for (uint32_t depth = 1; depth <= maxLen; ++depth) {
         const DInter it = d_trace(s, pos, d);
         if (!it.valid) break;
-        const DMat m = d_material(s, it.prim);
+        const DMat m = d_material(s, it);
         if (rng.uniform() < neeProb) {
             thr = thr / neeProb;
             const EmitterData& e = s.emitters[std::min<size_t>((size_t)(rng.uniform() * s.emitters.size()),
@@ -1959,7 +2050,7 @@ static D3 d_path(const Scene& s, const D3& origin, const D3& dir, uint32_t maxLe
 // the sampling tables, exactly as the frame produced them; every BSDF value, emitter colour,
 // product and derivative is computed in double from those float inputs and the evaluation
 // tables.  The BSDF derivatives are written out from the formula of brdf.cu:99-190 with
-// F0 = albedo (metal = 0) and checked against double-precision central differences.
+// F0 = mix(albedo, specular, metal) and checked against double-precision central differences.
 // ---------------------------------------------------------------------------------------
 
 namespace {
@@ -1973,9 +2064,9 @@ struct ADouble {
     D3 val, dA, dR;
 };
 
-ADouble brdf64(const Material& mf, const V3& nF, const V3& vF, const V3& lF) {
-    const D3 a(mf.diff), n(nF), v(vF), l(lF);
-    const double r = mf.roughness;
+ADouble brdf64(const DMat& m, const V3& nF, const V3& vF, const V3& lF) {
+    const D3 a = m.diff, f0 = m.spec, n(nF), v(vF), l(lF);
+    const double r = m.rough;
     const double nV = ddot(n, v), nL = ddot(n, l);
     const D3 h = dnorm(l + v);
     const double nH = ddot(h, n), lH = ddot(h, l);
@@ -1983,7 +2074,7 @@ ADouble brdf64(const Material& mf, const V3& nF, const V3& vF, const V3& lF) {
     ADouble o;
     D3 spec(0.0), dR(0.0);
     double dAs = 1.0 / pi;
-    if (ddot(a, a) > 1e-6) {   // F0 = albedo
+    if (ddot(f0, f0) > 1e-6) {   // F0 = albedo (1 - metal) + specular * metal
         const double a2 = r * r;
         const double q = nH * nH * (a2 - 1.0) + 1.0;
         const double D = a2 / (pi * q * q);
@@ -1992,17 +2083,17 @@ ADouble brdf64(const Material& mf, const V3& nF, const V3& vF, const V3& lF) {
         const double V = 0.5 / lam;
         const double x = std::max(0.0, 1.0 - lH);
         const double P = x * x * x * x * x;
-        const D3 F = a + (D3(1.0) - a) * P;
+        const D3 F = f0 + (D3(1.0) - f0) * P;
         spec = F * (D * V);
-        dAs += D * V * (1.0 - P);
+        dAs += D * V * (1.0 - P) * (1.0 - m.metal);
         const double dD = (q - 2.0 * a2 * nH * nH) / (pi * q * q * q);
         const double dlam = nL * (1.0 - nV * nV) / (2.0 * sv) + nV * (1.0 - nL * nL) / (2.0 * sl);
         const double dV = -0.5 * dlam / (lam * lam);
         dR = F * (2.0 * r * (dD * V + D * dV));
     }
     o.val = (a / pi + spec) * nL;
-    o.dA = D3(dAs * nL);
-    o.dR = dR * nL;
+    o.dA = m.diffParam ? D3(dAs * nL) : D3(0.0);
+    o.dR = m.roughParam ? dR * nL : D3(0.0);
     return o;
 }
 
@@ -2046,7 +2137,7 @@ bool nee64(AReplay& R, const Inter& it, PCG32& rng, bool requireValue, AFactor& 
     const float dist = length(es.pos - it.pos);
     if (trace_occ(R.s, it.pos, lightDir, 1e-3f, dist)) return false;
     color = D3(R.s.evalEmit[ei]);
-    const Material& m = R.s.evalMats[R.s.triMat[it.prim]];
+    const DMat m = resolve64(R.s, R.s.evalMats[R.s.triMat[it.prim]], it.prim, it.bu, it.bv);
     distFactor = 1.0 / ((double)dist * (double)dist);
     fac.mat = R.s.triMat[it.prim];
     fac.emit = ei;
@@ -2059,7 +2150,7 @@ bool nee64(AReplay& R, const Inter& it, PCG32& rng, bool requireValue, AFactor& 
     }
     const ADouble b = brdf64(m, it.normal, -it.inDir, lightDir);
     brdf = b.val;
-    emis = D3(m.emissive);
+    emis = m.emissive;
     fac.v = b.val * color * distFactor;
     fac.dA = b.dA * color * distFactor;
     fac.dR = b.dR * color * distFactor;
@@ -2068,7 +2159,8 @@ bool nee64(AReplay& R, const Inter& it, PCG32& rng, bool requireValue, AFactor& 
 }
 
 AFactor bsdf_factor(AReplay& R, const Inter& it, const V3& outDir) {
-    const ADouble b = brdf64(R.s.evalMats[R.s.triMat[it.prim]], it.normal, -it.inDir, outDir);
+    const ADouble b = brdf64(resolve64(R.s, R.s.evalMats[R.s.triMat[it.prim]], it.prim, it.bu, it.bv), it.normal,
+                             -it.inDir, outDir);
     AFactor f;
     f.v = b.val;
     f.dA = b.dA;
@@ -2384,6 +2476,21 @@ void* orc_scene_create(const cx_scene_desc* d, int bruteForce) {
         s->mats[m].spec = V3(cm.specF0[0], cm.specF0[1], cm.specF0[2]);
         s->mats[m].emissive = V3(cm.emissiveColor[0], cm.emissiveColor[1], cm.emissiveColor[2]);
         s->mats[m].roughness = cm.roughness;
+        if (d->materialTex) {
+            const cx_material_tex& mt = d->materialTex[m];
+            s->mats[m].diffTex = mt.diffTex;
+            s->mats[m].specTex = mt.specTex;
+            s->mats[m].roughTex = mt.roughTex;
+            s->mats[m].specUsage = mt.specTexUsage;
+        }
+    }
+    if (d->uvs) s->uv.assign(d->uvs, d->uvs + 2 * (size_t)d->numVertices);
+    s->textures.resize(d->numTextures);
+    for (uint32_t t = 0; t < d->numTextures; t++) {
+        s->textures[t].w = d->textures[t].width;
+        s->textures[t].h = d->textures[t].height;
+        s->textures[t].rgba.assign(d->textures[t].rgba,
+                                   d->textures[t].rgba + 4 * (size_t)d->textures[t].width * d->textures[t].height);
     }
     s->emitters.resize(d->numEmitters);
     for (uint32_t e = 0; e < d->numEmitters; e++) {
@@ -2738,7 +2845,7 @@ void orc_renderer_adjoint(void* rp, const float* dLdI, double* gA, double* gR, d
 void orc_scene_set_materials(void* sp, const cx_material* mats, uint32_t n, int tables) {
     Scene& s = *(Scene*)sp;
     for (uint32_t m = 0; m < n && m < s.mats.size(); m++) {
-        Material x;
+        Material x = s.mats[m];   // texture bindings stay
         x.diff = V3(mats[m].diffColor[0], mats[m].diffColor[1], mats[m].diffColor[2]);
         x.spec = V3(mats[m].specF0[0], mats[m].specF0[1], mats[m].specF0[2]);
         x.emissive = V3(mats[m].emissiveColor[0], mats[m].emissiveColor[1], mats[m].emissiveColor[2]);
@@ -2746,6 +2853,15 @@ void orc_scene_set_materials(void* sp, const cx_material* mats, uint32_t n, int 
         if (tables & 1) s.mats[m] = x;
         if (tables & 2) s.evalMats[m] = x;
     }
+}
+
+// Samples one texture at n (u, v) pairs with the canonical filter; out [n][3].
+void orc_texture_sample(const cx_texture* tx, const float* uv, uint32_t n, float* out) {
+    Texture t;
+    t.w = tx->width;
+    t.h = tx->height;
+    t.rgba.assign(tx->rgba, tx->rgba + 4 * (size_t)t.w * t.h);
+    for (uint32_t i = 0; i < n; i++) v3out(tex_fetch(t, uv[2 * i], uv[2 * i + 1]), &out[3 * i]);
 }
 
 void orc_scene_set_emitter_colors(void* sp, const float* colors, uint32_t n, int tables) {

@@ -91,6 +91,7 @@ def lib():
         L.orc_scene_set_materials.argtypes = [vp, vp, u32, i32]
         L.orc_scene_set_emitter_colors.argtypes = [vp, vp, u32, i32]
         L.orc_tonemap.argtypes = [vp, vp, u32]
+        L.orc_texture_sample.argtypes = [vp, vp, u32, vp]
         L.orc_num_threads.restype = i32
         L.orc_set_num_threads.argtypes = [i32]
         _lib = L
@@ -113,6 +114,16 @@ def tonemap(hdr: np.ndarray) -> np.ndarray:
     h = np.ascontiguousarray(hdr, np.float32)
     out = np.zeros(h.shape, np.uint8)
     lib().orc_tonemap(_p(h), _p(out), h.size)
+    return out
+
+
+def texture_sample(rgba: np.ndarray, uv: np.ndarray) -> np.ndarray:
+    """The canonical bilinear wrap sampler on one RGBA8 image [H, W, 4] at uv [n, 2]; [n, 3]."""
+    img = np.ascontiguousarray(rgba, np.uint8)
+    t = abi.Texture(img.ctypes.data, img.shape[1], img.shape[0])
+    q = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.zeros((q.shape[0], 3), np.float32)
+    lib().orc_texture_sample(C.byref(t), _p(q), q.shape[0], _p(out))
     return out
 
 

@@ -35,7 +35,7 @@ map_Kd lamp.png
 
 
 def test_parse_obj_and_mtl():
-    P, N, T, tm, mats = assets.parse_obj(OBJ, lambda name: MTL)
+    P, N, UV, T, tm, mats = assets.parse_obj(OBJ, lambda name: MTL)
     assert T.shape == (5, 3)                     # quad -> 2, then 3 triangles
     assert tm == ["red", "red", "lamp", "lamp", "undefined_one"]
     assert np.allclose(N[:6], [0, 0, 1])         # given normals
@@ -44,7 +44,39 @@ def test_parse_obj_and_mtl():
     e = np.cross(P[10] - P[9], P[11] - P[9])
     assert np.allclose(n, e / np.linalg.norm(e))
     assert mats["red"]["Pr"] == 0.35 and mats["lamp"]["Ke"] == (10.0, 10.0, 10.0)
-    assert mats["lamp"]["textures"] == ["lamp.png"]
+    assert mats["lamp"]["map_Kd"] == "lamp.png"
+    assert np.all(UV == 0)                       # no vt: uv (0, 0)
+
+
+def test_uvs_and_flip():
+    obj = "mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 1 1 0\nvt 0.25 0.5\nvt 0.75 0.125\nvt 2 -1\nusemtl m\nf 1/3 2/1 3/2\n"
+    mtl = "newmtl m\nKd 1 1 1\nKe 1 1 1\nmap_Kd -s 2 2 1 t.png\nmap_Ks t.png\nmap_Pr r.png\n"
+    P, N, UV, T, tm, mats = assets.parse_obj(obj, lambda n: mtl)
+    assert UV.tolist() == [[2, -1], [0.25, 0.5], [0.75, 0.125]]
+    loaded = []
+    img = {"t.png": np.full((2, 3, 4), 7, np.uint8), "r.png": np.full((1, 1, 4), 9, np.uint8)}
+
+    def tl(name):
+        loaded.append(name)
+        return img[name]
+
+    sc = assets.scene_from_obj(obj, lambda n: mtl, texture_loader=tl, flip_uv=True, spec_tex_usage=1)
+    assert loaded == ["t.png", "r.png"]          # one load per file
+    assert sc.material_tex.tolist() == [[0, 0, 1, 1]]
+    assert np.allclose(sc.uvs, [[2, 2], [0.25, 0.5], [0.75, 0.875]])   # v -> 1 - v
+
+
+def test_load_obj_reads_textures_from_files(tmp_path):
+    from crisoptix_amd import image
+    obj, mtl, tex = assets.cornell_box_obj(textured=True)
+    (tmp_path / "box.obj").write_text(obj)
+    (tmp_path / "cornell.mtl").write_text(mtl)
+    for name, im in tex.items():
+        image.write_png(str(tmp_path / name), im)
+    sc = assets.load_obj(str(tmp_path / "box.obj"), trace_emitters=False)
+    assert len(sc.textures) == 2                  # checker.dds was read as checker.png
+    assert np.array_equal(sc.textures[0], tex["checker.png"]) and np.array_equal(sc.textures[1], tex["rough.png"])
+    assert sc.uvs.shape == (sc.positions.shape[0], 2) and sc.uvs.max() == 3.0
 
 
 def test_scene_from_obj_materials_and_emitters():
@@ -59,7 +91,11 @@ def test_scene_from_obj_materials_and_emitters():
     assert sc2.num_triangles == 3 and sc2.emitter_face_offset.tolist() == [0, 2]
 
 
-def cornell_obj_scene(trace_emitters=False):
+def cornell_obj_scene(trace_emitters=False, textured=False):
+    if textured:
+        obj, mtl, tex = assets.cornell_box_obj(textured=True)
+        return assets.scene_from_obj(obj, lambda name: mtl, trace_emitters=trace_emitters, name="cornell_obj_tex",
+                                     texture_loader=lambda n: tex[n.replace(".dds", ".png")])
     obj, mtl = assets.cornell_box_obj()
     return assets.scene_from_obj(obj, lambda name: mtl, trace_emitters=trace_emitters, name="cornell_obj")
 
@@ -102,10 +138,28 @@ def test_oracle_renders_obj_scene(orc):
     assert img[:, -4:, 1].mean() > 1.5 * img[:, -4:, 0].mean()
 
 
+def test_textured_obj_changes_the_image(orc):
+    W = H = 24
+    cd = abi.CameraData.from_dict(obj_camera(W, H).data())
+    imgs = []
+    for textured in (False, True):
+        os_ = orc.OracleScene(cornell_obj_scene(textured=textured))
+        p = abi.workload_params("path_tracer", W, H, os_.size)
+        p.maxPathLen = 3
+        r = orc.OracleRenderer(os_, W, H, p, seed=2)
+        imgs.append(np.mean([r.frame(cd, i == 0).copy() for i in range(8)], axis=0))
+    # the checker floor (two blues on a white-Kd material) fills the lower rows
+    lo0, lo1 = imgs[0][-6:], imgs[1][-6:]
+    assert np.isfinite(imgs[1]).all()
+    assert lo1[..., 2].mean() / lo1[..., 0].mean() > 1.5 * lo0[..., 2].mean() / lo0[..., 0].mean()
+
+
 @pytest.mark.gpu
+@pytest.mark.parametrize("textured", [False, True])
 @pytest.mark.parametrize("mode", ["path_tracer", "conditional_restir"])
-def test_gpu_obj_scene_bit_exact(mode, orc, cx_runtime, cx_ctx):
-    sc = cornell_obj_scene()
+def test_gpu_obj_scene_bit_exact(mode, textured, orc, cx_runtime, cx_ctx):
+    print(f"obj scene {mode} textured={textured}", flush=True)
+    sc = cornell_obj_scene(textured=textured)
     W, H = 48, 40
     cd = abi.CameraData.from_dict(obj_camera(W, H).data())
     gs = cx_runtime.Scene(cx_ctx, sc)

@@ -39,7 +39,7 @@ def test_struct_layouts_match_header(tmp_path):
     fields = {
         "cx_launch_params": abi.LaunchParams, "cx_camera": abi.CameraData, "cx_material": abi.Material,
         "cx_scene_desc": abi.SceneDesc, "cx_scene_info": abi.SceneInfo, "cx_frame_stats": abi.FrameStats,
-        "cx_state_view": abi.StateView,
+        "cx_state_view": abi.StateView, "cx_texture": abi.Texture, "cx_material_tex": abi.MaterialTex,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in fields.items():

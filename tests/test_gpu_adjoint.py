@@ -11,6 +11,8 @@ and the path tracer:
   error below 1e-4 and all within the FP32 rounding bound (Richardson extrapolation with
   h = 5e-4 is printed beside them);
 * the torch.autograd.Function returns the same gradients as the C ABI call.
+The replay and FP64 checks also run on the textured variant of the scene (diffuse, roughness and
+OccRoughMetal specular textures), where texture-driven values have zero gradient.
 """
 import numpy as np
 import pytest
@@ -25,6 +27,9 @@ FRAMES = 3
 
 def gpu_setup(rt, ctx, mode):
     sc = scenes.inverse_scene()
+    if mode.endswith("_tex"):
+        sc = scenes.with_textures(sc)
+        mode = mode[:-4]
     gs = rt.Scene(ctx, sc)
     name = "restir_pt_cornell" if mode == "restir_pt" else "path_tracer"
     p = abi.workload_params(name, W, H, gs.size)
@@ -51,7 +56,7 @@ def gpu_adjoint(rt, sc, ren, g=None):
     return gA.numpy(), gR.numpy(), gE.numpy(), rep.numpy(), st.numpy()
 
 
-@pytest.mark.parametrize("mode", ["restir_pt", "path_tracer"])
+@pytest.mark.parametrize("mode", ["restir_pt", "path_tracer", "restir_pt_tex"])
 def test_replay_reproduces_frame(mode, cx_runtime, cx_ctx):
     sc, gs, ren, p, cd, out = gpu_setup(cx_runtime, cx_ctx, mode)
     _, _, _, rep, st = gpu_adjoint(cx_runtime, sc, ren)
@@ -60,7 +65,7 @@ def test_replay_reproduces_frame(mode, cx_runtime, cx_ctx):
     assert np.array_equal(rep.view(np.uint32), out.view(np.uint32))
 
 
-@pytest.mark.parametrize("mode", ["restir_pt", "path_tracer"])
+@pytest.mark.parametrize("mode", ["restir_pt", "path_tracer", "restir_pt_tex", "path_tracer_tex"])
 def test_adjoint_matches_fp64_oracle(mode, cx_runtime, cx_ctx, orc):
     sc, gs, ren, p, cd, out = gpu_setup(cx_runtime, cx_ctx, mode)
     g = np.random.default_rng(1).standard_normal((H, W, 3)).astype(np.float32)

@@ -23,12 +23,19 @@ CASES = {
                                   over=dict(gatherRadius=0.08, gatherN=2, suffixSpatialCount=2, windowRadius=6)),
     "cond_grid_orbit": dict(scene="grid3", mode="conditional_restir", W=48, H=32, frames=4, orbit=True,
                             over=dict(gatherRadius=0.3)),
+    # textured materials (diffuse, roughness and OccRoughMetal specular textures, wrapped uvs)
+    "pt_grid_tex": dict(scene="grid3_tex", mode="path_tracer", W=48, H=32, frames=3),
+    "restir_pt_grid_tex": dict(scene="grid3_tex", mode="restir_pt_cornell", W=48, H=32, frames=4),
+    "cond_grid_tex_orbit": dict(scene="grid3_tex", mode="conditional_restir", W=48, H=32, frames=4, orbit=True,
+                                over=dict(gatherRadius=0.3)),
 }
 
 
 def _setup(case, orc, rt, ctx, exec_mode="wavefront"):
     c = CASES[case]
     sc = scenes.cornell_box(1, 8) if c["scene"] == "cornell" else small_grid_scene()
+    if c["scene"].endswith("_tex"):
+        sc = scenes.with_textures(sc)
     W, H = c["W"], c["H"]
     gscene = rt.Scene(ctx, sc)
     oscene = orc.OracleScene(sc)

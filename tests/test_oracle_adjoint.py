@@ -12,8 +12,10 @@ from parity_util import small_grid_scene
 W, H = 48, 32
 
 
-def setup(orc, mode):
+def setup(orc, mode, textured=False):
     sc = small_grid_scene()
+    if textured:
+        sc = scenes.with_textures(sc)
     os_ = orc.OracleScene(sc)
     name = "restir_pt_cornell" if mode == "restir_pt" else "path_tracer"
     p = abi.workload_params(name, W, H, os_.size)
@@ -25,9 +27,11 @@ def setup(orc, mode):
     return sc, os_, r, out
 
 
-@pytest.mark.parametrize("mode", ["restir_pt", "path_tracer"])
-def test_fp64_adjoint_matches_central_differences(orc, mode):
-    sc, os_, r, out = setup(orc, mode)
+@pytest.mark.parametrize("mode,textured", [("restir_pt", False), ("path_tracer", False), ("restir_pt", True)])
+def test_fp64_adjoint_matches_central_differences(orc, mode, textured):
+    """Textured: texture-driven diffuse colours and roughnesses have zero gradient (and zero
+    central difference); OccRoughMetal materials carry the (1 - metal) factor on the albedo."""
+    sc, os_, r, out = setup(orc, mode, textured)
     g = np.random.default_rng(0).standard_normal((H, W, 3)).astype(np.float32)
     gA, gR, gE, rep, st = r.adjoint(g)
     assert st[0] > 100 and st[2] == 0
@@ -35,6 +39,12 @@ def test_fp64_adjoint_matches_central_differences(orc, mode):
     assert np.allclose(rep, out, rtol=1e-4, atol=1e-7)
     mats = sc.materials.astype(np.float32)
     emit = sc.emitter_color.astype(np.float32)
+    if textured:
+        mt = sc.material_tex
+        assert np.all(gA[mt[:, 0] >= 0] == 0)
+        assert np.all(gR[(mt[:, 2] >= 0) | (mt[:, 3] == abi.CX_SPEC_TEX_OCC_ROUGH_METAL)] == 0)
+        metal = (mt[:, 3] == abi.CX_SPEC_TEX_OCC_ROUGH_METAL) & (mt[:, 0] < 0)
+        assert np.abs(gA[metal]).max() > 0 and np.abs(gR[(mt[:, 2] < 0) & ~metal]).max() > 0
 
     def J(m, e):
         os_.set_materials(m, 2)
