@@ -100,6 +100,18 @@ struct cx_renderer {
     uint32_t maxLenSeen = 0;
     uint32_t maxPrefixSeen = 0;
     uint32_t gridLimit = 4096;   // blocks of the grid-stride queue kernels
+    uint64_t allocEpoch = 0;     // bumped whenever a buffer a frame uses is reallocated
+    // Graph mode: a frame is one hipGraphLaunch of a graph captured from the eager frame for
+    // each front-buffer parity; the camera, random and frame index come from `dyn`.
+    bool graphMode = false;
+    bool capturing = false;
+    struct {
+        hipGraphExec_t exec = nullptr;
+        std::vector<uint8_t> key;
+    } graphs[2];
+    FrameDyn* dyn = nullptr;
+    hipStream_t gstream = nullptr;   // capture / replay stream for frames on the null stream
+    uint32_t captures = 0;
 };
 
 namespace {
@@ -180,6 +192,7 @@ int entries_alloc(cx_ctx* ctx, uint32_t capacity, cx_entries** out) {
     A(dalloc(en->allocs, &en->E.keysAlt, n));
     A(dalloc(en->allocs, &en->E.valsAlt, n));
     A(dalloc(en->allocs, &en->E.numValid, 1));
+    A(dalloc(en->allocs, &en->T.activeLevels, 1));
     A(dalloc(en->allocs, &en->T.entrySorted, n));
     A(dalloc(en->allocs, &en->T.entryPixel, n));
     A(dalloc(en->allocs, &en->T.lo, nodes));
@@ -192,6 +205,8 @@ int entries_alloc(cx_ctx* ctx, uint32_t capacity, cx_entries** out) {
         e = hipMalloc(&en->sortTemp, std::max<size_t>(en->sortTempBytes, 16));
         if (e == hipSuccess) en->allocs.push_back(en->sortTemp);
     }
+    if (e == hipSuccess) e = hipMemset(en->T.activeLevels, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(en->E.numValid, 0, sizeof(uint32_t));
     if (e != hipSuccess) {
         free_all(en->allocs);
         delete en;
@@ -242,6 +257,7 @@ FrameArgs make_args(cx_renderer* r) {
     A.neighCount = r->neighCount;
     A.k = lp.gatherM > 1 ? (uint32_t)(lp.gatherM - 1) : 0u;
     A.counters = r->counters;
+    A.dyn = r->capturing ? r->dyn : nullptr;
     return A;
 }
 
@@ -274,6 +290,7 @@ int ensure_neighbors(cx_renderer* r, uint32_t k) {
     CX_HIP(r->ctx, dalloc(r->allocs, &r->neighIdx, (size_t)r->P * k));
     CX_HIP(r->ctx, dalloc(r->allocs, &r->neighDist, (size_t)r->P * k));
     r->kCap = k;
+    r->allocEpoch++;
     return CX_OK;
 }
 
@@ -343,6 +360,7 @@ int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
     }
     w.jobCap = jobs;
     w.ctxWords = words;
+    r->allocEpoch++;
     return CX_OK;
 }
 
@@ -857,6 +875,7 @@ int cx_renderer_create(cx_ctx* ctx, const cx_scene* scene, uint32_t w, uint32_t 
     A(dalloc(r->allocs, &r->gbuf, r->P));
     A(dalloc(r->allocs, &r->neighCount, r->P));
     A(dalloc(r->allocs, &r->counters, NUM_COUNTERS));
+    A(dalloc(r->allocs, &r->dyn, 1));
     if (e != hipSuccess) {
         free_all(r->allocs);
         delete r;
@@ -885,6 +904,9 @@ int cx_renderer_create(cx_ctx* ctx, const cx_scene* scene, uint32_t w, uint32_t 
 
 int cx_renderer_destroy(cx_renderer* r) {
     if (!r) return CX_OK;
+    for (auto& g : r->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (r->gstream) (void)hipStreamDestroy(r->gstream);
     if (r->evMade)
         for (auto& ev : r->ev) (void)hipEventDestroy(ev);
     for (auto& ev : r->traceEv) (void)hipEventDestroy(ev);
@@ -1013,23 +1035,21 @@ int cx_pass_launch(cx_renderer* r, cx_pass pass, const cx_launch_params* params,
     return rc;
 }
 
-int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, float* out, void* stream) {
-    int rc = cx_renderer_begin_frame(r, cam, cameraChanged, out);
-    if (rc != CX_OK) return rc;
+}  // extern "C"
+
+namespace {
+
+// The launches of one frame after cx_renderer_begin_frame (src/Renderer.cpp:259-371); also
+// what a graph-mode capture records.
+int frame_body(cx_renderer* r, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     cx_launch_params& lp = r->params;
-    if (r->timing && !r->evMade) {
-        // Timing-only events: no system-scope fence (cache writeback and invalidation) at
-        // each record, which would both slow the frame and inflate the measured intervals.
-        for (auto& ev : r->ev) CX_HIP(r->ctx, hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
-        r->evMade = true;
-    }
+    int rc = CX_OK;
     for (bool& b : r->passRan) b = false;
     r->traceEvUsed = 0;
     r->traceEvOk = true;
     CX_HIP(r->ctx, launch_zero_counters(r->counters, NUM_COUNTERS, s));
     rec(r, 0, s);
-    // src/Renderer.cpp:259-371
     auto run = [&](cx_pass p) -> int {
         r->passRan[p] = true;
         return cx_pass_launch(r, p, &lp, stream);
@@ -1071,6 +1091,107 @@ int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, f
     r->evValid = r->timing;
     return CX_OK;
 }
+
+
+// What a captured frame bakes in besides its parity: every parameter except the per-frame
+// random and index, the output buffer, the execution settings, the round bounds and the
+// buffers' allocation epoch.
+std::vector<uint8_t> graph_key(const cx_renderer* r) {
+    cx_launch_params p = r->params;
+    p.random = 0;
+    p.frameIdx = 0;
+    p.spatialRoundIdx = 0;
+    std::vector<uint8_t> k(sizeof(p));
+    std::memcpy(k.data(), &p, sizeof(p));
+    auto put = [&](const void* v, size_t n) { k.insert(k.end(), (const uint8_t*)v, (const uint8_t*)v + n); };
+    put(&r->out, sizeof(r->out));
+    put(&r->exec, sizeof(r->exec));
+    put(&r->gridLimit, sizeof(r->gridLimit));
+    put(&r->maxLenSeen, sizeof(r->maxLenSeen));
+    put(&r->maxPrefixSeen, sizeof(r->maxPrefixSeen));
+    put(&r->allocEpoch, sizeof(r->allocEpoch));
+    return k;
+}
+
+int graph_frame(cx_renderer* r, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) {
+        // the legacy null stream cannot be captured; a blocking stream is ordered with it
+        if (!r->gstream) CX_HIP(r->ctx, hipStreamCreate(&r->gstream));
+        s = r->gstream;
+    }
+    // every buffer the frame uses exists before the capture (no allocation inside it)
+    const cx_launch_params& lp = r->params;
+    int rc = ensure_neighbors(r, std::max(lp.gatherM > 1 ? (uint32_t)(lp.gatherM - 1) : 0u, 1u));
+    if (rc == CX_OK && r->exec == CX_EXEC_WAVEFRONT) rc = ensure_wave(r, lp);
+    if (rc != CX_OK) return rc;
+    auto& g = r->graphs[r->front];
+    if (!g.exec || g.key != graph_key(r)) {
+        if (g.exec) {
+            CX_HIP(r->ctx, hipStreamSynchronize(s));
+            (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+        }
+        CX_HIP(r->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        r->capturing = true;
+        rc = frame_body(r, s);
+        r->capturing = false;
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc != CX_OK || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc != CX_OK ? rc : hip_err(r->ctx, e, "frame capture");
+        }
+        const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            g.exec = nullptr;
+            return hip_err(r->ctx, ei, "frame graph instantiation");
+        }
+        // the rounds the capture used are max(seen, current), which is what `seen` now holds
+        g.key = graph_key(r);
+        r->captures++;
+    }
+    FrameDyn d;
+    std::memset(&d, 0, sizeof(d));
+    std::memcpy(d.cam, r->cam.pos, 12);
+    std::memcpy(d.cam + 3, r->cam.U, 12);
+    std::memcpy(d.cam + 6, r->cam.V, 12);
+    std::memcpy(d.cam + 9, r->cam.W, 12);
+    d.random = r->params.random;
+    d.frameIdx = r->params.frameIdx;
+    CX_HIP(r->ctx, launch_set_dyn(r->dyn, d, s));
+    CX_HIP(r->ctx, hipGraphLaunch(g.exec, s));
+    return CX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, float* out, void* stream) {
+    int rc = cx_renderer_begin_frame(r, cam, cameraChanged, out);
+    if (rc != CX_OK) return rc;
+    if (r->graphMode && !r->timing) {
+        r->evValid = false;
+        return graph_frame(r, stream);
+    }
+    if (r->timing && !r->evMade) {
+        // Timing-only events: no system-scope fence (cache writeback and invalidation) at
+        // each record, which would both slow the frame and inflate the measured intervals.
+        for (auto& ev : r->ev) CX_HIP(r->ctx, hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
+        r->evMade = true;
+    }
+    return frame_body(r, stream);
+}
+
+int cx_renderer_set_graph_mode(cx_renderer* r, int enable) {
+    if (!r) return CX_ERR_INVALID_ARG;
+    r->graphMode = enable != 0;
+    return CX_OK;
+}
+
+int cx_renderer_graph_captures(const cx_renderer* r) { return r ? (int)r->captures : -1; }
 
 int cx_renderer_set_queue_grid(cx_renderer* r, uint32_t maxBlocks) {
     if (!r) return CX_ERR_INVALID_ARG;

@@ -5,8 +5,6 @@
 #include "cx_internal.h"
 #include "cx_restir_dev.h"
 
-#include <rocprim/device/device_radix_sort.hpp>
-
 namespace cx {
 
 static constexpr int BLOCK = 128;
@@ -50,7 +48,7 @@ __device__ __forceinline__ void write_output(const FrameArgs& A, uint32_t p, V3 
     V3 r = rad;
     if (A.lp.enableAccum) {
         const V3 old = V3{o[0], o[1], o[2]};
-        const float blend = 1.0f / (float)(A.lp.frameIdx + 1);
+        const float blend = 1.0f / (float)(frame_index(A) + 1);
         r = blend * rad + (1.0f - blend) * old;
     }
     o[0] = r.x;
@@ -66,8 +64,9 @@ __device__ __forceinline__ void camera_ray(const FrameArgs& A, uint32_t x, uint3
     float uy = ((float)y + jy) / (float)A.H;
     ux = 2.0f * ux - 1.0f;
     uy = 2.0f * uy - 1.0f;
-    o = A.camPos;
-    d = normalize(A.camU * ux + A.camV * uy + A.camW);
+    const FrameCam c = frame_cam(A);
+    o = c.pos;
+    d = normalize(c.U * ux + c.V * uy + c.W);
 }
 
 __device__ __forceinline__ void store_gbuf(const FrameArgs& A, uint32_t p, uint64_t rng, uint32_t primaryValid,
@@ -83,7 +82,7 @@ __global__ __launch_bounds__(BLOCK) void k_prefix_gen_temp(FrameArgs A) {
     uint32_t x, y, p;
     if (tile_pixel(A, x, y, p)) {
         const cx_launch_params& lp = A.lp;
-        Rng rng = rng_seed(tea64(p, lp.random));
+        Rng rng = rng_seed(tea64(p, frame_random(A)));
         V3 origin, dir;
         camera_ray(A, x, y, rng, origin, dir);
         if (lp.rendererType == CX_CONDITIONAL_RESTIR || lp.rendererType == CX_RESTIR_PT) {
@@ -462,13 +461,164 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n, unsigned long long
     if (i < n) p[i] = (i == STAT_MIN) ? minInit : 0ull;
 }
 
+// ---- entry sort: least-significant-digit radix sort of the valid entries' Morton keys ----
+//
+// Four passes of 8 bits, each three launches with no host round trip and no memset (the
+// element count is read on the device): per-tile digit counts, a per-digit exclusive scan over
+// the tiles, and a stable scatter.  The first pass reads every slot of the entry arrays, drops
+// the invalid ones (key 0xFFFFFFFF; a Morton key has 30 bits) and takes the slot index as the
+// value, so it also compacts; later passes touch only the valid entries.  The result is the
+// valid entries ordered by (key, slot), a deterministic order.
+static constexpr int RS_THREADS = 256;
+static constexpr int RS_ITEMS = 4;
+static constexpr uint32_t RS_TILE = RS_THREADS * RS_ITEMS;
+static constexpr int RS_WAVES = RS_THREADS / 64;
+
+__device__ __forceinline__ uint32_t rs_count_of(bool first, uint32_t cap, const uint32_t* numValid) {
+    return first ? cap : *numValid;
+}
+
+// counts[d * stride + tile] = number of elements of the tile with digit d.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_count(const uint32_t* keys, uint32_t cap, const uint32_t* numValid,
+                                                        int first, uint32_t shift, uint32_t* counts, uint32_t stride) {
+    __shared__ uint32_t h[256];
+    const uint32_t n = rs_count_of(first != 0, cap, numValid);
+    const uint32_t t0 = blockIdx.x * RS_TILE;
+    if (t0 >= n) return;
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) {
+        const uint32_t idx = t0 + (uint32_t)i * RS_THREADS + threadIdx.x;
+        if (idx < n) {
+            const uint32_t key = keys[idx];
+            if (!first || key != 0xFFFFFFFFu) atomicAdd(&h[(key >> shift) & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    counts[threadIdx.x * stride + blockIdx.x] = h[threadIdx.x];
+}
+
+// Block-wide exclusive scan of one value per thread (RS_THREADS threads); returns the total.
+__device__ __forceinline__ uint32_t rs_block_scan(uint32_t v, uint32_t& total, uint32_t* s_w) {
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63u) s_w[w] = x;
+    __syncthreads();
+    uint32_t before = 0u;
+    total = 0u;
+#pragma unroll
+    for (int k = 0; k < RS_WAVES; k++) {
+        const uint32_t c = s_w[k];
+        if ((uint32_t)k < w) before += c;
+        total += c;
+    }
+    __syncthreads();
+    return before + x - v;
+}
+
+// One block per digit d: exclusive scan of counts[d][0, tiles) in place; totals[d] = the sum.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* counts, uint32_t stride, uint32_t cap,
+                                                       const uint32_t* numValid, int first, uint32_t* totals) {
+    __shared__ uint32_t s_w[RS_WAVES];
+    const uint32_t n = rs_count_of(first != 0, cap, numValid);
+    const uint32_t tiles = (n + RS_TILE - 1) / RS_TILE;
+    uint32_t* row = counts + (size_t)blockIdx.x * stride;
+    uint32_t carry = 0u;
+    for (uint32_t b0 = 0; b0 < tiles; b0 += RS_THREADS) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint32_t v = b < tiles ? row[b] : 0u;
+        uint32_t total;
+        const uint32_t ex = rs_block_scan(v, total, s_w);
+        if (b < tiles) row[b] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+// Stable scatter: element i of the tile goes to base[d] + (elements of digit d in earlier
+// tiles) + (elements of digit d before it in this tile).  Ranks within a wavefront come from
+// ballots over the digit's 8 bits; per-(item round, wavefront, digit) counts are scanned in
+// element order.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* keysIn, const uint32_t* valsIn,
+                                                          uint32_t* keysOut, uint32_t* valsOut, uint32_t cap,
+                                                          const uint32_t* numValid, int first, uint32_t shift,
+                                                          const uint32_t* counts, uint32_t stride,
+                                                          const uint32_t* totals) {
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_cnt[RS_ITEMS * RS_WAVES * 256];
+    __shared__ uint32_t s_w[RS_WAVES];
+    const uint32_t n = rs_count_of(first != 0, cap, numValid);
+    const uint32_t t0 = blockIdx.x * RS_TILE;
+    if (t0 >= n) return;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t dtotal;
+    const uint32_t dbase = rs_block_scan(totals[threadIdx.x], dtotal, s_w);
+    s_base[threadIdx.x] = dbase + counts[threadIdx.x * stride + blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(RS_ITEMS * RS_WAVES * 256); i += RS_THREADS) s_cnt[i] = 0u;
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t key[RS_ITEMS], val[RS_ITEMS], dig[RS_ITEMS], rank[RS_ITEMS];
+    bool ok[RS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) {
+        const uint32_t idx = t0 + (uint32_t)i * RS_THREADS + threadIdx.x;
+        key[i] = idx < n ? keysIn[idx] : 0xFFFFFFFFu;
+        val[i] = idx < n ? (first ? idx : valsIn[idx]) : 0u;
+        ok[i] = idx < n && (!first || key[i] != 0xFFFFFFFFu);
+        dig[i] = (key[i] >> shift) & 255u;
+        unsigned long long m = __ballot(ok[i]);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const unsigned long long b = __ballot(ok[i] && ((dig[i] >> bit) & 1u));
+            m &= ((dig[i] >> bit) & 1u) ? b : ~b;
+        }
+        rank[i] = (uint32_t)__popcll(m & below);
+        if (ok[i] && rank[i] == 0u) s_cnt[(i * RS_WAVES + w) * 256 + dig[i]] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    {
+        uint32_t run = 0u;
+        for (int g = 0; g < RS_ITEMS * RS_WAVES; g++) {
+            const uint32_t c = s_cnt[g * 256 + threadIdx.x];
+            s_cnt[g * 256 + threadIdx.x] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) {
+        if (!ok[i]) continue;
+        const uint32_t pos = s_base[dig[i]] + s_cnt[(i * RS_WAVES + w) * 256 + dig[i]] + rank[i];
+        keysOut[pos] = key[i];
+        valsOut[pos] = val[i];
+    }
+}
+
+// Levels of the tree over the valid entries: leaves = ceil(n / leafSize), rounded up to a
+// power of two (at least one leaf).
+__global__ void k_tree_prep(EntryTree T) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint32_t n = *T.numValid;
+    const uint32_t leaves = n > 0 ? (n + T.leafSize - 1) / T.leafSize : 1u;
+    uint32_t l = 0;
+    while ((1u << l) < leaves) l++;
+    *T.activeLevels = l;
+}
+
 // Leaves of the implicit entry tree: leaf j holds sorted entries [j*L, (j+1)*L) ∩ [0, nValid).
 // Boxes are the union of [pos - r, pos + r], padded outward so the float box test is
 // conservative for every entry whose distance test passes.
-__global__ void k_tree_leaves(EntryTree T, const uint32_t* sortedVals, const float4* entryPos, uint32_t numLeavesMax,
-                              uint32_t leafBase) {
+__global__ void k_tree_leaves(EntryTree T, const uint32_t* sortedVals, const float4* entryPos, uint32_t numLeavesMax) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= numLeavesMax) return;
+    const uint32_t leaves = 1u << *T.activeLevels;
+    if (j >= numLeavesMax || j >= leaves) return;
+    const uint32_t leafBase = leaves - 1u;
     const uint32_t n = *T.numValid;
     const uint32_t b = j * T.leafSize;
     const uint32_t e = min(b + T.leafSize, n);
@@ -502,9 +652,10 @@ __global__ void k_tree_leaves(EntryTree T, const uint32_t* sortedVals, const flo
     T.phi[leafBase + j] = phi;
 }
 
-__global__ void k_tree_level(EntryTree T, uint32_t levelBase, uint32_t count) {
+// Inner nodes of level l (nodes [2^l - 1, 2^(l+1) - 1)), when l is above the leaf level.
+__global__ void k_tree_level(EntryTree T, uint32_t level, uint32_t levelBase, uint32_t count) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= count) return;
+    if (j >= count || level >= *T.activeLevels) return;
     const uint32_t node = levelBase + j;
     const uint32_t c0 = 2 * node + 1, c1 = 2 * node + 2;
     const float4 l0 = T.lo[c0], l1 = T.lo[c1], h0 = T.hi[c0], h1 = T.hi[c1];
@@ -753,6 +904,17 @@ hipError_t launch_zero_counters(unsigned long long* c, uint32_t n, hipStream_t s
     return hipGetLastError();
 }
 
+__global__ void k_set_dyn(FrameDyn* dst, FrameDyn d) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = d;
+}
+
+// The per-frame inputs of a graph replay travel as a kernel argument, so the host's copy is
+// free as soon as the launch returns.
+hipError_t launch_set_dyn(FrameDyn* dst, const FrameDyn& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_dyn, dim3(1), dim3(64), 0, s, dst, d);
+    return hipGetLastError();
+}
+
 hipError_t launch_store_entries(const FrameArgs& A, const EntryBuild& E, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, E.numValid, 1u);
     hipLaunchKernelGGL(k_store_entries, tile_grid(A.W, A.H), dim3(BLOCK), 0, s, A, E);
@@ -766,29 +928,44 @@ hipError_t launch_entries_from_arrays(const float* pos, const float* r, const ui
     return hipGetLastError();
 }
 
+// Scratch of the entry sort: per-digit tile counts [256][tiles] and the digit totals.
 size_t sort_temp_bytes(uint32_t n) {
-    size_t bytes = 0;
-    (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                              (uint32_t*)nullptr, n, 0, 32, (hipStream_t)0);
-    return bytes;
+    const size_t tiles = (std::max<uint32_t>(n, 1u) + RS_TILE - 1) / RS_TILE;
+    return (256 * tiles + 256) * sizeof(uint32_t);
 }
 
-// Sort entries by Morton key and build the implicit tree bottom-up, one launch per level.
+// Sorts the valid entries by Morton key and builds the implicit tree over them bottom-up, one
+// launch per level (levels at or below the tree's leaf level return at once).  n = capacity.
 hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n, void* temp, size_t tempBytes,
                              hipStream_t s) {
-    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tree_prep, dim3(1), dim3(64), 0, s, T);
+    if (n == 0) return hipGetLastError();
     if (sort_temp_bytes(n) > tempBytes) return hipErrorInvalidValue;
-    size_t bytes = tempBytes;
-    hipError_t err = rocprim::radix_sort_pairs(temp, bytes, E.keys, E.keysAlt, E.vals, E.valsAlt, n, 0, 32, s);
-    if (err != hipSuccess) return err;
+    const uint32_t tiles = (n + RS_TILE - 1) / RS_TILE;
+    uint32_t* counts = (uint32_t*)temp;
+    uint32_t* totals = counts + (size_t)256 * tiles;
+    uint32_t* kIn = E.keys;
+    uint32_t* vIn = E.vals;
+    uint32_t* kOut = E.keysAlt;
+    uint32_t* vOut = E.valsAlt;
+    for (int pass = 0; pass < 4; pass++) {
+        const int first = pass == 0 ? 1 : 0;
+        const uint32_t shift = 8u * (uint32_t)pass;
+        hipLaunchKernelGGL(k_rs_count, dim3(tiles), dim3(RS_THREADS), 0, s, kIn, n, E.numValid, first, shift, counts,
+                           tiles);
+        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_THREADS), 0, s, counts, tiles, n, E.numValid, first, totals);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n, E.numValid,
+                           first, shift, counts, tiles, totals);
+        std::swap(kIn, kOut);
+        std::swap(vIn, vOut);
+    }
+    // after four passes the sorted entries are in kIn / vIn (= E.keys / E.vals)
     const uint32_t leaves = 1u << T.levels;
-    const uint32_t leafBase = leaves - 1u;
-    hipLaunchKernelGGL(k_tree_leaves, dim3((leaves + 255) / 256), dim3(256), 0, s, T, E.valsAlt, E.pos, leaves,
-                       leafBase);
+    hipLaunchKernelGGL(k_tree_leaves, dim3((leaves + 255) / 256), dim3(256), 0, s, T, vIn, E.pos, leaves);
     for (int l = (int)T.levels - 1; l >= 0; l--) {
         const uint32_t cnt = 1u << l;
         const uint32_t base = cnt - 1u;
-        hipLaunchKernelGGL(k_tree_level, dim3((cnt + 255) / 256), dim3(256), 0, s, T, base, cnt);
+        hipLaunchKernelGGL(k_tree_level, dim3((cnt + 255) / 256), dim3(256), 0, s, T, (uint32_t)l, base, cnt);
     }
     return hipGetLastError();
 }

@@ -175,11 +175,22 @@ struct EntryTree {
     float4* plo;          // [2^(levels+1) - 1] exact bounds of the entries' positions
     float4* phi;
     const uint32_t* numValid;
-    uint32_t levels;
+    uint32_t* activeLevels;  // device scalar: levels of the tree over the valid entries
+    uint32_t levels;         // levels at full capacity (sizes of the node arrays)
     uint32_t leafSize;
 };
 
 // ---- frame parameters passed by value to every pass kernel ----
+
+// Per-frame inputs of a frame replayed from a captured graph: the camera, the frame's random
+// number and index change every frame and are read from device memory (written by a one-thread
+// kernel ahead of the replay); everything else a graph bakes in stays fixed between captures.
+struct FrameDyn {
+    float cam[12];        // pos, U, V, W
+    uint32_t random;
+    uint32_t pad;
+    uint64_t frameIdx;
+};
 
 struct FrameArgs {
     DScene S;
@@ -198,7 +209,26 @@ struct FrameArgs {
     uint32_t* neighCount;        // [P]
     uint32_t k;
     unsigned long long* counters;   // [2 * pass + {0, 1}] closest / shadow rays, then STAT_*
+    const FrameDyn* dyn;            // null: camera, random and frameIdx are the fields above
 };
+
+struct FrameCam {
+    V3 pos, U, V, W;
+};
+
+__device__ __forceinline__ FrameCam frame_cam(const FrameArgs& A) {
+    if (A.dyn == nullptr) return FrameCam{A.camPos, A.camU, A.camV, A.camW};
+    const float* c = A.dyn->cam;
+    return FrameCam{V3{c[0], c[1], c[2]}, V3{c[3], c[4], c[5]}, V3{c[6], c[7], c[8]}, V3{c[9], c[10], c[11]}};
+}
+
+__device__ __forceinline__ uint32_t frame_random(const FrameArgs& A) {
+    return A.dyn == nullptr ? A.lp.random : A.dyn->random;
+}
+
+__device__ __forceinline__ uint64_t frame_index(const FrameArgs& A) {
+    return A.dyn == nullptr ? A.lp.frameIdx : A.dyn->frameIdx;
+}
 
 // Counter slots after the per-pass ray counters.
 static constexpr int STAT_TOTAL = 2 * CX_PASS_COUNT;
@@ -570,7 +600,7 @@ __device__ __forceinline__ uint32_t entries_query(const EntryTree& T, V3 q, uint
     top.init();
     const uint32_t n = *T.numValid;
     if (k == 0 || n == 0 || isnan(q.x) || isnan(q.y) || isnan(q.z)) return 0;
-    const uint32_t leafBase = (1u << T.levels) - 1u;
+    const uint32_t leafBase = (1u << *T.activeLevels) - 1u;
     float kth = INFINITY;   // k-th smallest distance once k candidates are held
     auto admits = [&](uint32_t node, float& bound) -> bool {
         if (!in_box(q, T.lo[node], T.hi[node])) return false;

@@ -92,7 +92,7 @@ __device__ __forceinline__ void cp_finish(const WaveArgs& W, uint32_t p, const C
     V3 r = rad;
     if (W.A.lp.enableAccum) {
         const V3 old = V3{o[0], o[1], o[2]};
-        const float blend = 1.0f / (float)(W.A.lp.frameIdx + 1);
+        const float blend = 1.0f / (float)(frame_index(W.A) + 1);
         r = blend * rad + (1.0f - blend) * old;
     }
     o[0] = r.x;
@@ -112,15 +112,16 @@ __global__ __launch_bounds__(WBLOCK) void k_w_prefix_init(WaveArgs W) {
     if (active) {
         uint32_t x, y;
         tile_xy(A, p, x, y);
-        Rng rng = rng_seed(tea64(p, A.lp.random));
+        Rng rng = rng_seed(tea64(p, frame_random(A)));
         const float jx = rng.f();
         const float jy = rng.f();
         float ux = ((float)x + jx) / (float)A.W;
         float uy = ((float)y + jy) / (float)A.H;
         ux = 2.0f * ux - 1.0f;
         uy = 2.0f * uy - 1.0f;
-        const V3 origin = A.camPos;
-        const V3 dir = normalize(A.camU * ux + A.camV * uy + A.camW);
+        const FrameCam c = frame_cam(A);
+        const V3 origin = c.pos;
+        const V3 dir = normalize(c.U * ux + c.V * uy + c.W);
         if (restir) {
             pg.rng = rng.s;
             pg.pre.primary = packed_invalid();
@@ -291,7 +292,7 @@ __device__ __forceinline__ void write_out(const FrameArgs& A, uint32_t p, V3 rad
     V3 r = rad;
     if (A.lp.enableAccum) {
         const V3 old = V3{o[0], o[1], o[2]};
-        const float blend = 1.0f / (float)(A.lp.frameIdx + 1);
+        const float blend = 1.0f / (float)(frame_index(A) + 1);
         r = blend * rad + (1.0f - blend) * old;
     }
     o[0] = r.x;
@@ -757,7 +758,8 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_init(WaveArgs W) {
             float uy = ((float)y + jy) / (float)A.H;
             ux = 2.0f * ux - 1.0f;
             uy = 2.0f * uy - 1.0f;
-            const V3 dir = normalize(A.camU * ux + A.camV * uy + A.camW);
+            const FrameCam c = frame_cam(A);
+            const V3 dir = normalize(c.U * ux + c.V * uy + c.W);
             W.camDir[p] = float4{dir.x, dir.y, dir.z, 0.0f};
             pg.rng = rng.s;
             pg.pre.primary = packed_invalid();
@@ -767,7 +769,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_init(WaveArgs W) {
             pg.lightDir = v3(0.0f);
             pg.dist = 0.0f;
             pg.color = v3(0.0f);
-            e = pg_start(A.lp, pg, A.camPos, dir);
+            e = pg_start(A.lp, pg, c.pos, dir);
             if (e.kind == EMIT_NONE) pg_finish(W, p, pg);
         }
     }
@@ -972,7 +974,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_fg_init(WaveArgs W) {
             s.p = 1.0f;
             const float4 cd = W.camDir[p];
             s.dir = V3{cd.x, cd.y, cd.z};
-            if (A.lp.prefixLen > 0) e = emit_closest(A.camPos, s.dir);
+            if (A.lp.prefixLen > 0) e = emit_closest(frame_cam(A).pos, s.dir);
             else fg_finish(W, p, s, true, packed_invalid());
         }
     }

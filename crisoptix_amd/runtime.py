@@ -177,6 +177,8 @@ def lib():
         L.cx_batch_frame.argtypes = [vp, vp, vp, vp, vp, u32, vp]
         L.cx_batch_sync.argtypes = [vp]
         L.cx_renderer_set_queue_grid.argtypes = [vp, u32]
+        L.cx_renderer_set_graph_mode.argtypes = [vp, i32]
+        L.cx_renderer_graph_captures.argtypes = [vp]
         L.cx_ctx_create.argtypes = [i32, C.POINTER(vp)]
         L.cx_ctx_destroy.argtypes = [vp]
         L.cx_last_error.restype = C.c_char_p
@@ -420,6 +422,13 @@ class Renderer:
 
     def set_queue_grid(self, max_blocks: int):
         self.ctx.check(lib().cx_renderer_set_queue_grid(self.h, max_blocks), "cx_renderer_set_queue_grid")
+
+    def set_graph_mode(self, enable: bool = True):
+        """Replay each frame from a captured hipGraph (cx_renderer_set_graph_mode); same images."""
+        self.ctx.check(lib().cx_renderer_set_graph_mode(self.h, 1 if enable else 0), "cx_renderer_set_graph_mode")
+
+    def graph_captures(self) -> int:
+        return int(lib().cx_renderer_graph_captures(self.h))
 
     def exec_mode(self) -> str:
         m = lib().cx_renderer_get_exec_mode(self.h)

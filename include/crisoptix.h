@@ -330,6 +330,18 @@ int cx_batch_sync(cx_ctx* ctx);
  * of 128 threads (default 4096); a tuning knob (crisoptix_amd/tune.py), results unchanged. */
 int cx_renderer_set_queue_grid(cx_renderer* r, uint32_t maxBlocks);
 
+/* Graph mode (no counterpart in the reference, which launches each pass with optixLaunch):
+ * cx_renderer_frame records the frame's launches once into a hipGraph per front-buffer parity
+ * and then replays it with one hipGraphLaunch per frame; the camera, the frame's random number
+ * and its index reach the replay through a small device buffer written by a one-thread kernel
+ * ahead of it.  A graph is recaptured when the parameters (other than random and frameIdx), the
+ * output buffer, the execution mode, the queue grid, the round bounds or a buffer allocation
+ * change.  Images are identical to the eager frame's.  Frames with timing enabled run eagerly.
+ * A frame on the null stream is captured and replayed on a renderer-owned blocking stream. */
+int cx_renderer_set_graph_mode(cx_renderer* r, int enable);
+/* Number of frame graphs captured so far (a diagnostic for the recapture rules). */
+int cx_renderer_graph_captures(const cx_renderer* r);
+
 /* Timings are gathered with hipEvents when enabled (off by default). */
 int cx_renderer_enable_timing(cx_renderer* r, int enable);
 int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* stats);

@@ -1,0 +1,111 @@
+"""Graph mode (cx_renderer_set_graph_mode): each frame is one replay of a hipGraph captured
+from the eager frame, per front-buffer parity, with the camera, random number and frame index
+read from device memory.  Every replay is checked bit for bit against an eager renderer of the
+same seed, with that renderer's eager frames and host synchronisations between the replays;
+parameter changes, a reset, the megakernel mode and a multi-view batch recapture or replay as
+the rules say.  The frame's entry sort and tree build are memset-free kernels, so a capture
+records kernel and copy nodes only."""
+import math
+
+import numpy as np
+import pytest
+
+from crisoptix_amd import abi, scenes
+from parity_util import small_grid_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(rt, ctx, mode, W, H, exec_mode="wavefront"):
+    sc = scenes.cornell_box(1, 8) if mode != "conditional_restir" else small_grid_scene()
+    gs = rt.Scene(ctx, sc)
+    p = abi.workload_params(mode, W, H, gs.size)
+    if mode == "conditional_restir":
+        p.gatherRadius = 0.3
+    make_cam = (lambda: scenes.cornell_camera(W, H)) if mode != "conditional_restir" else \
+        (lambda: scenes.orbit_camera(sc, W, H))
+    ce, cg = make_cam(), make_cam()
+    e = rt.Renderer(W, H, ctx, ce, gs, params=p, seed=77)
+    g = rt.Renderer(W, H, ctx, cg, gs, params=p, seed=77)
+    e.set_exec_mode(exec_mode)
+    g.set_exec_mode(exec_mode)
+    g.set_graph_mode(True)
+    return ce, cg, e, g, p
+
+
+def _step(ce, cg, e, g, orbit, f, tag):
+    if orbit:
+        ce.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
+        cg.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
+    changed = f == 0 or orbit
+    oe = e.launch_frame(abi.CameraData.from_dict(ce.data()), changed).numpy()
+    og = g.launch_frame(abi.CameraData.from_dict(cg.data()), changed).numpy()
+    assert np.array_equal(oe.view(np.uint32), og.view(np.uint32)), (tag, f)
+    return oe
+
+
+@pytest.mark.parametrize("mode,exec_mode", [("path_tracer", "wavefront"), ("restir_pt_cornell", "wavefront"),
+                                            ("conditional_restir", "wavefront"),
+                                            ("conditional_restir", "megakernel")])
+def test_graph_replays_equal_eager_frames(mode, exec_mode, cx_runtime, cx_ctx):
+    print(f"graph {mode} {exec_mode}", flush=True)
+    W, H = 48, 32
+    ce, cg, e, g, p = _pair(cx_runtime, cx_ctx, mode, W, H, exec_mode)
+    orbit = mode == "conditional_restir"
+    for f in range(6):
+        out = _step(ce, cg, e, g, orbit, f, "first")
+    assert g.graph_captures() == 2          # one capture per front-buffer parity
+    assert np.isfinite(out).all() and out.mean() > 0
+    # a parameter change recaptures both parities
+    p2 = abi.LaunchParams.from_buffer_copy(p)
+    p2.maxPathLen = max(2, int(p.maxPathLen) - 1)
+    e.set_params(p2)
+    g.set_params(p2)
+    for f in range(6, 10):
+        _step(ce, cg, e, g, orbit, f, "params")
+    assert g.graph_captures() == 4
+    # a reset restarts the parity sequence; the round bounds shrink, so both recapture again
+    e.reset()
+    g.reset()
+    for f in range(10, 14):
+        _step(ce, cg, e, g, orbit, f, "reset")
+    assert g.graph_captures() == 6
+
+
+def test_graph_batch_equals_eager(cx_runtime, cx_ctx):
+    hosts = [scenes.object_grid(2, seed=41, ground_quads=20, ico_subdiv=2),
+             scenes.object_grid(3, seed=42, ground_quads=30, ico_subdiv=2)]
+    gs = [cx_runtime.Scene(cx_ctx, h) for h in hosts]
+    specs = [(0, 64, 48), (1, 96, 56), (0, 40, 32), (1, 128, 72)]
+
+    def views(graph):
+        cams, rens = [], []
+        for i, (si, w, h) in enumerate(specs):
+            cam = scenes.orbit_camera(hosts[si], w, h)
+            cam.rotate_around_origin([0, 1, 0], 0.7 * i)
+            p = abi.workload_params("conditional_restir", w, h, gs[si].size)
+            p.maxPathLen = 6
+            r = cx_runtime.Renderer(w, h, cx_ctx, cam, gs[si], params=p, seed=50 + i)
+            r.set_graph_mode(graph)
+            rens.append(r)
+            cams.append(cam)
+        return cams, rens
+
+    def advance(cams):
+        out = []
+        for c in cams:
+            c.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
+            out.append(abi.CameraData.from_dict(c.data()))
+        return out
+
+    cp, rp = views(False)
+    cb, rb = views(True)
+    batch = cx_runtime.Batch(cx_ctx, rb)
+    for f in range(5):
+        for r, cd in zip(rp, advance(cp)):
+            r.launch_frame(cd, True)
+        batch.frame(advance(cb), [True] * len(rb))
+        batch.sync()
+        for i, (a, b) in enumerate(zip(rp, rb)):
+            assert np.array_equal(a.output.numpy().view(np.uint32), b.output.numpy().view(np.uint32)), (f, i)
+    assert all(r.graph_captures() == 2 for r in rb)
