@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--relmse-frames", type=int, default=4096,
                     help="accumulated frames of the rel-MSE measurement (0 = skip)")
     ap.add_argument("--bvh-width", type=int, choices=[2, 4, 8], default=4, help="children per BVH node")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run the entry build and neighbour query in sequence instead of beside the gather walks")
     return ap.parse_args()
 
 
@@ -160,6 +162,7 @@ def main():
     ren = runtime.Renderer(W, H, ctx, cam, gscene, params=params, seed=1)
     ren.enable_timing(True)
     ren.set_exec_mode(a.exec)
+    ren.set_overlap(not a.no_overlap)
 
     def frame():
         cam.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
@@ -251,6 +254,7 @@ def main():
         "config": {
             "workload": f"conditional_restir_{W}x{H}_objgrid{a.grid}",
             "exec_mode": a.exec,
+            "final_phase_overlap": not a.no_overlap,
             "resolution": [W, H],
             "triangles": scene.num_triangles,
             "max_path_len": int(params.maxPathLen),

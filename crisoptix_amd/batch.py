@@ -10,11 +10,12 @@ gathered suffixes.
 
 plain:   for each view, cx_renderer_frame on the default stream, one after another;
 batched: cx_batch_frame, every view on its own stream of the context's pool, so the small views'
-         launch-bound passes overlap with the others' work.
+         launch-bound passes overlap with the others' work; with --graph each view's frame is
+         one replay of its captured hipGraph (cx_renderer_set_graph_mode).
 Both run the same renderers' code on the same seeds: the per-view images are compared bit for
 bit every frame (equal images, so equal rel-MSE against any reference).
 
-    python -m crisoptix_amd.batch [--frames 8] [--tune]
+    python -m crisoptix_amd.batch [--frames 8] [--tune] [--graph]
 """
 from __future__ import annotations
 
@@ -54,7 +55,7 @@ def make_params(w, h, size):
 
 
 class Views:
-    def __init__(self, ctx, hosts, gscenes, specs, settings=None):
+    def __init__(self, ctx, hosts, gscenes, specs, settings=None, graph=False):
         self.cams, self.rens, self.specs = [], [], specs
         for i, s in enumerate(specs):
             sc, gs = hosts[s["scene"]], gscenes[s["scene"]]
@@ -67,6 +68,7 @@ class Views:
                 ren.set_exec_mode(st["exec"])
                 if st.get("queue_grid"):
                     ren.set_queue_grid(st["queue_grid"])
+            ren.set_graph_mode(graph)
             self.cams.append(cam)
             self.rens.append(ren)
 
@@ -104,7 +106,7 @@ def run(a) -> dict:
     gscenes = [runtime.Scene(ctx, sc, width=widths[i]) for i, sc in enumerate(hosts)]
     t_build = time.perf_counter() - t0
     plain = Views(ctx, hosts, gscenes, specs, settings)
-    batched = Views(ctx, hosts, gscenes, specs, settings)
+    batched = Views(ctx, hosts, gscenes, specs, settings, graph=a.graph)
     batch = runtime.Batch(ctx, batched.rens)
     P = sum(s["width"] * s["height"] for s in specs)
     mismatches = 0
@@ -144,6 +146,7 @@ def run(a) -> dict:
         "mpix_per_s_batched": P / ms_batch / 1e3,
         "image_mismatches": mismatches,
         "tuned": tuning,
+        "graph_mode": bool(a.graph),
         "build_seconds": t_build,
     }
 
@@ -154,6 +157,7 @@ def main(argv=None):
     ap.add_argument("--frames", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tune", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="batched views replay captured frame graphs")
     ap.add_argument("--tune-frames", type=int, default=4)
     ap.add_argument("--tune-cache")
     a = ap.parse_args(argv)

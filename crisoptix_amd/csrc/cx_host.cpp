@@ -112,6 +112,13 @@ struct cx_renderer {
     FrameDyn* dyn = nullptr;
     hipStream_t gstream = nullptr;   // capture / replay stream for frames on the null stream
     uint32_t captures = 0;
+    // Final-phase overlap (wavefront Conditional ReSTIR): the entry store and the entry-tree
+    // build run on `side` beside the gather's canonical prefix and suffix walks, joined by an
+    // event before the neighbour query.
+    bool overlap = true;
+    hipStream_t side = nullptr;       // non-blocking, default priority
+    hipEvent_t evFork = nullptr, evQuery = nullptr;
+    hipStream_t fgSide = nullptr;     // set while frame_body launches the final gather
 };
 
 namespace {
@@ -495,8 +502,14 @@ hipError_t wave_pass(cx_renderer* r, int pass, const FrameArgs& A, hipStream_t s
             return e;
         case CX_PASS_FINAL_GATHER: {
             const bool more = A.k > 0 && lp.gatherN > 1;
+            hipStream_t side = r->fgSide;
             e = run_machine(r, W, WK_GATHER_INIT, WK_PG_STEP, P, pgRounds, pass, s);
             if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SUFFIX_INIT, WK_SG_STEP, P, sgRounds, pass, s);
+            // the entry store and tree build, forked to `side`, join before the query
+            if (e == hipSuccess && side) {
+                e = hipEventRecord(r->evQuery, side);
+                if (e == hipSuccess) e = hipStreamWaitEvent(s, r->evQuery, 0);
+            }
             W.gIdx = 0;
             if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SETUP, WK_SH_STEP, shiftCap, shSuffixRounds, pass, s);
             if (e == hipSuccess) e = wave_launch(more ? WK_GATHER_COMBINE : WK_GATHER_COMBINE_LAST, W, P, s);
@@ -904,6 +917,10 @@ int cx_renderer_create(cx_ctx* ctx, const cx_scene* scene, uint32_t w, uint32_t 
 
 int cx_renderer_destroy(cx_renderer* r) {
     if (!r) return CX_OK;
+    if (r->side) (void)hipStreamSynchronize(r->side);
+    for (hipEvent_t ev : {r->evFork, r->evQuery})
+        if (ev) (void)hipEventDestroy(ev);
+    if (r->side) (void)hipStreamDestroy(r->side);
     for (auto& g : r->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (r->gstream) (void)hipStreamDestroy(r->gstream);
@@ -1039,6 +1056,18 @@ int cx_pass_launch(cx_renderer* r, cx_pass pass, const cx_launch_params* params,
 
 namespace {
 
+bool overlap_final(const cx_renderer* r) {
+    return r->overlap && r->exec == CX_EXEC_WAVEFRONT && r->params.rendererType == CX_CONDITIONAL_RESTIR;
+}
+
+int ensure_side(cx_renderer* r) {
+    if (r->side) return CX_OK;
+    CX_HIP(r->ctx, hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
+    for (hipEvent_t* ev : {&r->evFork, &r->evQuery})
+        CX_HIP(r->ctx, hipEventCreateWithFlags(ev, hipEventDisableTiming));
+    return CX_OK;
+}
+
 // The launches of one frame after cx_renderer_begin_frame (src/Renderer.cpp:259-371); also
 // what a graph-mode capture records.
 int frame_body(cx_renderer* r, void* stream) {
@@ -1076,16 +1105,32 @@ int frame_body(cx_renderer* r, void* stream) {
         }
     }
     rec(r, 4, s);
+    // With the overlap, the entry store and build go to the side stream, forked here, and the
+    // final gather joins it before its shifts; their time is then inside the final gather's.
+    const bool overlap = cond && overlap_final(r);
+    if (overlap) {
+        if ((rc = ensure_side(r))) return rc;
+        CX_HIP(r->ctx, hipEventRecord(r->evFork, s));
+        CX_HIP(r->ctx, hipStreamWaitEvent(r->side, r->evFork, 0));
+    }
+    void* entryStream = overlap ? (void*)r->side : stream;
+    auto runOn = [&](cx_pass p, void* st) -> int {
+        r->passRan[p] = true;
+        return cx_pass_launch(r, p, &lp, st);
+    };
     if (cond) {
-        if ((rc = run(CX_PASS_PREFIX_STORE_ENTRIES))) return rc;
+        if ((rc = runOn(CX_PASS_PREFIX_STORE_ENTRIES, entryStream))) return rc;
     }
     rec(r, 5, s);
     if (cond) {
-        if ((rc = run(CX_PASS_ENTRIES_BUILD))) return rc;
+        if ((rc = runOn(CX_PASS_ENTRIES_BUILD, entryStream))) return rc;
     }
     rec(r, 6, s);
     if (cond) {
-        if ((rc = run(CX_PASS_FINAL_GATHER))) return rc;
+        r->fgSide = overlap ? r->side : nullptr;
+        rc = run(CX_PASS_FINAL_GATHER);
+        r->fgSide = nullptr;
+        if (rc) return rc;
     }
     rec(r, 7, s);
     r->evValid = r->timing;
@@ -1110,6 +1155,7 @@ std::vector<uint8_t> graph_key(const cx_renderer* r) {
     put(&r->maxLenSeen, sizeof(r->maxLenSeen));
     put(&r->maxPrefixSeen, sizeof(r->maxPrefixSeen));
     put(&r->allocEpoch, sizeof(r->allocEpoch));
+    put(&r->overlap, sizeof(r->overlap));
     return k;
 }
 
@@ -1124,6 +1170,7 @@ int graph_frame(cx_renderer* r, void* stream) {
     const cx_launch_params& lp = r->params;
     int rc = ensure_neighbors(r, std::max(lp.gatherM > 1 ? (uint32_t)(lp.gatherM - 1) : 0u, 1u));
     if (rc == CX_OK && r->exec == CX_EXEC_WAVEFRONT) rc = ensure_wave(r, lp);
+    if (rc == CX_OK && overlap_final(r)) rc = ensure_side(r);
     if (rc != CX_OK) return rc;
     auto& g = r->graphs[r->front];
     if (!g.exec || g.key != graph_key(r)) {
@@ -1192,6 +1239,12 @@ int cx_renderer_set_graph_mode(cx_renderer* r, int enable) {
 }
 
 int cx_renderer_graph_captures(const cx_renderer* r) { return r ? (int)r->captures : -1; }
+
+int cx_renderer_set_overlap(cx_renderer* r, int enable) {
+    if (!r) return CX_ERR_INVALID_ARG;
+    r->overlap = enable != 0;
+    return CX_OK;
+}
 
 int cx_renderer_set_queue_grid(cx_renderer* r, uint32_t maxBlocks) {
     if (!r) return CX_ERR_INVALID_ARG;

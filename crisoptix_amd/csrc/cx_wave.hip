@@ -787,12 +787,15 @@ __device__ __forceinline__ uint32_t query_neighbors(const WaveArgs& W, uint32_t 
     float* nDist = W.A.neighDist + (size_t)p * k;
     const uint32_t count = entries_query<KR>(W.A.tree, q, k, stack, stride, nIdx, nDist);
     W.A.neighCount[p] = count;
+    return count;
+}
+
+__device__ __forceinline__ void neighbor_stats(const WaveArgs& W, uint32_t count) {
     if (W.A.lp.trackPrefixStats) {
         atomicAdd(&W.A.counters[STAT_TOTAL], (unsigned long long)count);
         atomicMin(&W.A.counters[STAT_MIN], (unsigned long long)count);
         atomicMax(&W.A.counters[STAT_MAX], (unsigned long long)count);
     }
-    return count;
 }
 
 // After the canonical prefix and suffix: neighbour query and the two shifts per valid
@@ -825,6 +828,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_setup(WaveArgs W) {
         }
         if (run) {
             count = query_neighbors<KR>(W, p, q, s_st + threadIdx.x, WBLOCK);
+            neighbor_stats(W, count);
             for (uint32_t i = 0; i < count; ++i) {
                 if (!((load_suffix_flags(A.suffixRes[A.front], A.P, A.neighIdx[(size_t)p * k + i]) >> 16) & 1u))
                     continue;

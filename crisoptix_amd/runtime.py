@@ -178,6 +178,7 @@ def lib():
         L.cx_batch_sync.argtypes = [vp]
         L.cx_renderer_set_queue_grid.argtypes = [vp, u32]
         L.cx_renderer_set_graph_mode.argtypes = [vp, i32]
+        L.cx_renderer_set_overlap.argtypes = [vp, i32]
         L.cx_renderer_graph_captures.argtypes = [vp]
         L.cx_ctx_create.argtypes = [i32, C.POINTER(vp)]
         L.cx_ctx_destroy.argtypes = [vp]
@@ -426,6 +427,10 @@ class Renderer:
     def set_graph_mode(self, enable: bool = True):
         """Replay each frame from a captured hipGraph (cx_renderer_set_graph_mode); same images."""
         self.ctx.check(lib().cx_renderer_set_graph_mode(self.h, 1 if enable else 0), "cx_renderer_set_graph_mode")
+
+    def set_overlap(self, enable: bool = True):
+        """Entry build and neighbour query beside the gather walks (cx_renderer_set_overlap)."""
+        self.ctx.check(lib().cx_renderer_set_overlap(self.h, 1 if enable else 0), "cx_renderer_set_overlap")
 
     def graph_captures(self) -> int:
         return int(lib().cx_renderer_graph_captures(self.h))

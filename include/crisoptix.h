@@ -330,6 +330,14 @@ int cx_batch_sync(cx_ctx* ctx);
  * of 128 threads (default 4096); a tuning knob (crisoptix_amd/tune.py), results unchanged. */
 int cx_renderer_set_queue_grid(cx_renderer* r, uint32_t maxBlocks);
 
+/* Final-phase overlap (on by default; wavefront Conditional ReSTIR): cx_renderer_frame runs the
+ * prefix-entry store and the entry-tree build (src/Renderer.cpp:351-366) on a renderer-owned
+ * stream beside the final gather's canonical prefix and suffix walks, which do not read them,
+ * and joins it before the gather's neighbour query.  Results are identical either way; with the
+ * overlap on, the per-pass timings of the store and build passes are part of the final
+ * gather's. */
+int cx_renderer_set_overlap(cx_renderer* r, int enable);
+
 /* Graph mode (no counterpart in the reference, which launches each pass with optixLaunch):
  * cx_renderer_frame records the frame's launches once into a hipGraph per front-buffer parity
  * and then replays it with one hipGraphLaunch per frame; the camera, the frame's random number

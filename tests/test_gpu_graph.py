@@ -109,3 +109,42 @@ def test_graph_batch_equals_eager(cx_runtime, cx_ctx):
         for i, (a, b) in enumerate(zip(rp, rb)):
             assert np.array_equal(a.output.numpy().view(np.uint32), b.output.numpy().view(np.uint32)), (f, i)
     assert all(r.graph_captures() == 2 for r in rb)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_final_phase_overlap_is_invisible(graph, cx_runtime, cx_ctx, orc):
+    """The entry store and tree build on the side stream (the default) against the sequential
+    order and the oracle: images, neighbour lists and the prefix statistics (trackPrefixStats)
+    agree bit for bit, frame after frame."""
+    print(f"overlap graph={graph}", flush=True)
+    sc = scenes.cornell_box(1, 8)          # a closed scene: many valid entries and neighbours
+    W, H = 48, 32
+    gs = cx_runtime.Scene(cx_ctx, sc)
+    p = abi.workload_params("conditional_restir", W, H, gs.size)
+    p.gatherRadius = 0.08
+    p.trackPrefixStats = 1
+    cams = [scenes.cornell_camera(W, H) for _ in range(3)]
+    on = cx_runtime.Renderer(W, H, cx_ctx, cams[0], gs, params=p, seed=5)
+    off = cx_runtime.Renderer(W, H, cx_ctx, cams[1], gs, params=p, seed=5)
+    off.set_overlap(False)
+    on.set_graph_mode(graph)
+    oren = orc.OracleRenderer(orc.OracleScene(sc), W, H, p, seed=5)
+    for f in range(5):
+        cds = [abi.CameraData.from_dict(c.data()) for c in cams]
+        a = on.launch_frame(cds[0], f == 0).numpy()
+        b = off.launch_frame(cds[1], f == 0).numpy()
+        o = oren.frame(cds[2], f == 0)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f
+        assert np.array_equal(a.view(np.uint32), o.view(np.uint32)), f
+        ia, da, ca = on.neighbors()
+        ib, db, cb = off.neighbors()
+        io, do, co = oren.neighbors()
+        assert np.array_equal(ca, cb) and np.array_equal(ca, co), f
+        for q in np.flatnonzero(ca):
+            n = ca[q]
+            assert np.array_equal(ia[q, :n], io[q, :n]) and np.array_equal(ib[q, :n], io[q, :n])
+            assert np.array_equal(da[q, :n].view(np.uint32), do[q, :n].view(np.uint32))
+        sa, sb = on.stats(), off.stats()
+        assert (sa.totalNeighCount, sa.minNeighCount, sa.maxNeighCount) == \
+               (sb.totalNeighCount, sb.minNeighCount, sb.maxNeighCount), f
+    assert ca.sum() > 50
