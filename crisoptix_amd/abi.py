@@ -96,7 +96,7 @@ class FrameStats(C.Structure):
         ("shadowRays", C.c_uint64), ("passClosestRays", C.c_uint64 * CX_PASS_COUNT),
         ("passShadowRays", C.c_uint64 * CX_PASS_COUNT), ("minNeighCount", C.c_uint32), ("maxNeighCount", C.c_uint32),
         ("totalNeighCount", C.c_uint64), ("numEntries", C.c_uint32), ("traceLaunches", C.c_uint32),
-        ("traceMs", C.c_float), ("pad1", C.c_uint32),
+        ("traceMs", C.c_float), ("queueOverflow", C.c_uint32),
     ]
 
 

@@ -318,6 +318,9 @@ size_t shift_jobs(const cx_renderer* r, const cx_launch_params& lp) {
     return (size_t)2 * slots * r->P;
 }
 
+// Two queues' stripe counters, then the shared overflow flag.
+constexpr size_t COUNT_WORDS = 2 * QSTRIPES * QCOUNT_STRIDE + 16;
+
 int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
     auto& w = r->wave;
     hipError_t e = hipSuccess;
@@ -336,14 +339,18 @@ int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
         A(dalloc(w.pixelAllocs, &w.radScratch, P));
         A(dalloc(w.pixelAllocs, &w.fgScratch, P));
         A(dalloc(w.pixelAllocs, &w.camDir, P));
-        A(dalloc(w.pixelAllocs, &w.count, 4));
+        A(dalloc(w.pixelAllocs, &w.count, COUNT_WORDS));
+        if (e == hipSuccess) e = hipMemset(w.count, 0, COUNT_WORDS * sizeof(uint32_t));
         if (e != hipSuccess) {
             free_all(w.pixelAllocs);
             return hip_err(r->ctx, e, "wavefront scratch allocation");
         }
     }
-    const size_t jobs = std::max<size_t>(shift_jobs(r, lp), r->P);
-    const size_t words = std::max<size_t>((size_t)PG_CHUNKS * r->P, (size_t)SH_CHUNKS * jobs);
+    // slots of the striped queues of the pixel machines and of the shift machines
+    const size_t capP = (size_t)QSTRIPES * queue_stripe_cap(r->P, r->P, r->gridLimit);
+    const size_t capShift = (size_t)QSTRIPES * queue_stripe_cap((uint32_t)shift_jobs(r, lp), r->P, r->gridLimit);
+    const size_t jobs = std::max(capP, capShift);
+    const size_t words = std::max<size_t>((size_t)PG_CHUNKS * capP, (size_t)SH_CHUNKS * capShift);
     static_assert(PG_CHUNKS >= SG_CHUNKS && PG_CHUNKS >= CP_CHUNKS && PG_CHUNKS >= FG_CHUNKS,
                   "path machines fit the prefix-generation context");
     if (jobs <= w.jobCap && words <= w.ctxWords) return CX_OK;
@@ -371,15 +378,20 @@ int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
     return CX_OK;
 }
 
-Queue queue_view(const cx_renderer* r, int side, uint32_t cap) {
+// A view of queue `side` for a machine of at most `jobs` jobs (ensure_wave allocated every
+// machine's striped size).
+Queue queue_view(const cx_renderer* r, int side, uint32_t jobs) {
     Queue q;
     q.ctx = r->wave.ctx[side];
     q.job = r->wave.job[side];
     q.ray = r->wave.ray[side];
     q.res = r->wave.res[side];
-    q.count = r->wave.count + 2 * side;
-    q.cap = cap;
+    q.count = r->wave.count + (size_t)side * QSTRIPES * QCOUNT_STRIDE;
+    q.overflow = r->wave.count + (size_t)2 * QSTRIPES * QCOUNT_STRIDE;
+    q.jobs = jobs;
     q.maxBlocks = r->gridLimit;
+    q.stripeCap = queue_stripe_cap(jobs, r->P, r->gridLimit);
+    q.cap = QSTRIPES * q.stripeCap;
     return q;
 }
 
@@ -434,6 +446,7 @@ hipError_t run_machine(cx_renderer* r, WaveArgs W, int setup, int step, uint32_t
                        int pass, hipStream_t s) {
     W.qin = queue_view(r, 1, cap);
     W.qout = queue_view(r, 0, cap);
+    if ((size_t)W.qout.cap > r->wave.jobCap) return hipErrorInvalidValue;   // ensure_wave sized every machine
     hipError_t e = wave_zero_counts(W.qout.count, s);
     if (e == hipSuccess) e = wave_launch(setup, W, r->P, s);
     int cur = 0;
@@ -455,7 +468,7 @@ hipError_t wave_pass(cx_renderer* r, int pass, const FrameArgs& A, hipStream_t s
     const cx_launch_params& lp = A.lp;
     WaveArgs W = make_wave_args(r, A);
     const uint32_t P = r->P;
-    const uint32_t shiftCap = (uint32_t)r->wave.jobCap;
+    const uint32_t shiftCap = (uint32_t)shift_jobs(r, lp);
     const uint32_t L = std::max(r->maxLenSeen, (uint32_t)lp.maxPathLen);
     const uint32_t PL = std::max(r->maxPrefixSeen, prefix_max_len(lp));
     // Longest trace chains, each round answering one ray per job:
@@ -1317,6 +1330,11 @@ int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* st) {
     uint32_t nv = 0;
     CX_HIP(r->ctx, hipMemcpy(&nv, r->entries->E.numValid, 4, hipMemcpyDeviceToHost));
     st->numEntries = nv;
+    if (r->wave.count) {
+        uint32_t ovf = 0;
+        CX_HIP(r->ctx, hipMemcpy(&ovf, r->wave.count + (size_t)2 * QSTRIPES * QCOUNT_STRIDE, 4, hipMemcpyDeviceToHost));
+        st->queueOverflow = ovf;
+    }
     if (r->evValid) {
         for (int i = 0; i < CX_PASS_COUNT; i++) {
             float ms = 0.0f;

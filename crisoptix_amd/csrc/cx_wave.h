@@ -4,10 +4,12 @@
 //
 // A machine step consumes the answer to the job's previous ray, runs the reference's logic up
 // to the next trace call, and either appends a new ray to the next queue or finishes.  Closest-
-// hit rays are appended from the front of a queue and shadow rays from the back, so each
-// traversal kernel walks one contiguous range.  Appends are wave-aggregated (one atomic per
-// wavefront, ballot + prefix count), so a wavefront's surviving jobs land in consecutive slots
-// and both the context stores and the next loads are coalesced.
+// hit rays are appended from the front of a queue stripe and shadow rays from its back.
+// Appends are wave-aggregated (one atomic per wavefront, ballot + prefix count), so a
+// wavefront's surviving jobs land in consecutive slots and both the context stores and the
+// next loads are coalesced.  A queue has 64 stripes with a counter pair each (wavefront w
+// appends to stripe w mod 64): device-scope atomics on one address serialise at about 11 ns
+// each on MI355X (372 us for the 32 k wavefronts of a 1080p pass), 64 addresses do not.
 //
 // Every machine computes exactly what the corresponding function of cx_restir_dev.h computes;
 // only the points where control leaves for a traversal differ.  Traversal is a pure function
@@ -22,15 +24,87 @@ namespace cx {
 
 static constexpr uint32_t EMIT_NONE = 0, EMIT_CLOSEST = 1, EMIT_SHADOW = 2;
 
+static constexpr uint32_t QSTRIPES = 64;       // one per lane of the wavefront that scans them
+static constexpr uint32_t QCOUNT_STRIDE = 16;  // words between stripes' counters (64 bytes)
+
 struct Queue {
     uint4* ctx;         // [chunks][cap]
     uint32_t* job;      // [cap]
     float4* ray;        // [2][cap] origin.xyz tmin | dir.xyz tmax
     uint4* res;         // [cap] closest: prim t u v; shadow: occluded
-    uint32_t* count;    // [2] closest count (front), shadow count (back)
-    uint32_t cap;
+    uint32_t* count;    // [QSTRIPES][QCOUNT_STRIDE]: closest count, shadow count of each stripe
+    uint32_t* overflow; // set when an append finds its stripe full (sizing makes that impossible)
+    uint32_t cap;       // QSTRIPES * stripeCap slots
+    uint32_t stripeCap; // slots of a stripe: closest rays from its start, shadow rays from its end
+    uint32_t jobs;      // most jobs the machine can have (sets the grid of its step kernels)
     uint32_t maxBlocks; // grid-stride kernels over this queue launch at most this many blocks
 };
+
+static constexpr uint32_t QUEUE_BLOCK = 128;   // threads per block of the queue kernels
+
+// Blocks of a grid-stride launch over a machine with at most `jobs` jobs: enough for all of
+// them, at most maxBlocks (the renderer's queue-grid setting).
+CX_HD uint32_t queue_grid_blocks(uint32_t jobs, uint32_t maxBlocks) {
+    const uint32_t need = (uint32_t)(((uint64_t)jobs + QUEUE_BLOCK - 1) / QUEUE_BLOCK);
+    const uint32_t cap = maxBlocks > 0 ? maxBlocks : 1u;
+    return need < 1u ? 1u : (need < cap ? need : cap);
+}
+
+// Slots per stripe for a machine of at most `jobs` jobs over P pixels: the most appends one
+// stripe can receive.  Wavefront w appends to stripe w mod 64, at most 64 rays per append
+// call.  A setup kernel (one thread per pixel, 2 wavefronts per block) makes ceil(jobs / P)
+// append calls per lane; a step kernel makes one per loop pass of its grid-stride loop.
+CX_HD uint32_t queue_stripe_cap(uint32_t jobs, uint32_t P, uint32_t maxBlocks) {
+    auto cdiv = [](uint64_t a, uint64_t b) { return (a + b - 1) / b; };
+    const uint64_t setupWaves = 2 * cdiv(P, QUEUE_BLOCK);
+    const uint64_t perLane = cdiv(jobs, P > 0 ? P : 1);
+    const uint64_t setup = cdiv(setupWaves, QSTRIPES) * 64 * perLane;
+    const uint64_t nb = queue_grid_blocks(jobs, maxBlocks);
+    const uint64_t step = cdiv(2 * nb, QSTRIPES) * 64 * cdiv(jobs, QUEUE_BLOCK * nb);
+    uint64_t c = cdiv(jobs, QSTRIPES);
+    if (setup > c) c = setup;
+    if (step > c) c = step;
+    return (uint32_t)c;
+}
+
+// Exclusive prefix sums of a queue's stripe counts, per block in LDS: the closest range of
+// the input is stripe 0's closest rays, then stripe 1's, ...; the shadow range likewise.
+struct QIndex {
+    uint32_t pc[QSTRIPES + 1];
+    uint32_t ps[QSTRIPES + 1];
+};
+
+// Every thread of the block calls it (one __syncthreads).
+__device__ __forceinline__ void qindex_load(const Queue& q, QIndex& x) {
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x;
+        uint32_t c = q.count[lane * QCOUNT_STRIDE], s = q.count[lane * QCOUNT_STRIDE + 1];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t yc = __shfl_up(c, off, 64), ys = __shfl_up(s, off, 64);
+            if (lane >= (uint32_t)off) {
+                c += yc;
+                s += ys;
+            }
+        }
+        x.pc[lane + 1] = c;
+        x.ps[lane + 1] = s;
+        if (lane == 0) {
+            x.pc[0] = 0u;
+            x.ps[0] = 0u;
+        }
+    }
+    __syncthreads();
+}
+
+// The stripe s with pre[s] <= i < pre[s + 1] (pre non-decreasing, pre[0] = 0 <= i).
+__device__ __forceinline__ uint32_t stripe_of(const uint32_t* pre, uint32_t i) {
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t step = QSTRIPES / 2; step > 0; step >>= 1)
+        if (pre[s + step] <= i) s += step;
+    return s;
+}
 
 struct Emit {
     uint32_t kind;
@@ -43,26 +117,40 @@ CX_HD Emit emit_closest(V3 o, V3 d) { return Emit{EMIT_CLOSEST, o, d, 1e-3f, 1e1
 CX_HD Emit emit_shadow(V3 o, V3 d, float tmax) { return Emit{EMIT_SHADOW, o, d, 1e-3f, tmax}; }
 
 // Slot of the input queue handled by thread i: the closest range [0, nc) first, then the
-// shadow range [cap - ns, cap).
-__device__ __forceinline__ uint32_t in_slot(const Queue& q, uint32_t i, uint32_t nc, uint32_t ns) {
-    return i < nc ? i : q.cap - ns + (i - nc);
+// shadow range, each walked stripe by stripe.
+__device__ __forceinline__ uint32_t in_slot(const Queue& q, const QIndex& x, uint32_t i) {
+    const uint32_t nc = x.pc[QSTRIPES];
+    if (i < nc) {
+        const uint32_t s = stripe_of(x.pc, i);
+        return s * q.stripeCap + (i - x.pc[s]);
+    }
+    const uint32_t j = i - nc;
+    const uint32_t s = stripe_of(x.ps, j);
+    return s * q.stripeCap + q.stripeCap - 1u - (j - x.ps[s]);
 }
 
-// Wave-aggregated append; every lane of the wavefront must call it.  Returns the slot.
+// Wave-aggregated append to the wavefront's stripe; every lane of the wavefront must call it.
+// Returns the slot.  A full stripe (never reached: the host sizes stripes for the most
+// appends any stripe can receive) raises the overflow flag and wraps inside the stripe, so no
+// access leaves the queue.
 __device__ __forceinline__ uint32_t append(const Queue& q, uint32_t kind) {
     const uint64_t mc = __ballot(kind == EMIT_CLOSEST);
     const uint64_t ms = __ballot(kind == EMIT_SHADOW);
     const uint32_t lane = __lane_id();
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t stripe = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % QSTRIPES;
+    uint32_t* cnt = q.count + stripe * QCOUNT_STRIDE;
     uint32_t bc = 0, bs = 0;
     if (lane == 0) {
-        if (mc) bc = atomicAdd(&q.count[0], (uint32_t)__popcll(mc));
-        if (ms) bs = atomicAdd(&q.count[1], (uint32_t)__popcll(ms));
+        if (mc) bc = atomicAdd(&cnt[0], (uint32_t)__popcll(mc));
+        if (ms) bs = atomicAdd(&cnt[1], (uint32_t)__popcll(ms));
+        if ((uint64_t)bc + __popcll(mc) + (uint64_t)bs + __popcll(ms) > q.stripeCap) atomicOr(q.overflow, 1u);
     }
     bc = __shfl(bc, 0, 64);
     bs = __shfl(bs, 0, 64);
-    if (kind == EMIT_CLOSEST) return bc + (uint32_t)__popcll(mc & lt);
-    if (kind == EMIT_SHADOW) return q.cap - 1u - (bs + (uint32_t)__popcll(ms & lt));
+    const uint32_t base = stripe * q.stripeCap;
+    if (kind == EMIT_CLOSEST) return base + (bc + (uint32_t)__popcll(mc & lt)) % q.stripeCap;
+    if (kind == EMIT_SHADOW) return base + q.stripeCap - 1u - (bs + (uint32_t)__popcll(ms & lt)) % q.stripeCap;
     return 0xFFFFFFFFu;
 }
 
@@ -222,6 +310,18 @@ __device__ __forceinline__ PgState pg_unpack(const Ctx<PG_CHUNKS>& c) {
 
 __device__ __forceinline__ uint32_t pg_max_len(const cx_launch_params& lp) {
     return is_pt(lp) ? (uint32_t)lp.maxPathLen : (uint32_t)lp.prefixLen;
+}
+
+// The generator fields the init kernels set before pg_start.
+__device__ __forceinline__ void pg_reset(PgState& s, uint64_t rng) {
+    s.rng = rng;
+    s.pre.primary = packed_invalid();
+    s.pre.recon = packed_invalid();
+    s.pre.last = packed_invalid();
+    s.pre.reconOutDir = v3(0.0f);
+    s.lightDir = v3(0.0f);
+    s.dist = 0.0f;
+    s.color = v3(0.0f);
 }
 
 // Start of TracePrefix with the camera ray: returns the first emission.
@@ -532,6 +632,65 @@ __device__ __forceinline__ CpState cp_unpack(const Ctx<CP_CHUNKS>& c) {
     s.cur = r.v(); s.dir = r.v(); s.thr = r.v(); s.it = r.pk(); s.lightDir = r.v(); s.dist = r.f(); s.color = r.v();
     s.radiance = v3(0.0f);
     return s;
+}
+
+// The state of a complete path before its first trace.
+__device__ __forceinline__ void cp_reset(CpState& s, uint64_t rng, V3 origin, V3 dir) {
+    s.rng = rng;
+    s.traceIdx = 0;
+    s.depth = 0;
+    s.state = CP_TRACE;
+    s.cur = origin;
+    s.dir = dir;
+    s.thr = v3(1.0f);
+    s.radiance = v3(0.0f);
+    s.it = packed_invalid();
+    s.lightDir = v3(0.0f);
+    s.dist = 0.0f;
+    s.color = v3(0.0f);
+}
+
+// ---- fresh contexts: a walk before its first trace is a function of its generator state and
+// its first ray, which the queue holds.  An init kernel stores only chunk 0 = {rng, CTX_FRESH}
+// (word 2 holds the prefix flags or the machine's state word, where bit 31 is never set) and
+// the first step rebuilds the rest, instead of a full context write and read per pixel. ----
+static constexpr uint32_t CTX_FRESH = 0x80000000u;
+
+__device__ __forceinline__ void store_fresh(const Queue& q, uint32_t slot, uint64_t rng) {
+    q.ctx[slot] = uint4{(uint32_t)rng, (uint32_t)(rng >> 32), CTX_FRESH, 0u};
+}
+
+__device__ __forceinline__ PgState pg_load(const Queue& q, uint32_t slot, const cx_launch_params& lp) {
+    const uint4 c0 = q.ctx[slot];
+    PgState s;
+    if (c0.z & CTX_FRESH) {
+        V3 o, d;
+        get_ray(q, slot, o, d);
+        pg_reset(s, (uint64_t)c0.x | ((uint64_t)c0.y << 32));
+        (void)pg_start(lp, s, o, d);   // re-emits the queued camera ray
+        return s;
+    }
+    Ctx<PG_CHUNKS> c;
+    c.w[0] = c0;
+#pragma unroll
+    for (int i = 1; i < PG_CHUNKS; i++) c.w[i] = q.ctx[(size_t)i * q.cap + slot];
+    return pg_unpack(c);
+}
+
+__device__ __forceinline__ CpState cp_load(const Queue& q, uint32_t slot) {
+    const uint4 c0 = q.ctx[slot];
+    CpState s;
+    if (c0.z & CTX_FRESH) {
+        V3 o, d;
+        get_ray(q, slot, o, d);
+        cp_reset(s, (uint64_t)c0.x | ((uint64_t)c0.y << 32), o, d);
+        return s;
+    }
+    Ctx<CP_CHUNKS> c;
+    c.w[0] = c0;
+#pragma unroll
+    for (int i = 1; i < CP_CHUNKS; i++) c.w[i] = q.ctx[(size_t)i * q.cap + slot];
+    return cp_unpack(c);
 }
 
 // Returns an emission; when none, `done` holds the radiance in s.radiance.

@@ -13,6 +13,7 @@
 namespace cx {
 
 static constexpr int WBLOCK = 128;
+static_assert(WBLOCK == (int)QUEUE_BLOCK, "queue kernels and the stripe sizing agree on the block size");
 static constexpr int SHORT_STACK = 16;
 
 __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -25,11 +26,13 @@ __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + thr
 template <int WIDTH>
 __global__ __launch_bounds__(WBLOCK) void k_q_trace(DScene S, Queue qin, Queue qout, unsigned long long* counters) {
     __shared__ int s_st[SHORT_STACK * WBLOCK];
-    const uint32_t nc = qin.count[0], ns = qin.count[1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        qout.count[0] = 0u;
-        qout.count[1] = 0u;
-        if (counters) {
+    __shared__ QIndex qi;
+    qindex_load(qin, qi);
+    const uint32_t nc = qi.pc[QSTRIPES], ns = qi.ps[QSTRIPES];
+    if (blockIdx.x == 0 && threadIdx.x < QSTRIPES) {
+        qout.count[threadIdx.x * QCOUNT_STRIDE] = 0u;
+        qout.count[threadIdx.x * QCOUNT_STRIDE + 1] = 0u;
+        if (threadIdx.x == 0 && counters) {
             if (nc) atomicAdd(&counters[0], (unsigned long long)nc);
             if (ns) atomicAdd(&counters[1], (unsigned long long)ns);
         }
@@ -39,7 +42,7 @@ __global__ __launch_bounds__(WBLOCK) void k_q_trace(DScene S, Queue qin, Queue q
     st.stride = WBLOCK;
     const uint32_t n = nc + ns;
     for (uint32_t i = blockIdx.x * WBLOCK + threadIdx.x; i < n; i += gridDim.x * WBLOCK) {
-        const uint32_t slot = in_slot(qin, i, nc, ns);
+        const uint32_t slot = in_slot(qin, qi, i);
         const float4 a = qin.ray[slot], b = qin.ray[(size_t)qin.cap + slot];
         Hit h;
         if (i < nc) {
@@ -56,7 +59,9 @@ __global__ __launch_bounds__(WBLOCK) void k_q_trace(DScene S, Queue qin, Queue q
 // Grid-stride loop over the jobs of qin whose bound is uniform per block, so every lane of a
 // wavefront reaches each wave-aggregated append.
 #define CX_FOR_JOBS(W)                                                         \
-    const uint32_t _nc = (W).qin.count[0], _ns = (W).qin.count[1];           \
+    __shared__ QIndex _qi;                                                    \
+    qindex_load((W).qin, _qi);                                                \
+    const uint32_t _nc = _qi.pc[QSTRIPES], _ns = _qi.ps[QSTRIPES];            \
     const uint32_t _n = _nc + _ns;                                            \
     for (uint32_t _base = blockIdx.x * WBLOCK; _base < _n; _base += gridDim.x * WBLOCK)
 
@@ -123,37 +128,18 @@ __global__ __launch_bounds__(WBLOCK) void k_w_prefix_init(WaveArgs W) {
         const V3 origin = c.pos;
         const V3 dir = normalize(c.U * ux + c.V * uy + c.W);
         if (restir) {
-            pg.rng = rng.s;
-            pg.pre.primary = packed_invalid();
-            pg.pre.recon = packed_invalid();
-            pg.pre.last = packed_invalid();
-            pg.pre.reconOutDir = v3(0.0f);
-            pg.lightDir = v3(0.0f);
-            pg.dist = 0.0f;
-            pg.color = v3(0.0f);
+            pg_reset(pg, rng.s);
             e = pg_start(A.lp, pg, origin, dir);
             if (e.kind == EMIT_NONE) pg_finish(W, p, pg);
         } else {
-            cp.rng = rng.s;
-            cp.traceIdx = 0;
-            cp.depth = 0;
-            cp.state = CP_TRACE;
-            cp.cur = origin;
-            cp.dir = dir;
-            cp.thr = v3(1.0f);
-            cp.radiance = v3(0.0f);
-            cp.it = packed_invalid();
-            cp.lightDir = v3(0.0f);
-            cp.dist = 0.0f;
-            cp.color = v3(0.0f);
+            cp_reset(cp, rng.s, origin, dir);
             if (A.lp.maxPathLen > 0) e = emit_closest(origin, dir);
             else cp_finish(W, p, cp);
         }
     }
     const uint32_t slot = append(W.qout, e.kind);
     if (active && e.kind != EMIT_NONE) {
-        if (restir) store_ctx(W.qout, slot, pg_pack(pg));
-        else store_ctx(W.qout, slot, cp_pack(cp));
+        store_fresh(W.qout, slot, restir ? pg.rng : cp.rng);
         emit_job(W.qout, slot, p, e);
     }
 }
@@ -166,9 +152,9 @@ __global__ __launch_bounds__(WBLOCK) void k_w_pg_step(WaveArgs W) {
     PgState s;
     uint32_t job = 0;
     if (active) {
-        const uint32_t slot = in_slot(W.qin, i, _nc, _ns);
+        const uint32_t slot = in_slot(W.qin, _qi, i);
         job = W.qin.job[slot];
-        s = pg_unpack(load_ctx<PG_CHUNKS>(W.qin, slot));
+        s = pg_load(W.qin, slot, W.A.lp);
         const uint4 r = W.qin.res[slot];
         if (s.state == PG_TRACE) {
             V3 o, d;
@@ -195,9 +181,9 @@ __global__ __launch_bounds__(WBLOCK) void k_w_cp_step(WaveArgs W) {
     CpState s;
     uint32_t job = 0;
     if (active) {
-        const uint32_t slot = in_slot(W.qin, i, _nc, _ns);
+        const uint32_t slot = in_slot(W.qin, _qi, i);
         job = W.qin.job[slot];
-        s = cp_unpack(load_ctx<CP_CHUNKS>(W.qin, slot));
+        s = cp_load(W.qin, slot);
         const uint4 r = W.qin.res[slot];
         if (s.state == CP_TRACE) {
             V3 o, d;
@@ -263,7 +249,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_sh_step(WaveArgs W) {
     ShState s;
     uint32_t job = 0;
     if (active) {
-        const uint32_t slot = in_slot(W.qin, i, _nc, _ns);
+        const uint32_t slot = in_slot(W.qin, _qi, i);
         job = W.qin.job[slot];
         s = sh_unpack(load_ctx<SH_CHUNKS>(W.qin, slot));
         const uint4 r = W.qin.res[slot];
@@ -622,7 +608,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_sg_step(WaveArgs W) {
     SgState s;
     uint32_t job = 0;
     if (active) {
-        const uint32_t slot = in_slot(W.qin, i, _nc, _ns);
+        const uint32_t slot = in_slot(W.qin, _qi, i);
         job = W.qin.job[slot];
         s = sg_unpack(load_ctx<SG_CHUNKS>(W.qin, slot));
         const uint4 r = W.qin.res[slot];
@@ -761,21 +747,14 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_init(WaveArgs W) {
             const FrameCam c = frame_cam(A);
             const V3 dir = normalize(c.U * ux + c.V * uy + c.W);
             W.camDir[p] = float4{dir.x, dir.y, dir.z, 0.0f};
-            pg.rng = rng.s;
-            pg.pre.primary = packed_invalid();
-            pg.pre.recon = packed_invalid();
-            pg.pre.last = packed_invalid();
-            pg.pre.reconOutDir = v3(0.0f);
-            pg.lightDir = v3(0.0f);
-            pg.dist = 0.0f;
-            pg.color = v3(0.0f);
+            pg_reset(pg, rng.s);
             e = pg_start(A.lp, pg, c.pos, dir);
             if (e.kind == EMIT_NONE) pg_finish(W, p, pg);
         }
     }
     const uint32_t slot = append(W.qout, e.kind);
     if (run && e.kind != EMIT_NONE) {
-        store_ctx(W.qout, slot, pg_pack(pg));
+        store_fresh(W.qout, slot, pg.rng);
         emit_job(W.qout, slot, p, e);
     }
 }
@@ -997,7 +976,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_fg_step(WaveArgs W) {
     FgState s;
     uint32_t job = 0;
     if (active) {
-        const uint32_t slot = in_slot(W.qin, i, _nc, _ns);
+        const uint32_t slot = in_slot(W.qin, _qi, i);
         job = W.qin.job[slot];
         s = fg_unpack(load_ctx<FG_CHUNKS>(W.qin, slot));
         V3 o, d;
@@ -1030,8 +1009,12 @@ __global__ __launch_bounds__(WBLOCK) void k_w_fg_step(WaveArgs W) {
     }
 }
 
+// Clears the stripe counters of a queue (one lane per stripe).
 __global__ void k_w_zero2(uint32_t* c) {
-    if (threadIdx.x < 2) c[threadIdx.x] = 0u;
+    if (threadIdx.x < QSTRIPES) {
+        c[threadIdx.x * QCOUNT_STRIDE] = 0u;
+        c[threadIdx.x * QCOUNT_STRIDE + 1] = 0u;
+    }
 }
 
 // ==========================================================================================
@@ -1043,9 +1026,7 @@ static dim3 grid_for(uint32_t n) { return dim3((n + WBLOCK - 1) / WBLOCK); }
 
 // Blocks of a grid-stride launch over a queue: enough for its capacity, at most q.maxBlocks
 // (the renderer's queue-grid setting, 4096 by default: 16 blocks of 128 threads per CU).
-static dim3 stride_grid(const Queue& q) {
-    return dim3(std::max(1u, std::min((q.cap + WBLOCK - 1) / WBLOCK, std::max(1u, q.maxBlocks))));
-}
+static dim3 stride_grid(const Queue& q) { return dim3(queue_grid_blocks(q.jobs, q.maxBlocks)); }
 
 hipError_t wave_zero_counts(uint32_t* c, hipStream_t s) {
     hipLaunchKernelGGL(k_w_zero2, dim3(1), dim3(64), 0, s, c);

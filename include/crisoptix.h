@@ -216,7 +216,9 @@ typedef struct cx_frame_stats {
        frame and their summed duration from HIP events around each launch (timing enabled). */
     uint32_t traceLaunches;
     float traceMs;
-    uint32_t pad1;
+    /* Nonzero if a wavefront queue append ever found its stripe full since the renderer was
+       created (the queues are sized so that it cannot; a diagnostic of that sizing). */
+    uint32_t queueOverflow;
 } cx_frame_stats;
 
 typedef struct cx_ctx cx_ctx;

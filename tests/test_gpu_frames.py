@@ -87,6 +87,7 @@ def test_frames_match_oracle(case, exec_mode, orc, cx_runtime, cx_ctx):
             for y, x in idx:
                 print("   px", x, y, out_o[y, x], out_g[y, x])
     assert worst == 0
+    assert ren.stats().queueOverflow == 0
     front, back, _, _ = oren.buffers()
     st = ren.state()
     assert (st.frontBufferIdx, st.backBufferIdx) == (front, back)
