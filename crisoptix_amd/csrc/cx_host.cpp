@@ -900,7 +900,7 @@ int cx_renderer_create(cx_ctx* ctx, const cx_scene* scene, uint32_t w, uint32_t 
     }
     A(dalloc(r->allocs, &r->gbuf, r->P));
     A(dalloc(r->allocs, &r->neighCount, r->P));
-    A(dalloc(r->allocs, &r->counters, NUM_COUNTERS));
+    A(dalloc(r->allocs, &r->counters, (size_t)COUNTER_STRIPES * NUM_COUNTERS));
     A(dalloc(r->allocs, &r->dyn, 1));
     if (e != hipSuccess) {
         free_all(r->allocs);
@@ -1090,7 +1090,7 @@ int frame_body(cx_renderer* r, void* stream) {
     for (bool& b : r->passRan) b = false;
     r->traceEvUsed = 0;
     r->traceEvOk = true;
-    CX_HIP(r->ctx, launch_zero_counters(r->counters, NUM_COUNTERS, s));
+    CX_HIP(r->ctx, launch_zero_counters(r->counters, COUNTER_STRIPES * NUM_COUNTERS, s));
     rec(r, 0, s);
     auto run = [&](cx_pass p) -> int {
         r->passRan[p] = true;
@@ -1315,9 +1315,18 @@ int cx_renderer_enable_timing(cx_renderer* r, int enable) {
 int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* st) {
     if (!r || !st) return CX_ERR_INVALID_ARG;
     std::memset(st, 0, sizeof(*st));
-    unsigned long long c[NUM_COUNTERS] = {0};
+    std::vector<unsigned long long> cs((size_t)COUNTER_STRIPES * NUM_COUNTERS);
     CX_HIP(r->ctx, hipDeviceSynchronize());
-    CX_HIP(r->ctx, hipMemcpy(c, r->counters, sizeof(c), hipMemcpyDeviceToHost));
+    CX_HIP(r->ctx, hipMemcpy(cs.data(), r->counters, cs.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    unsigned long long c[NUM_COUNTERS] = {0};
+    c[STAT_MIN] = ~0ull;
+    for (int stp = 0; stp < COUNTER_STRIPES; stp++)
+        for (int i = 0; i < NUM_COUNTERS; i++) {
+            const unsigned long long x = cs[(size_t)stp * NUM_COUNTERS + i];
+            if (i == STAT_MIN) c[i] = std::min(c[i], x);
+            else if (i == STAT_MAX) c[i] = std::max(c[i], x);
+            else c[i] += x;
+        }
     for (int i = 0; i < CX_PASS_COUNT; i++) {
         st->passClosestRays[i] = c[2 * i];
         st->passShadowRays[i] = c[2 * i + 1];

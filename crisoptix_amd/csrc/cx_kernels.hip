@@ -32,14 +32,16 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
     return v;
 }
 
-// One atomic per wavefront for the ray counters; counters[2 * pass + {0, 1}] hold the
-// closest-hit and shadow rays of each pass.
+// One atomic per wavefront for the ray counters, into the wavefront's counter stripe;
+// counters[2 * pass + {0, 1}] hold the closest-hit and shadow rays of each pass.
 __device__ __forceinline__ void flush_counters(const TraceCtx& tc, unsigned long long* counters) {
     const unsigned long long c = wave_sum((unsigned long long)tc.closest);
     const unsigned long long s = wave_sum((unsigned long long)tc.shadow);
     if ((threadIdx.x & 63) == 0 && counters != nullptr) {
-        if (c) atomicAdd(&counters[0], c);
-        if (s) atomicAdd(&counters[1], s);
+        unsigned long long* st =
+            counters + (((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % COUNTER_STRIPES) * NUM_COUNTERS;
+        if (c) atomicAdd(&st[0], c);
+        if (s) atomicAdd(&st[1], s);
     }
 }
 
@@ -432,7 +434,6 @@ __global__ __launch_bounds__(BLOCK) void k_store_entries(FrameArgs A, EntryBuild
     E.keys[p] = valid ? morton30(V3{e.x, e.y, e.z}, E.lo, E.scale) : 0xFFFFFFFFu;
     E.vals[p] = p;
     A.neighCount[p] = 0;
-    if (valid) atomicAdd(E.numValid, 1u);
 }
 
 // Standalone entry upload (cx_entries_build): same record as k_store_entries.
@@ -448,7 +449,6 @@ __global__ void k_entries_from_arrays(const float* pos, const float* r, const ui
     E.valid[i] = v ? 1 : 0;
     E.keys[i] = v ? morton30(V3{px, py, pz}, E.lo, E.scale) : 0xFFFFFFFFu;
     E.vals[i] = i;
-    if (v) atomicAdd(E.numValid, 1u);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -458,7 +458,7 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
 
 __global__ void k_zero_u64(unsigned long long* p, uint32_t n, unsigned long long minInit) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = (i == STAT_MIN) ? minInit : 0ull;
+    if (i < n) p[i] = (i % NUM_COUNTERS == (uint32_t)STAT_MIN) ? minInit : 0ull;
 }
 
 // ---- entry sort: least-significant-digit radix sort of the valid entries' Morton keys ----
@@ -560,6 +560,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* keysI
     uint32_t dtotal;
     const uint32_t dbase = rs_block_scan(totals[threadIdx.x], dtotal, s_w);
     s_base[threadIdx.x] = dbase + counts[threadIdx.x * stride + blockIdx.x];
+    // the first pass counts the valid entries: their number is the sum of the digit totals
+    if (first && blockIdx.x == 0 && threadIdx.x == 0) *const_cast<uint32_t*>(numValid) = dtotal;
     for (uint32_t i = threadIdx.x; i < (uint32_t)(RS_ITEMS * RS_WAVES * 256); i += RS_THREADS) s_cnt[i] = 0u;
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
@@ -707,11 +709,7 @@ __device__ __forceinline__ V3 gather_canon(const FrameArgs& A, TraceCtx& tc, uin
     float* nDist = A.neighDist + (size_t)p * k;
     const uint32_t neighCount = entries_query<KR>(A.tree, last.pos, k, tc.stack, tc.stride, nIdx, nDist);
     A.neighCount[p] = neighCount;
-    if (A.lp.trackPrefixStats) {
-        atomicAdd(&A.counters[STAT_TOTAL], (unsigned long long)neighCount);
-        atomicMin(&A.counters[STAT_MIN], (unsigned long long)neighCount);
-        atomicMax(&A.counters[STAT_MAX], (unsigned long long)neighCount);
-    }
+    if (A.lp.trackPrefixStats) neighbor_stat(A.counters, neighCount);
     const uint4* sfront = A.suffixRes[A.front];
     uint32_t validCount = 0, validFlags = 0;
     for (uint32_t i = 0; i < neighCount; ++i) {
@@ -756,11 +754,7 @@ __device__ __forceinline__ V3 gather_not_canon(const FrameArgs& A, TraceCtx& tc,
     float* nDist = A.neighDist + (size_t)p * k;
     const uint32_t neighCount = entries_query<KR>(A.tree, last.pos, k, tc.stack, tc.stride, nIdx, nDist);
     A.neighCount[p] = neighCount;
-    if (A.lp.trackPrefixStats) {
-        atomicAdd(&A.counters[STAT_TOTAL], (unsigned long long)neighCount);
-        atomicMin(&A.counters[STAT_MIN], (unsigned long long)neighCount);
-        atomicMax(&A.counters[STAT_MAX], (unsigned long long)neighCount);
-    }
+    if (A.lp.trackPrefixStats) neighbor_stat(A.counters, neighCount);
     const uint4* sfront = A.suffixRes[A.front];
     uint32_t validCount = 0, validFlags = 0;
     for (uint32_t i = 0; i < neighCount; ++i) {
@@ -900,7 +894,7 @@ hipError_t launch_reset(const FrameArgs& A, hipStream_t s) {
 }
 
 hipError_t launch_zero_counters(unsigned long long* c, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c, n, 0xFFFFFFFFull);
+    hipLaunchKernelGGL(k_zero_u64, dim3((n + 255) / 256), dim3(256), 0, s, c, n, 0xFFFFFFFFull);
     return hipGetLastError();
 }
 
@@ -938,8 +932,10 @@ size_t sort_temp_bytes(uint32_t n) {
 // launch per level (levels at or below the tree's leaf level return at once).  n = capacity.
 hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n, void* temp, size_t tempBytes,
                              hipStream_t s) {
-    hipLaunchKernelGGL(k_tree_prep, dim3(1), dim3(64), 0, s, T);
-    if (n == 0) return hipGetLastError();
+    if (n == 0) {
+        hipLaunchKernelGGL(k_tree_prep, dim3(1), dim3(64), 0, s, T);
+        return hipGetLastError();
+    }
     if (sort_temp_bytes(n) > tempBytes) return hipErrorInvalidValue;
     const uint32_t tiles = (n + RS_TILE - 1) / RS_TILE;
     uint32_t* counts = (uint32_t*)temp;
@@ -956,6 +952,7 @@ hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n
         hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_THREADS), 0, s, counts, tiles, n, E.numValid, first, totals);
         hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n, E.numValid,
                            first, shift, counts, tiles, totals);
+        if (first) hipLaunchKernelGGL(k_tree_prep, dim3(1), dim3(64), 0, s, T);   // needs the valid count
         std::swap(kIn, kOut);
         std::swap(vIn, vOut);
     }

@@ -235,6 +235,18 @@ static constexpr int STAT_TOTAL = 2 * CX_PASS_COUNT;
 static constexpr int STAT_MIN = STAT_TOTAL + 1;
 static constexpr int STAT_MAX = STAT_TOTAL + 2;
 static constexpr int NUM_COUNTERS = 32;
+// The counters are replicated in 64 stripes of NUM_COUNTERS (the host sums them; min and max
+// combine): device-scope atomics on one address serialise, so updates spread over stripes.
+static constexpr int COUNTER_STRIPES = 64;
+
+// Prefix-entry neighbour statistics of one pixel (PrefixAccelStruct::Stats), into the stripe
+// of the calling lane.
+__device__ __forceinline__ void neighbor_stat(unsigned long long* counters, uint32_t count) {
+    unsigned long long* c = counters + ((blockIdx.x * blockDim.x + threadIdx.x) % COUNTER_STRIPES) * NUM_COUNTERS;
+    atomicAdd(&c[STAT_TOTAL], (unsigned long long)count);
+    atomicMin(&c[STAT_MIN], (unsigned long long)count);
+    atomicMax(&c[STAT_MAX], (unsigned long long)count);
+}
 
 CX_HD bool is_pt(const cx_launch_params& lp) { return lp.rendererType == CX_RESTIR_PT; }
 CX_HD bool is_cond(const cx_launch_params& lp) { return lp.rendererType == CX_CONDITIONAL_RESTIR; }

@@ -770,11 +770,7 @@ __device__ __forceinline__ uint32_t query_neighbors(const WaveArgs& W, uint32_t 
 }
 
 __device__ __forceinline__ void neighbor_stats(const WaveArgs& W, uint32_t count) {
-    if (W.A.lp.trackPrefixStats) {
-        atomicAdd(&W.A.counters[STAT_TOTAL], (unsigned long long)count);
-        atomicMin(&W.A.counters[STAT_MIN], (unsigned long long)count);
-        atomicMax(&W.A.counters[STAT_MAX], (unsigned long long)count);
-    }
+    if (W.A.lp.trackPrefixStats) neighbor_stat(W.A.counters, count);
 }
 
 // After the canonical prefix and suffix: neighbour query and the two shifts per valid
