@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--relmse-frames", type=int, default=4096,
                     help="accumulated frames of the rel-MSE measurement (0 = skip)")
     ap.add_argument("--bvh-width", type=int, choices=[2, 4, 8], default=4, help="children per BVH node")
+    ap.add_argument("--wave-rounds", type=int, default=0,
+                    help="trace/step rounds per machine before the in-place tail kernel (0: all rounds)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="run the entry build and neighbour query in sequence instead of beside the gather walks")
     return ap.parse_args()
@@ -163,6 +165,7 @@ def main():
     ren.enable_timing(True)
     ren.set_exec_mode(a.exec)
     ren.set_overlap(not a.no_overlap)
+    ren.set_wave_rounds(a.wave_rounds)
 
     def frame():
         cam.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
@@ -178,6 +181,7 @@ def main():
     total_rays = 0
     trace_ms = 0.0
     trace_launches = 0
+    tail_rays = np.zeros(2)
     neigh = 0
     entries = 0
     gpu_ms = 0.0
@@ -195,6 +199,7 @@ def main():
         entries += st.numEntries
         trace_ms += st.traceMs
         trace_launches += st.traceLaunches
+        tail_rays += (st.tailClosestRays, st.tailShadowRays)
     ms_per_frame = gpu_ms / a.steps
     mrays = total_rays / (gpu_ms / 1000.0) / 1e6
     P = W * H
@@ -206,11 +211,13 @@ def main():
     ray_bytes = (pass_rays[:, 0].sum() * CLOSEST_RAY_BYTES + pass_rays[:, 1].sum() * SHADOW_RAY_BYTES) / a.steps
     if trace_launches:
         # Dominant kernel of the wavefront mode: the traversal launches, algorithmic bytes = the
-        # rays they answer x (32 B ray + 16 B hit, or 32 B ray + 4 B flag).
+        # rays they answer (all rays but those the tail kernels trace in place) x (32 B ray +
+        # 16 B hit, or 32 B ray + 4 B flag).
         roof_kernel = "k_q_trace"
         launches_per_frame = trace_launches / a.steps
         avg_launch_ms = trace_ms / trace_launches
-        bytes_per_launch = ray_bytes / launches_per_frame
+        launch_ray_bytes = ray_bytes - (tail_rays[0] * CLOSEST_RAY_BYTES + tail_rays[1] * SHADOW_RAY_BYTES) / a.steps
+        bytes_per_launch = launch_ray_bytes / launches_per_frame
     else:
         roof_kernel = "k_final_gather"
         avg_launch_ms = fg_ms
@@ -255,6 +262,7 @@ def main():
             "workload": f"conditional_restir_{W}x{H}_objgrid{a.grid}",
             "exec_mode": a.exec,
             "final_phase_overlap": not a.no_overlap,
+            "wave_rounds": a.wave_rounds,
             "resolution": [W, H],
             "triangles": scene.num_triangles,
             "max_path_len": int(params.maxPathLen),
@@ -266,6 +274,7 @@ def main():
             "entries_per_frame": int(entries / a.steps),
             "bvh_width": int(gscene.info.bvhWidth),
             "trace_ms_per_frame": round(trace_ms / a.steps, 4),
+            "tail_rays_per_frame": int(tail_rays.sum() / a.steps),
             "final_gather_pass": {"ms": round(fg_ms, 4), "algorithmic_bytes": int(fg_bytes),
                                   "algorithmic_GBps": round(fg_gbs, 2)},
             "rel_mse_vs_fp64_truth": relmse,

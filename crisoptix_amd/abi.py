@@ -53,7 +53,7 @@ class Material(C.Structure):
                 ("roughness", C.c_float)]
 
 
-ABI_VERSION = 6   # CX_ABI_VERSION of include/crisoptix.h
+ABI_VERSION = 7   # CX_ABI_VERSION of include/crisoptix.h
 CX_SPEC_TEX_COLOR = 0
 CX_SPEC_TEX_OCC_ROUGH_METAL = 1
 
@@ -97,6 +97,7 @@ class FrameStats(C.Structure):
         ("passShadowRays", C.c_uint64 * CX_PASS_COUNT), ("minNeighCount", C.c_uint32), ("maxNeighCount", C.c_uint32),
         ("totalNeighCount", C.c_uint64), ("numEntries", C.c_uint32), ("traceLaunches", C.c_uint32),
         ("traceMs", C.c_float), ("queueOverflow", C.c_uint32),
+        ("tailClosestRays", C.c_uint64), ("tailShadowRays", C.c_uint64),
     ]
 
 

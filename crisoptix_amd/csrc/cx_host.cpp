@@ -100,6 +100,7 @@ struct cx_renderer {
     uint32_t maxLenSeen = 0;
     uint32_t maxPrefixSeen = 0;
     uint32_t gridLimit = 4096;   // blocks of the grid-stride queue kernels
+    uint32_t waveRounds = 2;     // trace/step rounds per machine before the tail kernel (0: none)
     uint64_t allocEpoch = 0;     // bumped whenever a buffer a frame uses is reallocated
     // Graph mode: a frame is one hipGraphLaunch of a graph captured from the eager frame for
     // each front-buffer parity; the camera, random and frame index come from `dyn`.
@@ -450,7 +451,8 @@ hipError_t run_machine(cx_renderer* r, WaveArgs W, int setup, int step, uint32_t
     hipError_t e = wave_zero_counts(W.qout.count, s);
     if (e == hipSuccess) e = wave_launch(setup, W, r->P, s);
     int cur = 0;
-    for (uint32_t i = 0; i < rounds && e == hipSuccess; i++) {
+    const uint32_t waveRounds = (r->waveRounds > 0 && r->waveRounds < rounds) ? r->waveRounds : rounds;
+    for (uint32_t i = 0; i < waveRounds && e == hipSuccess; i++) {
         const Queue qin = queue_view(r, cur, cap), qout = queue_view(r, 1 - cur, cap);
         trace_rec(r, s);
         e = wave_trace(W.A.S, qin, qout, W.A.counters + 2 * pass, s);
@@ -459,6 +461,11 @@ hipError_t run_machine(cx_renderer* r, WaveArgs W, int setup, int step, uint32_t
         W.qout = qout;
         if (e == hipSuccess) e = wave_launch(step, W, 0, s);
         cur = 1 - cur;
+    }
+    if (e == hipSuccess && waveRounds < rounds) {   // the remaining rounds in one launch
+        W.qin = queue_view(r, cur, cap);
+        W.qout = queue_view(r, 1 - cur, cap);
+        e = wave_tail(step, W, rounds - waveRounds, pass, s);
     }
     return e;
 }
@@ -1169,6 +1176,7 @@ std::vector<uint8_t> graph_key(const cx_renderer* r) {
     put(&r->maxPrefixSeen, sizeof(r->maxPrefixSeen));
     put(&r->allocEpoch, sizeof(r->allocEpoch));
     put(&r->overlap, sizeof(r->overlap));
+    put(&r->waveRounds, sizeof(r->waveRounds));
     return k;
 }
 
@@ -1253,6 +1261,12 @@ int cx_renderer_set_graph_mode(cx_renderer* r, int enable) {
 
 int cx_renderer_graph_captures(const cx_renderer* r) { return r ? (int)r->captures : -1; }
 
+int cx_renderer_set_wave_rounds(cx_renderer* r, uint32_t rounds) {
+    if (!r) return CX_ERR_INVALID_ARG;
+    r->waveRounds = rounds;
+    return CX_OK;
+}
+
 int cx_renderer_set_overlap(cx_renderer* r, int enable) {
     if (!r) return CX_ERR_INVALID_ARG;
     r->overlap = enable != 0;
@@ -1333,6 +1347,8 @@ int cx_renderer_get_stats(cx_renderer* r, cx_frame_stats* st) {
         st->closestRays += c[2 * i];
         st->shadowRays += c[2 * i + 1];
     }
+    st->tailClosestRays = c[STAT_TAIL_CLOSEST];
+    st->tailShadowRays = c[STAT_TAIL_SHADOW];
     st->totalNeighCount = c[STAT_TOTAL];
     st->minNeighCount = (uint32_t)std::min<unsigned long long>(c[STAT_MIN], 0xFFFFFFFFull);
     st->maxNeighCount = (uint32_t)c[STAT_MAX];

@@ -68,6 +68,7 @@ enum WaveKernel {
 };
 
 hipError_t wave_zero_counts(uint32_t* c, hipStream_t s);
+hipError_t wave_tail(int step, const WaveArgs& W, uint32_t maxSteps, int pass, hipStream_t s);
 hipError_t wave_trace(const DScene& S, const Queue& qin, const Queue& qout, unsigned long long* counters,
                       hipStream_t s);
 hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStream_t s);

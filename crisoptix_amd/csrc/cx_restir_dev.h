@@ -234,6 +234,8 @@ __device__ __forceinline__ uint64_t frame_index(const FrameArgs& A) {
 static constexpr int STAT_TOTAL = 2 * CX_PASS_COUNT;
 static constexpr int STAT_MIN = STAT_TOTAL + 1;
 static constexpr int STAT_MAX = STAT_TOTAL + 2;
+static constexpr int STAT_TAIL_CLOSEST = STAT_TOTAL + 3;   // rays traced in place by tail kernels
+static constexpr int STAT_TAIL_SHADOW = STAT_TOTAL + 4;
 static constexpr int NUM_COUNTERS = 32;
 // The counters are replicated in 64 stripes of NUM_COUNTERS (the host sums them; min and max
 // combine): device-scope atomics on one address serialise, so updates spread over stripes.

@@ -144,6 +144,16 @@ __global__ __launch_bounds__(WBLOCK) void k_w_prefix_init(WaveArgs W) {
     }
 }
 
+// The step of each machine on the answer r to a job's last ray (origin o, direction d): the
+// reference's logic up to the job's next trace call or its end, where it stores the result.
+// Shared by the step kernels and the tail kernel.
+__device__ __forceinline__ Emit pg_advance(const WaveArgs& W, uint32_t job, PgState& s, uint4 r, V3 o, V3 d) {
+    const Emit e = s.state == PG_TRACE ? pg_on_hit(W.A.S, W.A.lp, s, inter_from_hit(W.A.S, r, d))
+                                       : pg_on_shadow(W.A.S, W.A.lp, s, r.x != 0);
+    if (e.kind == EMIT_NONE) pg_finish(W, job, s);
+    return e;
+}
+
 __global__ __launch_bounds__(WBLOCK) void k_w_pg_step(WaveArgs W) {
     CX_FOR_JOBS(W) {
     const uint32_t i = _base + threadIdx.x;
@@ -155,15 +165,9 @@ __global__ __launch_bounds__(WBLOCK) void k_w_pg_step(WaveArgs W) {
         const uint32_t slot = in_slot(W.qin, _qi, i);
         job = W.qin.job[slot];
         s = pg_load(W.qin, slot, W.A.lp);
-        const uint4 r = W.qin.res[slot];
-        if (s.state == PG_TRACE) {
-            V3 o, d;
-            get_ray(W.qin, slot, o, d);
-            e = pg_on_hit(W.A.S, W.A.lp, s, inter_from_hit(W.A.S, r, d));
-        } else {
-            e = pg_on_shadow(W.A.S, W.A.lp, s, r.x != 0);
-        }
-        if (e.kind == EMIT_NONE) pg_finish(W, job, s);
+        V3 o, d;
+        get_ray(W.qin, slot, o, d);
+        e = pg_advance(W, job, s, W.qin.res[slot], o, d);
     }
     const uint32_t slot = append(W.qout, e.kind);
     if (active && e.kind != EMIT_NONE) {
@@ -171,6 +175,13 @@ __global__ __launch_bounds__(WBLOCK) void k_w_pg_step(WaveArgs W) {
         emit_job(W.qout, slot, job, e);
     }
     }
+}
+
+__device__ __forceinline__ Emit cp_advance(const WaveArgs& W, uint32_t job, CpState& s, uint4 r, V3 o, V3 d) {
+    const Emit e = s.state == CP_TRACE ? cp_on_hit(W.A.S, W.A.lp, s, inter_from_hit(W.A.S, r, d))
+                                       : cp_on_shadow(W.A.S, s, r.x != 0);
+    if (e.kind == EMIT_NONE) cp_finish(W, job, s);
+    return e;
 }
 
 __global__ __launch_bounds__(WBLOCK) void k_w_cp_step(WaveArgs W) {
@@ -184,15 +195,9 @@ __global__ __launch_bounds__(WBLOCK) void k_w_cp_step(WaveArgs W) {
         const uint32_t slot = in_slot(W.qin, _qi, i);
         job = W.qin.job[slot];
         s = cp_load(W.qin, slot);
-        const uint4 r = W.qin.res[slot];
-        if (s.state == CP_TRACE) {
-            V3 o, d;
-            get_ray(W.qin, slot, o, d);
-            e = cp_on_hit(W.A.S, W.A.lp, s, inter_from_hit(W.A.S, r, d));
-        } else {
-            e = cp_on_shadow(W.A.S, s, r.x != 0);
-        }
-        if (e.kind == EMIT_NONE) cp_finish(W, job, s);
+        V3 o, d;
+        get_ray(W.qin, slot, o, d);
+        e = cp_advance(W, job, s, W.qin.res[slot], o, d);
     }
     const uint32_t slot = append(W.qout, e.kind);
     if (active && e.kind != EMIT_NONE) {
@@ -241,6 +246,19 @@ __global__ __launch_bounds__(WBLOCK) void k_w_prefix_temporal_setup(WaveArgs W) 
     }
 }
 
+__device__ __forceinline__ Emit sh_advance(const WaveArgs& W, uint32_t job, ShState& s, uint4 r, V3 o, V3 d) {
+    if (s.state == SH_TRACE) {
+        const Emit e = sh_on_hit(W.A.S, W.R, s, inter_from_hit(W.A.S, r, d));
+        if (e.kind == EMIT_NONE) shift_finish(W, job, v3(0.0f), 0.0f);
+        return e;
+    }
+    V3 f;
+    float J;
+    sh_on_shadow(W.A.S, W.R, s, r.x != 0, f, J);
+    shift_finish(W, job, f, J);
+    return emit_none();
+}
+
 __global__ __launch_bounds__(WBLOCK) void k_w_sh_step(WaveArgs W) {
     CX_FOR_JOBS(W) {
     const uint32_t i = _base + threadIdx.x;
@@ -252,18 +270,9 @@ __global__ __launch_bounds__(WBLOCK) void k_w_sh_step(WaveArgs W) {
         const uint32_t slot = in_slot(W.qin, _qi, i);
         job = W.qin.job[slot];
         s = sh_unpack(load_ctx<SH_CHUNKS>(W.qin, slot));
-        const uint4 r = W.qin.res[slot];
-        if (s.state == SH_TRACE) {
-            V3 o, d;
-            get_ray(W.qin, slot, o, d);
-            e = sh_on_hit(W.A.S, W.R, s, inter_from_hit(W.A.S, r, d));
-            if (e.kind == EMIT_NONE) shift_finish(W, job, v3(0.0f), 0.0f);
-        } else {
-            V3 f;
-            float J;
-            sh_on_shadow(W.A.S, W.R, s, r.x != 0, f, J);
-            shift_finish(W, job, f, J);
-        }
+        V3 o, d;
+        get_ray(W.qin, slot, o, d);
+        e = sh_advance(W, job, s, W.qin.res[slot], o, d);
     }
     const uint32_t slot = append(W.qout, e.kind);
     if (active && e.kind != EMIT_NONE) {
@@ -600,6 +609,15 @@ __global__ __launch_bounds__(WBLOCK) void k_w_suffix_init(WaveArgs W, int gather
     }
 }
 
+__device__ __forceinline__ Emit sg_advance(const WaveArgs& W, uint32_t job, SgState& s, uint4 r, V3 o, V3 d) {
+    Emit e;
+    if (s.state == SG_TRACE) e = sg_on_hit(W.A.S, W.A.lp, s, inter_from_hit(W.A.S, r, d), o);
+    else if (s.state == SG_NEE) e = sg_on_shadow(W.A.S, s, r.x != 0);
+    else e = sg_on_recon(s, inter_from_hit(W.A.S, r, d));
+    if (e.kind == EMIT_NONE) sg_finish_store(W, job, s);
+    return e;
+}
+
 __global__ __launch_bounds__(WBLOCK) void k_w_sg_step(WaveArgs W) {
     CX_FOR_JOBS(W) {
     const uint32_t i = _base + threadIdx.x;
@@ -611,13 +629,9 @@ __global__ __launch_bounds__(WBLOCK) void k_w_sg_step(WaveArgs W) {
         const uint32_t slot = in_slot(W.qin, _qi, i);
         job = W.qin.job[slot];
         s = sg_unpack(load_ctx<SG_CHUNKS>(W.qin, slot));
-        const uint4 r = W.qin.res[slot];
         V3 o, d;
         get_ray(W.qin, slot, o, d);
-        if (s.state == SG_TRACE) e = sg_on_hit(W.A.S, W.A.lp, s, inter_from_hit(W.A.S, r, d), o);
-        else if (s.state == SG_NEE) e = sg_on_shadow(W.A.S, s, r.x != 0);
-        else e = sg_on_recon(s, inter_from_hit(W.A.S, r, d));
-        if (e.kind == EMIT_NONE) sg_finish_store(W, job, s);
+        e = sg_advance(W, job, s, W.qin.res[slot], o, d);
     }
     const uint32_t slot = append(W.qout, e.kind);
     if (active && e.kind != EMIT_NONE) {
@@ -964,6 +978,29 @@ __global__ __launch_bounds__(WBLOCK) void k_w_fg_init(WaveArgs W) {
     }
 }
 
+__device__ __forceinline__ Emit fg_advance(const WaveArgs& W, uint32_t job, FgState& s, uint4 r, V3 o, V3 d) {
+    const Inter it = inter_from_hit(W.A.S, r, d);
+    if (!it.valid) {
+        fg_finish(W, job, s, false, packed_invalid());
+        return emit_none();
+    }
+    Rng rng;
+    rng.s = s.rng;
+    const BrdfSample bs = ggx_sample(W.A.S, it, rng);
+    s.rng = rng.s;
+    if (bs.pdf <= 0.0f) {
+        fg_finish(W, job, s, false, it.pack());
+        return emit_none();
+    }
+    s.thr = s.thr * bs.brdf;
+    s.p *= bs.pdf;
+    s.idx++;
+    s.dir = bs.outDir;
+    if (s.idx < (uint32_t)W.A.lp.prefixLen) return emit_closest(it.pos, bs.outDir);
+    fg_finish(W, job, s, true, it.pack());
+    return emit_none();
+}
+
 __global__ __launch_bounds__(WBLOCK) void k_w_fg_step(WaveArgs W) {
     CX_FOR_JOBS(W) {
     const uint32_t i = _base + threadIdx.x;
@@ -977,31 +1014,126 @@ __global__ __launch_bounds__(WBLOCK) void k_w_fg_step(WaveArgs W) {
         s = fg_unpack(load_ctx<FG_CHUNKS>(W.qin, slot));
         V3 o, d;
         get_ray(W.qin, slot, o, d);
-        const Inter it = inter_from_hit(W.A.S, W.qin.res[slot], d);
-        if (!it.valid) {
-            fg_finish(W, job, s, false, packed_invalid());
-        } else {
-            Rng rng;
-            rng.s = s.rng;
-            const BrdfSample bs = ggx_sample(W.A.S, it, rng);
-            s.rng = rng.s;
-            if (bs.pdf <= 0.0f) {
-                fg_finish(W, job, s, false, it.pack());
-            } else {
-                s.thr = s.thr * bs.brdf;
-                s.p *= bs.pdf;
-                s.idx++;
-                s.dir = bs.outDir;
-                if (s.idx < (uint32_t)W.A.lp.prefixLen) e = emit_closest(it.pos, bs.outDir);
-                else fg_finish(W, job, s, true, it.pack());
-            }
-        }
+        e = fg_advance(W, job, s, W.qin.res[slot], o, d);
     }
     const uint32_t slot = append(W.qout, e.kind);
     if (active && e.kind != EMIT_NONE) {
         store_ctx(W.qout, slot, fg_pack(s));
         emit_job(W.qout, slot, job, e);
     }
+    }
+}
+
+// ======================================================================================
+// Tail: a machine's last rounds in one launch
+// ======================================================================================
+
+template <int M>
+struct Mach;
+template <>
+struct Mach<WK_PG_STEP> {
+    using S = PgState;
+    __device__ static S load(const WaveArgs& W, uint32_t slot) { return pg_load(W.qin, slot, W.A.lp); }
+    __device__ static Emit advance(const WaveArgs& W, uint32_t job, S& s, uint4 r, V3 o, V3 d) {
+        return pg_advance(W, job, s, r, o, d);
+    }
+};
+template <>
+struct Mach<WK_CP_STEP> {
+    using S = CpState;
+    __device__ static S load(const WaveArgs& W, uint32_t slot) { return cp_load(W.qin, slot); }
+    __device__ static Emit advance(const WaveArgs& W, uint32_t job, S& s, uint4 r, V3 o, V3 d) {
+        return cp_advance(W, job, s, r, o, d);
+    }
+};
+template <>
+struct Mach<WK_SG_STEP> {
+    using S = SgState;
+    __device__ static S load(const WaveArgs& W, uint32_t slot) { return sg_unpack(load_ctx<SG_CHUNKS>(W.qin, slot)); }
+    __device__ static Emit advance(const WaveArgs& W, uint32_t job, S& s, uint4 r, V3 o, V3 d) {
+        return sg_advance(W, job, s, r, o, d);
+    }
+};
+template <>
+struct Mach<WK_SH_STEP> {
+    using S = ShState;
+    __device__ static S load(const WaveArgs& W, uint32_t slot) { return sh_unpack(load_ctx<SH_CHUNKS>(W.qin, slot)); }
+    __device__ static Emit advance(const WaveArgs& W, uint32_t job, S& s, uint4 r, V3 o, V3 d) {
+        return sh_advance(W, job, s, r, o, d);
+    }
+};
+template <>
+struct Mach<WK_FG_STEP> {
+    using S = FgState;
+    __device__ static S load(const WaveArgs& W, uint32_t slot) { return fg_unpack(load_ctx<FG_CHUNKS>(W.qin, slot)); }
+    __device__ static Emit advance(const WaveArgs& W, uint32_t job, S& s, uint4 r, V3 o, V3 d) {
+        return fg_advance(W, job, s, r, o, d);
+    }
+};
+
+// Finishes every job of qin in one launch: each thread traces its job's queued ray (the
+// traversal of k_q_trace, so the same answer), steps the job, and keeps tracing the job's next
+// rays in place until it ends, instead of one traversal launch and one step launch per round.
+// For a machine's last rounds, whose queues are short: every job then follows its own chain of
+// rays, without waiting each round for the slowest ray of the whole queue, and the near-empty
+// rounds cost nothing.  maxSteps is the machine's remaining round bound; a job still running
+// after it raises the queue's overflow flag (bit 1).
+template <int M, int WIDTH>
+__global__ __launch_bounds__(WBLOCK) void k_w_tail(WaveArgs W, uint32_t maxSteps, int pass) {
+    __shared__ int s_st[SHORT_STACK * WBLOCK];
+    TravStack<SHORT_STACK> st;
+    st.lds = s_st + threadIdx.x;
+    st.stride = WBLOCK;
+    const DScene& S = W.A.S;
+    uint32_t nClosest = 0, nShadow = 0;
+    CX_FOR_JOBS(W) {
+    const uint32_t i = _base + threadIdx.x;
+    if (i < _n) {
+        const uint32_t slot = in_slot(W.qin, _qi, i);
+        const uint32_t job = W.qin.job[slot];
+        typename Mach<M>::S s = Mach<M>::load(W, slot);
+        const float4 a = W.qin.ray[slot], b = W.qin.ray[(size_t)W.qin.cap + slot];
+        Emit e;
+        e.kind = i < _nc ? EMIT_CLOSEST : EMIT_SHADOW;
+        e.o = V3{a.x, a.y, a.z};
+        e.tmin = a.w;
+        e.d = V3{b.x, b.y, b.z};
+        e.tmax = b.w;
+        for (uint32_t k = 0; k < maxSteps && e.kind != EMIT_NONE; k++) {
+            Hit h;
+            uint4 r;
+            if (e.kind == EMIT_CLOSEST) {
+                traverse_w<false, WIDTH>(S, e.o, e.d, e.tmin, e.tmax, h, st);
+                const bool hit = h.prim != INVALID_PRIM;
+                r = uint4{h.prim, fb(hit ? h.t : 0.0f), fb(hit ? h.u : 0.0f), fb(hit ? h.v : 0.0f)};
+                nClosest++;
+            } else {
+                const bool occ = traverse_w<true, WIDTH>(S, e.o, e.d, e.tmin, e.tmax, h, st);
+                r = uint4{occ ? 1u : 0u, 0u, 0u, 0u};
+                nShadow++;
+            }
+            e = Mach<M>::advance(W, job, s, r, e.o, e.d);
+        }
+        if (e.kind != EMIT_NONE) atomicOr(W.qin.overflow, 2u);
+    }
+    }
+    unsigned long long c = nClosest, sh = nShadow;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        c += __shfl_xor(c, off, 64);
+        sh += __shfl_xor(sh, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && W.A.counters != nullptr) {
+        unsigned long long* cs =
+            W.A.counters + (((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % COUNTER_STRIPES) * NUM_COUNTERS;
+        if (c) {
+            atomicAdd(&cs[2 * pass], c);
+            atomicAdd(&cs[STAT_TAIL_CLOSEST], c);
+        }
+        if (sh) {
+            atomicAdd(&cs[2 * pass + 1], sh);
+            atomicAdd(&cs[STAT_TAIL_SHADOW], sh);
+        }
     }
 }
 
@@ -1036,6 +1168,26 @@ hipError_t wave_trace(const DScene& S, const Queue& qin, const Queue& qout, unsi
     else if (S.width == 8) hipLaunchKernelGGL(k_q_trace<8>, g, b, 0, s, S, qin, qout, counters);
     else hipLaunchKernelGGL(k_q_trace<2>, g, b, 0, s, S, qin, qout, counters);
     return hipGetLastError();
+}
+
+template <int M>
+static hipError_t tail_launch_w(const WaveArgs& W, uint32_t maxSteps, int pass, hipStream_t s) {
+    const dim3 g = stride_grid(W.qin), b(WBLOCK);
+    if (W.A.S.width == 4) hipLaunchKernelGGL((k_w_tail<M, 4>), g, b, 0, s, W, maxSteps, pass);
+    else if (W.A.S.width == 8) hipLaunchKernelGGL((k_w_tail<M, 8>), g, b, 0, s, W, maxSteps, pass);
+    else hipLaunchKernelGGL((k_w_tail<M, 2>), g, b, 0, s, W, maxSteps, pass);
+    return hipGetLastError();
+}
+
+hipError_t wave_tail(int step, const WaveArgs& W, uint32_t maxSteps, int pass, hipStream_t s) {
+    switch (step) {
+        case WK_PG_STEP: return tail_launch_w<WK_PG_STEP>(W, maxSteps, pass, s);
+        case WK_CP_STEP: return tail_launch_w<WK_CP_STEP>(W, maxSteps, pass, s);
+        case WK_SG_STEP: return tail_launch_w<WK_SG_STEP>(W, maxSteps, pass, s);
+        case WK_SH_STEP: return tail_launch_w<WK_SH_STEP>(W, maxSteps, pass, s);
+        case WK_FG_STEP: return tail_launch_w<WK_FG_STEP>(W, maxSteps, pass, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStream_t s) {

@@ -179,6 +179,7 @@ def lib():
         L.cx_renderer_set_queue_grid.argtypes = [vp, u32]
         L.cx_renderer_set_graph_mode.argtypes = [vp, i32]
         L.cx_renderer_set_overlap.argtypes = [vp, i32]
+        L.cx_renderer_set_wave_rounds.argtypes = [vp, u32]
         L.cx_renderer_graph_captures.argtypes = [vp]
         L.cx_ctx_create.argtypes = [i32, C.POINTER(vp)]
         L.cx_ctx_destroy.argtypes = [vp]
@@ -431,6 +432,10 @@ class Renderer:
     def set_overlap(self, enable: bool = True):
         """Entry build and neighbour query beside the gather walks (cx_renderer_set_overlap)."""
         self.ctx.check(lib().cx_renderer_set_overlap(self.h, 1 if enable else 0), "cx_renderer_set_overlap")
+
+    def set_wave_rounds(self, rounds: int):
+        """Trace/step rounds per machine before the in-place tail kernel (0: all rounds)."""
+        self.ctx.check(lib().cx_renderer_set_wave_rounds(self.h, rounds), "cx_renderer_set_wave_rounds")
 
     def graph_captures(self) -> int:
         return int(lib().cx_renderer_graph_captures(self.h))

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define CX_ABI_VERSION 6
+#define CX_ABI_VERSION 7
 
 typedef enum cx_status {
     CX_OK = 0,
@@ -219,6 +219,10 @@ typedef struct cx_frame_stats {
     /* Nonzero if a wavefront queue append ever found its stripe full since the renderer was
        created (the queues are sized so that it cannot; a diagnostic of that sizing). */
     uint32_t queueOverflow;
+    /* Rays traced in place by the tail kernels (cx_renderer_set_wave_rounds), included in the
+       counts above; the traversal launches answered the rest. */
+    uint64_t tailClosestRays;
+    uint64_t tailShadowRays;
 } cx_frame_stats;
 
 typedef struct cx_ctx cx_ctx;
@@ -331,6 +335,13 @@ int cx_batch_sync(cx_ctx* ctx);
 /* Wavefront mode: the grid-stride traversal and step kernels launch at most maxBlocks blocks
  * of 128 threads (default 4096); a tuning knob (crisoptix_amd/tune.py), results unchanged. */
 int cx_renderer_set_queue_grid(cx_renderer* r, uint32_t maxBlocks);
+
+/* Wavefront mode: each path or shift machine runs at most `rounds` rounds of one traversal
+ * launch and one step launch, then finishes its remaining jobs in one kernel that traces each
+ * job's rays in place (the same traversal, so the same answers and ray counts).  Default 2;
+ * 0 runs every round as traversal + step.  Results are identical for every value; a tuning knob
+ * (crisoptix_amd/tune.py). */
+int cx_renderer_set_wave_rounds(cx_renderer* r, uint32_t rounds);
 
 /* Final-phase overlap (on by default; wavefront Conditional ReSTIR): cx_renderer_frame runs the
  * prefix-entry store and the entry-tree build (src/Renderer.cpp:351-366) on a renderer-owned

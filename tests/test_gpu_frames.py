@@ -3,7 +3,9 @@ shader logic (shader/*.cu, include/graph/restir/*.h) on identical seeded inputs.
 
 Per-pixel radiance must agree within 1e-4 relative after every frame; reservoir records,
 RNG states and neighbour sets are compared bit for bit.  Both execution modes of the HIP
-library (wavefront state machines and one megakernel per pass) are checked."""
+library (wavefront state machines and one megakernel per pass) are checked; the wavefront mode
+with its default (two rounds per machine, then the in-place tail kernel), with every round as
+traversal + step (rounds0) and with the tail after one round (rounds1)."""
 import numpy as np
 import pytest
 
@@ -45,7 +47,11 @@ def _setup(case, orc, rt, ctx, exec_mode="wavefront"):
         setattr(p, k, v)
     cam = scenes.cornell_camera(W, H) if c["scene"] == "cornell" else scenes.orbit_camera(sc, W, H)
     ren = rt.Renderer(W, H, ctx, cam, gscene, params=p, seed=1234)
-    ren.set_exec_mode(exec_mode)
+    if exec_mode.startswith("rounds"):   # wavefront rounds (0: all), then the in-place tail kernel
+        ren.set_exec_mode("wavefront")
+        ren.set_wave_rounds(int(exec_mode[6:]))
+    else:
+        ren.set_exec_mode(exec_mode)
     oren = orc.OracleRenderer(oscene, W, H, p, seed=1234)
     return c, cam, ren, oren
 
@@ -63,7 +69,7 @@ def _compare_prefix(g, o, which_valid):
     assert np.array_equal(g["last_prim"][v], o["lastInt"]["prim"][v])
 
 
-@pytest.mark.parametrize("exec_mode", ["wavefront", "megakernel"])
+@pytest.mark.parametrize("exec_mode", ["wavefront", "megakernel", "rounds0", "rounds1"])
 @pytest.mark.parametrize("case", list(CASES))
 def test_frames_match_oracle(case, exec_mode, orc, cx_runtime, cx_ctx):
     print(f"case {case} exec {exec_mode}", flush=True)
