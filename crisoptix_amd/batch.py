@@ -68,6 +68,8 @@ class Views:
                 ren.set_exec_mode(st["exec"])
                 if st.get("queue_grid"):
                     ren.set_queue_grid(st["queue_grid"])
+                if st.get("wave_rounds") is not None:
+                    ren.set_wave_rounds(st["wave_rounds"])
             ren.set_graph_mode(graph)
             self.cams.append(cam)
             self.rens.append(ren)

@@ -4,6 +4,7 @@ For one scene and one view it times the frame under every combination of
   * BVH width (children per node: 2, 4, 8; cx_scene_desc.bvhWidth),
   * queue grid (blocks of the grid-stride traversal and step kernels; cx_renderer_set_queue_grid),
   * execution mode (wavefront queues or one kernel per pass; cx_renderer_set_exec_mode),
+  * wavefront rounds per machine before the in-place tail kernel (cx_renderer_set_wave_rounds),
 and keeps the fastest.  All settings give bit-identical images (the tests check every one), so
 the choice is a pure performance decision.  Results are cached in a JSON file keyed by the
 scene's name and triangle count, the resolution and the renderer type.
@@ -26,6 +27,7 @@ from . import abi, runtime, scenes
 WIDTHS = (2, 4, 8)
 QUEUE_GRIDS = (1024, 2048, 4096, 8192)
 EXEC_MODES = ("wavefront", "megakernel")
+WAVE_ROUNDS = (0, 1, 2, 3)
 
 
 def _key(scene, w, h, params) -> str:
@@ -53,8 +55,11 @@ def time_frames(ren, cam, frames: int, warmup: int = 2) -> float:
 
 def tune(ctx, scene, w: int, h: int, params, camera_factory, frames: int = 5, gscenes: dict | None = None,
          cache_path: str | None = None, log=None) -> dict:
-    """Returns {"bvh_width", "queue_grid", "exec", "ms", "table"} for the view; `gscenes` maps a
-    BVH width to an already built runtime.Scene of `scene` (built here when missing)."""
+    """Returns {"bvh_width", "queue_grid", "wave_rounds", "exec", "ms", "table"} for the view;
+    `gscenes` maps a BVH width to an already built runtime.Scene of `scene` (built here when
+    missing).  The queue grid is swept at the default wave rounds and the wave rounds at the best
+    queue grid of each width, so the table holds 4 + 4 wavefront rows and one megakernel row per
+    width."""
     key = _key(scene, w, h, params)
     cache = {}
     if cache_path and os.path.exists(cache_path):
@@ -71,21 +76,30 @@ def tune(ctx, scene, w: int, h: int, params, camera_factory, frames: int = 5, gs
         gs = gscenes[width]
         if gs.info.bvhWidth != width:
             continue   # the wide tree's stack bound did not fit; that width is the binary one
-        for mode in EXEC_MODES:
-            for grid in (QUEUE_GRIDS if mode == "wavefront" else (0,)):
-                cam = camera_factory()
-                ren = runtime.Renderer(w, h, ctx, cam, gs, params=params, seed=1)
-                ren.set_exec_mode(mode)
-                if grid:
-                    ren.set_queue_grid(grid)
-                ms = time_frames(ren, cam, frames)
-                row = {"bvh_width": width, "exec": mode, "queue_grid": grid or None, "ms": ms}
-                table.append(row)
-                if log:
-                    log(f"  {key}: {row}")
-                if best is None or ms < best["ms"]:
-                    best = dict(row)
-                del ren
+        def run(mode, grid=None, rounds=None):
+            nonlocal best
+            cam = camera_factory()
+            ren = runtime.Renderer(w, h, ctx, cam, gs, params=params, seed=1)
+            ren.set_exec_mode(mode)
+            if grid:
+                ren.set_queue_grid(grid)
+            if rounds is not None:
+                ren.set_wave_rounds(rounds)
+            ms = time_frames(ren, cam, frames)
+            row = {"bvh_width": width, "exec": mode, "queue_grid": grid, "wave_rounds": rounds, "ms": ms}
+            table.append(row)
+            if log:
+                log(f"  {key}: {row}")
+            if best is None or ms < best["ms"]:
+                best = dict(row)
+            del ren
+            return ms
+
+        grid_ms = {g: run("wavefront", g, None) for g in QUEUE_GRIDS}
+        g_best = min(grid_ms, key=grid_ms.get)
+        for rounds in WAVE_ROUNDS:
+            run("wavefront", g_best, rounds)
+        run("megakernel")
     best["table"] = table
     if cache_path:
         cache[key] = best

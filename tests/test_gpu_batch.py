@@ -1,7 +1,8 @@
 """Multi-view batch (cx_batch_frame): views of two scenes at uneven resolutions rendered on
 the context's stream pool give, frame after frame, the same images bit for bit as the plain
 per-view frames on the default stream; and every launch setting the autotuner chooses from
-(BVH width, queue grid, execution mode) renders the same image."""
+(BVH width, queue grid, wavefront rounds before the tail kernel, execution mode) renders the
+same image."""
 import math
 
 import numpy as np
@@ -57,7 +58,8 @@ def test_tuning_settings_render_the_same_image(cx_runtime, cx_ctx):
     ref = None
     for width in (2, 4, 8):
         g = cx_runtime.Scene(cx_ctx, host, width=width)
-        for mode, grid in (("wavefront", 1024), ("wavefront", 8192), ("megakernel", None)):
+        for mode, grid, rounds in (("wavefront", 1024, 0), ("wavefront", 8192, 1), ("wavefront", 4096, 3),
+                                   ("megakernel", None, None)):
             cam = scenes.orbit_camera(host, W, H)
             p = abi.workload_params("conditional_restir", W, H, g.size)
             p.maxPathLen = 6
@@ -65,8 +67,10 @@ def test_tuning_settings_render_the_same_image(cx_runtime, cx_ctx):
             r.set_exec_mode(mode)
             if grid:
                 r.set_queue_grid(grid)
+            if rounds is not None:
+                r.set_wave_rounds(rounds)
             for f in range(3):
                 img = r.launch_frame(advance([cam])[0], True).numpy()
             if ref is None:
                 ref = img
-            assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (width, mode, grid)
+            assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), (width, mode, grid, rounds)
