@@ -62,7 +62,7 @@ def parse():
     ap.add_argument("--relmse-frames", type=int, default=4096,
                     help="accumulated frames of the rel-MSE measurement (0 = skip)")
     ap.add_argument("--bvh-width", type=int, choices=[2, 4, 8], default=4, help="children per BVH node")
-    ap.add_argument("--wave-rounds", type=int, default=0,
+    ap.add_argument("--wave-rounds", type=int, default=2,
                     help="trace/step rounds per machine before the in-place tail kernel (0: all rounds)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="run the entry build and neighbour query in sequence instead of beside the gather walks")
