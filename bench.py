@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--relmse-frames", type=int, default=4096,
                     help="accumulated frames of the rel-MSE measurement (0 = skip)")
     ap.add_argument("--bvh-width", type=int, choices=[2, 4, 8], default=4, help="children per BVH node")
+    ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
+                    help="binned SAH on the host or PLOC on the GPU (same hits, different trees)")
     ap.add_argument("--wave-rounds", type=int, default=2,
                     help="trace/step rounds per machine before the in-place tail kernel (0: all rounds)")
     ap.add_argument("--no-overlap", action="store_true",
@@ -151,10 +153,11 @@ def main():
     t0 = time.perf_counter()
     scene = scenes.object_grid(a.grid, WORKLOAD_SEED)
     ctx = runtime.Context(0)
-    gscene = runtime.Scene(ctx, scene, width=a.bvh_width)
-    log(f"scene: {scene.num_triangles} triangles, BVH {gscene.info.numNodes} binary nodes, depth {gscene.info.maxDepth}, "
-        f"width {gscene.info.bvhWidth} ({gscene.info.numWideNodes} wide nodes, stack bound {gscene.info.stackBound}), "
-        f"built in {time.perf_counter() - t0:.1f}s")
+    gscene = runtime.Scene(ctx, scene, width=a.bvh_width, builder=a.bvh_builder)
+    log(f"scene: {scene.num_triangles} triangles, BVH ({a.bvh_builder}) {gscene.info.numNodes} binary nodes, depth "
+        f"{gscene.info.maxDepth}, width {gscene.info.bvhWidth} ({gscene.info.numWideNodes} wide nodes, stack bound "
+        f"{gscene.info.stackBound}), BVH build {gscene.info.buildMs:.1f} ms, scene ready in "
+        f"{time.perf_counter() - t0:.1f}s")
 
     def params_for(w, h, size):
         return abi.workload_params("conditional_restir", w, h, size)
@@ -273,6 +276,8 @@ def main():
             "frame_algorithmic_GBps": round(frame_bytes / (ms_per_frame / 1000.0) / 1e9, 2),
             "entries_per_frame": int(entries / a.steps),
             "bvh_width": int(gscene.info.bvhWidth),
+            "bvh_builder": a.bvh_builder,
+            "bvh_build_ms": round(float(gscene.info.buildMs), 2),
             "trace_ms_per_frame": round(trace_ms / a.steps, 4),
             "tail_rays_per_frame": int(tail_rays.sum() / a.steps),
             "final_gather_pass": {"ms": round(fg_ms, 4), "algorithmic_bytes": int(fg_bytes),

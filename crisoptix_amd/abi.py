@@ -53,9 +53,12 @@ class Material(C.Structure):
                 ("roughness", C.c_float)]
 
 
-ABI_VERSION = 7   # CX_ABI_VERSION of include/crisoptix.h
+ABI_VERSION = 8   # CX_ABI_VERSION of include/crisoptix.h
 CX_SPEC_TEX_COLOR = 0
 CX_SPEC_TEX_OCC_ROUGH_METAL = 1
+CX_BVH_BUILD_HOST = 0
+CX_BVH_BUILD_DEVICE = 1
+BVH_BUILDERS = {"host": CX_BVH_BUILD_HOST, "device": CX_BVH_BUILD_DEVICE}
 
 
 class SceneDesc(C.Structure):
@@ -66,7 +69,7 @@ class SceneDesc(C.Structure):
         ("emitterFaces", C.c_void_p), ("emitterFaceOffset", C.c_void_p), ("emitterColor", C.c_void_p),
         ("numEmitters", C.c_uint32), ("bvhMaxLeafSize", C.c_uint32), ("bvhWidth", C.c_uint32),
         ("uvs", C.c_void_p), ("materialTex", C.c_void_p), ("textures", C.c_void_p), ("numTextures", C.c_uint32),
-        ("pad0", C.c_uint32),
+        ("bvhBuilder", C.c_uint32),
     ]
 
 
@@ -86,7 +89,8 @@ class SceneInfo(C.Structure):
         ("maxDepth", C.c_uint32), ("numEmitters", C.c_uint32), ("numEmitterFaces", C.c_uint32),
         ("sceneSize", C.c_float), ("boundsMin", C.c_float * 3), ("boundsMax", C.c_float * 3),
         ("deviceBytes", C.c_uint64), ("bvhWidth", C.c_uint32), ("numWideNodes", C.c_uint32),
-        ("stackBound", C.c_uint32), ("numTextures", C.c_uint32),
+        ("stackBound", C.c_uint32), ("numTextures", C.c_uint32), ("bvhBuilder", C.c_uint32),
+        ("buildMs", C.c_float),
     ]
 
 

@@ -2,6 +2,7 @@
 // search structure, and the renderer that runs Renderer::LaunchFrame's phase order
 // (src/Renderer.cpp:209-406) as HIP kernels on the caller's stream with no device sync.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -616,20 +617,34 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
                 return set_err(ctx, CX_ERR_INVALID_ARG, "unknown specTexUsage");
             matTex[m] = make_int4(mt.diffTex, mt.specTex, mt.roughTex, (int32_t)mt.specTexUsage);
         }
-    BvhHost bvh;
-    std::string err;
-    if (!build_bvh(d->positions, d->indices, d->numTriangles, d->bvhMaxLeafSize ? d->bvhMaxLeafSize : 4, bvh, err))
-        return set_err(ctx, CX_ERR_BUILD, err);
     const uint32_t wantWidth = d->bvhWidth ? d->bvhWidth : 4u;
     if (wantWidth != 2 && wantWidth != 4 && wantWidth != 8)
         return set_err(ctx, CX_ERR_INVALID_ARG, "bvhWidth must be 0, 2, 4 or 8");
+    if (d->bvhBuilder != CX_BVH_BUILD_HOST && d->bvhBuilder != CX_BVH_BUILD_DEVICE)
+        return set_err(ctx, CX_ERR_INVALID_ARG, "unknown bvhBuilder");
+    const bool onDevice = d->bvhBuilder == CX_BVH_BUILD_DEVICE;
+    if (onDevice)   // the clustering compares box areas: it needs finite coordinates
+        for (size_t i = 0; i < 3 * (size_t)d->numVertices; i++)
+            if (!std::isfinite(d->positions[i]))
+                return set_err(ctx, CX_ERR_INVALID_ARG, "non-finite vertex position (device BVH build)");
+    const uint32_t maxLeaf = d->bvhMaxLeafSize ? d->bvhMaxLeafSize : 4;
+    const auto tBuild = std::chrono::steady_clock::now();
+    BvhHost bvh;
+    std::string err;
     std::vector<float4> wide;
-    int32_t wroot = bvh.root;
-    uint32_t stackBound = bvh.maxDepth;
-    uint32_t width = 2;
-    if (wantWidth > 2 && collapse_bvh(bvh, (int)wantWidth, (uint32_t)STACK_DEPTH, wide, wroot, stackBound))
-        width = wantWidth;
-    if (width == 2) stackBound = bvh.maxDepth;
+    int32_t root = 0, wroot = 0;
+    uint32_t stackBound = 0, width = 2, maxDepth = 0, numNodes = 0, numWide = 0;
+    if (!onDevice) {
+        if (!build_bvh(d->positions, d->indices, d->numTriangles, maxLeaf, bvh, err))
+            return set_err(ctx, CX_ERR_BUILD, err);
+        root = wroot = bvh.root;
+        stackBound = maxDepth = bvh.maxDepth;
+        if (wantWidth > 2 && collapse_bvh(bvh, (int)wantWidth, (uint32_t)STACK_DEPTH, wide, wroot, stackBound))
+            width = wantWidth;
+        if (width == 2) stackBound = bvh.maxDepth;
+        numNodes = (uint32_t)(bvh.nodes.size() / 4);
+        numWide = width > 2 ? (uint32_t)(wide.size() / (2 * width)) : 0u;
+    }
     cx_scene* s = new (std::nothrow) cx_scene();
     if (!s) return set_err(ctx, CX_ERR_OUT_OF_MEMORY, "host allocation failed");
     s->ctx = ctx;
@@ -654,10 +669,12 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     std::vector<uint32_t> texels((size_t)numTexels);
     for (uint32_t t = 0; t < d->numTextures; t++)
         std::memcpy(texels.data() + texInfo[t].x, d->textures[t].rgba, (size_t)texInfo[t].y * texInfo[t].z * 4);
-    const float4* nodesSrc = bvh.nodes.empty() ? nullptr : bvh.nodes.data();
-    s->d.nodes = (const float4*)up(nodesSrc, bvh.nodes.size(), sizeof(float4));
-    s->d.wnodes = width > 2 ? (const float4*)up(wide.data(), wide.size(), sizeof(float4)) : nullptr;
-    s->d.tris = (const float4*)up(bvh.tris.data(), bvh.tris.size(), sizeof(float4));
+    if (!onDevice) {
+        const float4* nodesSrc = bvh.nodes.empty() ? nullptr : bvh.nodes.data();
+        s->d.nodes = (const float4*)up(nodesSrc, bvh.nodes.size(), sizeof(float4));
+        s->d.wnodes = width > 2 ? (const float4*)up(wide.data(), wide.size(), sizeof(float4)) : nullptr;
+        s->d.tris = (const float4*)up(bvh.tris.data(), bvh.tris.size(), sizeof(float4));
+    }
     s->d.pos = (const float*)up(d->positions, 3 * (size_t)d->numVertices, sizeof(float));
     s->d.nrm = (const float*)up(d->normals, 3 * (size_t)d->numVertices, sizeof(float));
     s->d.idx = (const uint32_t*)up(d->indices, 3 * (size_t)d->numTriangles, sizeof(uint32_t));
@@ -679,9 +696,63 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
         delete s;
         return hip_err(ctx, e, "scene upload");
     }
+    uint32_t builderUsed = d->bvhBuilder;
+    if (onDevice) {
+        DeviceBvh db;
+        hipError_t he = hipSuccess;
+        std::vector<void*> bvhAllocs;
+        if (!build_bvh_device(s->d.pos, s->d.idx, d->numTriangles, maxLeaf, (int)wantWidth, db, bvhAllocs, err, he)) {
+            free_all(bvhAllocs);
+            if (!db.tooDeep) {
+                free_all(s->allocs);
+                delete s;
+                if (he != hipSuccess && he != hipErrorUnknown) return hip_err(ctx, he, err.c_str());
+                return set_err(ctx, CX_ERR_BUILD, err);
+            }
+            // a tree deeper than the traversal stack (heavily duplicated geometry): the host
+            // build bounds its depth, so it takes over; cx_scene_info.bvhBuilder reports it
+            builderUsed = CX_BVH_BUILD_HOST;
+            if (!build_bvh(d->positions, d->indices, d->numTriangles, maxLeaf, bvh, err)) {
+                free_all(s->allocs);
+                delete s;
+                return set_err(ctx, CX_ERR_BUILD, err);
+            }
+            db = DeviceBvh{};
+            db.root = db.wroot = bvh.root;
+            db.stackBound = db.maxDepth = bvh.maxDepth;
+            if (wantWidth > 2 && collapse_bvh(bvh, (int)wantWidth, (uint32_t)STACK_DEPTH, wide, db.wroot, db.stackBound))
+                db.width = wantWidth;
+            if (db.width == 2) db.stackBound = bvh.maxDepth;
+            db.numNodes = (uint32_t)(bvh.nodes.size() / 4);
+            db.numWide = db.width > 2 ? (uint32_t)(wide.size() / (2 * db.width)) : 0u;
+            db.nodes = (float4*)up(bvh.nodes.empty() ? nullptr : bvh.nodes.data(), bvh.nodes.size(), sizeof(float4));
+            db.wnodes = db.width > 2 ? (float4*)up(wide.data(), wide.size(), sizeof(float4)) : nullptr;
+            db.tris = (float4*)up(bvh.tris.data(), bvh.tris.size(), sizeof(float4));
+        } else {
+            s->allocs.insert(s->allocs.end(), bvhAllocs.begin(), bvhAllocs.end());
+        }
+        s->d.nodes = db.nodes ? db.nodes : (const float4*)up(nullptr, 0, sizeof(float4));
+        s->d.tris = db.tris ? db.tris : (const float4*)up(nullptr, 0, sizeof(float4));
+        s->d.wnodes = db.wnodes;
+        if (db.width > 2 && !db.wnodes) s->d.wnodes = (const float4*)up(nullptr, 0, sizeof(float4));
+        if (e != hipSuccess) {
+            free_all(s->allocs);
+            delete s;
+            return hip_err(ctx, e, "scene upload");
+        }
+        root = db.root;
+        wroot = db.wroot;
+        width = db.width;
+        maxDepth = db.maxDepth;
+        stackBound = db.stackBound;
+        numNodes = db.numNodes;
+        numWide = db.numWide;
+    }
+    const float buildMs =
+        std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - tBuild).count();
     s->d.numTris = d->numTriangles;
     s->d.numEmitters = d->numEmitters;
-    s->d.root = bvh.root;
+    s->d.root = root;
     s->d.wroot = wroot;
     s->d.width = width;
     // Scene::GetSceneSize (src/Scene.cpp:48-71): bbox of all vertices and the origin.
@@ -700,15 +771,18 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     }
     s->info.numTriangles = d->numTriangles;
     s->info.numVertices = d->numVertices;
-    s->info.numNodes = (uint32_t)(bvh.nodes.size() / 4);
-    s->info.maxDepth = bvh.maxDepth;
+    s->info.numNodes = numNodes;
+    s->info.maxDepth = maxDepth;
     s->info.numEmitters = d->numEmitters;
     s->info.numEmitterFaces = numFaces;
     s->info.bvhWidth = width;
-    s->info.numWideNodes = width > 2 ? (uint32_t)(wide.size() / (2 * width)) : 0u;
+    s->info.numWideNodes = numWide;
     s->info.stackBound = stackBound;
     s->info.numTextures = d->numTextures;
-    s->info.deviceBytes = (bvh.nodes.size() + wide.size() + bvh.tris.size()) * sizeof(float4) +
+    s->info.bvhBuilder = builderUsed;
+    s->info.buildMs = buildMs;
+    s->info.deviceBytes = (4 * (size_t)numNodes + 2 * (size_t)width * numWide + 3 * (size_t)d->numTriangles) *
+                              sizeof(float4) +
                           (size_t)d->numVertices * (d->uvs ? 32 : 24) + (size_t)d->numTriangles * 16 +
                           mats.size() * sizeof(DMaterial) + texels.size() * 4 + texInfo.size() * sizeof(uint4);
     *out = s;

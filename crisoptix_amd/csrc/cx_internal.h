@@ -20,6 +20,8 @@ hipError_t launch_store_entries(const FrameArgs& A, const EntryBuild& E, hipStre
 hipError_t launch_entries_from_arrays(const float* pos, const float* r, const uint8_t* valid, uint32_t n,
                                       const EntryBuild& E, hipStream_t s);
 size_t sort_temp_bytes(uint32_t n);
+hipError_t launch_key_sort(uint32_t* keys, uint32_t* vals, uint32_t* keysAlt, uint32_t* valsAlt, uint32_t n,
+                           uint32_t* numValid, void* temp, size_t tempBytes, hipStream_t s);
 hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n, void* temp, size_t tempBytes,
                              hipStream_t s);
 hipError_t launch_entries_query(const EntryTree& T, const float* q, uint32_t nq, uint32_t k, uint32_t* idx,
@@ -83,6 +85,21 @@ struct BvhHost {
 
 bool build_bvh(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t maxLeaf, BvhHost& out,
                std::string& err);
+
+// Device BVH build (cx_bvh_device.hip) from device positions and indices: binary nodes, triangle
+// slots and, when wantWidth is 4 or 8 and its stack bound fits, wide nodes, in the formats of
+// the host build.  The arrays it returns are appended to `allocs` (owned by the caller).
+struct DeviceBvh {
+    float4* nodes = nullptr;     // 4 float4 per binary inner node
+    float4* wnodes = nullptr;    // 2 * width float4 per wide node, or null
+    float4* tris = nullptr;      // 3 float4 per triangle slot
+    uint32_t numNodes = 0, numWide = 0;
+    int32_t root = 0, wroot = 0;
+    uint32_t width = 2, maxDepth = 0, stackBound = 0;
+    bool tooDeep = false;        // the binary tree is deeper than the traversal stack
+};
+bool build_bvh_device(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t maxLeaf, int wantWidth,
+                      DeviceBvh& out, std::vector<void*>& allocs, std::string& err, hipError_t& herr);
 
 // Wide (4- or 8-child) node array from a binary BVH; false if its worst-case traversal stack
 // would not fit in `stackLimit` entries.

@@ -928,6 +928,35 @@ size_t sort_temp_bytes(uint32_t n) {
     return (256 * tiles + 256) * sizeof(uint32_t);
 }
 
+// Sorts the n slots of keys (0xFFFFFFFF = invalid, dropped) by key, stably: afterwards keys[0,
+// *numValid) are the valid keys in order and vals[] their slot indices.  keysAlt / valsAlt are
+// the ping-pong buffers; temp holds sort_temp_bytes(n).
+hipError_t launch_key_sort(uint32_t* keys, uint32_t* vals, uint32_t* keysAlt, uint32_t* valsAlt, uint32_t n,
+                           uint32_t* numValid, void* temp, size_t tempBytes, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (sort_temp_bytes(n) > tempBytes) return hipErrorInvalidValue;
+    const uint32_t tiles = (n + RS_TILE - 1) / RS_TILE;
+    uint32_t* counts = (uint32_t*)temp;
+    uint32_t* totals = counts + (size_t)256 * tiles;
+    uint32_t* kIn = keys;
+    uint32_t* vIn = vals;
+    uint32_t* kOut = keysAlt;
+    uint32_t* vOut = valsAlt;
+    for (int pass = 0; pass < 4; pass++) {
+        const int first = pass == 0 ? 1 : 0;
+        const uint32_t shift = 8u * (uint32_t)pass;
+        hipLaunchKernelGGL(k_rs_count, dim3(tiles), dim3(RS_THREADS), 0, s, kIn, n, numValid, first, shift, counts,
+                           tiles);
+        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_THREADS), 0, s, counts, tiles, n, numValid, first, totals);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n, numValid,
+                           first, shift, counts, tiles, totals);
+        std::swap(kIn, kOut);
+        std::swap(vIn, vOut);
+    }
+    // four passes: the sorted data is back in keys / vals
+    return hipGetLastError();
+}
+
 // Sorts the valid entries by Morton key and builds the implicit tree over them bottom-up, one
 // launch per level (levels at or below the tree's leaf level return at once).  n = capacity.
 hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n, void* temp, size_t tempBytes,
@@ -936,29 +965,11 @@ hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n
         hipLaunchKernelGGL(k_tree_prep, dim3(1), dim3(64), 0, s, T);
         return hipGetLastError();
     }
-    if (sort_temp_bytes(n) > tempBytes) return hipErrorInvalidValue;
-    const uint32_t tiles = (n + RS_TILE - 1) / RS_TILE;
-    uint32_t* counts = (uint32_t*)temp;
-    uint32_t* totals = counts + (size_t)256 * tiles;
-    uint32_t* kIn = E.keys;
-    uint32_t* vIn = E.vals;
-    uint32_t* kOut = E.keysAlt;
-    uint32_t* vOut = E.valsAlt;
-    for (int pass = 0; pass < 4; pass++) {
-        const int first = pass == 0 ? 1 : 0;
-        const uint32_t shift = 8u * (uint32_t)pass;
-        hipLaunchKernelGGL(k_rs_count, dim3(tiles), dim3(RS_THREADS), 0, s, kIn, n, E.numValid, first, shift, counts,
-                           tiles);
-        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_THREADS), 0, s, counts, tiles, n, E.numValid, first, totals);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(tiles), dim3(RS_THREADS), 0, s, kIn, vIn, kOut, vOut, n, E.numValid,
-                           first, shift, counts, tiles, totals);
-        if (first) hipLaunchKernelGGL(k_tree_prep, dim3(1), dim3(64), 0, s, T);   // needs the valid count
-        std::swap(kIn, kOut);
-        std::swap(vIn, vOut);
-    }
-    // after four passes the sorted entries are in kIn / vIn (= E.keys / E.vals)
+    const hipError_t e = launch_key_sort(E.keys, E.vals, E.keysAlt, E.valsAlt, n, E.numValid, temp, tempBytes, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tree_prep, dim3(1), dim3(64), 0, s, T);   // needs the valid count
     const uint32_t leaves = 1u << T.levels;
-    hipLaunchKernelGGL(k_tree_leaves, dim3((leaves + 255) / 256), dim3(256), 0, s, T, vIn, E.pos, leaves);
+    hipLaunchKernelGGL(k_tree_leaves, dim3((leaves + 255) / 256), dim3(256), 0, s, T, E.vals, E.pos, leaves);
     for (int l = (int)T.levels - 1; l >= 0; l--) {
         const uint32_t cnt = 1u << l;
         const uint32_t base = cnt - 1u;

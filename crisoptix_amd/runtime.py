@@ -288,12 +288,14 @@ class Context:
 class Scene:
     """Device scene: flattened triangles, BVH, materials and explicit emitters."""
 
-    def __init__(self, ctx: Context, scene, max_leaf: int = 0, width: int = 0):
+    def __init__(self, ctx: Context, scene, max_leaf: int = 0, width: int = 0, builder: str = "host"):
+        """builder: "host" (binned SAH on the CPU) or "device" (PLOC on the GPU); same hits."""
         self.ctx = ctx
         self._keep: list = []
         desc = abi.scene_desc(scene, self._keep)
         desc.bvhMaxLeafSize = max_leaf
         desc.bvhWidth = width
+        desc.bvhBuilder = abi.BVH_BUILDERS[builder]
         h = C.c_void_p()
         ctx.check(lib().cx_scene_build(ctx.h, C.byref(desc), C.byref(h)), "cx_scene_build")
         self.h = h.value

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define CX_ABI_VERSION 7
+#define CX_ABI_VERSION 8
 
 typedef enum cx_status {
     CX_OK = 0,
@@ -141,6 +141,18 @@ typedef struct cx_material_tex {
  * alpha = frac(x), beta = frac(y), each rounded to a multiple of 1/256 (the 8 fractional bits
  * the CUDA programming guide gives for linear filtering); the value is the exact weighted sum
  * of the 8-bit texels rounded once to float: sum(w_i * texel_i) / (65536 * 255). */
+/* Where cx_scene_build builds the triangle BVH.
+ *   HOST:   binned SAH (32 bins) on the host, collapsed to the requested width, uploaded.
+ *   DEVICE: on the GPU from the uploaded vertices: 30-bit Morton sort of the triangle
+ *           centroids, PLOC clustering (Meister and Bittner 2018, radius 64), SAH leaf
+ *           formation, collapse to the requested width; deterministic.  Vertex positions must
+ *           be finite.  A tree deeper than the traversal stack is replaced by the host build
+ *           (cx_scene_info.bvhBuilder then reports HOST). */
+typedef enum cx_bvh_builder {
+    CX_BVH_BUILD_HOST = 0,
+    CX_BVH_BUILD_DEVICE = 1,
+} cx_bvh_builder;
+
 typedef struct cx_texture {
     const uint8_t* rgba;           /* host, [height][width][4] */
     uint32_t width;
@@ -179,7 +191,8 @@ typedef struct cx_scene_desc {
                                       CX_SPEC_TEX_COLOR for every material */
     const cx_texture* textures;    /* host, [numTextures] */
     uint32_t numTextures;
-    uint32_t pad0;
+    uint32_t bvhBuilder;           /* cx_bvh_builder: where the BVH is built (the traversal's
+                                      answers are the same for either tree) */
 } cx_scene_desc;
 
 typedef struct cx_scene_info {
@@ -197,6 +210,9 @@ typedef struct cx_scene_info {
     uint32_t numWideNodes;
     uint32_t stackBound;           /* worst-case traversal stack entries */
     uint32_t numTextures;
+    uint32_t bvhBuilder;           /* cx_bvh_builder used */
+    float buildMs;                 /* wall time of the BVH build and its upload (host build) or of
+                                      the device build, ms */
 } cx_scene_info;
 
 /* Per-pass timing of the last frame, ms (src/Renderer.cpp:396-402), plus ray counts. */
@@ -238,8 +254,9 @@ int cx_abi_version(void);
 
 /* ---- scene: replaces Mesh::BuildAccel, Model::BuildAccel, Scene::BuildAccel and
  *      Scene::BuildEmitterTable (src/Mesh.cpp:54-135, src/Model.cpp:376-457,
- *      src/Scene.cpp:73-204).  Builds a binary SAH BVH on the host, collapses it to
- *      desc->bvhWidth children per node and uploads it. ---- */
+ *      src/Scene.cpp:73-204).  Builds a binary BVH (desc->bvhBuilder: binned SAH on the
+ *      host, or PLOC on the device), collapses it to desc->bvhWidth children per node and
+ *      keeps it in device memory.  Synchronous. ---- */
 int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* desc, cx_scene** out);
 int cx_scene_destroy(cx_scene* scene);
 int cx_scene_get_info(const cx_scene* scene, cx_scene_info* info);

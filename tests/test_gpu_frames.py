@@ -25,6 +25,9 @@ CASES = {
                                   over=dict(gatherRadius=0.08, gatherN=2, suffixSpatialCount=2, windowRadius=6)),
     "cond_grid_orbit": dict(scene="grid3", mode="conditional_restir", W=48, H=32, frames=4, orbit=True,
                             over=dict(gatherRadius=0.3)),
+    # the same with the BVH built on the GPU (PLOC): same images as with the host-built tree
+    "cond_grid_orbit_devbvh": dict(scene="grid3", mode="conditional_restir", W=48, H=32, frames=4, orbit=True,
+                                   over=dict(gatherRadius=0.3), builder="device"),
     # textured materials (diffuse, roughness and OccRoughMetal specular textures, wrapped uvs)
     "pt_grid_tex": dict(scene="grid3_tex", mode="path_tracer", W=48, H=32, frames=3),
     "restir_pt_grid_tex": dict(scene="grid3_tex", mode="restir_pt_cornell", W=48, H=32, frames=4),
@@ -39,7 +42,7 @@ def _setup(case, orc, rt, ctx, exec_mode="wavefront"):
     if c["scene"].endswith("_tex"):
         sc = scenes.with_textures(sc)
     W, H = c["W"], c["H"]
-    gscene = rt.Scene(ctx, sc)
+    gscene = rt.Scene(ctx, sc, builder=c.get("builder", "host"))
     oscene = orc.OracleScene(sc)
     assert abs(gscene.size - oscene.size) == 0.0
     p = abi.workload_params(c["mode"], W, H, gscene.size)
