@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--relmse-frames", type=int, default=4096,
                     help="accumulated frames of the rel-MSE measurement (0 = skip)")
     ap.add_argument("--bvh-width", type=int, choices=[2, 4, 8], default=4, help="children per BVH node")
+    ap.add_argument("--bvh-max-leaf", type=int, default=0,
+                    help="most triangles per BVH leaf (0: the library's default, 2)")
     ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
                     help="binned SAH on the host or PLOC on the GPU (same hits, different trees)")
     ap.add_argument("--wave-rounds", type=int, default=2,
@@ -153,7 +155,7 @@ def main():
     t0 = time.perf_counter()
     scene = scenes.object_grid(a.grid, WORKLOAD_SEED)
     ctx = runtime.Context(0)
-    gscene = runtime.Scene(ctx, scene, width=a.bvh_width, builder=a.bvh_builder)
+    gscene = runtime.Scene(ctx, scene, width=a.bvh_width, builder=a.bvh_builder, max_leaf=a.bvh_max_leaf)
     log(f"scene: {scene.num_triangles} triangles, BVH ({a.bvh_builder}) {gscene.info.numNodes} binary nodes, depth "
         f"{gscene.info.maxDepth}, width {gscene.info.bvhWidth} ({gscene.info.numWideNodes} wide nodes, stack bound "
         f"{gscene.info.stackBound}), BVH build {gscene.info.buildMs:.1f} ms, scene ready in "
@@ -277,6 +279,7 @@ def main():
             "entries_per_frame": int(entries / a.steps),
             "bvh_width": int(gscene.info.bvhWidth),
             "bvh_builder": a.bvh_builder,
+            "bvh_max_leaf": a.bvh_max_leaf or 2,
             "bvh_build_ms": round(float(gscene.info.buildMs), 2),
             "trace_ms_per_frame": round(trace_ms / a.steps, 4),
             "tail_rays_per_frame": int(tail_rays.sum() / a.steps),

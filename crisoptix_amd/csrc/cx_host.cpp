@@ -627,7 +627,7 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
         for (size_t i = 0; i < 3 * (size_t)d->numVertices; i++)
             if (!std::isfinite(d->positions[i]))
                 return set_err(ctx, CX_ERR_INVALID_ARG, "non-finite vertex position (device BVH build)");
-    const uint32_t maxLeaf = d->bvhMaxLeafSize ? d->bvhMaxLeafSize : 4;
+    const uint32_t maxLeaf = d->bvhMaxLeafSize ? d->bvhMaxLeafSize : DEFAULT_MAX_LEAF;
     const auto tBuild = std::chrono::steady_clock::now();
     BvhHost bvh;
     std::string err;
@@ -845,7 +845,8 @@ int cx_bvh_host_trace(const cx_scene_desc* d, const float* rays, uint32_t n, int
     if (wantWidth != 2 && wantWidth != 4 && wantWidth != 8) return CX_ERR_INVALID_ARG;
     BvhHost bvh;
     std::string err;
-    if (!build_bvh(d->positions, d->indices, d->numTriangles, d->bvhMaxLeafSize ? d->bvhMaxLeafSize : 4, bvh, err))
+    if (!build_bvh(d->positions, d->indices, d->numTriangles, d->bvhMaxLeafSize ? d->bvhMaxLeafSize : DEFAULT_MAX_LEAF,
+                   bvh, err))
         return CX_ERR_BUILD;
     std::vector<float4> wide;
     int32_t wroot = bvh.root;

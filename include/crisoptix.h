@@ -181,7 +181,7 @@ typedef struct cx_scene_desc {
     const uint32_t* emitterFaceOffset; /* host, [numEmitters + 1] prefix offsets into faces */
     const float* emitterColor;     /* host, [numEmitters][3] */
     uint32_t numEmitters;
-    uint32_t bvhMaxLeafSize;       /* 0 = default (4) */
+    uint32_t bvhMaxLeafSize;       /* most triangles per leaf, 1..15; 0 = default (2) */
     uint32_t bvhWidth;             /* children per BVH node: 2, 4 or 8; 0 = default (4).  A wide
                                       BVH whose worst-case traversal stack would not fit falls
                                       back to 2 (cx_scene_info.bvhWidth reports the width used) */
