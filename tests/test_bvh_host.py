@@ -27,3 +27,17 @@ def test_host_traversal_matches_oracle(name, width, max_leaf, orc):
     oo = orc.OracleScene(sc).trace_occluded(srays)
     og, _ = runtime.host_trace(sc, srays, any_hit=True, width=width, max_leaf=max_leaf)
     assert np.array_equal(oo.astype(np.uint32), og)
+
+
+def test_wide8_collapse_falls_back_to_greedy_on_large_scene():
+    """The 1.06 M-triangle object grid of the 1080p benchmark: the least-area collapse at width 8
+    needs more than 64 traversal-stack entries, so the collapse takes the greedy rule, which fits;
+    width 8 is kept (not the binary fallback) and its answers equal the 4-wide tree's."""
+    sc = scenes.object_grid(16, 2)
+    rays = random_rays(sc, 3000, seed=31)
+    *a8, used8 = runtime.host_trace(sc, rays, width=8)
+    *a4, used4 = runtime.host_trace(sc, rays, width=4)
+    assert (used8, used4) == (8, 4)
+    assert np.mean(a4[0] != 0xFFFFFFFF) > 0.3
+    for x, y in zip(a8, a4):
+        assert np.array_equal(bits(x), bits(y))
