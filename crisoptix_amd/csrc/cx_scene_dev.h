@@ -171,9 +171,18 @@ CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float
                                           float tmax, Hit& best) {
     const uint32_t code = ~(uint32_t)ref;
     const uint32_t first = code >> 4, cnt = code & 15u;
+    // the next triangle's vertices are fetched before the current one is tested, so a leaf's
+    // fetches overlap instead of queueing behind each test (a leaf holds at least one triangle)
+    const float4* tv = S.tris + 3 * (size_t)first;
+    float4 n0 = tv[0], n1 = tv[1], n2 = tv[2];
     for (uint32_t i = 0; i < cnt; i++) {
-        const float4* tv = S.tris + 3 * (size_t)(first + i);
-        const float4 p0 = tv[0], p1 = tv[1], p2 = tv[2];
+        const float4 p0 = n0, p1 = n1, p2 = n2;
+        if (i + 1 < cnt) {
+            const float4* nv = tv + 3 * (size_t)(i + 1);
+            n0 = nv[0];
+            n1 = nv[1];
+            n2 = nv[2];
+        }
         float t, u, v;
         if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, tmax, t, u, v))
             continue;

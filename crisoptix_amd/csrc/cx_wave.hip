@@ -23,8 +23,12 @@ __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + thr
 // Answers every ray of qin: closest-hit rays in [0, nc), shadow rays in [cap - ns, cap).
 // Grid-stride over the device-side counts, so the host never waits for them.  Also clears the
 // counts of qout (the next step kernel's output queue) and adds the rays to the pass counters.
+// The 2- and 4-wide kernels are held to 64 VGPRs (8 waves per SIMD): the leaf loop's prefetch of
+// the next triangle would otherwise take the 4-wide kernel to 66 and 7 waves (measured on the
+// 1080p frame: traversal 4.71 ms at 8 waves, 4.78-4.80 at 7, 4.79-4.81 without the prefetch).
 template <int WIDTH>
-__global__ __launch_bounds__(WBLOCK) void k_q_trace(DScene S, Queue qin, Queue qout, unsigned long long* counters) {
+__global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH == 8 ? 1 : 8, 8))) void k_q_trace(
+    DScene S, Queue qin, Queue qout, unsigned long long* counters) {
     __shared__ int s_st[SHORT_STACK * WBLOCK];
     __shared__ QIndex qi;
     qindex_load(qin, qi);
