@@ -95,7 +95,10 @@ __device__ uint2 block_scan2(uint2 v, uint2& total) {
     return make_uint2(before.x + x.x - v.x, before.y + x.y - v.y);
 }
 
-__global__ __launch_bounds__(BB) void k_scan_tiles(const uint2* a, uint32_t n, uint2* tileSums) {
+// The element count is n, or *nDev when nDev is not null (the grid then covers an upper bound).
+__global__ __launch_bounds__(BB) void k_scan_tiles(const uint2* a, uint32_t n, const uint32_t* nDev,
+                                                   uint2* tileSums) {
+    if (nDev) n = *nDev;
     const uint32_t t0 = blockIdx.x * SCAN_TILE;
     uint2 s = make_uint2(0u, 0u);
 #pragma unroll
@@ -108,7 +111,9 @@ __global__ __launch_bounds__(BB) void k_scan_tiles(const uint2* a, uint32_t n, u
     if (threadIdx.x == 0) tileSums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(BB) void k_scan_top(uint2* tileSums, uint32_t tiles, uint2* total) {
+__global__ __launch_bounds__(BB) void k_scan_top(uint2* tileSums, uint32_t tiles, const uint32_t* nDev,
+                                                 uint2* total) {
+    if (nDev) tiles = (*nDev + SCAN_TILE - 1) / SCAN_TILE;
     uint2 carry = make_uint2(0u, 0u);
     for (uint32_t b0 = 0; b0 < tiles; b0 += BB) {
         const uint32_t b = b0 + threadIdx.x;
@@ -121,7 +126,10 @@ __global__ __launch_bounds__(BB) void k_scan_top(uint2* tileSums, uint32_t tiles
     if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ __launch_bounds__(BB) void k_scan_apply(uint2* a, uint32_t n, const uint2* tileSums) {
+__global__ __launch_bounds__(BB) void k_scan_apply(uint2* a, uint32_t n, const uint32_t* nDev,
+                                                   const uint2* tileSums) {
+    if (nDev) n = *nDev;
+    if (blockIdx.x * SCAN_TILE >= n) return;   // a tile past the count (uniform per block)
     const uint32_t t0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     uint2 v[SCAN_ITEMS];
     uint2 s = make_uint2(0u, 0u);
@@ -243,8 +251,16 @@ __global__ __launch_bounds__(BB) void k_ploc_init(const uint32_t* vals, const fl
 
 // nn[i] = the position j != i within PLOC_RADIUS whose merged box with i has the smallest
 // area, ties to the smaller (|i - j|, min(i, j) odd, min(i, j)); i itself when there is none.
-__global__ __launch_bounds__(BB) void k_ploc_nn(const uint32_t* clusters, uint32_t m, const float4* nlo,
+// Device state of the clustering: the current cluster count, the inner nodes made so far, the
+// iterations run, and a flag for an iteration that merged nothing.
+struct PlocState {
+    uint32_t m, nodeCount, iters, stalled;
+};
+
+__global__ __launch_bounds__(BB) void k_ploc_nn(const uint32_t* clusters, const PlocState* st, const float4* nlo,
                                                 const float4* nhi, uint32_t* nn) {
+    const uint32_t m = st->m;
+    if (m <= 1 || blockIdx.x * BB >= m) return;   // finished, or a block past the clusters
     constexpr int W = BB + 2 * PLOC_RADIUS;
     __shared__ float4 s_lo[W], s_hi[W];
     const int base = (int)(blockIdx.x * BB) - PLOC_RADIUS;
@@ -282,9 +298,9 @@ __global__ __launch_bounds__(BB) void k_ploc_nn(const uint32_t* clusters, uint32
 
 // flags[i] = (cluster i survives, cluster i creates a node): mutual nearest neighbours merge
 // at the lower position.
-__global__ __launch_bounds__(BB) void k_ploc_flags(const uint32_t* nn, uint32_t m, uint2* flags) {
-    const uint32_t i = tid();
-    if (i >= m) return;
+__global__ __launch_bounds__(BB) void k_ploc_flags(const uint32_t* nn, const PlocState* st, uint2* flags) {
+    const uint32_t m = st->m, i = tid();
+    if (m <= 1 || i >= m) return;
     const uint32_t j = nn[i];
     const bool mutual = j != i && nn[j] == i;
     flags[i] = make_uint2(mutual && i > j ? 0u : 1u, mutual && i < j ? 1u : 0u);
@@ -292,11 +308,11 @@ __global__ __launch_bounds__(BB) void k_ploc_flags(const uint32_t* nn, uint32_t 
 
 // After the scan: flags[i] = (output position, rank among this iteration's new nodes).
 __global__ __launch_bounds__(BB) void k_ploc_merge(const uint32_t* cin, uint32_t* cout, const uint32_t* nn,
-                                                   const uint2* flags, uint32_t m, uint32_t n, uint32_t nodeBase,
+                                                   const uint2* flags, const PlocState* st, uint32_t n,
                                                    uint32_t maxLeaf, float4* nlo, float4* nhi, uint2* kids,
                                                    uint32_t* cnt, float* cost, uint32_t* leafOK) {
-    const uint32_t i = tid();
-    if (i >= m) return;
+    const uint32_t m = st->m, nodeBase = st->nodeCount, i = tid();
+    if (m <= 1 || i >= m) return;
     const uint32_t j = nn[i];
     const bool mutual = j != i && nn[j] == i;
     if (mutual && i > j) return;   // merged into the node its partner creates
@@ -320,6 +336,22 @@ __global__ __launch_bounds__(BB) void k_ploc_merge(const uint32_t* cin, uint32_t
     cost[id] = ok ? leafC : split;
     leafOK[id] = ok ? 1u : 0u;
     cout[f.x] = id;
+}
+
+// Ends an iteration: the scan's totals give the surviving clusters and the new nodes; the node
+// count after it goes to iterEnd[iteration].  No-op once one cluster is left or after a stall.
+__global__ void k_ploc_advance(PlocState* st, const uint2* total, uint32_t* iterEnd, uint32_t iterCap) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (st->m <= 1 || st->stalled) return;
+    const uint2 t = *total;
+    if (t.y == 0 || t.x + t.y != st->m || st->iters >= iterCap) {
+        st->stalled = 1u;
+        return;
+    }
+    st->nodeCount += t.y;
+    st->m = t.x;
+    iterEnd[st->iters] = st->nodeCount;
+    st->iters += 1u;
 }
 
 // ---- top-down: triangle slots (off = position of the subtree's first triangle), node states ----
@@ -615,12 +647,13 @@ struct Arena {
 struct Scan {
     uint2* tileSums;
     uint2* total;
-    hipError_t run(uint2* a, uint32_t n, hipStream_t s) const {
+    // n elements, or *nDev elements (at most n) when nDev is given
+    hipError_t run(uint2* a, uint32_t n, hipStream_t s, const uint32_t* nDev = nullptr) const {
         if (n == 0) return hipMemsetAsync(total, 0, sizeof(uint2), s);
         const uint32_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-        hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(BB), 0, s, a, n, tileSums);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BB), 0, s, tileSums, tiles, total);
-        hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(BB), 0, s, a, n, tileSums);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(BB), 0, s, a, n, nDev, tileSums);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(BB), 0, s, tileSums, tiles, nDev, total);
+        hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(BB), 0, s, a, n, nDev, tileSums);
         return hipGetLastError();
     }
 };
@@ -649,7 +682,8 @@ bool build_bvh_device(const float* pos, const uint32_t* idx, uint32_t n, uint32_
         struct {
             float4 *triLo, *triHi, *nlo, *nhi;
             uint32_t *keys, *vals, *keysAlt, *valsAlt, *numValid, *cA, *cB, *nn, *cnt, *leafOK, *state, *off, *depth;
-            uint32_t *scalars;
+            uint32_t *scalars, *iterEnd;
+            PlocState* ploc;
             float *partial, *bounds, *cost;
             uint2 *kids, *flags, *tileSums, *total, *frontA, *frontB;
             void* sortTemp;
@@ -672,6 +706,8 @@ bool build_bvh_device(const float* pos, const uint32_t* idx, uint32_t n, uint32_
         b.off = A.take<uint32_t>(nodesAll);
         b.depth = A.take<uint32_t>(nodesAll);
         b.scalars = A.take<uint32_t>(4);
+        b.ploc = A.take<PlocState>(1);
+        b.iterEnd = A.take<uint32_t>(n);
         b.partial = A.take<float>(6 * (size_t)triBlocks);
         b.bounds = A.take<float>(6);
         b.cost = A.take<float>(nodesAll);
@@ -717,33 +753,39 @@ bool build_bvh_device(const float* pos, const uint32_t* idx, uint32_t n, uint32_
     const uint32_t* order = B.vals;   // leaf s holds triangle order[s]
     hipLaunchKernelGGL(k_ploc_init, blocks(n), dim3(BB), 0, s, order, B.triLo, B.triHi, n, B.nlo, B.nhi, B.cnt,
                        B.cost, B.leafOK, B.cA);
-    // clustering, one host round trip per iteration for the new cluster count
+    // Clustering.  The cluster count lives on the device (PlocState); the host enqueues
+    // PLOC_BATCH iterations at a time, sized for the last count it read (an upper bound: the
+    // count only falls), and reads the state back once per batch.
     const Scan scan{B.tileSums, B.total};
-    std::vector<uint32_t> iterEnd;   // inner nodes created after each iteration
-    uint32_t m = n, nodeCount = 0;
+    const PlocState st0{n, 0u, 0u, 0u};
+    e = hipMemcpyAsync(B.ploc, &st0, sizeof(st0), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return fail(e, "clustering state");
+    PlocState st = st0;
     uint32_t* cin = B.cA;
     uint32_t* cout = B.cB;
-    while (m > 1) {
-        hipLaunchKernelGGL(k_ploc_nn, blocks(m), dim3(BB), 0, s, cin, m, B.nlo, B.nhi, B.nn);
-        hipLaunchKernelGGL(k_ploc_flags, blocks(m), dim3(BB), 0, s, B.nn, m, B.flags);
-        e = scan.run(B.flags, m, s);
-        if (e != hipSuccess) return fail(e, "scan");
-        hipLaunchKernelGGL(k_ploc_merge, blocks(m), dim3(BB), 0, s, cin, cout, B.nn, B.flags, m, n, nodeCount, maxLeaf,
-                           B.nlo, B.nhi, B.kids, B.cnt, B.cost, B.leafOK);
-        uint2 tot;
-        e = hipMemcpy(&tot, B.total, sizeof(tot), hipMemcpyDeviceToHost);
+    constexpr int PLOC_BATCH = 16;
+    while (st.m > 1) {
+        for (int it = 0; it < PLOC_BATCH && e == hipSuccess; it++) {
+            hipLaunchKernelGGL(k_ploc_nn, blocks(st.m), dim3(BB), 0, s, cin, B.ploc, B.nlo, B.nhi, B.nn);
+            hipLaunchKernelGGL(k_ploc_flags, blocks(st.m), dim3(BB), 0, s, B.nn, B.ploc, B.flags);
+            e = scan.run(B.flags, st.m, s, &B.ploc->m);
+            hipLaunchKernelGGL(k_ploc_merge, blocks(st.m), dim3(BB), 0, s, cin, cout, B.nn, B.flags, B.ploc, n, maxLeaf,
+                               B.nlo, B.nhi, B.kids, B.cnt, B.cost, B.leafOK);
+            hipLaunchKernelGGL(k_ploc_advance, dim3(1), dim3(64), 0, s, B.ploc, B.total, B.iterEnd, n);
+            std::swap(cin, cout);
+        }
+        if (e == hipSuccess) e = hipMemcpy(&st, B.ploc, sizeof(st), hipMemcpyDeviceToHost);
         if (e != hipSuccess) return fail(e, "clustering");
-        if (tot.y == 0 || tot.x + tot.y != m) {
+        if (st.stalled) {
             herr = hipErrorUnknown;
-            err = "device BVH build did not converge (non-finite vertex positions?)";
+            err = "device BVH build did not converge";
             return false;
         }
-        nodeCount += tot.y;
-        iterEnd.push_back(nodeCount);
-        m = tot.x;
-        std::swap(cin, cout);
     }
-    if (nodeCount != inner) return fail(hipErrorUnknown, "clustering left the wrong number of inner nodes");
+    if (st.nodeCount != inner) return fail(hipErrorUnknown, "clustering left the wrong number of inner nodes");
+    std::vector<uint32_t> iterEnd(st.iters);   // inner nodes created after each iteration
+    if (st.iters) e = hipMemcpy(iterEnd.data(), B.iterEnd, st.iters * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(e, "clustering");
     const uint32_t root = n == 1 ? 0u : n + inner - 1;   // the last node created
     // top-down: slots, states, depths
     hipLaunchKernelGGL(k_td_root, dim3(1), dim3(64), 0, s, root, B.leafOK, B.state, B.off, B.depth);
