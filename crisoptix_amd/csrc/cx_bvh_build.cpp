@@ -434,7 +434,7 @@ bool collapse_bvh(const BvhHost& bin, int width, uint32_t stackLimit, std::vecto
     root = bin.root;
     stackBound = 0;
     if (width != 4 && width != 8) return false;
-    if (bin.root < 0) return true;   // a single leaf: no inner node
+    if (bin.root < 0 || bin.nodes.empty()) return true;   // a single leaf, or no triangle: no inner node
     bool ok = collapse_with(bin, width, false, stackLimit, out, root, stackBound) ||
               collapse_with(bin, width, true, stackLimit, out, root, stackBound);
     // Every reference must be empty, an inner node of `out` or a leaf inside the triangle slots.

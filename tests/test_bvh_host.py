@@ -41,3 +41,16 @@ def test_wide8_collapse_falls_back_to_greedy_on_large_scene():
     assert np.mean(a4[0] != 0xFFFFFFFF) > 0.3
     for x, y in zip(a8, a4):
         assert np.array_equal(bits(x), bits(y))
+
+
+def test_empty_geometry_misses():
+    """A scene with no triangles (explicit emitters only): every query misses, at every width."""
+    import dataclasses
+    sc = scenes.cornell_box(1, 8)
+    rays = random_rays(sc, 200, seed=32)
+    sc = dataclasses.replace(sc, indices=np.zeros((0, 3), np.uint32), tri_material=np.zeros(0, np.uint32))
+    for width in (2, 4, 8):
+        prim, t, u, v, _ = runtime.host_trace(sc, rays, width=width)
+        assert (prim == 0xFFFFFFFF).all() and (t == 0).all()
+        occ, _ = runtime.host_trace(sc, rays, any_hit=True, width=width)
+        assert not occ.any()
