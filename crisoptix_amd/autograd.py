@@ -12,9 +12,10 @@ autograd, so each backward must come before the next forward of the same rendere
 gradient is the path-replay estimate W * df(x)/dtheta of crisoptix_amd/csrc/cx_adjoint.hip (the
 reservoir-selected path x and its contribution weight W held fixed).
 
-The library owns all device memory: tensors cross the boundary as host arrays (the parameters
-are a few kilobytes and an image is a few megabytes), so torch's own HIP runtime never has to
-share pointers, streams or events with the library's.
+Here tensors cross the boundary as host arrays (the parameters are a few kilobytes and an image
+is a few megabytes), the simplest form for a module in a larger autograd graph.  A loop whose
+parameters stay on the device, and that records each step as one graph, is
+crisoptix_amd.inverse.DeviceStep: it hands device pointers and torch's stream to the library.
 """
 from __future__ import annotations
 

@@ -340,7 +340,7 @@ __device__ __forceinline__ void adjoint_restir_pt(const AdjCtx& C, uint32_t p, V
     const PrefixRes res = load_prefix(A.prefixRes[A.front], A.P, p);
     const bool primaryValid = A.gbuf[p].z != 0u;
     if (primaryValid && res.s.valid() && res.wSum != 0.0f) {
-        Rng canon = rng_seed(tea64(p, A.lp.random));
+        Rng canon = rng_seed(tea64(p, frame_random(A)));
         canon.f();
         canon.f();
         Factors F;
@@ -382,15 +382,16 @@ __device__ __forceinline__ void adjoint_path_tracer(const AdjCtx& C, uint32_t p,
     const AdjArgs& G = C.G;
     const DScene& S = A.S;
     const cx_launch_params& lp = A.lp;
-    Rng rng = rng_seed(tea64(p, lp.random));
+    Rng rng = rng_seed(tea64(p, frame_random(A)));
     const float jx = rng.f();
     const float jy = rng.f();
     float ux = ((float)x + jx) / (float)A.W;
     float uy = ((float)y + jy) / (float)A.H;
     ux = 2.0f * ux - 1.0f;
     uy = 2.0f * uy - 1.0f;
-    V3 pos = A.camPos;
-    V3 dir = normalize(A.camU * ux + A.camV * uy + A.camW);
+    const FrameCam cam = frame_cam(A);
+    V3 pos = cam.pos;
+    V3 dir = normalize(cam.U * ux + cam.V * uy + cam.W);
     Factors F;
     F.n = 0;
     F.overflow = false;

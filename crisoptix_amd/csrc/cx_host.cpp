@@ -112,6 +112,8 @@ struct cx_renderer {
         std::vector<uint8_t> key;
     } graphs[2];
     FrameDyn* dyn = nullptr;
+    bool deviceRandom = false;   // frames draw their random number from devRng on the device
+    uint64_t* devRng = nullptr;
     hipStream_t gstream = nullptr;   // capture / replay stream for frames on the null stream
     uint32_t captures = 0;
     // Final-phase overlap (wavefront Conditional ReSTIR): the entry store and the entry-tree
@@ -266,7 +268,7 @@ FrameArgs make_args(cx_renderer* r) {
     A.neighCount = r->neighCount;
     A.k = lp.gatherM > 1 ? (uint32_t)(lp.gatherM - 1) : 0u;
     A.counters = r->counters;
-    A.dyn = r->capturing ? r->dyn : nullptr;
+    A.dyn = (r->capturing || r->deviceRandom) ? r->dyn : nullptr;
     return A;
 }
 
@@ -984,6 +986,7 @@ int cx_renderer_create(cx_ctx* ctx, const cx_scene* scene, uint32_t w, uint32_t 
     A(dalloc(r->allocs, &r->neighCount, r->P));
     A(dalloc(r->allocs, &r->counters, (size_t)COUNTER_STRIPES * NUM_COUNTERS));
     A(dalloc(r->allocs, &r->dyn, 1));
+    A(dalloc(r->allocs, &r->devRng, 1));
     if (e != hipSuccess) {
         free_all(r->allocs);
         delete r;
@@ -1172,6 +1175,16 @@ int frame_body(cx_renderer* r, void* stream) {
     for (bool& b : r->passRan) b = false;
     r->traceEvUsed = 0;
     r->traceEvOk = true;
+    if (r->deviceRandom) {   // camera and frame index from the host, the random number on the device
+        FrameDyn d;
+        std::memset(&d, 0, sizeof(d));
+        std::memcpy(d.cam, r->cam.pos, 12);
+        std::memcpy(d.cam + 3, r->cam.U, 12);
+        std::memcpy(d.cam + 6, r->cam.V, 12);
+        std::memcpy(d.cam + 9, r->cam.W, 12);
+        d.frameIdx = r->params.frameIdx;
+        CX_HIP(r->ctx, launch_next_dyn(r->dyn, r->devRng, d, s));
+    }
     CX_HIP(r->ctx, launch_zero_counters(r->counters, COUNTER_STRIPES * NUM_COUNTERS, s));
     rec(r, 0, s);
     auto run = [&](cx_pass p) -> int {
@@ -1252,6 +1265,7 @@ std::vector<uint8_t> graph_key(const cx_renderer* r) {
     put(&r->allocEpoch, sizeof(r->allocEpoch));
     put(&r->overlap, sizeof(r->overlap));
     put(&r->waveRounds, sizeof(r->waveRounds));
+    put(&r->deviceRandom, sizeof(r->deviceRandom));
     return k;
 }
 
@@ -1339,6 +1353,26 @@ int cx_renderer_graph_captures(const cx_renderer* r) { return r ? (int)r->captur
 int cx_renderer_set_wave_rounds(cx_renderer* r, uint32_t rounds) {
     if (!r) return CX_ERR_INVALID_ARG;
     r->waveRounds = rounds;
+    return CX_OK;
+}
+
+int cx_renderer_set_device_random(cx_renderer* r, int enable, uint64_t seed) {
+    if (!r) return CX_ERR_INVALID_ARG;
+    if (enable) {
+        const uint64_t st = pcg_seed(seed);
+        CX_HIP(r->ctx, hipMemcpy(r->devRng, &st, sizeof(st), hipMemcpyHostToDevice));
+    }
+    r->deviceRandom = enable != 0;
+    return CX_OK;
+}
+
+int cx_scene_set_materials_device(cx_scene* s, const float* rows, uint32_t n, int tables, void* stream) {
+    if (!s || (n && !rows)) return CX_ERR_INVALID_ARG;
+    if (n != s->numMaterials) return set_err(s->ctx, CX_ERR_INVALID_ARG, "material count differs from the scene's");
+    if (tables & ~(CX_TABLE_SAMPLING | CX_TABLE_EVAL)) return set_err(s->ctx, CX_ERR_INVALID_ARG, "unknown table");
+    const hipStream_t st = (hipStream_t)stream;
+    if (tables & CX_TABLE_SAMPLING) CX_HIP(s->ctx, launch_set_materials((DMaterial*)s->d.mats, rows, n, st));
+    if (tables & CX_TABLE_EVAL) CX_HIP(s->ctx, launch_set_materials(s->evalMats, rows, n, st));
     return CX_OK;
 }
 

@@ -16,6 +16,8 @@ hipError_t launch_pass(int pass, const FrameArgs& A, hipStream_t s);
 hipError_t launch_reset(const FrameArgs& A, hipStream_t s);
 hipError_t launch_zero_counters(unsigned long long* c, uint32_t n, hipStream_t s);
 hipError_t launch_set_dyn(FrameDyn* dst, const FrameDyn& d, hipStream_t s);
+hipError_t launch_next_dyn(FrameDyn* dst, uint64_t* state, const FrameDyn& d, hipStream_t s);
+hipError_t launch_set_materials(DMaterial* dst, const float* rows, uint32_t n, hipStream_t s);
 hipError_t launch_store_entries(const FrameArgs& A, const EntryBuild& E, hipStream_t s);
 hipError_t launch_entries_from_arrays(const float* pos, const float* r, const uint8_t* valid, uint32_t n,
                                       const EntryBuild& E, hipStream_t s);

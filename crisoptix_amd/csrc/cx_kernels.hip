@@ -909,6 +909,39 @@ hipError_t launch_set_dyn(FrameDyn* dst, const FrameDyn& d, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Device-random frames: the camera and frame index come as arguments, the frame's random number
+// from a PCG32 whose state stays in device memory (the host generator's step, include/util/
+// random.h:52-68), so a frame recorded into a graph draws a new number on every replay.
+__global__ void k_next_dyn(FrameDyn* dst, uint64_t* state, FrameDyn d) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint64_t old = *state;
+    *state = old * PCG_MULT + PCG_INC;
+    const uint32_t xs = (uint32_t)(((old >> 18) ^ old) >> 27);
+    const uint32_t rot = (uint32_t)(old >> 59);
+    d.random = (xs >> rot) | (xs << ((32u - rot) & 31u));
+    *dst = d;
+}
+
+hipError_t launch_next_dyn(FrameDyn* dst, uint64_t* state, const FrameDyn& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_next_dyn, dim3(1), dim3(64), 0, s, dst, state, d);
+    return hipGetLastError();
+}
+
+// Material table rows from device memory (cx_material layout, 10 floats); texture bindings stay.
+__global__ void k_set_materials(DMaterial* dst, const float* rows, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = rows + 10 * (size_t)i;
+    dst[i].diffRough = make_float4(r[0], r[1], r[2], r[9]);
+    dst[i].spec = make_float4(r[3], r[4], r[5], 0.0f);
+    dst[i].emissive = make_float4(r[6], r[7], r[8], 0.0f);
+}
+
+hipError_t launch_set_materials(DMaterial* dst, const float* rows, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_set_materials, dim3((n + 255) / 256), dim3(256), 0, s, dst, rows, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_store_entries(const FrameArgs& A, const EntryBuild& E, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, E.numValid, 1u);
     hipLaunchKernelGGL(k_store_entries, tile_grid(A.W, A.H), dim3(BLOCK), 0, s, A, E);

@@ -2,6 +2,7 @@
 
     python -m crisoptix_amd.inverse gradcheck [--width 512 --height 512]
     python -m crisoptix_amd.inverse optimize  [--steps 200 --views 8 --width 640 --height 480]
+                                              [--exec graph|device|host]
 
 gradcheck: the 100 k-triangle object grid (64 object materials + ground, 16 emitters), ReSTIR
 PT (temporal and spatial reuse, max path length 4) at 512 x 512.  The loss is the L2 image
@@ -24,7 +25,10 @@ view at the true parameters (median of 8 batch means); the L2 loss leaves out pi
 10 x the target's 99.9th percentile.  Each step renders from a reset history (the
 reservoirs of the previous step were built with other parameters).  Reports parameter
 relative error (all materials, and those covering at least --min-pixels primary hits over the
-views) and ms per step.
+views) and ms per step.  --exec graph (default) records the device-resident step (DeviceStep)
+once after 3 eager steps and replays it; --exec device enqueues the same step eagerly; --exec
+host runs the views one after another with host round trips (materials, images and gradients
+as host arrays).
 """
 from __future__ import annotations
 
@@ -181,6 +185,111 @@ def gradcheck(a) -> dict:
     }
 
 
+class DeviceStep:
+    """One optimisation step of the inverse-rendering loop with every operand in device memory:
+    the material rows are assembled from the parameter tensors and written with
+    cx_scene_set_materials_device; each view renders a residual frame into a tensor, the masked
+    residual becomes dL/dI on the GPU, an independent frame is rendered and its adjoint
+    accumulates into one pair of float64 gradient tensors shared by all views; Adam
+    (capturable) updates the parameters and they are clamped.  Each view runs on a stream of its
+    own, forked from and joined back into `stream` (small views leave most of the GPU idle on
+    their own; their adjoints add into the shared gradients with atomics).  Frames draw their
+    random numbers on the device (cx_renderer_set_device_random), so the whole step can be
+    recorded once with torch.cuda.graph and replayed; the fork and join are recorded with it.
+    Nothing in a step synchronises with the host.
+
+    views: list of (runtime.Renderer, abi.CameraData); targets: [H, W, 3] float32 tensors on the
+    device; clips: per-view residual mask levels; base: [M, 10] material rows (specular and
+    emission are kept, albedo and roughness come from the parameters)."""
+
+    def __init__(self, gscene, views, targets, clips, base, albedo, rough, lr, seed=0):
+        import torch
+        self.torch = torch
+        dev = torch.device("cuda")
+        self.stream = torch.cuda.Stream()
+        self.sp = self.stream.cuda_stream
+        self.vstreams = [torch.cuda.Stream() for _ in views]
+        self.gscene, self.views = gscene, views
+        self.targets = [t.to(dev) for t in targets]
+        self.clips = [float(c) for c in clips]
+        self.base = torch.from_numpy(np.ascontiguousarray(base, np.float32)).to(dev)
+        self.rows = self.base.clone()
+        self.M = self.base.shape[0]
+        self.albedo = torch.tensor(np.asarray(albedo, np.float32), device=dev)
+        self.rough = torch.tensor(np.asarray(rough, np.float32), device=dev)
+        self.albedo.grad = torch.zeros_like(self.albedo)
+        self.rough.grad = torch.zeros_like(self.rough)
+        self.opt = torch.optim.Adam([self.albedo, self.rough], lr=lr, capturable=True)
+        H, W = self.targets[0].shape[0], self.targets[0].shape[1]
+        self.npx = float(W * H * 3)
+        self.gA = torch.zeros((self.M, 3), dtype=torch.float64, device=dev)
+        self.gR = torch.zeros(self.M, dtype=torch.float64, device=dev)
+        self.imgs = [torch.zeros((H, W, 3), dtype=torch.float32, device=dev) for _ in views]
+        self.imgs2 = [torch.zeros_like(i) for i in self.imgs]
+        self.dls = [torch.zeros_like(i) for i in self.imgs]
+        self.zero = torch.zeros((), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()   # earlier work on the renderers ends before their generators are seeded
+        for v, (ren, _) in enumerate(views):
+            ren.set_device_random(True, seed + v)
+        self.graph = None
+
+    def step(self):
+        torch = self.torch
+        with torch.cuda.stream(self.stream), torch.no_grad():
+            self.rows.copy_(self.base)
+            self.rows[:, 0:3] = self.albedo
+            self.rows[:, 9] = self.rough
+            self.gscene.set_materials_device(self.rows.data_ptr(), self.M, 3, self.sp)
+            self.gA.zero_()
+            self.gR.zero_()
+        for v, (ren, cam) in enumerate(self.views):
+            vs = self.vstreams[v]
+            vs.wait_stream(self.stream)
+            sp = vs.cuda_stream
+            with torch.cuda.stream(vs), torch.no_grad():
+                # decorrelated estimate: residual from one frame, adjoint of an independent one
+                ren.reset(stream=sp)
+                ren.launch_frame(cam, False, output=runtime.DevicePtr(self.imgs[v].data_ptr()), stream=sp)
+                img, tgt = self.imgs[v], self.targets[v]
+                torch.where(img <= self.clips[v], (img - tgt) * (2.0 / self.npx), self.zero, out=self.dls[v])
+                ren.reset(stream=sp)
+                ren.launch_frame(cam, False, output=runtime.DevicePtr(self.imgs2[v].data_ptr()), stream=sp)
+                ren.adjoint(self.dls[v].data_ptr(), self.gA.data_ptr(), self.gR.data_ptr(), None, stream=sp)
+        for vs in self.vstreams:
+            self.stream.wait_stream(vs)
+        with torch.cuda.stream(self.stream), torch.no_grad():
+            self.albedo.grad.copy_(self.gA)
+            self.rough.grad.copy_(self.gR)
+        with torch.cuda.stream(self.stream):
+            self.opt.step()
+            with torch.no_grad():
+                self.albedo.clamp_(0.01, 0.99)
+                self.rough.clamp_(0.02, 1.0)
+
+    def capture(self):
+        """Records step() into a graph (after at least one eager step has allocated everything)."""
+        torch = self.torch
+        self.stream.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=self.stream):
+            self.step()
+
+    def replay(self):
+        self.graph.replay()
+
+    def sync(self):
+        """Waits for the enqueued steps; tensors of the step may then be read on any stream."""
+        self.stream.synchronize()
+
+    def loss(self) -> float:
+        """Masked L2 loss of the last step's residual frames (synchronises)."""
+        torch = self.torch
+        with torch.cuda.stream(self.stream):
+            tot = sum(float(torch.where(i <= c, (i - t) ** 2, self.zero).double().sum()) / self.npx
+                      for i, t, c in zip(self.imgs, self.targets, self.clips))
+        return tot
+
+
 def optimize(a) -> dict:
     import torch
 
@@ -229,6 +338,8 @@ def optimize(a) -> dict:
     e0, e0o = rel_err(), rel_err(observed)
     npx = W * H * 3
     hist = []
+    if a.exec != "host":
+        return _optimize_device(a, sc, gs, views, targets, clips, truth, albedo, rough, observed, t_target, e0, e0o)
     runtime.synchronize()
     t0 = time.perf_counter()
     for step in range(a.steps):
@@ -282,6 +393,59 @@ def optimize(a) -> dict:
     }
 
 
+def _optimize_device(a, sc, gs, views, targets, clips, truth, albedo, rough, observed, t_target, e0, e0o) -> dict:
+    """The loop of optimize() with DeviceStep: eagerly (--exec device) or as one captured graph
+    per step replayed after WARMUP eager steps (--exec graph)."""
+    import torch
+    M = truth.shape[0]
+    ds = DeviceStep(gs, [(dv.ren, dv.cam) for dv in views], targets, clips, truth, albedo.detach().numpy(),
+                    rough.detach().numpy(), a.lr, seed=1000)
+
+    def rel_err(mask=None):
+        m = np.ones(M, bool) if mask is None else mask
+        al = ds.albedo.detach().cpu().numpy()[m]
+        ro = ds.rough.detach().cpu().numpy()[m]
+        ea = np.linalg.norm(al - truth[m, 0:3]) / np.linalg.norm(truth[m, 0:3])
+        er = np.linalg.norm(ro - truth[m, 9]) / np.linalg.norm(truth[m, 9])
+        return float(ea), float(er)
+
+    hist = []
+    warm = min(WARMUP, a.steps) if a.exec == "graph" else 0
+    for _ in range(warm):
+        ds.step()
+    if a.exec == "graph":
+        ds.capture()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for step in range(warm, a.steps):
+        if a.exec == "graph":
+            ds.replay()
+        else:
+            ds.step()
+        if step % 20 == 0 or step == a.steps - 1:
+            hist.append({"step": step, "loss": ds.loss(), "rel_err": rel_err(), "rel_err_observed": rel_err(observed)})
+            print(f"step {step}: loss {hist[-1]['loss']:.6f}, rel err albedo {hist[-1]['rel_err'][0]:.4f} "
+                  f"roughness {hist[-1]['rel_err'][1]:.4f}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / max(1, a.steps - warm)
+    e1, e1o = rel_err(), rel_err(observed)
+    return {
+        "workload": f"inverse_rendering_{a.views}x{a.width}x{a.height}_objgrid16_subdiv3",
+        "exec": a.exec, "triangles": sc.num_triangles, "materials": M, "views": a.views, "steps": a.steps,
+        "lr": a.lr, "target_frames": a.target_frames, "target_seconds": t_target, "ms_per_step": ms,
+        "frames_per_step": 2 * a.views, "observed_materials": int(observed.sum()),
+        "observed_min_pixels": a.min_pixels,
+        "rel_err_initial": {"albedo": e0[0], "roughness": e0[1]},
+        "rel_err_final": {"albedo": e1[0], "roughness": e1[1]},
+        "rel_err_observed_initial": {"albedo": e0o[0], "roughness": e0o[1]},
+        "rel_err_observed_final": {"albedo": e1o[0], "roughness": e1o[1]},
+        "history": hist,
+    }
+
+
+WARMUP = 3   # eager steps before a step graph is captured
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=["gradcheck", "optimize"])
@@ -295,6 +459,9 @@ def main(argv=None):
     ap.add_argument("--views", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-2)
     ap.add_argument("--min-pixels", type=int, default=200)
+    ap.add_argument("--exec", choices=["host", "device", "graph"], default="graph",
+                    help="optimize: host round trips per view (host), a device-resident step (device), or that "
+                         "step captured once and replayed (graph)")
     a = ap.parse_args(argv)
     if a.mode == "gradcheck":
         a.width, a.height = a.width or 512, a.height or 512

@@ -82,6 +82,14 @@ def synchronize():
     _hcheck(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+class DevicePtr:
+    """A device pointer owned elsewhere (a torch tensor's data_ptr(), say), where the API takes
+    a buffer object."""
+
+    def __init__(self, ptr: int):
+        self.ptr = int(ptr)
+
+
 class DeviceArray:
     """A device allocation with a numpy dtype and shape."""
 
@@ -180,6 +188,8 @@ def lib():
         L.cx_renderer_set_graph_mode.argtypes = [vp, i32]
         L.cx_renderer_set_overlap.argtypes = [vp, i32]
         L.cx_renderer_set_wave_rounds.argtypes = [vp, u32]
+        L.cx_renderer_set_device_random.argtypes = [vp, i32, u64]
+        L.cx_scene_set_materials_device.argtypes = [vp, vp, u32, i32, vp]
         L.cx_renderer_graph_captures.argtypes = [vp]
         L.cx_ctx_create.argtypes = [i32, C.POINTER(vp)]
         L.cx_ctx_destroy.argtypes = [vp]
@@ -317,6 +327,12 @@ class Scene:
         self.ctx.check(lib().cx_scene_set_materials(self.h, m.ctypes.data, m.shape[0], tables),
                        "cx_scene_set_materials")
 
+    def set_materials_device(self, rows_ptr: int, n: int, tables: int = 3, stream=None):
+        """Material rows (n x 10 float32, cx_material layout) from device memory, enqueued on
+        `stream` (cx_scene_set_materials_device)."""
+        self.ctx.check(lib().cx_scene_set_materials_device(self.h, rows_ptr, n, tables, stream),
+                       "cx_scene_set_materials_device")
+
     def set_emitter_colors(self, colors: np.ndarray, tables: int = 3):
         c = np.ascontiguousarray(colors, np.float32).reshape(-1, 3)
         self.ctx.check(lib().cx_scene_set_emitter_colors(self.h, c.ctypes.data, c.shape[0], tables),
@@ -414,8 +430,8 @@ class Renderer:
     def seed(self, seed: int):
         lib().cx_renderer_seed(self.h, seed)
 
-    def reset(self):
-        self.ctx.check(lib().cx_renderer_reset(self.h, None), "cx_renderer_reset")
+    def reset(self, stream=None):
+        self.ctx.check(lib().cx_renderer_reset(self.h, stream), "cx_renderer_reset")
 
     def enable_timing(self, on: bool = True):
         lib().cx_renderer_enable_timing(self.h, 1 if on else 0)
@@ -438,6 +454,12 @@ class Renderer:
     def set_wave_rounds(self, rounds: int):
         """Trace/step rounds per machine before the in-place tail kernel (0: all rounds)."""
         self.ctx.check(lib().cx_renderer_set_wave_rounds(self.h, rounds), "cx_renderer_set_wave_rounds")
+
+    def set_device_random(self, enable: bool = True, seed: int = 0):
+        """Frames draw their random number on the device (cx_renderer_set_device_random), so a
+        frame recorded into a caller's graph draws a new one on every replay."""
+        self.ctx.check(lib().cx_renderer_set_device_random(self.h, 1 if enable else 0, seed),
+                       "cx_renderer_set_device_random")
 
     def graph_captures(self) -> int:
         return int(lib().cx_renderer_graph_captures(self.h))

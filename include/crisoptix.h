@@ -360,6 +360,16 @@ int cx_renderer_set_queue_grid(cx_renderer* r, uint32_t maxBlocks);
  * (crisoptix_amd/tune.py). */
 int cx_renderer_set_wave_rounds(cx_renderer* r, uint32_t rounds);
 
+/* Device-random frames (off by default).  With enable != 0, each cx_renderer_frame draws its
+ * frame random number (the value src/Renderer.cpp:220 takes from the host generator) from a PCG32
+ * whose state lives in device memory, seeded here with `seed` (the host generator's seeding), and
+ * reads the camera, that number and the frame index from a device parameter block written by a
+ * one-thread kernel at the start of the frame.  A frame enqueued inside a caller's stream capture
+ * (a hipGraph of a whole optimisation step, say) then draws a new random number on every replay.
+ * No counterpart in the reference; for callers that record frames into their own graphs.  The
+ * adjoint of a frame reads the same block.  cx_pass_launch keeps the host generator. */
+int cx_renderer_set_device_random(cx_renderer* r, int enable, uint64_t seed);
+
 /* Final-phase overlap (on by default; wavefront Conditional ReSTIR): cx_renderer_frame runs the
  * prefix-entry store and the entry-tree build (src/Renderer.cpp:351-366) on a renderer-owned
  * stream beside the final gather's canonical prefix and suffix walks, which do not read them,
@@ -422,6 +432,10 @@ enum {
 /* Sets the constants of every material; texture bindings stay as the scene was built. */
 int cx_scene_set_materials(cx_scene* scene, const cx_material* materials, uint32_t n, int tables);
 /* colors: host, [n][3] emitter radiance (Emitter colour, include/model/Emitter.h:13-55) */
+/* Material rows from device memory (`rows`: n cx_material in device memory, enqueued on
+ * `stream`); like cx_scene_set_materials otherwise.  For optimisation loops whose parameters stay
+ * on the device.  No counterpart in the reference. */
+int cx_scene_set_materials_device(cx_scene* scene, const float* rows, uint32_t n, int tables, void* stream);
 int cx_scene_set_emitter_colors(cx_scene* scene, const float* colors, uint32_t n, int tables);
 
 /* Path-replay adjoint of the last frame's estimate (ReSTIR PT or the path tracer), with no

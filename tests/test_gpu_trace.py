@@ -118,17 +118,21 @@ def _subset(sc, k):
     return dataclasses.replace(sc, indices=sc.indices[:k].copy(), tri_material=sc.tri_material[:k].copy())
 
 
-@pytest.mark.parametrize("k", [1, 2, 3, 5, 17])
+@pytest.mark.parametrize("k", [0, 1, 2, 3, 5, 17])
 def test_device_build_tiny_scenes(k, orc, cx_runtime, cx_ctx):
-    """1 to 17 triangles: a single leaf holding every triangle (k <= max leaf) and small trees."""
-    sc = _subset(scenes.cornell_box(1, 8), k)
-    rays = np.concatenate([random_rays(sc, 3000, seed=19), centroid_rays(sc, seed=20)])
+    """0 to 17 triangles: no geometry, a single leaf holding every triangle (k <= max leaf) and
+    small trees."""
+    full = scenes.cornell_box(1, 8)
+    sc = _subset(full, k)
+    rays = random_rays(full, 3000, seed=19)   # aimed at the full box (its vertices stay in the scene)
+    if k:
+        rays = np.concatenate([rays, centroid_rays(sc, seed=20)])
     srays = shadow_rays(sc, 3000, seed=21)
     oscene = orc.OracleScene(sc)
     for width in (2, 4, 8):
         for max_leaf in (1, 4, 8):
             gscene = cx_runtime.Scene(cx_ctx, sc, width=width, max_leaf=max_leaf, builder="device")
-            if k == 1:
+            if k <= 1:
                 assert gscene.info.numNodes == 0
             _check_against_oracle(sc, gscene, oscene, rays, srays)
 
