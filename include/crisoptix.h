@@ -367,7 +367,10 @@ int cx_renderer_set_wave_rounds(cx_renderer* r, uint32_t rounds);
  * one-thread kernel at the start of the frame.  A frame enqueued inside a caller's stream capture
  * (a hipGraph of a whole optimisation step, say) then draws a new random number on every replay.
  * No counterpart in the reference; for callers that record frames into their own graphs.  The
- * adjoint of a frame reads the same block.  cx_pass_launch keeps the host generator. */
+ * adjoint of a frame reads the same block.  cx_pass_launch keeps the host generator.  Recording
+ * by a caller is tested for ReSTIR PT and path-tracer frames (torch.cuda.graph, views on forked
+ * streams); Conditional ReSTIR frames are recorded with cx_renderer_set_graph_mode, whose capture
+ * is tested, as a caller's recording of them has not been. */
 int cx_renderer_set_device_random(cx_renderer* r, int enable, uint64_t seed);
 
 /* Final-phase overlap (on by default; wavefront Conditional ReSTIR): cx_renderer_frame runs the
