@@ -66,8 +66,9 @@ def parse():
                     help="most triangles per BVH leaf (0: the library's default, 2)")
     ap.add_argument("--bvh-builder", choices=["host", "device"], default="host",
                     help="binned SAH on the host or PLOC on the GPU (same hits, different trees)")
-    ap.add_argument("--wave-rounds", type=int, default=2,
-                    help="trace/step rounds per machine before the in-place tail kernel (0: all rounds)")
+    ap.add_argument("--wave-rounds", type=int, default=None,
+                    help="trace/step rounds of every machine before the in-place tail kernel (0: all rounds; "
+                         "default: suffix generation 1, the other machines 2)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="run the entry build and neighbour query in sequence instead of beside the gather walks")
     return ap.parse_args()
@@ -170,7 +171,8 @@ def main():
     ren.enable_timing(True)
     ren.set_exec_mode(a.exec)
     ren.set_overlap(not a.no_overlap)
-    ren.set_wave_rounds(a.wave_rounds)
+    if a.wave_rounds is not None:
+        ren.set_wave_rounds(a.wave_rounds)
 
     def frame():
         cam.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
@@ -267,7 +269,7 @@ def main():
             "workload": f"conditional_restir_{W}x{H}_objgrid{a.grid}",
             "exec_mode": a.exec,
             "final_phase_overlap": not a.no_overlap,
-            "wave_rounds": a.wave_rounds,
+            "wave_rounds": a.wave_rounds if a.wave_rounds is not None else "suffix generation 1, other machines 2",
             "resolution": [W, H],
             "triangles": scene.num_triangles,
             "max_path_len": int(params.maxPathLen),

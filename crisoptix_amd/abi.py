@@ -56,6 +56,7 @@ class Material(C.Structure):
 ABI_VERSION = 8   # CX_ABI_VERSION of include/crisoptix.h
 CX_SPEC_TEX_COLOR = 0
 CX_SPEC_TEX_OCC_ROUGH_METAL = 1
+CX_WAVE_ROUNDS_DEFAULT = 0xFFFFFFFF   # per-machine default wavefront rounds
 CX_BVH_BUILD_HOST = 0
 CX_BVH_BUILD_DEVICE = 1
 BVH_BUILDERS = {"host": CX_BVH_BUILD_HOST, "device": CX_BVH_BUILD_DEVICE}

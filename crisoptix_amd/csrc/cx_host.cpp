@@ -101,7 +101,7 @@ struct cx_renderer {
     uint32_t maxLenSeen = 0;
     uint32_t maxPrefixSeen = 0;
     uint32_t gridLimit = 4096;   // blocks of the grid-stride queue kernels
-    uint32_t waveRounds = 2;     // trace/step rounds per machine before the tail kernel (0: none)
+    uint32_t waveRounds = CX_WAVE_ROUNDS_DEFAULT;   // rounds before the tail kernel, every machine (0: all)
     uint64_t allocEpoch = 0;     // bumped whenever a buffer a frame uses is reallocated
     // Graph mode: a frame is one hipGraphLaunch of a graph captured from the eager frame for
     // each front-buffer parity; the camera, random and frame index come from `dyn`.
@@ -446,6 +446,12 @@ void trace_rec(cx_renderer* r, hipStream_t s) {
     (void)hipEventRecord(r->traceEv[r->traceEvUsed++], s);
 }
 
+// Wavefront rounds before the tail kernel by machine.  Measured on the 1080p Conditional ReSTIR
+// frame: suffix generation 1 / 2 / 3 rounds -> 8.62 / 8.81 / 8.99 ms per frame (its walks have
+// few long survivors after the first bounce); prefix generation 1 / 2 / 3 -> 9.05 / 8.81 / 8.83;
+// shifts 1 / 2 / 3 -> 8.80 / 8.81 / 8.85.
+uint32_t default_wave_rounds(int step) { return step == WK_SG_STEP ? 1u : 2u; }
+
 hipError_t run_machine(cx_renderer* r, WaveArgs W, int setup, int step, uint32_t cap, uint32_t rounds,
                        int pass, hipStream_t s) {
     W.qin = queue_view(r, 1, cap);
@@ -454,7 +460,8 @@ hipError_t run_machine(cx_renderer* r, WaveArgs W, int setup, int step, uint32_t
     hipError_t e = wave_zero_counts(W.qout.count, s);
     if (e == hipSuccess) e = wave_launch(setup, W, r->P, s);
     int cur = 0;
-    const uint32_t waveRounds = (r->waveRounds > 0 && r->waveRounds < rounds) ? r->waveRounds : rounds;
+    const uint32_t wr = r->waveRounds != CX_WAVE_ROUNDS_DEFAULT ? r->waveRounds : default_wave_rounds(step);
+    const uint32_t waveRounds = (wr > 0 && wr < rounds) ? wr : rounds;
     for (uint32_t i = 0; i < waveRounds && e == hipSuccess; i++) {
         const Queue qin = queue_view(r, cur, cap), qout = queue_view(r, 1 - cur, cap);
         trace_rec(r, s);
