@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--wave-rounds", type=int, default=None,
                     help="trace/step rounds of every machine before the in-place tail kernel (0: all rounds; "
                          "default: suffix generation 1, the other machines 2)")
+    ap.add_argument("--queue-grid", type=int, default=0,
+                    help="most blocks of the grid-stride queue kernels (0: the library's default, 8192)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="run the entry build and neighbour query in sequence instead of beside the gather walks")
     return ap.parse_args()
@@ -173,6 +175,8 @@ def main():
     ren.set_overlap(not a.no_overlap)
     if a.wave_rounds is not None:
         ren.set_wave_rounds(a.wave_rounds)
+    if a.queue_grid:
+        ren.set_queue_grid(a.queue_grid)
 
     def frame():
         cam.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
@@ -269,6 +273,7 @@ def main():
             "workload": f"conditional_restir_{W}x{H}_objgrid{a.grid}",
             "exec_mode": a.exec,
             "final_phase_overlap": not a.no_overlap,
+            "queue_grid": a.queue_grid or 8192,
             "wave_rounds": a.wave_rounds if a.wave_rounds is not None else "suffix generation 1, other machines 2",
             "resolution": [W, H],
             "triangles": scene.num_triangles,

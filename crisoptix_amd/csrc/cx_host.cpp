@@ -100,7 +100,9 @@ struct cx_renderer {
     // trace/step rounds a machine can need (records of earlier frames take part in shifts).
     uint32_t maxLenSeen = 0;
     uint32_t maxPrefixSeen = 0;
-    uint32_t gridLimit = 4096;   // blocks of the grid-stride queue kernels
+    // Most blocks of the grid-stride queue kernels.  1080p Conditional ReSTIR frame: 2048 / 4096 /
+    // 8192 / 16384 -> 9.53 / 8.59-8.62 / 8.45 / 8.47 ms; a machine of fewer jobs launches fewer.
+    uint32_t gridLimit = 8192;
     uint32_t waveRounds = CX_WAVE_ROUNDS_DEFAULT;   // rounds before the tail kernel, every machine (0: all)
     uint64_t allocEpoch = 0;     // bumped whenever a buffer a frame uses is reallocated
     // Graph mode: a frame is one hipGraphLaunch of a graph captured from the eager frame for

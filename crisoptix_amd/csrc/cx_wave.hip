@@ -1157,7 +1157,7 @@ __global__ void k_w_zero2(uint32_t* c) {
 static dim3 grid_for(uint32_t n) { return dim3((n + WBLOCK - 1) / WBLOCK); }
 
 // Blocks of a grid-stride launch over a queue: enough for its capacity, at most q.maxBlocks
-// (the renderer's queue-grid setting, 4096 by default: 16 blocks of 128 threads per CU).
+// (the renderer's queue-grid setting, 8192 by default: 32 blocks of 128 threads per CU).
 static dim3 stride_grid(const Queue& q) { return dim3(queue_grid_blocks(q.jobs, q.maxBlocks)); }
 
 hipError_t wave_zero_counts(uint32_t* c, hipStream_t s) {

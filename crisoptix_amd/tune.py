@@ -25,7 +25,7 @@ import numpy as np
 from . import abi, runtime, scenes
 
 WIDTHS = (2, 4, 8)
-QUEUE_GRIDS = (1024, 2048, 4096, 8192)
+QUEUE_GRIDS = (2048, 4096, 8192, 16384)
 EXEC_MODES = ("wavefront", "megakernel")
 WAVE_ROUNDS = (0, 1, 2, 3)
 

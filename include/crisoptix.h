@@ -350,7 +350,7 @@ int cx_batch_frame(cx_ctx* ctx, cx_renderer* const* renderers, const cx_camera* 
 int cx_batch_sync(cx_ctx* ctx);
 
 /* Wavefront mode: the grid-stride traversal and step kernels launch at most maxBlocks blocks
- * of 128 threads (default 4096); a tuning knob (crisoptix_amd/tune.py), results unchanged. */
+ * of 128 threads (default 8192); a tuning knob (crisoptix_amd/tune.py), results unchanged. */
 int cx_renderer_set_queue_grid(cx_renderer* r, uint32_t maxBlocks);
 
 /* Wavefront mode: each path or shift machine runs at most `rounds` rounds of one traversal
