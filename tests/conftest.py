@@ -21,7 +21,7 @@ def orc():
 
 @pytest.fixture(scope="session")
 def cx_runtime():
-    """The HIP product; on a GPU box a missing library is an error, never a skip."""
+    """The HIP product; on a machine with a GPU a missing library is an error, never a skip."""
     from crisoptix_amd import runtime as rt
     rt.lib()
     if rt.device_count() < 1:
