@@ -4,7 +4,9 @@ recorded once with torch.cuda.graph: frames draw their random numbers on the dev
 (cx_scene_set_materials_device), so each replay must equal the eager step that starts from the
 same state.  With learning rate 0 the parameters stay fixed and the steps differ only by their
 random numbers: every replay's residual image equals the eager one bit for bit, and its
-gradients up to the summation order of the adjoint's double-precision atomics."""
+gradients up to the summation order of the adjoint's double-precision atomics.  Also: materials
+written from device memory render exactly like the host upload, and device-random frames depend
+only on their seed."""
 import numpy as np
 import pytest
 
@@ -84,3 +86,43 @@ def test_graph_steps_follow_eager_parameters(cx_runtime, cx_ctx):
     assert not np.allclose(ea, al)   # the parameters moved
     np.testing.assert_allclose(ga, ea, rtol=0, atol=1e-5)
     np.testing.assert_allclose(gr, er, rtol=0, atol=1e-5)
+
+
+def test_device_materials_equal_host_upload(cx_runtime, cx_ctx):
+    """cx_scene_set_materials_device writes the same tables as cx_scene_set_materials: frames
+    rendered after either upload of perturbed materials are bit-identical."""
+    import torch
+    sc = scenes.inverse_scene()
+    mats = scenes.perturbed_materials(sc.materials, seed=8)
+    out = []
+    for device in (False, True):
+        gs = cx_runtime.Scene(cx_ctx, sc)
+        if device:
+            rows = torch.from_numpy(np.ascontiguousarray(mats, np.float32)).cuda()
+            gs.set_materials_device(rows.data_ptr(), rows.shape[0], 3, None)
+            torch.cuda.synchronize()
+        else:
+            gs.set_materials(mats, 3)
+        cam = scenes.orbit_camera(sc, W, H)
+        ren = cx_runtime.Renderer(W, H, cx_ctx, cam, gs, params=inverse._params(gs.size, W, H), seed=21)
+        cd = abi.CameraData.from_dict(cam.data())
+        out.append([ren.launch_frame(cd, f == 0).numpy().copy() for f in range(2)])
+    for a, b in zip(*out):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_device_random_frames_reproducible(cx_runtime, cx_ctx):
+    """Device-random frames depend only on the seed: two renderers seeded alike render the same
+    frames, and consecutive frames differ."""
+    sc = scenes.inverse_scene()
+    gs = cx_runtime.Scene(cx_ctx, sc)
+    frames = []
+    for _ in range(2):
+        cam = scenes.orbit_camera(sc, W, H)
+        ren = cx_runtime.Renderer(W, H, cx_ctx, cam, gs, params=inverse._params(gs.size, W, H), seed=1)
+        ren.set_device_random(True, 99)
+        cd = abi.CameraData.from_dict(cam.data())
+        frames.append([ren.launch_frame(cd, f == 0).numpy().copy() for f in range(3)])
+    for a, b in zip(*frames):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert not np.array_equal(frames[0][1], frames[0][2])
