@@ -54,3 +54,44 @@ def test_empty_geometry_misses():
         assert (prim == 0xFFFFFFFF).all() and (t == 0).all()
         occ, _ = runtime.host_trace(sc, rays, any_hit=True, width=width)
         assert not occ.any()
+
+
+def _soup(seed):
+    """A random triangle soup with degenerate triangles mixed in: zero-area (repeated vertex),
+    collinear, exact duplicates, and clusters at very different scales."""
+    import dataclasses
+    rng = np.random.default_rng(seed)
+    base = scenes.cornell_box(1, 8)
+    n = int(rng.integers(1, 600))
+    c = rng.normal(size=(n, 1, 3)) * rng.choice([1e-3, 1.0, 50.0], size=(n, 1, 1))
+    tri = c + rng.normal(size=(n, 3, 3)) * rng.choice([1e-4, 0.05, 2.0], size=(n, 1, 1))
+    k = n // 8
+    tri[:k, 2] = tri[:k, 1]                                          # zero area
+    tri[k:2 * k, 2] = 2.0 * tri[k:2 * k, 1] - tri[k:2 * k, 0]        # collinear
+    if n > 4 * k:
+        tri[2 * k:3 * k] = tri[3 * k:4 * k]                          # duplicates
+    pos = tri.reshape(-1, 3).astype(np.float32)
+    return dataclasses.replace(
+        base, positions=pos, normals=np.tile(np.float32([[0, 0, 1]]), (pos.shape[0], 1)),
+        indices=np.arange(pos.shape[0], dtype=np.uint32).reshape(-1, 3),
+        tri_material=np.zeros(n, np.uint32), uvs=None, material_tex=None, textures=[])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_soups_match_oracle(seed, orc):
+    """Random soups with degenerate triangles: every width and leaf size gives the oracle's
+    closest hits and occlusion bit for bit (the builds and both collapses on odd trees)."""
+    sc = _soup(seed)
+    rays = random_rays(sc, 3000, seed=40 + seed)
+    srays = shadow_rays(sc, 2000, seed=50 + seed)
+    oscene = orc.OracleScene(sc)
+    ref = oscene.trace_closest(rays)
+    assert np.mean(ref[0] != 0xFFFFFFFF) > 0.1   # about a quarter of the rays hit
+    occ = oscene.trace_occluded(srays).astype(np.uint32)
+    for width in (2, 4, 8):
+        for max_leaf in (1, 2, 4, 15):
+            *got, _ = runtime.host_trace(sc, rays, width=width, max_leaf=max_leaf)
+            for x, y in zip(ref, got):
+                assert np.array_equal(bits(x), bits(y)), (width, max_leaf)
+            og, _ = runtime.host_trace(sc, srays, any_hit=True, width=width, max_leaf=max_leaf)
+            assert np.array_equal(occ, og), (width, max_leaf)
