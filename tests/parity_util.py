@@ -62,3 +62,24 @@ def rel_mismatch(a, b, rtol=1e-4, atol=1e-6):
     bad = np.abs(a - b) > rtol * np.maximum(np.abs(a), np.abs(b)) + atol
     bad |= np.isnan(a) != np.isnan(b)
     return bad
+
+
+def random_soup(seed):
+    """A random triangle soup with degenerate triangles mixed in: zero-area (repeated vertex),
+    collinear, exact duplicates, and clusters at very different scales."""
+    import dataclasses
+    rng = np.random.default_rng(seed)
+    base = scenes.cornell_box(1, 8)
+    n = int(rng.integers(1, 600))
+    c = rng.normal(size=(n, 1, 3)) * rng.choice([1e-3, 1.0, 50.0], size=(n, 1, 1))
+    tri = c + rng.normal(size=(n, 3, 3)) * rng.choice([1e-4, 0.05, 2.0], size=(n, 1, 1))
+    k = n // 8
+    tri[:k, 2] = tri[:k, 1]                                          # zero area
+    tri[k:2 * k, 2] = 2.0 * tri[k:2 * k, 1] - tri[k:2 * k, 0]        # collinear
+    if n > 4 * k:
+        tri[2 * k:3 * k] = tri[3 * k:4 * k]                          # duplicates
+    pos = tri.reshape(-1, 3).astype(np.float32)
+    return dataclasses.replace(
+        base, positions=pos, normals=np.tile(np.float32([[0, 0, 1]]), (pos.shape[0], 1)),
+        indices=np.arange(pos.shape[0], dtype=np.uint32).reshape(-1, 3),
+        tri_material=np.zeros(n, np.uint32), uvs=None, material_tex=None, textures=[])

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from crisoptix_amd import runtime, scenes
-from parity_util import bits, random_rays, shadow_rays, small_grid_scene
+from parity_util import bits, random_rays, random_soup, shadow_rays, small_grid_scene
 
 pytestmark = pytest.mark.skipif(not __import__("os").path.exists(runtime.LIB_PATH), reason="HIP library not built")
 
@@ -56,32 +56,11 @@ def test_empty_geometry_misses():
         assert not occ.any()
 
 
-def _soup(seed):
-    """A random triangle soup with degenerate triangles mixed in: zero-area (repeated vertex),
-    collinear, exact duplicates, and clusters at very different scales."""
-    import dataclasses
-    rng = np.random.default_rng(seed)
-    base = scenes.cornell_box(1, 8)
-    n = int(rng.integers(1, 600))
-    c = rng.normal(size=(n, 1, 3)) * rng.choice([1e-3, 1.0, 50.0], size=(n, 1, 1))
-    tri = c + rng.normal(size=(n, 3, 3)) * rng.choice([1e-4, 0.05, 2.0], size=(n, 1, 1))
-    k = n // 8
-    tri[:k, 2] = tri[:k, 1]                                          # zero area
-    tri[k:2 * k, 2] = 2.0 * tri[k:2 * k, 1] - tri[k:2 * k, 0]        # collinear
-    if n > 4 * k:
-        tri[2 * k:3 * k] = tri[3 * k:4 * k]                          # duplicates
-    pos = tri.reshape(-1, 3).astype(np.float32)
-    return dataclasses.replace(
-        base, positions=pos, normals=np.tile(np.float32([[0, 0, 1]]), (pos.shape[0], 1)),
-        indices=np.arange(pos.shape[0], dtype=np.uint32).reshape(-1, 3),
-        tri_material=np.zeros(n, np.uint32), uvs=None, material_tex=None, textures=[])
-
-
 @pytest.mark.parametrize("seed", range(6))
 def test_random_soups_match_oracle(seed, orc):
     """Random soups with degenerate triangles: every width and leaf size gives the oracle's
     closest hits and occlusion bit for bit (the builds and both collapses on odd trees)."""
-    sc = _soup(seed)
+    sc = random_soup(seed)
     rays = random_rays(sc, 3000, seed=40 + seed)
     srays = shadow_rays(sc, 2000, seed=50 + seed)
     oscene = orc.OracleScene(sc)

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from crisoptix_amd import scenes
-from parity_util import bits, random_rays, shadow_rays, small_grid_scene
+from parity_util import bits, random_rays, random_soup, shadow_rays, small_grid_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -192,3 +192,16 @@ def test_device_build_deterministic(cx_runtime, cx_ctx):
     h = cx_runtime.Scene(cx_ctx, sc, builder="host").trace_closest(rays)
     for x, y, z in zip(h, a.trace_closest(rays), b.trace_closest(rays)):
         assert np.array_equal(bits(x), bits(y)) and np.array_equal(bits(y), bits(z))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_device_build_random_soups(seed, orc, cx_runtime, cx_ctx):
+    """Random soups with zero-area, collinear and duplicated triangles at mixed scales: the device
+    build's answers equal the oracle's at every width."""
+    sc = random_soup(seed)
+    rays = np.concatenate([random_rays(sc, 3000, seed=60 + seed), centroid_rays(sc, seed=70 + seed)])
+    srays = shadow_rays(sc, 2000, seed=80 + seed)
+    oscene = orc.OracleScene(sc)
+    for width in (2, 4, 8):
+        gscene = cx_runtime.Scene(cx_ctx, sc, width=width, builder="device")
+        _check_against_oracle(sc, gscene, oscene, rays, srays)
