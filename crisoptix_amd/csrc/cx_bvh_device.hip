@@ -491,11 +491,122 @@ __device__ __forceinline__ void bin_children(const float4* nodes, int32_t node, 
     c1 = WChild{{b.x, b.z, c.z}, {b.y, b.w, c.w}, r.y};
 }
 
-// Children of the wide node opened from binary node `node`: take its two children and keep
-// replacing the inner child of largest surface area by its two children.
+// Least-area collapse tables per binary node (the host build's dynamic programme, cx_bvh_build.cpp):
+// cost[b][i] = least summed area of wide nodes for binary node b spread over at most i slots of
+// its parent, use[b][i] = the slot count that optimum uses (1: b becomes a wide node), split[b][j]
+// = the left child's share of j slots.  use == null selects the greedy rule.
+struct WideDP {
+    float* cost;
+    uint8_t* use;
+    uint8_t* split;
+};
+
+// Tables of the kept binary nodes made in one clustering iteration (ids n + [lo, hi)); their kept
+// children were made in earlier iterations, so their tables are complete.
+template <int N>
+__global__ __launch_bounds__(BB) void k_wide_dp(uint32_t n, uint32_t lo, uint32_t hi, const uint32_t* state,
+                                                const uint2* index, const float4* nodes, uint32_t numNodes,
+                                                WideDP dp, uint32_t* bad) {
+    const uint32_t k = lo + tid();
+    if (k >= hi || state[n + k] != NODE_KEPT) return;
+    const uint32_t b = index[k].x;
+    if (b >= numNodes) {
+        atomicOr(bad, 64u);
+        return;
+    }
+    const float4* q = nodes + 4 * (size_t)b;
+    const float4 x = q[0], y = q[1], z = q[2];
+    const int4 r = __builtin_bit_cast(int4, q[3]);
+    WChild c[2];
+    c[0] = WChild{{x.x, x.z, z.x}, {x.y, x.w, z.y}, r.x};
+    c[1] = WChild{{y.x, y.z, z.z}, {y.y, y.w, z.w}, r.y};
+    if ((r.x >= 0 && (uint32_t)r.x >= b) || (r.y >= 0 && (uint32_t)r.y >= b)) {   // children come first
+        atomicOr(bad, 64u);
+        return;
+    }
+    WChild u;
+    for (int a = 0; a < 3; a++) {
+        u.lo[a] = fminf(c[0].lo[a], c[1].lo[a]);
+        u.hi[a] = fmaxf(c[0].hi[a], c[1].hi[a]);
+    }
+    constexpr int S = N + 1;
+    auto child_cost = [&](int ci, int i) -> float {
+        return c[ci].ref < 0 ? 0.0f : dp.cost[(size_t)c[ci].ref * S + i];
+    };
+    float D[N + 1];
+    const size_t base = (size_t)b * S;
+    for (int j = 2; j <= N; j++) {
+        D[j] = INFINITY;
+        for (int kk = 1; kk < j; kk++) {
+            const float v = child_cost(0, kk) + child_cost(1, j - kk);
+            if (v < D[j]) {
+                D[j] = v;
+                dp.split[base + j] = (uint8_t)kk;
+            }
+        }
+    }
+    dp.cost[base + 1] = wchild_area(u) + D[N];
+    dp.use[base + 1] = 1;
+    for (int i = 2; i <= N; i++) {
+        if (D[i] < dp.cost[base + i - 1]) {
+            dp.cost[base + i] = D[i];
+            dp.use[base + i] = (uint8_t)i;
+        } else {
+            dp.cost[base + i] = dp.cost[base + i - 1];
+            dp.use[base + i] = dp.use[base + i - 1];
+        }
+    }
+}
+
+// Children of the wide node opened from binary node `node`.  With tables: the node spread over N
+// slots by the least-area optimum.  Without: take its two children and keep replacing the inner
+// child of largest surface area by its two children.
 template <int N>
 __device__ __forceinline__ int open_wide(const float4* nodes, uint32_t numNodes, int32_t node, WChild (&ch)[N],
-                                        bool& ok) {
+                                        bool& ok, const WideDP& dp) {
+    if (dp.use) {
+        constexpr int S = N + 1;
+        WChild pend[N + 1];
+        int pendI[N + 1];
+        int ps = 0, n = 0;
+        auto distribute = [&](int32_t nd, int j) {
+            WChild l, r;
+            bin_children(nodes, nd, l, r);
+            const int kk = dp.split[(size_t)nd * S + j];
+            if (kk < 1 || kk >= j || ps + 2 > N + 1) {
+                ok = false;
+                return;
+            }
+            pend[ps] = r;
+            pendI[ps++] = j - kk;
+            pend[ps] = l;
+            pendI[ps++] = kk;
+        };
+        distribute(node, N);
+        while (ok && ps > 0) {
+            const WChild c = pend[--ps];
+            const int i = pendI[ps];
+            if (c.ref < 0) {
+                if (n < N) ch[n++] = c;
+                else ok = false;
+                continue;
+            }
+            if ((uint32_t)c.ref >= numNodes) {
+                ok = false;
+                break;
+            }
+            const int u = dp.use[(size_t)c.ref * S + i];
+            if (u == 1) {
+                if (n < N) ch[n++] = c;
+                else ok = false;
+            } else if (u >= 2 && u <= i) {
+                distribute(c.ref, u);
+            } else {
+                ok = false;
+            }
+        }
+        return n;
+    }
     int n = 2;
     bin_children(nodes, node, ch[0], ch[1]);
     while (n < N) {
@@ -521,7 +632,7 @@ __device__ __forceinline__ int open_wide(const float4* nodes, uint32_t numNodes,
 
 template <int N>
 __global__ __launch_bounds__(BB) void k_wide_count(const uint2* front, uint32_t fN, const float4* nodes,
-                                                   uint32_t numNodes, uint2* counts, uint32_t* bad) {
+                                                   uint32_t numNodes, WideDP dp, uint2* counts, uint32_t* bad) {
     const uint32_t k = tid();
     if (k >= fN) return;
     if (front[k].x >= numNodes) {
@@ -531,7 +642,7 @@ __global__ __launch_bounds__(BB) void k_wide_count(const uint2* front, uint32_t 
     }
     WChild ch[N];
     bool ok = true;
-    const int n = open_wide<N>(nodes, numNodes, (int32_t)front[k].x, ch, ok);
+    const int n = open_wide<N>(nodes, numNodes, (int32_t)front[k].x, ch, ok, dp);
     if (!ok) atomicOr(bad, 16u);
     uint32_t inner = 0u;
     for (int i = 0; i < n; i++) inner += ch[i].ref >= 0 ? 1u : 0u;
@@ -543,8 +654,8 @@ __global__ __launch_bounds__(BB) void k_wide_count(const uint2* front, uint32_t 
 template <int N>
 __global__ __launch_bounds__(BB) void k_wide_write(const uint2* front, uint32_t fN, uint32_t levelBase,
                                                    uint32_t nextBase, const uint2* counts, const float4* nodes,
-                                                   uint32_t numNodes, float4* wnodes, uint2* next, uint32_t nextCap,
-                                                   uint32_t* maxStack, uint32_t* bad) {
+                                                   uint32_t numNodes, WideDP dp, float4* wnodes, uint2* next,
+                                                   uint32_t nextCap, uint32_t* maxStack, uint32_t* bad) {
     __shared__ uint32_t s_m[BB / 64];
     const uint32_t k = tid();
     uint32_t smax = 0u;
@@ -559,7 +670,7 @@ __global__ __launch_bounds__(BB) void k_wide_write(const uint2* front, uint32_t 
         WChild ch[N];
         const uint2 fk = front[k];
         bool ok = true;
-        const int n = open_wide<N>(nodes, numNodes, (int32_t)fk.x, ch, ok);
+        const int n = open_wide<N>(nodes, numNodes, (int32_t)fk.x, ch, ok, dp);
         if (!ok) atomicOr(bad, 32u);
         const uint32_t stack = fk.y + (uint32_t)(n - 1);
         smax = stack;
@@ -844,43 +955,79 @@ bool build_bvh_device(const float* pos, const uint32_t* idx, uint32_t n, uint32_
         out.width = (uint32_t)wantWidth;   // a single leaf: no inner node at any width
     } else if (wantWidth == 4 || wantWidth == 8) {
         const int words = 2 * wantWidth;
+        const size_t S = (size_t)wantWidth + 1;
         float4* wnodes = nullptr;
+        WideDP dp{nullptr, nullptr, nullptr};
         e = hipMalloc((void**)&wnodes, (size_t)numNodes * words * sizeof(float4));
-        if (e != hipSuccess) return fail(e, "allocation of the wide nodes failed");
-        uint2 rootEntry = make_uint2((uint32_t)out.root, 0u);
-        e = hipMemcpy(B.frontA, &rootEntry, sizeof(rootEntry), hipMemcpyHostToDevice);
-        uint2* front = B.frontA;
-        uint2* next = B.frontB;
-        uint32_t fN = 1, levelBase = 0;
-        while (fN && e == hipSuccess) {
-            if (wantWidth == 4)
-                hipLaunchKernelGGL(k_wide_count<4>, blocks(fN), dim3(BB), 0, s, front, fN, out.nodes, numNodes, B.flags,
-                                   bad);
-            else
-                hipLaunchKernelGGL(k_wide_count<8>, blocks(fN), dim3(BB), 0, s, front, fN, out.nodes, numNodes, B.flags,
-                                   bad);
-            e = scan.run(B.flags, fN, s);
-            if (e != hipSuccess) break;
-            const uint32_t nextBase = levelBase + fN;
-            if (wantWidth == 4)
-                hipLaunchKernelGGL(k_wide_write<4>, blocks(fN), dim3(BB), 0, s, front, fN, levelBase, nextBase, B.flags,
-                                   out.nodes, numNodes, wnodes, next, n, maxStack, bad);
-            else
-                hipLaunchKernelGGL(k_wide_write<8>, blocks(fN), dim3(BB), 0, s, front, fN, levelBase, nextBase, B.flags,
-                                   out.nodes, numNodes, wnodes, next, n, maxStack, bad);
-            uint2 tot;
-            e = hipMemcpy(&tot, B.total, sizeof(tot), hipMemcpyDeviceToHost);
-            if (e != hipSuccess) break;
-            if (nextBase + tot.x > numNodes) {
-                e = hipErrorUnknown;
-                break;
+        if (e == hipSuccess) e = hipMalloc((void**)&dp.cost, (size_t)numNodes * S * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void**)&dp.use, (size_t)numNodes * S);
+        if (e == hipSuccess) e = hipMalloc((void**)&dp.split, (size_t)numNodes * S);
+        struct Tables {   // the programme's tables live only through the collapse
+            WideDP& t;
+            ~Tables() {
+                (void)hipStreamSynchronize(nullptr);
+                (void)hipFree(t.cost);
+                (void)hipFree(t.use);
+                (void)hipFree(t.split);
             }
-            levelBase = nextBase;
-            fN = tot.x;
-            std::swap(front, next);
+        } tables{dp};
+        if (e != hipSuccess) {
+            (void)hipFree(wnodes);
+            return fail(e, "allocation of the wide nodes failed");
         }
-        const uint32_t numWide = levelBase;
-        if (e == hipSuccess) e = hipMemcpy(sc, B.scalars, sizeof(sc), hipMemcpyDeviceToHost);
+        // the tables, one clustering iteration at a time (B.flags still holds the kept ranks)
+        for (size_t it = 0; it < iterEnd.size(); it++) {
+            const uint32_t lo = it ? iterEnd[it - 1] : 0u, hi = iterEnd[it];
+            if (wantWidth == 4)
+                hipLaunchKernelGGL(k_wide_dp<4>, blocks(hi - lo), dim3(BB), 0, s, n, lo, hi, B.state, B.flags, out.nodes,
+                                   numNodes, dp, bad);
+            else
+                hipLaunchKernelGGL(k_wide_dp<8>, blocks(hi - lo), dim3(BB), 0, s, n, lo, hi, B.state, B.flags, out.nodes,
+                                   numNodes, dp, bad);
+        }
+        // one level of wide nodes per round; the least-area tables first, the greedy rule if
+        // their tree's stack bound does not fit
+        uint32_t numWide = 0;
+        auto collapse = [&](const WideDP& rule) -> hipError_t {
+            hipError_t ce = hipMemsetAsync(maxStack, 0, sizeof(uint32_t), s);
+            uint2 rootEntry = make_uint2((uint32_t)out.root, 0u);
+            if (ce == hipSuccess) ce = hipMemcpy(B.frontA, &rootEntry, sizeof(rootEntry), hipMemcpyHostToDevice);
+            uint2* front = B.frontA;
+            uint2* next = B.frontB;
+            uint32_t fN = 1, levelBase = 0;
+            while (fN && ce == hipSuccess) {
+                if (wantWidth == 4)
+                    hipLaunchKernelGGL(k_wide_count<4>, blocks(fN), dim3(BB), 0, s, front, fN, out.nodes, numNodes, rule,
+                                       B.flags, bad);
+                else
+                    hipLaunchKernelGGL(k_wide_count<8>, blocks(fN), dim3(BB), 0, s, front, fN, out.nodes, numNodes, rule,
+                                       B.flags, bad);
+                ce = scan.run(B.flags, fN, s);
+                if (ce != hipSuccess) break;
+                const uint32_t nextBase = levelBase + fN;
+                if (wantWidth == 4)
+                    hipLaunchKernelGGL(k_wide_write<4>, blocks(fN), dim3(BB), 0, s, front, fN, levelBase, nextBase,
+                                       B.flags, out.nodes, numNodes, rule, wnodes, next, n, maxStack, bad);
+                else
+                    hipLaunchKernelGGL(k_wide_write<8>, blocks(fN), dim3(BB), 0, s, front, fN, levelBase, nextBase,
+                                       B.flags, out.nodes, numNodes, rule, wnodes, next, n, maxStack, bad);
+                uint2 tot;
+                ce = hipMemcpy(&tot, B.total, sizeof(tot), hipMemcpyDeviceToHost);
+                if (ce != hipSuccess) break;
+                if (nextBase + tot.x > numNodes) {
+                    ce = hipErrorUnknown;
+                    break;
+                }
+                levelBase = nextBase;
+                fN = tot.x;
+                std::swap(front, next);
+            }
+            numWide = levelBase;
+            if (ce == hipSuccess) ce = hipMemcpy(sc, B.scalars, sizeof(sc), hipMemcpyDeviceToHost);
+            return ce;
+        };
+        e = collapse(dp);
+        if (e == hipSuccess && sc[1] + 1 > (uint32_t)STACK_DEPTH) e = collapse(WideDP{nullptr, nullptr, nullptr});
         if (e == hipSuccess && sc[1] + 1 <= (uint32_t)STACK_DEPTH) {
             hipLaunchKernelGGL(k_check_refs, blocks(numWide), dim3(BB), 0, s, wnodes, numWide, words, n, bad);
             allocs.push_back(wnodes);
