@@ -791,12 +791,18 @@ __device__ __forceinline__ void neighbor_stats(const WaveArgs& W, uint32_t count
     if (W.A.lp.trackPrefixStats) neighbor_stat(W.A.counters, count);
 }
 
-// After the canonical prefix and suffix: neighbour query and the two shifts per valid
-// neighbour (final_gather.cu:90-177).  gIdx > 0: a gather prefix of FinalGatherNotCanon
-// (final_gather.cu:200-279), one shift per neighbour onto its traced last vertex.
+// After the canonical prefix and suffix: the neighbour query and the valid-neighbour mask
+// (final_gather.cu:90-133); gIdx > 0: the same for a gather prefix of FinalGatherNotCanon
+// (final_gather.cu:209-246).  Writes the
+// neighbours, the mask of those with a valid suffix (validMask) and pixFlags bit 3 (queried).
+// The tree walk is a chain of dependent fetches, so it runs apart from the register-heavy shift
+// starts (k_w_gather_setup), at more waves per SIMD.  The depth-first walk defers at most one
+// sibling per internal level of the path it is on, so its LDS stack (dynamic shared memory,
+// gather_query_lds bytes) holds the tree's allocated levels, not a fixed 32 entries.
 template <int KR>
-__global__ __launch_bounds__(WBLOCK) void k_w_gather_setup(WaveArgs W) {
-    __shared__ int s_st[32 * WBLOCK];
+__global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(KR <= 8 ? 8 : 1, 8))) void k_w_gather_query(
+    WaveArgs W) {
+    extern __shared__ int s_st[];
     const FrameArgs& A = W.A;
     const uint32_t p = gid();
     const bool px = p < A.P;
@@ -830,6 +836,21 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_setup(WaveArgs W) {
         }
         W.pixFlags[p] = (W.pixFlags[p] & 7u) | (run ? 8u : 0u);
         W.validMask[p] = validFlags;
+    }
+}
+
+// The two shifts per valid neighbour of a queried pixel (final_gather.cu:135-177); gIdx > 0: one
+// shift per neighbour onto the gather prefix's traced last vertex (final_gather.cu:248-279).
+__global__ __launch_bounds__(WBLOCK) void k_w_gather_setup(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    const uint32_t k = A.k;
+    bool run = false;
+    uint32_t validFlags = 0;
+    if (p < A.P) {
+        const uint32_t pf = W.pixFlags[p];
+        run = (pf & 1u) && (pf & 8u);
+        if (run) validFlags = W.validMask[p];
     }
     const uint32_t dirs = W.gIdx == 0 ? 2u : 1u;
     for (uint32_t i = 0; i < k; ++i) {
@@ -1194,6 +1215,12 @@ hipError_t wave_tail(int step, const WaveArgs& W, uint32_t maxSteps, int pass, h
     }
 }
 
+// LDS of one k_w_gather_query block: a stack entry per level of the entry tree's allocation
+// (activeLevels never exceeds it) for each thread.
+static size_t gather_query_lds(const WaveArgs& W) {
+    return (size_t)std::max(1u, W.A.tree.levels) * WBLOCK * sizeof(int);
+}
+
 hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStream_t s) {
     const dim3 b(WBLOCK);
     switch (kernel) {
@@ -1220,8 +1247,9 @@ hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStrea
         case WK_SUFFIX_SPATIAL_RESAMPLE: hipLaunchKernelGGL(k_w_suffix_spatial_resample, g, b, 0, s, W); break;
         case WK_GATHER_INIT: hipLaunchKernelGGL(k_w_gather_init, g, b, 0, s, W); break;
         case WK_GATHER_SETUP:
-            if (W.A.k <= 8) hipLaunchKernelGGL(k_w_gather_setup<8>, g, b, 0, s, W);
-            else hipLaunchKernelGGL(k_w_gather_setup<32>, g, b, 0, s, W);
+            if (W.A.k <= 8) hipLaunchKernelGGL(k_w_gather_query<8>, g, b, gather_query_lds(W), s, W);
+            else hipLaunchKernelGGL(k_w_gather_query<32>, g, b, gather_query_lds(W), s, W);
+            hipLaunchKernelGGL(k_w_gather_setup, g, b, 0, s, W);
             break;
         case WK_GATHER_COMBINE: hipLaunchKernelGGL(k_w_gather_combine, g, b, 0, s, W, 0); break;
         case WK_GATHER_COMBINE_LAST: hipLaunchKernelGGL(k_w_gather_combine, g, b, 0, s, W, 1); break;
