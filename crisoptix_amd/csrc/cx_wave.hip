@@ -841,7 +841,8 @@ __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(KR <= 8 
 
 // The two shifts per valid neighbour of a queried pixel (final_gather.cu:135-177); gIdx > 0: one
 // shift per neighbour onto the gather prefix's traced last vertex (final_gather.cu:248-279).
-__global__ __launch_bounds__(WBLOCK) void k_w_gather_setup(WaveArgs W) {
+// Held to 5 waves per SIMD: 92 VGPRs without spilling, against 104 and 4 waves unconstrained.
+__global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_w_gather_setup(WaveArgs W) {
     const FrameArgs& A = W.A;
     const uint32_t p = gid();
     const uint32_t k = A.k;
