@@ -53,7 +53,7 @@ class Material(C.Structure):
                 ("roughness", C.c_float)]
 
 
-ABI_VERSION = 8   # CX_ABI_VERSION of include/crisoptix.h
+ABI_VERSION = 9   # CX_ABI_VERSION of include/crisoptix.h
 CX_SPEC_TEX_COLOR = 0
 CX_SPEC_TEX_OCC_ROUGH_METAL = 1
 CX_WAVE_ROUNDS_DEFAULT = 0xFFFFFFFF   # per-machine default wavefront rounds

@@ -22,7 +22,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "
           "-Wno-unused-function", "-Wno-unused-result"]
 DEVICE = [f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["cx_kernels.hip", "cx_wave.hip", "cx_adjoint.hip", "cx_image.hip", "cx_bvh_device.hip", "cx_host.cpp",
+SOURCES = ["cx_kernels.hip", "cx_wave.hip", "cx_adjoint.hip", "cx_image.hip", "cx_bvh_device.hip", "cx_noalloc.hip", "cx_host.cpp",
            "cx_bvh_build.cpp"]
 
 

@@ -178,13 +178,6 @@ uint64_t pcg_seed(uint64_t initstate) {
     return s;
 }
 
-uint32_t tree_levels(uint32_t capacity, uint32_t leafSize) {
-    const uint32_t leaves = std::max(1u, (capacity + leafSize - 1) / leafSize);
-    uint32_t l = 0;
-    while ((1u << l) < leaves) l++;
-    return l;
-}
-
 int entries_alloc(cx_ctx* ctx, uint32_t capacity, cx_entries** out) {
     cx_entries* en = new (std::nothrow) cx_entries();
     if (!en) return set_err(ctx, CX_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -227,16 +220,6 @@ int entries_alloc(cx_ctx* ctx, uint32_t capacity, cx_entries** out) {
     }
     *out = en;
     return CX_OK;
-}
-
-void set_quantisation(EntryBuild& E, const float* mn, const float* mx) {
-    E.lo = V3{mn[0], mn[1], mn[2]};
-    float sc[3];
-    for (int a = 0; a < 3; a++) {
-        const float w = mx[a] - mn[a];
-        sc[a] = w > 0.0f ? 1023.0f / w : 0.0f;
-    }
-    E.scale = V3{sc[0], sc[1], sc[2]};
 }
 
 FrameArgs make_args(cx_renderer* r) {
@@ -560,6 +543,13 @@ hipError_t wave_pass(cx_renderer* r, int pass, const FrameArgs& A, hipStream_t s
 
 }  // namespace
 
+namespace cx {
+void ctx_set_error(cx_ctx* ctx, const char* msg) {
+    if (ctx) ctx->err = msg;
+}
+int ctx_device(const cx_ctx* ctx) { return ctx ? ctx->device : -1; }
+}  // namespace cx
+
 extern "C" {
 
 int cx_abi_version(void) { return CX_ABI_VERSION; }
@@ -570,6 +560,8 @@ int cx_ctx_create(int device, cx_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CX_ERR_NO_DEVICE;
     if (device < 0 || device >= n) return CX_ERR_INVALID_ARG;
     if (hipSetDevice(device) != hipSuccess) return CX_ERR_HIP;
+    // the static scratch of cx_entries_search_static: its address on this device, taken once
+    if (noalloc_setup(device) != hipSuccess) return CX_ERR_HIP;
     cx_ctx* c = new (std::nothrow) cx_ctx();
     if (!c) return CX_ERR_OUT_OF_MEMORY;
     c->device = device;

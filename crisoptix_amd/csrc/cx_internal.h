@@ -28,6 +28,32 @@ hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n
                              hipStream_t s);
 hipError_t launch_entries_query(const EntryTree& T, const float* q, uint32_t nq, uint32_t k, uint32_t* idx,
                                 float* dist, uint32_t* count, hipStream_t s);
+// Levels of the implicit entry tree over `capacity` entries in leaves of `leafSize`.
+inline uint32_t tree_levels(uint32_t capacity, uint32_t leafSize) {
+    const uint32_t leaves = capacity > leafSize ? (capacity + leafSize - 1) / leafSize : 1u;
+    uint32_t l = 0;
+    while ((1u << l) < leaves) l++;
+    return l;
+}
+
+// Morton quantisation of the entry positions over the box [mn, mx] (10 bits per axis).
+inline void set_quantisation(EntryBuild& E, const float* mn, const float* mx) {
+    E.lo = V3{mn[0], mn[1], mn[2]};
+    float sc[3];
+    for (int a = 0; a < 3; a++) {
+        const float w = mx[a] - mn[a];
+        sc[a] = w > 0.0f ? 1023.0f / w : 0.0f;
+    }
+    E.scale = V3{sc[0], sc[1], sc[2]};
+}
+
+// Static scratch of cx_entries_search_static (cx_noalloc.hip): takes the address of the
+// file-scope device array on the current device, once per device.  Called by cx_ctx_create.
+hipError_t noalloc_setup(int device);
+// The context's last-error message and device, for entry points outside cx_host.cpp.
+void ctx_set_error(cx_ctx* ctx, const char* msg);
+int ctx_device(const cx_ctx* ctx);
+
 hipError_t launch_trace(const DScene& S, const float* rays, uint32_t n, int anyHit, uint32_t* prim, float* t,
                         float* u, float* v, uint32_t* occ, hipStream_t s);
 

@@ -204,6 +204,7 @@ def lib():
         L.cx_entries_destroy.argtypes = [vp]
         L.cx_entries_build.argtypes = [vp, vp, vp, vp, vp, u32, vp, vp, vp]
         L.cx_entries_query.argtypes = [vp, vp, vp, u32, u32, vp, vp, vp, vp]
+        L.cx_entries_search_static.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp, u32, u32, vp, vp, vp, vp]
         L.cx_default_params.argtypes = [vp, u32, u32, C.POINTER(abi.LaunchParams)]
         L.cx_renderer_create.argtypes = [vp, vp, u32, u32, C.POINTER(vp)]
         L.cx_renderer_destroy.argtypes = [vp]
@@ -399,6 +400,27 @@ class Entries:
         if getattr(self, "h", None):
             lib().cx_entries_destroy(self.h)
             self.h = None
+
+
+def entries_search_static(ctx: Context, pos, radius, valid, scene_min, scene_max, q, k: int):
+    """cx_entries_search_static: the entry structure built and queried in one call that
+    allocates no device memory in the library (the arrays here are the caller's).  Returns
+    (idx [nq, k], dist [nq, k], count [nq]) as Entries.build + Entries.query do."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, 3)
+    n, nq = pos.shape[0], q.shape[0]
+    d = [DeviceArray.from_numpy(pos), DeviceArray.from_numpy(np.ascontiguousarray(radius, np.float32)),
+         DeviceArray.from_numpy(np.ascontiguousarray(valid, np.uint8)), DeviceArray.from_numpy(q)]
+    idx = DeviceArray((nq, k), np.uint32)
+    dist = DeviceArray((nq, k), np.float32)
+    cnt = DeviceArray((nq,), np.uint32)
+    mn = np.ascontiguousarray(scene_min, np.float32)
+    mx = np.ascontiguousarray(scene_max, np.float32)
+    ctx.check(lib().cx_entries_search_static(ctx.h, d[0].ptr, d[1].ptr, d[2].ptr, n, mn.ctypes.data, mx.ctypes.data,
+                                             d[3].ptr, nq, k, idx.ptr, dist.ptr, cnt.ptr, None),
+              "cx_entries_search_static")
+    synchronize()
+    return idx.numpy(), dist.numpy(), cnt.numpy()
 
 
 class Renderer:

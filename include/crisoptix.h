@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define CX_ABI_VERSION 8
+#define CX_ABI_VERSION 9
 
 typedef enum cx_status {
     CX_OK = 0,
@@ -298,6 +298,19 @@ int cx_entries_build(cx_ctx* ctx, cx_entries* entries, const float* d_pos, const
 /* d_query: device [nq][3]; outputs: d_idx/d_dist device [nq][k], d_count device [nq]. */
 int cx_entries_query(cx_ctx* ctx, const cx_entries* entries, const float* d_query, uint32_t nq,
                      uint32_t k, uint32_t* d_idx, float* d_dist, uint32_t* d_count, void* stream);
+/* No counterpart in the reference: cx_entries_build followed by cx_entries_query in one call that
+ * allocates no device memory, for callers that cannot allocate (code under hipGraph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The entry records, sort buffers and tree
+ * live in a fixed device array of the library (about 4.4 MB per device, held until the process
+ * ends) whose address cx_ctx_create takes on its device; the context's device must be current.
+ * Arguments mean what they mean for cx_entries_build / cx_entries_query, and the outputs are
+ * theirs bit for bit.  n above 65536 (a 256 x 256 frame's entries) returns CX_ERR_INVALID_ARG.
+ * Every call on one device shares that array: calls that overlap on one device (on two
+ * streams) race, and each call overwrites what the previous one left. */
+int cx_entries_search_static(cx_ctx* ctx, const float* d_pos, const float* d_radius, const uint8_t* d_valid,
+                             uint32_t n, const float* sceneMin, const float* sceneMax, const float* d_query,
+                             uint32_t nq, uint32_t k, uint32_t* d_idx, float* d_dist, uint32_t* d_count,
+                             void* stream);
 
 /* ---- renderer: Renderer(w, h, ctx, Camera&, const Scene&) and LaunchFrame
  *      (include/graph/Renderer.h:11-21, src/Renderer.cpp:13-406). ---- */
