@@ -53,10 +53,11 @@ class Material(C.Structure):
                 ("roughness", C.c_float)]
 
 
-ABI_VERSION = 9   # CX_ABI_VERSION of include/crisoptix.h
+ABI_VERSION = 10   # CX_ABI_VERSION of include/crisoptix.h
 CX_SPEC_TEX_COLOR = 0
 CX_SPEC_TEX_OCC_ROUGH_METAL = 1
 CX_WAVE_ROUNDS_DEFAULT = 0xFFFFFFFF   # per-machine default wavefront rounds
+CX_VIS_SURFACE, CX_VIS_OCCLUSION, CX_VIS_ALL = 1, 2, 3   # cx_scene_desc.triMask bits
 CX_BVH_BUILD_HOST = 0
 CX_BVH_BUILD_DEVICE = 1
 BVH_BUILDERS = {"host": CX_BVH_BUILD_HOST, "device": CX_BVH_BUILD_DEVICE}
@@ -70,7 +71,7 @@ class SceneDesc(C.Structure):
         ("emitterFaces", C.c_void_p), ("emitterFaceOffset", C.c_void_p), ("emitterColor", C.c_void_p),
         ("numEmitters", C.c_uint32), ("bvhMaxLeafSize", C.c_uint32), ("bvhWidth", C.c_uint32),
         ("uvs", C.c_void_p), ("materialTex", C.c_void_p), ("textures", C.c_void_p), ("numTextures", C.c_uint32),
-        ("bvhBuilder", C.c_uint32),
+        ("bvhBuilder", C.c_uint32), ("triMask", C.c_void_p),
     ]
 
 
@@ -138,6 +139,12 @@ def scene_desc(scene, keep: list) -> SceneDesc:
     d.numEmitters = arrs["emitterColor"].shape[0]
     d.bvhMaxLeafSize = 0
     d.bvhWidth = 0
+    tri_mask = getattr(scene, "tri_mask", None)
+    if tri_mask is not None:
+        arrs["triMask"] = np.ascontiguousarray(tri_mask, np.uint8).reshape(-1)
+        if arrs["triMask"].shape[0] != d.numTriangles:
+            raise ValueError("tri_mask must have one entry per triangle")
+        d.triMask = arrs["triMask"].ctypes.data
     uvs = getattr(scene, "uvs", None)
     if uvs is not None:
         arrs["uvs"] = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)

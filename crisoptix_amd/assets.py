@@ -19,10 +19,10 @@ occlusion/roughness/metal maps).  Image files are read by image.read_png (8-bit 
 Emitters (Scene::BuildEmitterTable, src/Scene.cpp:176-204): every group of faces whose material
 has |Ke| > 0.01 becomes an implicit emitter over those faces with colour (1, 1, 1) (the reference
 forces implicit emitters to unit colour, Scene.cpp:194); `emitter_color` overrides that.  The
-reference also traces those faces as geometry; `trace_emitters=False` keeps them out of the
-traced triangles, which the benchmark scenes do with their explicit emitters (a shadow ray
-aimed at a point of a traced emitter would otherwise report its own emitter as the occluder
-whenever t rounds below the distance).
+reference also traces those faces as geometry, and so does the loader by default: closest-hit
+rays see them and occlusion rays pass over them (tri_mask CX_VIS_SURFACE), so a shadow ray aimed
+at a point of a light is not reported occluded by that light whenever t rounds below the
+distance.  `trace_emitters=False` keeps them out of the traced triangles.
 """
 from __future__ import annotations
 
@@ -189,13 +189,16 @@ def scene_from_obj(text: str, mtl_loader=None, trace_emitters: bool = True, emit
     remap[used] = np.arange(len(used), dtype=np.uint32)
     mt = np.asarray(mtex, np.int32).reshape(-1, 4)
     any_tex = bool(textures) or bool(np.any(mt[:, 3] != 0))
+    # traced emissive faces are seen by closest-hit rays only (CX_VIS_SURFACE)
+    vis = np.where(emissive[mat_of[keep]], 1, 3).astype(np.uint8)
     return Scene(
         positions=P[used], normals=N[used], indices=remap[T2], tri_material=mat_of[keep],
         materials=materials, emitter_faces=np.concatenate(faces).astype(np.float32),
         emitter_face_offset=np.asarray(offs, np.uint32),
         emitter_color=np.asarray(cols, np.float32).reshape(-1, 3), name=name,
         meta={"half": float(np.abs(P).max()) if len(P) else 1.0},
-        uvs=UV[used] if any_tex else None, material_tex=mt if any_tex else None, textures=textures)
+        uvs=UV[used] if any_tex else None, material_tex=mt if any_tex else None, textures=textures,
+        tri_mask=vis if (vis != 3).any() else None)
 
 
 def cornell_box_obj(textured: bool = False):

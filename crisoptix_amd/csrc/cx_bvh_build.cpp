@@ -46,6 +46,7 @@ struct Box {
 struct Builder {
     const float* pos;
     const uint32_t* idx;
+    const uint8_t* mask = nullptr;   // CX_VIS_* bits per triangle, or null (all visible)
     uint32_t maxLeaf;
     std::vector<Box> triBox;
     std::vector<float> cent;   // [T][3]
@@ -73,10 +74,12 @@ struct Builder {
         const uint32_t first = (uint32_t)(out->tris.size() / 3);
         for (uint32_t i = b; i < e; i++) {
             const uint32_t t = order[i];
+            const uint32_t skip = mask ? (~(uint32_t)mask[t] & 3u) : 0u;
             for (int k = 0; k < 3; k++) {
                 const float* v = pos + 3 * (size_t)idx[3 * (size_t)t + k];
                 float w = 0.0f;
                 if (k == 0) std::memcpy(&w, &t, 4);
+                if (k == 1) std::memcpy(&w, &skip, 4);
                 out->tris.push_back(make_float4(v[0], v[1], v[2], w));
             }
         }
@@ -189,7 +192,7 @@ struct Builder {
 }  // namespace
 
 bool build_bvh(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t maxLeaf, BvhHost& out,
-               std::string& err) {
+               std::string& err, const uint8_t* mask) {
     out.nodes.clear();
     out.tris.clear();
     out.root = 0;
@@ -202,6 +205,7 @@ bool build_bvh(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t
     Builder B;
     B.pos = pos;
     B.idx = idx;
+    B.mask = mask;
     B.maxLeaf = std::max(1u, std::min(15u, maxLeaf));
     B.out = &out;
     B.triBox.resize(numTris);

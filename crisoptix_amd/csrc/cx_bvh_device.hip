@@ -389,9 +389,11 @@ __global__ __launch_bounds__(BB) void k_td(uint32_t n, uint32_t lo, uint32_t hi,
     }
 }
 
-// Triangle of leaf s to its slot, 3 float4 per triangle (vertex 0's w = the primitive index).
+// Triangle of leaf s to its slot, 3 float4 per triangle (vertex 0's w = the primitive index,
+// vertex 1's w = the skip bits, the complement of the triangle's CX_VIS_* mask).
 __global__ __launch_bounds__(BB) void k_tri_slots(const uint32_t* vals, const uint32_t* off, uint32_t n,
-                                                  const float* pos, const uint32_t* idx, float4* tris, uint32_t* bad) {
+                                                  const float* pos, const uint32_t* idx, const uint8_t* mask,
+                                                  float4* tris, uint32_t* bad) {
     const uint32_t s = tid();
     if (s >= n) return;
     const uint32_t t = vals[s], slot = off[s];
@@ -400,9 +402,11 @@ __global__ __launch_bounds__(BB) void k_tri_slots(const uint32_t* vals, const ui
         return;
     }
     float4* o = tris + 3 * (size_t)slot;
+    const uint32_t skip = mask ? (~(uint32_t)mask[t] & 3u) : 0u;
     for (int k = 0; k < 3; k++) {
         const float* v = pos + 3 * (size_t)idx[3 * (size_t)t + k];
-        o[k] = make_float4(v[0], v[1], v[2], k == 0 ? __builtin_bit_cast(float, t) : 0.0f);
+        const uint32_t w = k == 0 ? t : (k == 1 ? skip : 0u);
+        o[k] = make_float4(v[0], v[1], v[2], __builtin_bit_cast(float, w));
     }
 }
 
@@ -773,8 +777,8 @@ inline dim3 blocks(uint32_t n) { return dim3((std::max(n, 1u) + BB - 1) / BB); }
 
 }  // namespace
 
-bool build_bvh_device(const float* pos, const uint32_t* idx, uint32_t n, uint32_t maxLeaf, int wantWidth,
-                      DeviceBvh& out, std::vector<void*>& allocs, std::string& err, hipError_t& herr) {
+bool build_bvh_device(const float* pos, const uint32_t* idx, const uint8_t* mask, uint32_t n, uint32_t maxLeaf,
+                      int wantWidth, DeviceBvh& out, std::vector<void*>& allocs, std::string& err, hipError_t& herr) {
     out = DeviceBvh{};
     herr = hipSuccess;
     if (n == 0) return true;
@@ -930,7 +934,7 @@ bool build_bvh_device(const float* pos, const uint32_t* idx, uint32_t n, uint32_
     uint32_t* maxDepth = B.scalars;
     uint32_t* maxStack = B.scalars + 1;
     uint32_t* bad = B.scalars + 2;
-    hipLaunchKernelGGL(k_tri_slots, blocks(n), dim3(BB), 0, s, order, B.off, n, pos, idx, out.tris, bad);
+    hipLaunchKernelGGL(k_tri_slots, blocks(n), dim3(BB), 0, s, order, B.off, n, pos, idx, mask, out.tris, bad);
     if (numNodes)
         hipLaunchKernelGGL(k_bin_nodes, blocks(inner), dim3(BB), 0, s, n, B.kids, B.state, B.flags, B.off, B.cnt,
                            B.depth, B.nlo, B.nhi, out.nodes, numNodes, maxDepth, bad);

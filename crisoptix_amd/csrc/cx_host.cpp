@@ -638,7 +638,7 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     int32_t root = 0, wroot = 0;
     uint32_t stackBound = 0, width = 2, maxDepth = 0, numNodes = 0, numWide = 0;
     if (!onDevice) {
-        if (!build_bvh(d->positions, d->indices, d->numTriangles, maxLeaf, bvh, err))
+        if (!build_bvh(d->positions, d->indices, d->numTriangles, maxLeaf, bvh, err, d->triMask))
             return set_err(ctx, CX_ERR_BUILD, err);
         root = wroot = bvh.root;
         stackBound = maxDepth = bvh.maxDepth;
@@ -704,7 +704,15 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
         DeviceBvh db;
         hipError_t he = hipSuccess;
         std::vector<void*> bvhAllocs;
-        if (!build_bvh_device(s->d.pos, s->d.idx, d->numTriangles, maxLeaf, (int)wantWidth, db, bvhAllocs, err, he)) {
+        const uint8_t* dmask =
+            d->triMask ? (const uint8_t*)up(d->triMask, d->numTriangles, sizeof(uint8_t)) : nullptr;
+        if (e != hipSuccess) {
+            free_all(s->allocs);
+            delete s;
+            return hip_err(ctx, e, "scene upload");
+        }
+        if (!build_bvh_device(s->d.pos, s->d.idx, dmask, d->numTriangles, maxLeaf, (int)wantWidth, db, bvhAllocs, err,
+                              he)) {
             free_all(bvhAllocs);
             if (!db.tooDeep) {
                 free_all(s->allocs);
@@ -715,7 +723,7 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
             // a tree deeper than the traversal stack (heavily duplicated geometry): the host
             // build bounds its depth, so it takes over; cx_scene_info.bvhBuilder reports it
             builderUsed = CX_BVH_BUILD_HOST;
-            if (!build_bvh(d->positions, d->indices, d->numTriangles, maxLeaf, bvh, err)) {
+            if (!build_bvh(d->positions, d->indices, d->numTriangles, maxLeaf, bvh, err, d->triMask)) {
                 free_all(s->allocs);
                 delete s;
                 return set_err(ctx, CX_ERR_BUILD, err);
@@ -849,7 +857,7 @@ int cx_bvh_host_trace(const cx_scene_desc* d, const float* rays, uint32_t n, int
     BvhHost bvh;
     std::string err;
     if (!build_bvh(d->positions, d->indices, d->numTriangles, d->bvhMaxLeafSize ? d->bvhMaxLeafSize : DEFAULT_MAX_LEAF,
-                   bvh, err))
+                   bvh, err, d->triMask))
         return CX_ERR_BUILD;
     std::vector<float4> wide;
     int32_t wroot = bvh.root;

@@ -116,8 +116,9 @@ struct BvhHost {
     uint32_t maxDepth = 0;
 };
 
+// mask: CX_VIS_* bits per triangle (host), or null; stored as skip bits in the triangle slots.
 bool build_bvh(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t maxLeaf, BvhHost& out,
-               std::string& err);
+               std::string& err, const uint8_t* mask = nullptr);
 
 // Device BVH build (cx_bvh_device.hip) from device positions and indices: binary nodes, triangle
 // slots and, when wantWidth is 4 or 8 and its stack bound fits, wide nodes, in the formats of
@@ -131,8 +132,9 @@ struct DeviceBvh {
     uint32_t width = 2, maxDepth = 0, stackBound = 0;
     bool tooDeep = false;        // the binary tree is deeper than the traversal stack
 };
-bool build_bvh_device(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t maxLeaf, int wantWidth,
-                      DeviceBvh& out, std::vector<void*>& allocs, std::string& err, hipError_t& herr);
+// mask: CX_VIS_* bits per triangle in device memory, or null.
+bool build_bvh_device(const float* pos, const uint32_t* idx, const uint8_t* mask, uint32_t numTris, uint32_t maxLeaf,
+                      int wantWidth, DeviceBvh& out, std::vector<void*>& allocs, std::string& err, hipError_t& herr);
 
 // Wide (4- or 8-child) node array from a binary BVH; false if its worst-case traversal stack
 // would not fit in `stackLimit` entries.

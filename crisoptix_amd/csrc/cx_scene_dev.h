@@ -22,7 +22,9 @@ static constexpr float PI_F = 3.14159265358979323846264f;   // shader/brdf.cu:12
 //   a = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)   b = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
 //   c = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)   d = (ref0, ref1, -, -)
 // ref >= 0: inner node index; ref < 0: leaf, ~ref = (first triangle slot << 4) | count.
-// Triangle slots hold 3 float4: (v0.xyz, primitive index bits), (v1.xyz, 0), (v2.xyz, 0).
+// Triangle slots hold 3 float4: (v0.xyz, primitive index bits), (v1.xyz, skip bits), (v2.xyz, 0);
+// the skip bits are the complement of the triangle's CX_VIS_* mask: bit 0 hides it from
+// closest-hit rays, bit 1 from occlusion rays.
 struct DMaterial {
     float4 diffRough;   // diffuse rgb, roughness
     float4 spec;        // specF0 rgb, -
@@ -165,7 +167,9 @@ struct TravStack {
 };
 
 // Closest-hit / any-hit leaf loop shared by the binary and wide traversals.  Returns true when
-// an any-hit query is answered.
+// an any-hit query is answered.  Triangles whose skip bits hide them from the ray kind (closest
+// hit: bit 0, any hit: bit 1) are passed over, as OptiX passes over an instance whose
+// visibilityMask shares no bit with the ray's.
 template <bool ANY>
 CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float tmin,
                                           float tmax, Hit& best) {
@@ -183,6 +187,7 @@ CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float
             n1 = nv[1];
             n2 = nv[2];
         }
+        if (__builtin_bit_cast(uint32_t, p1.w) & (ANY ? 2u : 1u)) continue;   // invisible to this ray kind
         float t, u, v;
         if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, tmax, t, u, v))
             continue;

@@ -296,7 +296,7 @@ def optimize(a) -> dict:
     from .autograd import DiffRenderer
 
     ctx = runtime.Context(0)
-    sc = scenes.object_grid(16, seed=3, ico_subdiv=3)
+    sc = scenes.object_grid(16, seed=3, ico_subdiv=3, trace_emitters=False, lights="panels")
     gs = runtime.Scene(ctx, sc)
     W, H = a.width, a.height
     truth = sc.materials.astype(np.float32)

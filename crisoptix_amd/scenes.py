@@ -6,8 +6,12 @@ on every machine.  Geometry is generated in float64 and rounded once to float32.
 
 Scenes follow the flattening of the renderer's C ABI: one triangle list with per-triangle
 material ids (the reference's Model/Mesh/ModelInstance tree always rehydrates with
-transforms[0], include/graph/Interaction.h:143), plus explicit emitters that are sampled
-but not traced (Scene::BuildEmitterTable, src/Scene.cpp:176-204).
+transforms[0], include/graph/Interaction.h:143), plus explicit emitters that NEE samples
+(Scene::BuildEmitterTable, src/Scene.cpp:176-204).  With ``trace_emitters`` the emitter
+triangles are also traced geometry, as the reference's scenes have them (implicit emitters are
+mesh instances, src/Scene.cpp:184-197; an explicit Emitter wraps a traced mesh,
+src/main.cpp:164-167): closest-hit rays see them and occlusion rays pass over them (their
+``tri_mask`` is CX_VIS_SURFACE), so a shadow ray that ends on a light is not blocked by it.
 """
 from __future__ import annotations
 
@@ -63,6 +67,7 @@ class Scene:
     material_tex: np.ndarray | None = None  # int32 [M,4] diffuse, specular, roughness texture
     #                                         (-1: none), specular usage (0 Color, 1 OccRoughMetal)
     textures: list = field(default_factory=list)  # uint8 [H,W,4] RGBA images
+    tri_mask: np.ndarray | None = None     # uint8 [T] CX_VIS_* bits per triangle, or None (all 3)
 
     @property
     def num_triangles(self) -> int:
@@ -82,6 +87,7 @@ class _Builder:
         self.nrm: list[np.ndarray] = []
         self.idx: list[np.ndarray] = []
         self.mat: list[np.ndarray] = []
+        self.vis: list[np.ndarray] = []
         self.nv = 0
         self.materials: list[list[float]] = []
 
@@ -89,12 +95,23 @@ class _Builder:
         self.materials.append([*diff, *spec, *emissive, rough])
         return len(self.materials) - 1
 
-    def add_mesh(self, pos: np.ndarray, nrm: np.ndarray, idx: np.ndarray, mat: int):
+    def add_mesh(self, pos: np.ndarray, nrm: np.ndarray, idx: np.ndarray, mat: int, vis: int = 3):
         self.pos.append(np.asarray(pos, np.float64))
         self.nrm.append(np.asarray(nrm, np.float64))
         self.idx.append(np.asarray(idx, np.int64) + self.nv)
         self.mat.append(np.full(len(idx), mat, np.uint32))
+        self.vis.append(np.full(len(idx), vis, np.uint8))
         self.nv += len(pos)
+
+    def add_emitter_geometry(self, emitters, mat: int):
+        """The emitters' triangles as traced geometry seen only by closest-hit rays (flat,
+        normal (v1 - v0) x (v2 - v0), the side the reference's emitters light)."""
+        for tris, _ in emitters:
+            tris = np.asarray(tris, np.float64).reshape(-1, 3, 3)
+            for tri in tris:
+                n = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+                n = n / np.linalg.norm(n)
+                self.add_mesh(tri, np.tile(n, (3, 1)), np.asarray([[0, 1, 2]]), mat, vis=1)
 
     def add_grid_quad(self, origin, eu, ev, n, normal, mat):
         """An n x n tessellated parallelogram origin + s*eu + t*ev, flat normal."""
@@ -136,6 +153,7 @@ class _Builder:
         nrm = np.concatenate(self.nrm).astype(np.float32)
         idx = np.concatenate(self.idx).astype(np.uint32)
         mat = np.concatenate(self.mat).astype(np.uint32)
+        vis = np.concatenate(self.vis).astype(np.uint8)
         faces = []
         offs = [0]
         cols = []
@@ -150,7 +168,7 @@ class _Builder:
             emitter_faces=np.concatenate(faces).astype(np.float32),
             emitter_face_offset=np.asarray(offs, np.uint32),
             emitter_color=np.asarray(cols, np.float32).reshape(-1, 3),
-            name=name, meta=meta or {})
+            name=name, meta=meta or {}, tri_mask=None if (vis == 3).all() else vis)
 
 
 def _icosphere(subdiv: int):
@@ -200,12 +218,13 @@ def _random_rotation(rng: Pcg32) -> np.ndarray:
     ])
 
 
-def cornell_box(seed: int = 1, tess: int = 8) -> Scene:
+def cornell_box(seed: int = 1, tess: int = 8, trace_emitters: bool = False) -> Scene:
     """Closed Cornell-style box [-1,1] x [0,2] x [-3,1]: 6 wall quads and 2 axis-aligned boxes,
     each face tessellated tess x tess (2,304 triangles at tess = 8), and one 2-triangle emitter
     of radiance 15 hanging 0.2 below the ceiling (the reference's emitter has no cosine term,
     so a light flush with the ceiling would give the ceiling unbounded 1/d^2 weights).
-    Materials: albedo U[0.2, 0.8]^3 per surface, roughness 1."""
+    Materials: albedo U[0.2, 0.8]^3 per surface, roughness 1.  trace_emitters adds the light's
+    two triangles as geometry for closest-hit rays (white diffuse panel, albedo 0.8)."""
     rng = Pcg32(seed)
     b = _Builder()
 
@@ -227,15 +246,29 @@ def cornell_box(seed: int = 1, tess: int = 8) -> Scene:
     lz = -1.5
     light = [[(-0.3, y, lz - 0.3), (0.3, y, lz - 0.3), (0.3, y, lz + 0.3)],
              [(-0.3, y, lz - 0.3), (0.3, y, lz + 0.3), (-0.3, y, lz + 0.3)]]
-    return b.build([(light, (15.0, 15.0, 15.0))], name=f"cornell_seed{seed}_tess{tess}")
+    emitters = [(light, (15.0, 15.0, 15.0))]
+    if trace_emitters:
+        b.add_emitter_geometry(emitters, b.add_material([0.8, 0.8, 0.8], 1.0))
+    return b.build(emitters, name=f"cornell_seed{seed}_tess{tess}" + ("_tl" if trace_emitters else ""))
 
 
 def object_grid(grid: int = 16, seed: int = 2, ground_quads: int = 200, ico_subdiv: int = 4,
-                ico_prob: float = 0.75, num_lights: int = 16, spacing: float = 4.0) -> Scene:
+                ico_prob: float = 0.75, num_lights: int = 16, spacing: float = 4.0,
+                trace_emitters: bool = True, lights: str = "tent") -> Scene:
     """grid x grid seeded icospheres (subdivision `ico_subdiv`) and boxes, scale U[0.3, 1.5],
-    random rotation, on a ground plane of ground_quads^2 quads.  GGX roughness U[0.05, 1],
-    albedo U[0.1, 0.9], specF0 0.04.  `num_lights` emitter triangles of radiance
-    lognormal(mu=2, sigma=0.5).  grid 16 gives about 1.06 M triangles."""
+    random rotation, on a ground plane of ground_quads^2 quads (half-width `half` = grid *
+    spacing / 2 + spacing).  GGX roughness U[0.05, 1], albedo U[0.1, 0.9], specF0 0.04.
+    `num_lights` emitter triangles of radiance lognormal(mu=2, sigma=0.5), one emitter each:
+      lights="tent":   the triangles are the faces of a regular pyramid over the grid (apex at
+                       height `half` above the centre, base a num_lights-gon of radius `half`
+                       on the ground), facing inward, so ground and tent close the scene as the
+                       reference's Cornell box is closed: a path ends by missing geometry only
+                       where it leaves between the faces' edges, never into an open sky;
+      lights="panels": small triangles facing down from height 4 * spacing + 4 at seeded
+                       positions (an open scene: most bounces escape and end the path).
+    With trace_emitters (the default) the triangles are also geometry for closest-hit rays, a
+    white diffuse material (albedo 0.8, roughness 1, no emission: light reaches paths through
+    NEE), the last material.  grid 16 gives about 1.06 M triangles."""
     rng = Pcg32(seed)
     b = _Builder()
     half = grid * spacing / 2.0 + spacing
@@ -257,24 +290,44 @@ def object_grid(grid: int = 16, seed: int = 2, ground_quads: int = 200, ico_subd
                 b.add_mesh(pos, nrm, ifc, m)
             else:
                 b.add_box(center, (scale, scale, scale), rot, 1, m)
-    lights = []
-    h = 4.0 * spacing + 4.0
-    for _ in range(num_lights):
-        cx = rng.uniform(-half * 0.7, half * 0.7)
-        cz = rng.uniform(-half * 0.7, half * 0.7)
-        s = spacing * 0.75
-        radiance = math.exp(2.0 + 0.5 * rng.normal())
-        tri = [(cx - s, h, cz - s), (cx + s, h, cz - s), (cx, h, cz + s)]
-        lights.append(([tri], (radiance, radiance, radiance)))
-    return b.build(lights, name=f"objgrid{grid}_seed{seed}",
-                   meta={"grid": grid, "spacing": spacing, "half": half})
+    emitters = []
+    if lights == "tent":
+        if num_lights < 3:
+            raise ValueError("a tent needs at least 3 light triangles")
+        top = np.asarray([0.0, half, 0.0])
+        for i in range(num_lights):
+            a0, a1 = 2.0 * math.pi * i / num_lights, 2.0 * math.pi * (i + 1) / num_lights
+            p0 = np.asarray([half * math.cos(a0), 0.0, half * math.sin(a0)])
+            p1 = np.asarray([half * math.cos(a1), 0.0, half * math.sin(a1)])
+            # wound so that (v1 - v0) x (v2 - v0) points into the tent
+            tri = [top, p1, p0] if np.dot(np.cross(p0 - top, p1 - top), p0 + p1) > 0 else [top, p0, p1]
+            radiance = math.exp(2.0 + 0.5 * rng.normal())
+            emitters.append(([tri], (radiance, radiance, radiance)))
+    elif lights == "panels":
+        h = 4.0 * spacing + 4.0
+        for _ in range(num_lights):
+            cx = rng.uniform(-half * 0.7, half * 0.7)
+            cz = rng.uniform(-half * 0.7, half * 0.7)
+            s = spacing * 0.75
+            radiance = math.exp(2.0 + 0.5 * rng.normal())
+            tri = [(cx - s, h, cz - s), (cx + s, h, cz - s), (cx, h, cz + s)]
+            emitters.append(([tri], (radiance, radiance, radiance)))
+    else:
+        raise ValueError(f"unknown lights {lights!r}")
+    if trace_emitters:
+        b.add_emitter_geometry(emitters, b.add_material([0.8, 0.8, 0.8], 1.0))
+    name = f"objgrid{grid}_seed{seed}" + ("" if lights == "tent" else "_" + lights) + \
+        ("" if trace_emitters else "_untraced")
+    return b.build(emitters, name=name, meta={"grid": grid, "spacing": spacing, "half": half,
+                                              "trace_emitters": trace_emitters, "lights": lights})
 
 
 def inverse_scene(seed: int = 4) -> Scene:
     """The differentiable-rendering scene: the object grid at 8 x 8 objects (icospheres of
     subdivision 3 and boxes) on a 140 x 140-quad ground, about 100 k triangles, with 64 object
-    materials plus the ground's and 16 emitter triangles."""
-    return object_grid(grid=8, seed=seed, ground_quads=140, ico_subdiv=3, num_lights=16)
+    materials plus the ground's and 16 emitter triangles (hanging panels, not traced)."""
+    return object_grid(grid=8, seed=seed, ground_quads=140, ico_subdiv=3, num_lights=16, trace_emitters=False,
+                       lights="panels")
 
 
 def with_textures(scene: Scene, seed: int = 7, scale: float = 0.3) -> Scene:
@@ -314,7 +367,7 @@ def with_textures(scene: Scene, seed: int = 7, scale: float = 0.3) -> Scene:
                  tri_material=scene.tri_material, materials=mats, emitter_faces=scene.emitter_faces,
                  emitter_face_offset=scene.emitter_face_offset, emitter_color=scene.emitter_color,
                  name=scene.name + "_textured", meta=dict(scene.meta), uvs=uv, material_tex=mt,
-                 textures=[t0, t1, t2])
+                 textures=[t0, t1, t2], tri_mask=scene.tri_mask)
 
 
 def perturbed_materials(materials: np.ndarray, seed: int, albedo: float = 0.2, roughness: float = 0.3) -> np.ndarray:
@@ -380,8 +433,14 @@ def cornell_camera(width: int, height: int) -> Camera:
 
 
 def orbit_camera(scene: Scene, width: int, height: int) -> Camera:
+    """Camera of the orbit workloads, aimed at the origin (Camera::RotateAroundOrigin turns it):
+    inside the tent of an object grid lit by one (half the tent's radius out, a quarter of its
+    height up), otherwise above and beside the grid."""
     half = scene.meta.get("half", 34.0)
-    pos = np.asarray([0.0, 0.55 * half, 1.25 * half], np.float64)
+    if scene.meta.get("lights") == "tent":
+        pos = np.asarray([0.0, 0.25 * half, 0.5 * half], np.float64)
+    else:
+        pos = np.asarray([0.0, 0.55 * half, 1.25 * half], np.float64)
     return Camera(pos=pos.astype(np.float32), view_dir=(-pos / np.linalg.norm(pos)).astype(np.float32),
                   up=np.asarray([0.0, 1.0, 0.0], np.float32), aspect=width / height, fov=math.radians(50.0))
 

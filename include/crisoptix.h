@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define CX_ABI_VERSION 9
+#define CX_ABI_VERSION 10
 
 typedef enum cx_status {
     CX_OK = 0,
@@ -166,8 +166,21 @@ typedef struct cx_texture {
  * Each triangle carries the dense id of its material (the MeshSbtData callable indices of
  * include/model/Mesh.h:10-16 become material ids).  Emitters are the explicit emitters of
  * Scene::BuildEmitterTable (src/Scene.cpp:176-204): world-space triangles and one colour
- * each; they are not part of the traced geometry.
+ * each, sampled by NEE.  An emitter's triangles are traced only if they are also listed as
+ * geometry, as the reference's scenes do (implicit emitters are mesh instances,
+ * src/Scene.cpp:184-197; an explicit Emitter wraps the mesh of a traced instance,
+ * src/main.cpp:164-167).  triMask then keeps them out of occlusion queries: a shadow ray aimed
+ * at a point of a traced light would otherwise find the light itself at t = tmax.
  */
+/* Visibility of a triangle (cx_scene_desc.triMask), the per-instance visibilityMask of an
+ * OptiX IAS reduced to two bits: closest-hit (surface) rays see triangles with bit 0 set,
+ * occlusion rays those with bit 1 set (the reference traces with mask 1 and gives every
+ * instance 1, include/graph/trace.h:25-88). */
+enum {
+    CX_VIS_SURFACE = 1,
+    CX_VIS_OCCLUSION = 2,
+    CX_VIS_ALL = 3
+};
 typedef struct cx_scene_desc {
     const float* positions;        /* host, [numVertices][3] */
     const float* normals;          /* host, [numVertices][3] */
@@ -193,6 +206,10 @@ typedef struct cx_scene_desc {
     uint32_t numTextures;
     uint32_t bvhBuilder;           /* cx_bvh_builder: where the BVH is built (the traversal's
                                       answers are the same for either tree) */
+    const uint8_t* triMask;        /* host, [numTriangles] CX_VIS_* bits of each triangle, or NULL:
+                                      every triangle CX_VIS_ALL.  Emitter geometry is given
+                                      CX_VIS_SURFACE: surface rays hit lights, shadow rays ignore
+                                      them */
 } cx_scene_desc;
 
 typedef struct cx_scene_info {
@@ -265,8 +282,9 @@ int cx_scene_get_info(const cx_scene* scene, cx_scene_info* info);
  *      (include/graph/trace.h:25-88, shader/mesh_hit.cu:17-69, shader/miss.cu:8-26).
  *      rays: device, [n][8] = origin xyz, direction xyz, tmin, tmax.
  *      Closest hit writes primitive index (0xFFFFFFFF on a miss), t and the OptiX
- *      barycentrics (u weights vertex 1, v weights vertex 2).  Occlusion writes 1 if any
- *      triangle is hit in [tmin, tmax], else 0. ---- */
+ *      barycentrics (u weights vertex 1, v weights vertex 2) over the triangles with
+ *      CX_VIS_SURFACE.  Occlusion writes 1 if any triangle with CX_VIS_OCCLUSION is hit in
+ *      [tmin, tmax], else 0. ---- */
 int cx_trace_closest(cx_ctx* ctx, const cx_scene* scene, const float* d_rays, uint32_t n,
                      uint32_t* d_prim, float* d_t, float* d_u, float* d_v, void* stream);
 int cx_trace_occluded(cx_ctx* ctx, const cx_scene* scene, const float* d_rays, uint32_t n,

@@ -13,7 +13,9 @@
 //  C5 watertight ray/triangle test, closest hit = min (t, primitive), [tmin, tmax] inclusive;
 //  C6 sin/cos/pow are the project-written functions of orc_math.h;
 //  C7 rehydrated normals are not normalised; C8 one instance, identity transform;
-//  C9 explicit emitters are not traced geometry.
+//  C9' emitter triangles are traced geometry only where the scene lists them, and then only
+//      closest-hit rays see them: occlusion rays pass over triangles without CX_VIS_OCCLUSION
+//      (an OptiX visibility mask), so a shadow ray ending on a light is not occluded by it.
 //
 // Parity pins (tests/test_oracle_*.py): PCG32 against the PCG reference demo output and an
 // independent restatement from the PCG paper, TEA against the published TEA test vector,
@@ -154,6 +156,7 @@ struct Scene {
     std::vector<Texture> textures;
     std::vector<uint32_t> idx;      // 3 per triangle
     std::vector<uint32_t> triMat;
+    std::vector<uint8_t> triVis;    // CX_VIS_* bits per triangle (C9')
     std::vector<Material> mats;
     std::vector<EmitterData> emitters;
     // Evaluation tables of the FP64 adjoint (values and derivatives); `mats` and the emitters'
@@ -299,6 +302,7 @@ static Hit trace_closest(const Scene& s, const V3& o, const V3& d, float tmin, f
     Hit best{INVALID_PRIM, tmax, 0.0f, 0.0f};
     const RayPre rp = ray_pre(d);
     auto consider = [&](uint32_t tri) {
+        if (!(s.triVis[tri] & CX_VIS_SURFACE)) return;
         float t, u, v;
         if (!tri_hit(o, rp, s.vtx(tri, 0), s.vtx(tri, 1), s.vtx(tri, 2), tmin, tmax, t, u, v)) return;
         if (best.prim == INVALID_PRIM || t < best.t || (t == best.t && tri < best.prim)) best = {tri, t, u, v};
@@ -328,9 +332,13 @@ static Hit trace_closest(const Scene& s, const V3& o, const V3& d, float tmin, f
 static bool trace_occluded(const Scene& s, const V3& o, const V3& d, float tmin, float tmax) {
     const RayPre rp = ray_pre(d);
     float t, u, v;
+    auto blocks = [&](uint32_t tri) {
+        return (s.triVis[tri] & CX_VIS_OCCLUSION) &&
+               tri_hit(o, rp, s.vtx(tri, 0), s.vtx(tri, 1), s.vtx(tri, 2), tmin, tmax, t, u, v);
+    };
     if (s.bruteForce || s.nodes.empty()) {
         for (uint32_t tri = 0; tri < s.numTris(); tri++)
-            if (tri_hit(o, rp, s.vtx(tri, 0), s.vtx(tri, 1), s.vtx(tri, 2), tmin, tmax, t, u, v)) return true;
+            if (blocks(tri)) return true;
         return false;
     }
     const V3 inv(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -341,10 +349,8 @@ static bool trace_occluded(const Scene& s, const V3& o, const V3& d, float tmin,
         const Node& n = s.nodes[stack[--sp]];
         if (!box_visit(n, o, inv, tmin, tmax)) continue;
         if (n.right < 0) {
-            for (int i = 0; i < -n.right; i++) {
-                const uint32_t tri = s.triOrder[n.left + i];
-                if (tri_hit(o, rp, s.vtx(tri, 0), s.vtx(tri, 1), s.vtx(tri, 2), tmin, tmax, t, u, v)) return true;
-            }
+            for (int i = 0; i < -n.right; i++)
+                if (blocks(s.triOrder[n.left + i])) return true;
         } else {
             stack[sp++] = n.right;
             stack[sp++] = n.left;
@@ -2469,6 +2475,8 @@ void* orc_scene_create(const cx_scene_desc* d, int bruteForce) {
     }
     s->idx.assign(d->indices, d->indices + 3 * (size_t)d->numTriangles);
     s->triMat.assign(d->triMaterial, d->triMaterial + d->numTriangles);
+    if (d->triMask) s->triVis.assign(d->triMask, d->triMask + d->numTriangles);
+    else s->triVis.assign(d->numTriangles, (uint8_t)CX_VIS_ALL);
     s->mats.resize(d->numMaterials);
     for (uint32_t m = 0; m < d->numMaterials; m++) {
         const cx_material& cm = d->materials[m];

@@ -37,9 +37,16 @@ CASES = {
     "cond_cornell_32x24": dict(scene=("cornell_box", dict(seed=1, tess=8)), mode="conditional_restir", W=32,
                                H=24, frames=3, over=dict(gatherRadius=0.08, suffixSpatialCount=2)),
     "cond_grid3_orbit_32x24": dict(scene=("object_grid", dict(grid=3, seed=5, ground_quads=20, ico_subdiv=2,
-                                                               num_lights=4)),
+                                                               num_lights=4, lights="tent", trace_emitters=True)),
                                    mode="conditional_restir", W=32, H=24, frames=3, orbit=True,
                                    over=dict(gatherRadius=0.3)),
+    # the Conditional ReSTIR workload reduced: 16 traced light triangles forming the tent, k = 8,
+    # neighbour counts from 1 to 8 with a few hundred pixels at 8
+    "cond_tent16_orbit_48x32": dict(scene=("object_grid", dict(grid=3, seed=5, ground_quads=20, ico_subdiv=2,
+                                                                num_lights=16, lights="tent",
+                                                                trace_emitters=True)),
+                                    mode="conditional_restir", W=48, H=32, frames=3, orbit=True,
+                                    over=dict(gatherRadius=0.6)),
 }
 
 

@@ -24,13 +24,14 @@ sys.path.insert(0, ROOT)
 from crisoptix_amd import abi, scenes  # noqa: E402
 
 # Scenes and views of the rel-MSE measurements: the Cornell box of the ReSTIR PT workload, and
-# the 1.06 M-triangle object grid of the Conditional ReSTIR workload seen from the orbit
-# camera's first position, at a resolution the CPU can converge.
+# the 1.06 M-triangle object grid of the Conditional ReSTIR workload (under its tent of 16 traced
+# light triangles) seen from the orbit camera's first position, at a resolution the CPU can
+# converge.
 TRUTHS = {
     "cornell_64x64": dict(scene=("cornell_box", dict(seed=1, tess=8)), camera="cornell", W=64, H=64,
                           maxPathLen=4, neeProb=0.3, spp=65536),
-    "objgrid16_96x54": dict(scene=("object_grid", dict(grid=16, seed=2)), camera="orbit", W=96, H=54,
-                            maxPathLen=8, neeProb=0.3, spp=16384),
+    "objgrid16_96x54": dict(scene=("object_grid", dict(grid=16, seed=2, lights="tent", trace_emitters=True)),
+                            camera="orbit", W=96, H=54, maxPathLen=8, neeProb=0.3, spp=65536),
 }
 
 

@@ -5,8 +5,16 @@ from crisoptix_amd import scenes
 
 
 def small_grid_scene():
-    """A reduced object grid: 3x3 objects (icosphere subdivision 2 or boxes) on a 20x20 ground."""
+    """A reduced object grid: 3x3 objects (icosphere subdivision 2 or boxes) on a 20x20 ground,
+    under a tent of 4 traced light triangles."""
     return scenes.object_grid(grid=3, seed=5, ground_quads=20, ico_subdiv=2, num_lights=4)
+
+
+def tent_grid_scene():
+    """The Conditional ReSTIR workload's scene reduced: 3x3 objects on a 20x20 ground under the
+    tent of 16 traced light triangles of the full scene, a closed scene in which about a third
+    of the pixels hold a valid suffix."""
+    return scenes.object_grid(grid=3, seed=5, ground_quads=20, ico_subdiv=2, num_lights=16)
 
 
 def random_rays(scene, n, seed, kind="mixed"):

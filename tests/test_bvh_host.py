@@ -32,8 +32,10 @@ def test_host_traversal_matches_oracle(name, width, max_leaf, orc):
 def test_wide8_collapse_falls_back_to_greedy_on_large_scene():
     """The 1.06 M-triangle object grid of the 1080p benchmark: the least-area collapse at width 8
     needs more than 64 traversal-stack entries, so the collapse takes the greedy rule, which fits;
-    width 8 is kept (not the binary fallback) and its answers equal the 4-wide tree's."""
-    sc = scenes.object_grid(16, 2)
+    width 8 is kept (not the binary fallback) and its answers equal the 4-wide tree's.  (With its
+    16 light panels traced as well, the greedy tree does not fit either and width 8 falls back
+    to the binary tree, which test_traced_emitters_* cover.)"""
+    sc = scenes.object_grid(16, 2, trace_emitters=False)
     rays = random_rays(sc, 3000, seed=31)
     *a8, used8 = runtime.host_trace(sc, rays, width=8)
     *a4, used4 = runtime.host_trace(sc, rays, width=4)
@@ -74,3 +76,51 @@ def test_random_soups_match_oracle(seed, orc):
                 assert np.array_equal(bits(x), bits(y)), (width, max_leaf)
             og, _ = runtime.host_trace(sc, srays, any_hit=True, width=width, max_leaf=max_leaf)
             assert np.array_equal(occ, og), (width, max_leaf)
+
+
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_traced_emitters_are_invisible_to_occlusion(width, orc):
+    """C9': the same scene with and without its light panels as geometry.  Occlusion rays pass
+    over triangles that lack CX_VIS_OCCLUSION, so every occlusion answer is the same for both
+    scenes; closest-hit rays see the panels, so a closest hit changes only where the traced
+    scene's hit is a panel triangle no farther than the untraced scene's hit.  Host traversal
+    and oracle agree bit for bit on the traced scene."""
+    traced = scenes.object_grid(grid=3, seed=5, ground_quads=20, ico_subdiv=2, num_lights=4)
+    plain = scenes.object_grid(grid=3, seed=5, ground_quads=20, ico_subdiv=2, num_lights=4, trace_emitters=False)
+    nplain = plain.num_triangles
+    assert traced.num_triangles == nplain + 4
+    assert (traced.tri_mask[:nplain] == 3).all() and (traced.tri_mask[nplain:] == 1).all()
+    # shadow rays from random ground / object points to random points of the lights
+    rng = np.random.default_rng(61)
+    n = 4000
+    tri = plain.indices[rng.integers(0, nplain, n)]
+    bary = rng.dirichlet((1, 1, 1), n)
+    src = np.einsum("nk,nkd->nd", bary, plain.positions[tri].astype(np.float64))
+    faces = traced.emitter_faces.astype(np.float64)
+    fb = rng.dirichlet((1, 1, 1), n)
+    dst = np.einsum("nk,nkd->nd", fb, faces[rng.integers(0, faces.shape[0], n)])
+    v = dst - src
+    dist = np.linalg.norm(v, axis=1)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = src
+    rays[:, 3:6] = v / dist[:, None]
+    rays[:, 6] = 1e-3
+    rays[:, 7] = dist
+    occ_t, _ = runtime.host_trace(traced, rays, any_hit=True, width=width)
+    occ_p, _ = runtime.host_trace(plain, rays, any_hit=True, width=width)
+    assert np.array_equal(occ_t, occ_p)
+    assert 0.05 < occ_t.mean() < 0.95
+    assert np.array_equal(orc.OracleScene(traced).trace_occluded(rays).astype(np.uint32), occ_t)
+    # the same rays as closest-hit queries without the length limit: the unoccluded ones end on
+    # the light they aim at
+    rays[:, 7] = 1e16
+    pt, tt, ut, vt, _ = runtime.host_trace(traced, rays, width=width)
+    pp, tp, _, _, _ = runtime.host_trace(plain, rays, width=width)
+    ref = orc.OracleScene(traced).trace_closest(rays)
+    for x, y in zip(ref, (pt, tt, ut, vt)):
+        assert np.array_equal(bits(x), bits(y))
+    panel = pt >= nplain
+    assert panel[occ_t == 0].mean() > 0.99
+    same = pt == pp
+    assert (same | panel).all()
+    assert (tt[panel & (pp != 0xFFFFFFFF)] <= tp[panel & (pp != 0xFFFFFFFF)]).all()

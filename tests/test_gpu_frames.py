@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from crisoptix_amd import abi, scenes
-from parity_util import rel_mismatch, small_grid_scene
+from parity_util import rel_mismatch, small_grid_scene, tent_grid_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -33,14 +33,30 @@ CASES = {
     "restir_pt_grid_tex": dict(scene="grid3_tex", mode="restir_pt_cornell", W=48, H=32, frames=4),
     "cond_grid_tex_orbit": dict(scene="grid3_tex", mode="conditional_restir", W=48, H=32, frames=4, orbit=True,
                                 over=dict(gatherRadius=0.3)),
+    # the Conditional ReSTIR workload reduced (tent of 16 traced lights, k = 8): the final gather
+    # sees every neighbour count from 1 to 8, and at least 250 pixels gather all 8 neighbours
+    "cond_tent_orbit": dict(scene="tent16", mode="conditional_restir", W=48, H=32, frames=4, orbit=True,
+                            over=dict(gatherRadius=0.6), full_k=250),
+    # the closed Cornell box with its light traced as geometry (C9')
+    "cond_cornell_traced_light": dict(scene="cornell_tl", mode="conditional_restir", W=40, H=32, frames=3,
+                                      over=dict(gatherRadius=0.08)),
 }
+
+
+def _scene(name):
+    if name == "cornell":
+        return scenes.cornell_box(1, 8)
+    if name == "cornell_tl":
+        return scenes.cornell_box(1, 8, trace_emitters=True)
+    if name == "tent16":
+        return tent_grid_scene()
+    sc = small_grid_scene()
+    return scenes.with_textures(sc) if name.endswith("_tex") else sc
 
 
 def _setup(case, orc, rt, ctx, exec_mode="wavefront"):
     c = CASES[case]
-    sc = scenes.cornell_box(1, 8) if c["scene"] == "cornell" else small_grid_scene()
-    if c["scene"].endswith("_tex"):
-        sc = scenes.with_textures(sc)
+    sc = _scene(c["scene"])
     W, H = c["W"], c["H"]
     gscene = rt.Scene(ctx, sc, builder=c.get("builder", "host"))
     oscene = orc.OracleScene(sc)
@@ -48,7 +64,7 @@ def _setup(case, orc, rt, ctx, exec_mode="wavefront"):
     p = abi.workload_params(c["mode"], W, H, gscene.size)
     for k, v in c.get("over", {}).items():
         setattr(p, k, v)
-    cam = scenes.cornell_camera(W, H) if c["scene"] == "cornell" else scenes.orbit_camera(sc, W, H)
+    cam = scenes.cornell_camera(W, H) if c["scene"].startswith("cornell") else scenes.orbit_camera(sc, W, H)
     ren = rt.Renderer(W, H, ctx, cam, gscene, params=p, seed=1234)
     if exec_mode.startswith("rounds"):   # wavefront rounds (0: all), then the in-place tail kernel
         ren.set_exec_mode("wavefront")
@@ -123,4 +139,9 @@ def test_frames_match_oracle(case, exec_mode, orc, cx_runtime, cx_ctx):
             n = oc[p]
             assert np.array_equal(gi[p, :n], oi[p, :n])
             assert np.array_equal(gd[p, :n].view(np.uint32), od[p, :n].view(np.uint32))
-        print(f"{case}: mean neighbours {oc.mean():.2f}")
+        k = int(c.get("over", {}).get("gatherM", 9)) - 1
+        hist = np.bincount(oc, minlength=k + 1)
+        print(f"{case}: mean neighbours {oc.mean():.2f}, histogram {hist.tolist()}, "
+              f"valid suffixes {v.mean():.2f}")
+        if "full_k" in c:
+            assert (hist[1:] > 0).all() and hist[k] >= c["full_k"], hist

@@ -105,7 +105,8 @@ def test_textured_bsdf_channel_mapping(orc):
     """A textured material evaluates like the untextured material whose constants are the
     texture's values at the hit (diffuse colour; roughness from red; OccRoughMetal: roughness
     from green, metal from blue, specF0 = mix(diffuse, 1, metal))."""
-    base = scenes.object_grid(grid=2, seed=3, ground_quads=4, ico_subdiv=1, num_lights=1)
+    base = scenes.object_grid(grid=2, seed=3, ground_quads=4, ico_subdiv=1, num_lights=1, trace_emitters=False,
+                              lights="panels")
     tex = scenes.with_textures(base)
     M = tex.materials.shape[0]
     rng = np.random.default_rng(7)
