@@ -3,15 +3,19 @@
 Workload: 1920x1080 Conditional ReSTIR (prefix/suffix reservoirs, entry-tree neighbour search,
 final gather with 8 suffixes per pixel) on the ~1.06 M-triangle object grid (16 x 16 seeded
 icospheres of subdivision 4 and boxes, scale U[0.3, 1.5], random rotation, on a 200 x 200-quad
-ground plane, GGX roughness U[0.05, 1], albedo U[0.1, 0.9], 16 emitter triangles of radiance
-lognormal(2, 0.5)), max path length 8, camera orbiting by 2*pi/64 per frame.  Synthetic,
+ground plane, GGX roughness U[0.05, 1], albedo U[0.1, 0.9]) lit by 16 emitter triangles of
+radiance lognormal(2, 0.5) that form a tent over the grid and are traced as geometry (closest-hit
+rays only), max path length 8, camera inside the tent orbiting by 2*pi/64 per frame.  Synthetic,
 seeded scene data; every frame runs the full pass sequence of Renderer::LaunchFrame.
 
-One step = one frame.  value = counted rays (closest-hit + shadow) per second over the timed
-frames; ms_per_step = GPU time per frame from hipEvents around the timed frames.  roofline: the
-dominant kernel is the ray traversal (k_q_trace, about half of the frame), timed by HIP events
-around each of its launches on the frame's stream; its algorithmic bytes are 48 B per
-closest-hit ray (32 B ray + 16 B hit) and 36 B per shadow ray (32 B ray + 4 B flag).
+One step = one frame.  ms_per_step = the median over the timed frames (frames W+1 .. W+K) of
+HIP-event time around one frame; value = counted rays (closest-hit + shadow) of the timed frames
+per second of their summed time.  roofline: the dominant kernel is the ray traversal
+(k_q_trace), timed by HIP events around each of its launches on the frame's stream; its
+algorithmic bytes are 48 B per closest-hit ray (32 B ray + 16 B hit) and 36 B per shadow ray
+(32 B ray + 4 B flag).  The line reports the workload's reuse density (valid suffixes, prefix
+entries, neighbours per gathering pixel) and exits with status 3 if a wavefront queue ever
+overflowed.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -31,11 +35,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/frame + Mrays/s at 1080p Conditional ReSTIR; rel-MSE vs FP64 truth"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per pixel of the final gather (primary interaction 40, RNG 32, neighbour
-# list write 64 + read 128, k = 8 neighbour suffix reservoirs read twice 2 x 8 x 152, output
-# 24) and per ray (48 B closest hit, 36 B shadow), from the reference's record sizes.
-FINAL_GATHER_STATE_BYTES_PER_PX = 2720
-FRAME_STATE_BYTES_PER_PX = 6704
+# Algorithmic bytes, from the reference's record sizes: per pixel of the final gather without
+# its neighbours (primary interaction 40, RNG 32, neighbour list write 64 + read 128, output
+# 24) and of the whole frame without them (6,704 - 2,432), plus 2 x 152 B per neighbour found
+# (its suffix reservoir read by the validity and the shift loops), counted from the neighbour
+# statistics of each frame; per ray 48 B (closest hit) or 36 B (shadow).
+FINAL_GATHER_STATE_BYTES_PER_PX = 288
+FRAME_STATE_BYTES_PER_PX = 4272
+NEIGHBOR_BYTES = 2 * 152
 CLOSEST_RAY_BYTES = 48
 SHADOW_RAY_BYTES = 36
 WORKLOAD_SEED = 2
@@ -54,9 +61,9 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--grid", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-scale", type=int, default=1,
+    ap.add_argument("--cpu-sample-scale", type=int, default=4,
                     help="CPU baseline renders (W/scale) x (H/scale) pixels of the same workload")
-    ap.add_argument("--cpu-frames", type=int, default=6)
+    ap.add_argument("--cpu-frames", type=int, default=6, help="measured CPU frames (after 2 warm-up frames)")
     ap.add_argument("--exec", choices=["wavefront", "megakernel"], default="wavefront",
                     help="execution mode of the HIP passes (identical results)")
     ap.add_argument("--relmse-frames", type=int, default=4096,
@@ -116,7 +123,10 @@ def rel_mse_vs_truth(runtime, ctx, gscene, scene, frames, exec_mode):
     return out
 
 
-def cpu_baseline(scene, params_for, W, H, scale, frames=2):
+def cpu_baseline(scene, params_for, W, H, scale, frames=6, warmup=2):
+    """The oracle (host C++17 restatement of the shader logic, OpenMP over pixels) on the same
+    workload at (W/scale) x (H/scale): `warmup` frames discarded, then the median frame time of
+    `frames` frames; Mrays/s = rays of the measured frames / their summed time."""
     from crisoptix_amd import abi, scenes
     from oracle import oracle as O
 
@@ -126,23 +136,24 @@ def cpu_baseline(scene, params_for, W, H, scale, frames=2):
     cam = scenes.orbit_camera(scene, w, h)
     ren = O.OracleRenderer(os_, w, h, p, seed=1)
     rays = 0
-    secs = 0.0
-    for f in range(frames + 1):
+    times = []
+    for f in range(warmup + frames):
         cam.rotate_around_origin([0, 1, 0], 2 * math.pi / 64)
         t0 = time.perf_counter()
         ren.frame(abi.CameraData.from_dict(cam.data()), True)
         dt = time.perf_counter() - t0
-        if f > 0:   # first frame warms the history
-            secs += dt
+        if f >= warmup:
+            times.append(dt)
             rays += int(ren.rays().sum())
+    med = float(np.median(times))
     return {
-        "value": rays / secs / 1e6,
+        "value": rays / sum(times) / 1e6,
         "unit": "Mrays/s",
         "cores": O.num_threads(),
         "kind": "port",
-        "sample": f"{frames} frames (after 1 warm-up frame) of the same workload at {w}x{h}, "
-                  f"oracle host C++ with OpenMP, {secs * 1000 / frames:.0f} ms/frame",
-        "ms_per_frame_at_sample": secs * 1000 / frames,
+        "sample": f"{frames} frames after {warmup} warm-up frames of the same workload at {w}x{h}, oracle host "
+                  f"C++ with OpenMP; median {med * 1000:.0f} ms/frame, {rays / frames / (w * h):.2f} rays/px",
+        "ms_per_frame_at_sample": med * 1000,
     }
 
 
@@ -168,6 +179,7 @@ def main():
         return abi.workload_params("conditional_restir", w, h, size)
 
     params = params_for(W, H, gscene.size)
+    params.trackPrefixStats = 1   # the neighbour statistics of PrefixAccelStruct::Stats, every frame
     cam = scenes.orbit_camera(scene, W, H)
     ren = runtime.Renderer(W, H, ctx, cam, gscene, params=params, seed=1)
     ren.enable_timing(True)
@@ -193,31 +205,47 @@ def main():
     trace_ms = 0.0
     trace_launches = 0
     tail_rays = np.zeros(2)
-    neigh = 0
     entries = 0
-    gpu_ms = 0.0
+    neigh_total = 0
+    neigh_max = 0
+    overflow = 0
+    frame_ms = []
     for i in range(a.steps):
         start.record()
         frame()
         end.record()
         end.synchronize()
-        gpu_ms += start.elapsed_ms(end)
+        frame_ms.append(start.elapsed_ms(end))
         st = ren.stats()
         pass_ms += np.asarray(st.passMs)
         pass_rays[:, 0] += np.asarray(st.passClosestRays)
         pass_rays[:, 1] += np.asarray(st.passShadowRays)
         total_rays += st.closestRays + st.shadowRays
         entries += st.numEntries
+        neigh_total += st.totalNeighCount
+        neigh_max = max(neigh_max, st.maxNeighCount)
+        overflow |= st.queueOverflow
         trace_ms += st.traceMs
         trace_launches += st.traceLaunches
         tail_rays += (st.tailClosestRays, st.tailShadowRays)
-    ms_per_frame = gpu_ms / a.steps
+    gpu_ms = float(np.sum(frame_ms))
+    ms_per_frame = float(np.median(frame_ms))
     mrays = total_rays / (gpu_ms / 1000.0) / 1e6
     P = W * H
+    # reuse density of the last timed frame: valid suffixes (front buffer) and the neighbour
+    # count of every pixel (0 for pixels that did not gather)
+    sv = ren.state()
+    suffix_flags = runtime.download(sv.suffixRes[sv.frontBufferIdx], P * 16, np.uint32)[2::4]
+    valid_suffix = float(np.mean((suffix_flags >> 16) & 1))
+    ncount = runtime.download(sv.neighCount, P * 4, np.uint32)
+    k = int(params.gatherM) - 1
+    hist = np.bincount(ncount, minlength=k + 1)
+    gathering = int((ncount > 0).sum())
+    neigh_per_frame = neigh_total / a.steps
     fg = abi.PASS_NAMES.index("final_gather")
     fg_ms = pass_ms[fg] / a.steps
-    fg_bytes = FINAL_GATHER_STATE_BYTES_PER_PX * P + (pass_rays[fg, 0] * CLOSEST_RAY_BYTES +
-                                                      pass_rays[fg, 1] * SHADOW_RAY_BYTES) / a.steps
+    fg_bytes = FINAL_GATHER_STATE_BYTES_PER_PX * P + NEIGHBOR_BYTES * neigh_per_frame + (
+        pass_rays[fg, 0] * CLOSEST_RAY_BYTES + pass_rays[fg, 1] * SHADOW_RAY_BYTES) / a.steps
     fg_gbs = fg_bytes / (fg_ms / 1000.0) / 1e9
     ray_bytes = (pass_rays[:, 0].sum() * CLOSEST_RAY_BYTES + pass_rays[:, 1].sum() * SHADOW_RAY_BYTES) / a.steps
     if trace_launches:
@@ -234,14 +262,15 @@ def main():
         avg_launch_ms = fg_ms
         bytes_per_launch = fg_bytes
     achieved = bytes_per_launch / (avg_launch_ms / 1000.0) / 1e9
-    frame_bytes = FRAME_STATE_BYTES_PER_PX * P + (pass_rays[:, 0].sum() * CLOSEST_RAY_BYTES +
-                                                  pass_rays[:, 1].sum() * SHADOW_RAY_BYTES) / a.steps
+    frame_bytes = FRAME_STATE_BYTES_PER_PX * P + NEIGHBOR_BYTES * neigh_per_frame + ray_bytes
     out = runtime.DeviceArray.numpy(ren.output)
     log("per-pass ms: " + ", ".join(f"{n} {m / a.steps:.3f}" for n, m in zip(abi.PASS_NAMES, pass_ms)))
     log("per-pass rays (closest, shadow): " + ", ".join(
         f"{n} {int(c / a.steps)}/{int(s / a.steps)}" for n, (c, s) in zip(abi.PASS_NAMES, pass_rays)))
-    log(f"rays/frame {total_rays / a.steps:.0f} ({total_rays / a.steps / P:.2f}/px), entries/frame "
-        f"{entries / a.steps:.0f}, mean radiance {float(out.mean()):.5f}, finite {bool(np.isfinite(out).all())}")
+    log(f"frame ms median {ms_per_frame:.3f} (min {min(frame_ms):.3f}, max {max(frame_ms):.3f}); rays/frame "
+        f"{total_rays / a.steps:.0f} ({total_rays / a.steps / P:.2f}/px), valid suffixes {valid_suffix:.3f}, "
+        f"entries/frame {entries / a.steps:.0f}, neighbours/frame {neigh_per_frame:.0f} (histogram {hist.tolist()}), "
+        f"mean radiance {float(out.mean()):.5f}, finite {bool(np.isfinite(out).all())}, queue overflow {overflow}")
 
     relmse = None
     if a.relmse_frames > 0 and a.grid == 16:
@@ -254,7 +283,7 @@ def main():
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
         cpu = cpu_baseline(scene, params_for, W, H, a.cpu_sample_scale, a.cpu_frames)
-        log(f"cpu baseline: {cpu['value']:.2f} Mrays/s on {cpu['cores']} threads")
+        log(f"cpu baseline: {cpu['value']:.2f} Mrays/s on {cpu['cores']} threads ({cpu['sample']})")
 
     line = {
         "metric": METRIC,
@@ -270,7 +299,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded procedural scene, orbiting camera)",
         "config": {
-            "workload": f"conditional_restir_{W}x{H}_objgrid{a.grid}",
+            "workload": f"conditional_restir_{W}x{H}_objgrid{a.grid}_tent",
             "exec_mode": a.exec,
             "final_phase_overlap": not a.no_overlap,
             "queue_grid": a.queue_grid or 8192,
@@ -279,11 +308,19 @@ def main():
             "triangles": scene.num_triangles,
             "max_path_len": int(params.maxPathLen),
             "gather_m": int(params.gatherM),
+            "ms_per_frame_median": round(ms_per_frame, 4),
+            "ms_per_frame_mean": round(gpu_ms / a.steps, 4),
             "rays_per_frame": int(total_rays / a.steps),
             "rays_per_pixel": round(total_rays / a.steps / P, 3),
+            "valid_suffix_fraction": round(valid_suffix, 4),
+            "entries_per_frame": int(entries / a.steps),
+            "neighbors_per_frame": int(neigh_per_frame),
+            "neighbors_per_gathering_pixel": round(float(ncount[ncount > 0].mean()) if gathering else 0.0, 3),
+            "neighbors_max": int(neigh_max),
+            "neighbor_histogram_last_frame": hist.tolist(),
+            "queue_overflow": int(overflow),
             "pass_ms": {n: round(m / a.steps, 4) for n, m in zip(abi.PASS_NAMES, pass_ms)},
             "frame_algorithmic_GBps": round(frame_bytes / (ms_per_frame / 1000.0) / 1e9, 2),
-            "entries_per_frame": int(entries / a.steps),
             "bvh_width": int(gscene.info.bvhWidth),
             "bvh_builder": a.bvh_builder,
             "bvh_max_leaf": a.bvh_max_leaf or 2,
@@ -309,6 +346,9 @@ def main():
                                                   for k, v in cpu.items() if k != "ms_per_frame_at_sample"},
     }
     print(json.dumps(line), flush=True)
+    if overflow:
+        log("a wavefront queue overflowed (cx_frame_stats.queueOverflow): the frames are not valid")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

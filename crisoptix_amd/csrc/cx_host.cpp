@@ -125,6 +125,9 @@ struct cx_renderer {
     hipStream_t side = nullptr;       // non-blocking, default priority
     hipEvent_t evFork = nullptr, evQuery = nullptr;
     hipStream_t fgSide = nullptr;     // set while frame_body launches the final gather
+    // The caller's stream is recording a graph (hipStreamIsCapturing): the frame forks no side
+    // stream and nothing may allocate, create streams or events, or synchronise.
+    bool callerCapture = false;
 };
 
 namespace {
@@ -275,6 +278,10 @@ int check_params(cx_renderer* r, const cx_launch_params& p) {
 
 int ensure_neighbors(cx_renderer* r, uint32_t k) {
     if (k <= r->kCap) return CX_OK;
+    if (r->callerCapture)
+        return set_err(r->ctx, CX_ERR_INVALID_ARG,
+                       "the neighbour buffers would grow inside the caller's stream capture: run one frame "
+                       "with these parameters eagerly first");
     if (r->neighIdx) {
         (void)hipFree(r->neighIdx);
         (void)hipFree(r->neighDist);
@@ -317,6 +324,10 @@ int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
         if (e == hipSuccess) e = x;
     };
     if (w.pixelAllocs.empty()) {
+        if (r->callerCapture)
+            return set_err(r->ctx, CX_ERR_INVALID_ARG,
+                           "the wavefront buffers would be allocated inside the caller's stream capture: run one "
+                           "frame eagerly first");
         const size_t P = r->P;
         A(dalloc(w.pixelAllocs, &w.prefixScratch, (size_t)PREFIX_CHUNKS * P));
         A(dalloc(w.pixelAllocs, &w.suffixScratch, (size_t)SUFFIX_CHUNKS * P));
@@ -343,6 +354,10 @@ int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
     static_assert(PG_CHUNKS >= SG_CHUNKS && PG_CHUNKS >= CP_CHUNKS && PG_CHUNKS >= FG_CHUNKS,
                   "path machines fit the prefix-generation context");
     if (jobs <= w.jobCap && words <= w.ctxWords) return CX_OK;
+    if (r->callerCapture)
+        return set_err(r->ctx, CX_ERR_INVALID_ARG,
+                       "the wavefront queues would grow inside the caller's stream capture: run one frame with "
+                       "these parameters eagerly first");
     if (!w.queueAllocs.empty()) {
         e = hipDeviceSynchronize();
         if (e != hipSuccess) return hip_err(r->ctx, e, "wavefront queue reallocation");
@@ -539,6 +554,26 @@ hipError_t wave_pass(cx_renderer* r, int pass, const FrameArgs& A, hipStream_t s
         }
         default: return hipErrorInvalidValue;
     }
+}
+
+}  // namespace
+
+namespace {
+
+// Whether the caller's stream is recording a graph.  Inside the library's own graph-mode capture
+// (r->capturing) the frame is not the caller's; a stream whose capture was invalidated is
+// refused.
+int detect_caller_capture(cx_renderer* r, void* stream) {
+    if (r->capturing) return CX_OK;
+    r->callerCapture = false;
+    if (!stream) return CX_OK;   // the legacy null stream cannot be captured
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    const hipError_t e = hipStreamIsCapturing((hipStream_t)stream, &cs);
+    if (e != hipSuccess) return hip_err(r->ctx, e, "hipStreamIsCapturing");
+    if (cs == hipStreamCaptureStatusInvalidated)
+        return set_err(r->ctx, CX_ERR_INVALID_ARG, "the caller's stream capture has been invalidated");
+    r->callerCapture = cs == hipStreamCaptureStatusActive;
+    return CX_OK;
 }
 
 }  // namespace
@@ -1108,7 +1143,9 @@ int cx_renderer_begin_frame(cx_renderer* r, const cx_camera* cam, int cameraChan
 int cx_pass_launch(cx_renderer* r, cx_pass pass, const cx_launch_params* params, void* stream) {
     if (!r || !params) return CX_ERR_INVALID_ARG;
     if (!r->out) return set_err(r->ctx, CX_ERR_INVALID_ARG, "no frame begun (cx_renderer_begin_frame)");
-    int prc = check_params(r, *params);
+    int prc = detect_caller_capture(r, stream);
+    if (prc != CX_OK) return prc;
+    prc = check_params(r, *params);
     if (prc != CX_OK) return prc;
     prc = ensure_neighbors(r, std::max(params->gatherM > 1 ? (uint32_t)(params->gatherM - 1) : 0u, 1u));
     if (prc != CX_OK) return prc;
@@ -1163,8 +1200,12 @@ int cx_pass_launch(cx_renderer* r, cx_pass pass, const cx_launch_params* params,
 
 namespace {
 
+// The side-stream fork of the final phase: not inside a caller's stream capture, where the
+// frame keeps to the caller's stream (the recorded graph is then a chain of kernel and copy
+// nodes, the form the caller-capture tests cover).
 bool overlap_final(const cx_renderer* r) {
-    return r->overlap && r->exec == CX_EXEC_WAVEFRONT && r->params.rendererType == CX_CONDITIONAL_RESTIR;
+    return r->overlap && !r->callerCapture && r->exec == CX_EXEC_WAVEFRONT &&
+           r->params.rendererType == CX_CONDITIONAL_RESTIR;
 }
 
 int ensure_side(cx_renderer* r) {
@@ -1336,7 +1377,21 @@ int graph_frame(cx_renderer* r, void* stream) {
 extern "C" {
 
 int cx_renderer_frame(cx_renderer* r, const cx_camera* cam, int cameraChanged, float* out, void* stream) {
-    int rc = cx_renderer_begin_frame(r, cam, cameraChanged, out);
+    if (!r) return CX_ERR_INVALID_ARG;
+    int rc = detect_caller_capture(r, stream);
+    if (rc != CX_OK) return rc;
+    if (r->callerCapture) {
+        if (r->graphMode || r->timing)
+            return set_err(r->ctx, CX_ERR_INVALID_ARG,
+                           "recording a frame into the caller's graph needs graph mode and timing off (they "
+                           "record or create their own graphs and events)");
+        // every buffer must exist already: these only check, and refuse growth, under capture
+        const cx_launch_params& lp = r->params;
+        rc = ensure_neighbors(r, std::max(lp.gatherM > 1 ? (uint32_t)(lp.gatherM - 1) : 0u, 1u));
+        if (rc == CX_OK && r->exec == CX_EXEC_WAVEFRONT) rc = ensure_wave(r, lp);
+        if (rc != CX_OK) return rc;
+    }
+    rc = cx_renderer_begin_frame(r, cam, cameraChanged, out);
     if (rc != CX_OK) return rc;
     if (r->graphMode && !r->timing) {
         r->evValid = false;

@@ -32,8 +32,10 @@ struct Queue {
     uint32_t* job;      // [cap]
     float4* ray;        // [2][cap] origin.xyz tmin | dir.xyz tmax
     uint4* res;         // [cap] closest: prim t u v; shadow: occluded
-    uint32_t* count;    // [QSTRIPES][QCOUNT_STRIDE]: closest count, shadow count of each stripe
-    uint32_t* overflow; // set when an append finds its stripe full (sizing makes that impossible)
+    uint32_t* count;    // [QSTRIPES][QCOUNT_STRIDE]: closest count, shadow count of each stripe (one
+                        // 64-bit word, closest in the low half)
+    uint32_t* overflow; // bit 0: an append found its stripe full (sizing makes that impossible);
+                        // bit 1: a tail kernel ended a job still running at its step bound
     uint32_t cap;       // QSTRIPES * stripeCap slots
     uint32_t stripeCap; // slots of a stripe: closest rays from its start, shadow rays from its end
     uint32_t jobs;      // most jobs the machine can have (sets the grid of its step kernels)
@@ -130,21 +132,26 @@ __device__ __forceinline__ uint32_t in_slot(const Queue& q, const QIndex& x, uin
 }
 
 // Wave-aggregated append to the wavefront's stripe; every lane of the wavefront must call it.
-// Returns the slot.  A full stripe (never reached: the host sizes stripes for the most
-// appends any stripe can receive) raises the overflow flag and wraps inside the stripe, so no
-// access leaves the queue.
+// Returns the slot.  The stripe's closest and shadow counts are the two halves of one 64-bit
+// word (closest low), advanced by one atomic, so every append sees both totals: a stripe whose
+// two ranges would meet (never reached: the host sizes stripes for the most appends any stripe
+// can receive) raises bit 0 of the overflow flag and wraps inside the stripe, so no access
+// leaves the queue.
 __device__ __forceinline__ uint32_t append(const Queue& q, uint32_t kind) {
     const uint64_t mc = __ballot(kind == EMIT_CLOSEST);
     const uint64_t ms = __ballot(kind == EMIT_SHADOW);
     const uint32_t lane = __lane_id();
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t stripe = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % QSTRIPES;
-    uint32_t* cnt = q.count + stripe * QCOUNT_STRIDE;
+    // 8-byte aligned: the counters start a 256-byte aligned allocation, QCOUNT_STRIDE words apart
+    unsigned long long* cnt = (unsigned long long*)(q.count + stripe * QCOUNT_STRIDE);
     uint32_t bc = 0, bs = 0;
-    if (lane == 0) {
-        if (mc) bc = atomicAdd(&cnt[0], (uint32_t)__popcll(mc));
-        if (ms) bs = atomicAdd(&cnt[1], (uint32_t)__popcll(ms));
-        if ((uint64_t)bc + __popcll(mc) + (uint64_t)bs + __popcll(ms) > q.stripeCap) atomicOr(q.overflow, 1u);
+    if (lane == 0 && (mc | ms)) {
+        const uint32_t nc = (uint32_t)__popcll(mc), ns = (uint32_t)__popcll(ms);
+        const unsigned long long old = atomicAdd(cnt, (unsigned long long)nc | ((unsigned long long)ns << 32));
+        bc = (uint32_t)old;
+        bs = (uint32_t)(old >> 32);
+        if ((uint64_t)bc + nc + (uint64_t)bs + ns > q.stripeCap) atomicOr(q.overflow, 1u);
     }
     bc = __shfl(bc, 0, 64);
     bs = __shfl(bs, 0, 64);

@@ -249,8 +249,11 @@ typedef struct cx_frame_stats {
        frame and their summed duration from HIP events around each launch (timing enabled). */
     uint32_t traceLaunches;
     float traceMs;
-    /* Nonzero if a wavefront queue append ever found its stripe full since the renderer was
-       created (the queues are sized so that it cannot; a diagnostic of that sizing). */
+    /* Nonzero if the wavefront queues ever failed since the renderer was created: bit 0, an
+       append found its queue stripe full (the queues are sized so that it cannot; a diagnostic of
+       that sizing); bit 1, a tail kernel ended a job that was still running at its step bound
+       (the bound is the longest trace chain of the machine, so it cannot either).  A frame with
+       either bit set has dropped work; bench.py refuses its numbers. */
     uint32_t queueOverflow;
     /* Rays traced in place by the tail kernels (cx_renderer_set_wave_rounds), included in the
        counts above; the traversal launches answered the rest. */
@@ -400,10 +403,10 @@ int cx_renderer_set_wave_rounds(cx_renderer* r, uint32_t rounds);
  * one-thread kernel at the start of the frame.  A frame enqueued inside a caller's stream capture
  * (a hipGraph of a whole optimisation step, say) then draws a new random number on every replay.
  * No counterpart in the reference; for callers that record frames into their own graphs.  The
- * adjoint of a frame reads the same block.  cx_pass_launch keeps the host generator.  Recording
- * by a caller is tested for ReSTIR PT and path-tracer frames (torch.cuda.graph, views on forked
- * streams); Conditional ReSTIR frames are recorded with cx_renderer_set_graph_mode, whose capture
- * is tested, as a caller's recording of them has not been. */
+ * adjoint of a frame reads the same block.  cx_pass_launch keeps the host generator.  A frame
+ * enqueued on a stream that the caller is capturing keeps to that stream (no side-stream fork)
+ * and must not allocate: run one frame with the same parameters eagerly first; graph mode and
+ * timing are refused there (CX_ERR_INVALID_ARG). */
 int cx_renderer_set_device_random(cx_renderer* r, int enable, uint64_t seed);
 
 /* Final-phase overlap (on by default; wavefront Conditional ReSTIR): cx_renderer_frame runs the

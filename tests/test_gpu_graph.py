@@ -148,3 +148,54 @@ def test_final_phase_overlap_is_invisible(graph, cx_runtime, cx_ctx, orc):
         assert (sa.totalNeighCount, sa.minNeighCount, sa.maxNeighCount) == \
                (sb.totalNeighCount, sb.minNeighCount, sb.maxNeighCount), f
     assert ca.sum() > 50
+
+
+def test_caller_graph_records_conditional_restir_frame(cx_runtime, cx_ctx):
+    """A Conditional ReSTIR frame recorded into the caller's graph (torch.cuda.graph on the
+    caller's stream): inside a caller's capture the frame keeps to that stream (no side-stream
+    fork) and allocates nothing, so each replay must equal an eager renderer's next frame bit for
+    bit (device-random frames, accumulation off, static camera), with eager frames and host
+    synchronisations between the replays.  Graph mode inside a caller's capture, and a frame
+    whose buffers would first be allocated there, are refused with an error before anything is
+    enqueued."""
+    import torch
+    W, H = 32, 24
+    sc = scenes.cornell_box(1, 8)
+    gs = cx_runtime.Scene(cx_ctx, sc)
+    p = abi.workload_params("conditional_restir", W, H, gs.size)
+    p.gatherRadius = 0.08
+    cam = scenes.cornell_camera(W, H)
+    cd = abi.CameraData.from_dict(cam.data())
+    eager = cx_runtime.Renderer(W, H, cx_ctx, cam, gs, params=p, seed=5)
+    rec = cx_runtime.Renderer(W, H, cx_ctx, cam, gs, params=p, seed=5)
+    for r in (eager, rec):
+        r.set_device_random(True, 11)
+    ref = [eager.launch_frame(cd, f == 0).numpy().copy() for f in range(5)]
+    assert np.isfinite(ref[-1]).all() and ref[-1].mean() > 0
+    stream = torch.cuda.Stream()
+    out = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    dst = cx_runtime.DevicePtr(out.data_ptr())
+    with torch.cuda.stream(stream):
+        rec.launch_frame(cd, True, output=dst, stream=stream.cuda_stream)   # eager: allocates every buffer
+    stream.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ref[0].view(np.uint32))
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        rec.launch_frame(cd, False, output=dst, stream=stream.cuda_stream)
+    side = torch.ones(1024, device="cuda")
+    for f in range(1, 5):
+        graph.replay()
+        stream.synchronize()
+        got = out.cpu().numpy()
+        print(f"caller graph replay {f}: mean {got.mean():.5f}", flush=True)
+        assert np.array_equal(got.view(np.uint32), ref[f].view(np.uint32)), f
+        side.mul_(1.0001)                     # eager work and a synchronisation between replays
+        torch.cuda.synchronize()
+    # refused inside a capture: graph mode, and a renderer that never ran a frame
+    fresh = cx_runtime.Renderer(W, H, cx_ctx, cam, gs, params=p, seed=5)
+    rec.set_graph_mode(True)
+    for r, what in ((rec, "graph mode"), (fresh, "eagerly first")):
+        g2 = torch.cuda.CUDAGraph()
+        with pytest.raises(cx_runtime.CxError, match=what):
+            with torch.cuda.graph(g2, stream=stream):
+                r.launch_frame(cd, False, output=dst, stream=stream.cuda_stream)
