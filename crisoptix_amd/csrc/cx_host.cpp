@@ -187,9 +187,6 @@ int entries_alloc(cx_ctx* ctx, uint32_t capacity, cx_entries** out) {
     en->ctx = ctx;
     en->capacity = capacity;
     const uint32_t n = std::max(1u, capacity);
-    en->T.leafSize = 8;
-    en->T.levels = tree_levels(n, en->T.leafSize);
-    const size_t nodes = (size_t(2) << en->T.levels) - 1;
     hipError_t e = hipSuccess;
     auto A = [&](hipError_t r) {
         if (e == hipSuccess) e = r;
@@ -201,20 +198,19 @@ int entries_alloc(cx_ctx* ctx, uint32_t capacity, cx_entries** out) {
     A(dalloc(en->allocs, &en->E.keysAlt, n));
     A(dalloc(en->allocs, &en->E.valsAlt, n));
     A(dalloc(en->allocs, &en->E.numValid, 1));
-    A(dalloc(en->allocs, &en->T.activeLevels, 1));
     A(dalloc(en->allocs, &en->T.entrySorted, n));
     A(dalloc(en->allocs, &en->T.entryPixel, n));
-    A(dalloc(en->allocs, &en->T.lo, nodes));
-    A(dalloc(en->allocs, &en->T.hi, nodes));
-    A(dalloc(en->allocs, &en->T.plo, nodes));
-    A(dalloc(en->allocs, &en->T.phi, nodes));
+    A(dalloc(en->allocs, &en->T.nodes, 4 * (size_t)n));
+    A(dalloc(en->allocs, &en->T.parent, n));
+    A(dalloc(en->allocs, &en->T.leafParent, n));
+    A(dalloc(en->allocs, &en->T.arrivals, n));
     en->T.numValid = en->E.numValid;
+    en->T.sortedKeys = en->E.keys;   // the sort leaves its output in keys / vals
     en->sortTempBytes = sort_temp_bytes(n);
     if (e == hipSuccess) {
         e = hipMalloc(&en->sortTemp, std::max<size_t>(en->sortTempBytes, 16));
         if (e == hipSuccess) en->allocs.push_back(en->sortTemp);
     }
-    if (e == hipSuccess) e = hipMemset(en->T.activeLevels, 0, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(en->E.numValid, 0, sizeof(uint32_t));
     if (e != hipSuccess) {
         free_all(en->allocs);

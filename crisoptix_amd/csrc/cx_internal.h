@@ -28,14 +28,6 @@ hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n
                              hipStream_t s);
 hipError_t launch_entries_query(const EntryTree& T, const float* q, uint32_t nq, uint32_t k, uint32_t* idx,
                                 float* dist, uint32_t* count, hipStream_t s);
-// Levels of the implicit entry tree over `capacity` entries in leaves of `leafSize`.
-inline uint32_t tree_levels(uint32_t capacity, uint32_t leafSize) {
-    const uint32_t leaves = capacity > leafSize ? (capacity + leafSize - 1) / leafSize : 1u;
-    uint32_t l = 0;
-    while ((1u << l) < leaves) l++;
-    return l;
-}
-
 // Morton quantisation of the entry positions over the box [mn, mx] (10 bits per axis).
 inline void set_quantisation(EntryBuild& E, const float* mn, const float* mx) {
     E.lo = V3{mn[0], mn[1], mn[2]};

@@ -602,70 +602,93 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* keysI
     }
 }
 
-// Levels of the tree over the valid entries: leaves = ceil(n / leafSize), rounded up to a
-// power of two (at least one leaf).
-__global__ void k_tree_prep(EntryTree T) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const uint32_t n = *T.numValid;
-    const uint32_t leaves = n > 0 ? (n + T.leafSize - 1) / T.leafSize : 1u;
-    uint32_t l = 0;
-    while ((1u << l) < leaves) l++;
-    *T.activeLevels = l;
+// ---- entry LBVH (Karras 2012) over the sorted valid entries ----
+
+// Longest common prefix of sorted keys i and j extended by their slot index (equal keys compare
+// their slots), or -1 outside [0, n).
+__device__ __forceinline__ int lbvh_delta(const uint32_t* keys, uint32_t n, int i, int j) {
+    if (j < 0 || j >= (int)n) return -1;
+    const uint32_t a = keys[i], b = keys[j];
+    if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+    return __clz(a ^ b);
 }
 
-// Leaves of the implicit entry tree: leaf j holds sorted entries [j*L, (j+1)*L) ∩ [0, nValid).
-// Boxes are the union of [pos - r, pos + r], padded outward so the float box test is
-// conservative for every entry whose distance test passes.
-__global__ void k_tree_leaves(EntryTree T, const uint32_t* sortedVals, const float4* entryPos, uint32_t numLeavesMax) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t leaves = 1u << *T.activeLevels;
-    if (j >= numLeavesMax || j >= leaves) return;
-    const uint32_t leafBase = leaves - 1u;
+// Sorted slot i: its entry record and pixel; the arrival counter of internal node i cleared.
+__global__ void k_lbvh_slots(EntryTree T, const uint32_t* sortedVals, const float4* entryPos) {
     const uint32_t n = *T.numValid;
-    const uint32_t b = j * T.leafSize;
-    const uint32_t e = min(b + T.leafSize, n);
-    float4 lo = float4{INFINITY, INFINITY, INFINITY, 0.0f}, hi = float4{-INFINITY, -INFINITY, -INFINITY, 0.0f};
-    float4 plo = lo, phi = hi;
-    for (uint32_t i = b; i < e; i++) {
-        const uint32_t pix = sortedVals[i];
-        const float4 q = entryPos[pix];
-        T.entrySorted[i] = q;
-        T.entryPixel[i] = pix;
-        plo = float4{fminf(plo.x, q.x), fminf(plo.y, q.y), fminf(plo.z, q.z), 0.0f};
-        phi = float4{fmaxf(phi.x, q.x), fmaxf(phi.y, q.y), fmaxf(phi.z, q.z), 0.0f};
-        const float r = q.w;
-        lo.x = fminf(lo.x, q.x - r);
-        lo.y = fminf(lo.y, q.y - r);
-        lo.z = fminf(lo.z, q.z - r);
-        hi.x = fmaxf(hi.x, q.x + r);
-        hi.y = fmaxf(hi.y, q.y + r);
-        hi.z = fmaxf(hi.z, q.z + r);
-    }
-    if (b < e) {
-        const float ex = fmaxf(fabsf(lo.x), fabsf(hi.x)) * 3.0517578125e-5f + 1e-20f;
-        const float ey = fmaxf(fabsf(lo.y), fabsf(hi.y)) * 3.0517578125e-5f + 1e-20f;
-        const float ez = fmaxf(fabsf(lo.z), fabsf(hi.z)) * 3.0517578125e-5f + 1e-20f;
-        lo.x -= ex; lo.y -= ey; lo.z -= ez;
-        hi.x += ex; hi.y += ey; hi.z += ez;
-    }
-    T.lo[leafBase + j] = lo;
-    T.hi[leafBase + j] = hi;
-    T.plo[leafBase + j] = plo;
-    T.phi[leafBase + j] = phi;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t pix = sortedVals[i];
+    T.entrySorted[i] = entryPos[pix];
+    T.entryPixel[i] = pix;
+    T.arrivals[i] = 0u;
 }
 
-// Inner nodes of level l (nodes [2^l - 1, 2^(l+1) - 1)), when l is above the leaf level.
-__global__ void k_tree_level(EntryTree T, uint32_t level, uint32_t levelBase, uint32_t count) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= count || level >= *T.activeLevels) return;
-    const uint32_t node = levelBase + j;
-    const uint32_t c0 = 2 * node + 1, c1 = 2 * node + 2;
-    const float4 l0 = T.lo[c0], l1 = T.lo[c1], h0 = T.hi[c0], h1 = T.hi[c1];
-    T.lo[node] = float4{fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.0f};
-    T.hi[node] = float4{fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.0f};
-    const float4 pl0 = T.plo[c0], pl1 = T.plo[c1], ph0 = T.phi[c0], ph1 = T.phi[c1];
-    T.plo[node] = float4{fminf(pl0.x, pl1.x), fminf(pl0.y, pl1.y), fminf(pl0.z, pl1.z), 0.0f};
-    T.phi[node] = float4{fmaxf(ph0.x, ph1.x), fmaxf(ph0.y, ph1.y), fmaxf(ph0.z, ph1.z), 0.0f};
+// Internal node i: its key range and split (the paper's determineRange and findSplit), the
+// children's references (a range of at most ENTRY_LEAF_MAX entries becomes a leaf range) and the
+// parent links of the full binary tree that the bounds pass climbs.
+__global__ void k_lbvh_nodes(EntryTree T) {
+    const uint32_t n = *T.numValid;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (n < 2 || i >= (int)n - 1) return;
+    const uint32_t* K = T.sortedKeys;
+    const int d = lbvh_delta(K, n, i, i + 1) > lbvh_delta(K, n, i, i - 1) ? 1 : -1;
+    const int dmin = lbvh_delta(K, n, i, i - d);
+    int lmax = 2;
+    while (lbvh_delta(K, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+        if (lbvh_delta(K, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = lbvh_delta(K, n, i, j);
+    int s = 0;
+    for (int t = (l + 1) / 2;; t = (t + 1) / 2) {
+        if (lbvh_delta(K, n, i, i + (s + t) * d) > dnode) s += t;
+        if (t == 1) break;
+    }
+    const int g = i + s * d + (d < 0 ? -1 : 0);
+    const int first = min(i, j), last = max(i, j);
+    const uint32_t nl = (uint32_t)(g - first + 1), nr = (uint32_t)(last - g);
+    const int refL = nl <= ENTRY_LEAF_MAX ? (int)~(((uint32_t)first << 4) | nl) : g;
+    const int refR = nr <= ENTRY_LEAF_MAX ? (int)~(((uint32_t)(g + 1) << 4) | nr) : g + 1;
+    float4* nd = T.nodes + 4 * (size_t)i;
+    nd[1].w = __int_as_float(refL);
+    nd[3].w = __int_as_float(refR);
+    const uint32_t up = (uint32_t)i << 1;
+    if (first == g) T.leafParent[g] = up; else T.parent[g] = up;
+    if (last == g + 1) T.leafParent[g + 1] = up | 1u; else T.parent[g + 1] = up | 1u;
+    if (i == 0) T.parent[0] = 0xFFFFFFFFu;
+}
+
+// Bounds, bottom-up: a thread per sorted entry writes its box into its parent's child slot and
+// climbs while it is the second child to arrive at a node (the first one stops there), so every
+// node's two slots are complete when its box is formed.  The fence before each arrival makes
+// the slot visible device-wide before the counter; the fence after it makes the sibling's slot
+// visible to this thread.  Min and max are exact, so the result does not depend on the order.
+__global__ void k_lbvh_bounds(EntryTree T) {
+    const uint32_t n = *T.numValid;
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < 2 || e >= n) return;
+    const float4 p = T.entrySorted[e];
+    float4 lo = float4{p.x, p.y, p.z, p.w}, hi = float4{p.x, p.y, p.z, 0.0f};
+    uint32_t slot = T.leafParent[e];
+    while (true) {
+        const uint32_t node = slot >> 1, side = slot & 1u;
+        float4* nd = T.nodes + 4 * (size_t)node;
+        nd[2 * side] = lo;   // lo.xyz and the largest radius
+        float* h = (float*)(nd + 2 * side + 1);
+        h[0] = hi.x;         // hi.xyz; .w keeps the child reference
+        h[1] = hi.y;
+        h[2] = hi.z;
+        __threadfence();
+        if (atomicAdd(&T.arrivals[node], 1u) == 0u) return;
+        __threadfence();
+        const float4 a0 = nd[0], a1 = nd[1], c0 = nd[2], c1 = nd[3];
+        lo = float4{fminf(a0.x, c0.x), fminf(a0.y, c0.y), fminf(a0.z, c0.z), fmaxf(a0.w, c0.w)};
+        hi = float4{fmaxf(a1.x, c1.x), fmaxf(a1.y, c1.y), fmaxf(a1.z, c1.z), 0.0f};
+        slot = T.parent[node];
+        if (slot == 0xFFFFFFFFu) return;
+    }
 }
 
 template <int KR>
@@ -678,8 +701,11 @@ __global__ __launch_bounds__(BLOCK) void k_entries_query(EntryTree T, const floa
         idx[(size_t)j * k + i] = 0xFFFFFFFFu;
         dist[(size_t)j * k + i] = 0.0f;
     }
-    count[j] = entries_query<KR>(T, V3{q[3 * j], q[3 * j + 1], q[3 * j + 2]}, k, s_stack + threadIdx.x, BLOCK,
-                                 idx + (size_t)j * k, dist + (size_t)j * k);
+    QueryStack<STACK_DEPTH / 2> st;
+    st.lds = s_stack + threadIdx.x;
+    st.stride = BLOCK;
+    count[j] = entries_query<KR>(T, V3{q[3 * j], q[3 * j + 1], q[3 * j + 2]}, k, st, idx + (size_t)j * k,
+                                 dist + (size_t)j * k);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -707,7 +733,10 @@ __device__ __forceinline__ V3 gather_canon(const FrameArgs& A, TraceCtx& tc, uin
     const Inter last = rehydrate(A.S, cs.lastPrefix);
     uint32_t* nIdx = A.neighIdx + (size_t)p * k;
     float* nDist = A.neighDist + (size_t)p * k;
-    const uint32_t neighCount = entries_query<KR>(A.tree, last.pos, k, tc.stack, tc.stride, nIdx, nDist);
+    QueryStack<STACK_DEPTH / 2> qst;
+    qst.lds = tc.stack;
+    qst.stride = tc.stride;
+    const uint32_t neighCount = entries_query<KR>(A.tree, last.pos, k, qst, nIdx, nDist);
     A.neighCount[p] = neighCount;
     if (A.lp.trackPrefixStats) neighbor_stat(A.counters, neighCount);
     const uint4* sfront = A.suffixRes[A.front];
@@ -752,7 +781,10 @@ __device__ __forceinline__ V3 gather_not_canon(const FrameArgs& A, TraceCtx& tc,
     if (!trace_prefix_fg(A, tc, thr, pp, last, origin, dir, (uint32_t)A.lp.prefixLen, rng)) return v3(0.0f);
     uint32_t* nIdx = A.neighIdx + (size_t)p * k;
     float* nDist = A.neighDist + (size_t)p * k;
-    const uint32_t neighCount = entries_query<KR>(A.tree, last.pos, k, tc.stack, tc.stride, nIdx, nDist);
+    QueryStack<STACK_DEPTH / 2> qst;
+    qst.lds = tc.stack;
+    qst.stride = tc.stride;
+    const uint32_t neighCount = entries_query<KR>(A.tree, last.pos, k, qst, nIdx, nDist);
     A.neighCount[p] = neighCount;
     if (A.lp.trackPrefixStats) neighbor_stat(A.counters, neighCount);
     const uint4* sfront = A.suffixRes[A.front];
@@ -990,24 +1022,17 @@ hipError_t launch_key_sort(uint32_t* keys, uint32_t* vals, uint32_t* keysAlt, ui
     return hipGetLastError();
 }
 
-// Sorts the valid entries by Morton key and builds the implicit tree over them bottom-up, one
-// launch per level (levels at or below the tree's leaf level return at once).  n = capacity.
+// Sorts the valid entries by Morton key and builds the LBVH over them: slots, nodes, then the
+// bounds pass.  Grids cover the capacity n; every kernel reads the valid count on the device.
 hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n, void* temp, size_t tempBytes,
                              hipStream_t s) {
-    if (n == 0) {
-        hipLaunchKernelGGL(k_tree_prep, dim3(1), dim3(64), 0, s, T);
-        return hipGetLastError();
-    }
+    if (n == 0) return hipGetLastError();   // numValid was cleared when the entries were written
     const hipError_t e = launch_key_sort(E.keys, E.vals, E.keysAlt, E.valsAlt, n, E.numValid, temp, tempBytes, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_tree_prep, dim3(1), dim3(64), 0, s, T);   // needs the valid count
-    const uint32_t leaves = 1u << T.levels;
-    hipLaunchKernelGGL(k_tree_leaves, dim3((leaves + 255) / 256), dim3(256), 0, s, T, E.vals, E.pos, leaves);
-    for (int l = (int)T.levels - 1; l >= 0; l--) {
-        const uint32_t cnt = 1u << l;
-        const uint32_t base = cnt - 1u;
-        hipLaunchKernelGGL(k_tree_level, dim3((cnt + 255) / 256), dim3(256), 0, s, T, (uint32_t)l, base, cnt);
-    }
+    const dim3 g((n + 255) / 256), b(256);
+    hipLaunchKernelGGL(k_lbvh_slots, g, b, 0, s, T, E.vals, E.pos);
+    hipLaunchKernelGGL(k_lbvh_nodes, g, b, 0, s, T);
+    hipLaunchKernelGGL(k_lbvh_bounds, g, b, 0, s, T);
     return hipGetLastError();
 }
 

@@ -17,26 +17,18 @@ namespace cx {
 namespace {
 
 constexpr uint32_t NOALLOC_MAX_ENTRIES = 65536;   // a 256 x 256 frame's prefix entries
-constexpr uint32_t NOALLOC_LEAF = 8;              // entries per tree leaf, as cx_entries uses
 constexpr uint32_t NOALLOC_MAX_DEVICES = 64;
 
 constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-constexpr uint32_t max_levels() {
-    uint32_t l = 0;
-    while ((1u << l) < NOALLOC_MAX_ENTRIES / NOALLOC_LEAF) l++;
-    return l;
-}
-
 // Byte offsets of the buffers inside the array.
 struct Layout {
-    size_t pos, valid, keys, vals, keysAlt, valsAlt, numValid, activeLevels;
-    size_t entrySorted, entryPixel, lo, hi, plo, phi, sortTemp, sortTempBytes, total;
+    size_t pos, valid, keys, vals, keysAlt, valsAlt, numValid;
+    size_t entrySorted, entryPixel, nodes, parent, leafParent, arrivals, sortTemp, sortTempBytes, total;
 };
 
 constexpr Layout make_layout() {
     constexpr size_t n = NOALLOC_MAX_ENTRIES;
-    constexpr size_t nodes = (size_t(2) << max_levels()) - 1;
     // the sort's per-digit tile counts and digit totals (sort_temp_bytes: tiles of 1024 keys)
     constexpr size_t sortBytes = (256 * ((n + 1023) / 1024) + 256) * sizeof(uint32_t);
     Layout L{};
@@ -53,13 +45,12 @@ constexpr Layout make_layout() {
     L.keysAlt = take(n * sizeof(uint32_t));
     L.valsAlt = take(n * sizeof(uint32_t));
     L.numValid = take(sizeof(uint32_t));
-    L.activeLevels = take(sizeof(uint32_t));
     L.entrySorted = take(n * sizeof(float4));
     L.entryPixel = take(n * sizeof(uint32_t));
-    L.lo = take(nodes * sizeof(float4));
-    L.hi = take(nodes * sizeof(float4));
-    L.plo = take(nodes * sizeof(float4));
-    L.phi = take(nodes * sizeof(float4));
+    L.nodes = take(4 * n * sizeof(float4));
+    L.parent = take(n * sizeof(uint32_t));
+    L.leafParent = take(n * sizeof(uint32_t));
+    L.arrivals = take(n * sizeof(uint32_t));
     L.sortTemp = take(sortBytes);
     L.sortTempBytes = sortBytes;
     L.total = o;
@@ -67,7 +58,7 @@ constexpr Layout make_layout() {
 }
 
 constexpr Layout LAYOUT = make_layout();
-static_assert(LAYOUT.total <= (size_t(5) << 20), "static scratch stays below 5 MiB");
+static_assert(LAYOUT.total <= (size_t(9) << 20), "static scratch stays below 9 MiB");
 
 std::mutex g_mu;
 void* g_ptr[NOALLOC_MAX_DEVICES] = {};   // the array's address on each device, once taken
@@ -132,18 +123,17 @@ extern "C" int cx_entries_search_static(cx_ctx* ctx, const float* pos, const flo
     EntryTree T{};
     T.entrySorted = (float4*)(b + LAYOUT.entrySorted);
     T.entryPixel = (uint32_t*)(b + LAYOUT.entryPixel);
-    T.lo = (float4*)(b + LAYOUT.lo);
-    T.hi = (float4*)(b + LAYOUT.hi);
-    T.plo = (float4*)(b + LAYOUT.plo);
-    T.phi = (float4*)(b + LAYOUT.phi);
+    T.nodes = (float4*)(b + LAYOUT.nodes);
+    T.parent = (uint32_t*)(b + LAYOUT.parent);
+    T.leafParent = (uint32_t*)(b + LAYOUT.leafParent);
+    T.arrivals = (uint32_t*)(b + LAYOUT.arrivals);
+    T.sortedKeys = E.keys;   // the sort leaves its output in keys / vals
     T.numValid = E.numValid;
-    T.activeLevels = (uint32_t*)(b + LAYOUT.activeLevels);
-    T.leafSize = NOALLOC_LEAF;
-    T.levels = tree_levels(n > 0 ? n : 1u, NOALLOC_LEAF);   // <= max_levels(): the node arrays fit
     const hipStream_t s = (hipStream_t)stream;
     // every value the kernels read is written earlier in this call: the entry records for all
-    // n slots, the valid count (cleared by a fill kernel, then set by the sort's first pass),
-    // the active levels and every node of the active tree
+    // n slots, the valid count (cleared by a fill kernel, then set by the sort's first pass), the
+    // sorted slots, the arrival counters (cleared by the slot kernel) and the nodes, parents and
+    // bounds of the tree over the valid entries
     hipError_t e = launch_entries_from_arrays(pos, radius, valid, n, E, s);
     if (e == hipSuccess) e = launch_tree_build(E, T, n, b + LAYOUT.sortTemp, LAYOUT.sortTempBytes, s);
     if (e == hipSuccess) e = launch_entries_query(T, query, nq, k, idx, dist, count, s);

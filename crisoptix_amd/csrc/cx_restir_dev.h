@@ -164,20 +164,32 @@ struct EntryBuild {
     V3 lo, scale;         // Morton quantisation: (q - lo) * scale in [0, 1023]
 };
 
-// Implicit complete binary tree over Morton-sorted valid entries: leaf j holds sorted
-// entries [j * leafSize, (j + 1) * leafSize); node i has children 2i+1, 2i+2; the leaf level
-// starts at node 2^levels - 1.  Boxes are padded unions of the entries' radius boxes.
+// Linear BVH over the Morton-sorted valid entries (Karras, "Maximizing Parallelism in the
+// Construction of BVHs, Octrees, and k-d Trees", HPG 2012): internal node i of the n - 1 covers a
+// range of sorted entries whose keys share a prefix, split where the next key bit changes, so a
+// node's entries form one aligned Morton cell (ties between equal keys split by slot).  Each
+// internal node holds both children, 4 float4 = 64 bytes:
+//   {lo0.xyz, rmax0} {hi0.xyz, ref0} {lo1.xyz, rmax1} {hi1.xyz, ref1}
+// lo/hi are the exact bounds of the child's entry positions and rmax the largest entry radius
+// among them; ref >= 0 is an internal node, ref < 0 a leaf range of sorted entries,
+// ~ref = (first << 4) | count with count in 1..ENTRY_LEAF_MAX (a subtree of at most that many
+// entries is scanned, not walked).  Node 0 is the root when more than ENTRY_LEAF_MAX entries are
+// valid; with fewer the query scans them all.
+static constexpr uint32_t ENTRY_LEAF_MAX = 8;
+// Deepest walk: keys are 30-bit Morton codes extended by the slot index, and the common-prefix
+// length of a node's range grows strictly from parent to child, so a root-to-leaf path has at most
+// 62 internal nodes and a depth-first walk defers at most one sibling per node of its path.
+static constexpr int ENTRY_STACK = 64;
+
 struct EntryTree {
-    float4* entrySorted;  // [capacity] pos.xyz, radius
-    uint32_t* entryPixel; // [capacity]
-    float4* lo;           // [2^(levels+1) - 1] union of the entries' [pos - r, pos + r], padded
-    float4* hi;
-    float4* plo;          // [2^(levels+1) - 1] exact bounds of the entries' positions
-    float4* phi;
+    float4* entrySorted;   // [capacity] pos.xyz, radius, in key order
+    uint32_t* entryPixel;  // [capacity] entry index (pixel) of each sorted slot
+    float4* nodes;         // [capacity][4] internal nodes (n - 1 of them are used)
+    uint32_t* parent;      // [capacity] internal node -> (parent << 1) | side; the root 0xFFFFFFFF
+    uint32_t* leafParent;  // [capacity] sorted slot -> (parent << 1) | side
+    uint32_t* arrivals;    // [capacity] children whose bounds have reached each internal node
+    const uint32_t* sortedKeys;  // [capacity] keys in sorted order (the sort's output)
     const uint32_t* numValid;
-    uint32_t* activeLevels;  // device scalar: levels of the tree over the valid entries
-    uint32_t levels;         // levels at full capacity (sizes of the node arrays)
-    uint32_t leafSize;
 };
 
 // ---- frame parameters passed by value to every pass kernel ----
@@ -597,73 +609,98 @@ __device__ __forceinline__ float box_distance(V3 q, float4 lo, float4 hi) {
     return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
-__device__ __forceinline__ bool in_box(V3 q, float4 lo, float4 hi) {
-    return q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y && q.z >= lo.z && q.z <= hi.z;
-}
+// Deferred siblings of the entry query: (child reference, its distance bound).  The first SHORT
+// entries live in a per-thread LDS column (two ints per entry, lds[(2 i) * stride] and
+// lds[(2 i + 1) * stride]), deeper ones in a per-lane private array.
+template <int SHORT>
+struct QueryStack {
+    int* lds;
+    int stride;
+    uint2 ovf[ENTRY_STACK - SHORT > 0 ? ENTRY_STACK - SHORT : 1];
+    __device__ __forceinline__ void put(int i, int ref, float bound) {
+        if (SHORT >= ENTRY_STACK || i < SHORT) {
+            lds[(2 * i) * stride] = ref;
+            lds[(2 * i + 1) * stride] = __float_as_int(bound);
+        } else {
+            ovf[i - SHORT] = uint2{(uint32_t)ref, __float_as_uint(bound)};
+        }
+    }
+    __device__ __forceinline__ void get(int i, int& ref, float& bound) const {
+        if (SHORT >= ENTRY_STACK || i < SHORT) {
+            ref = lds[(2 * i) * stride];
+            bound = __int_as_float(lds[(2 * i + 1) * stride]);
+        } else {
+            const uint2 v = ovf[i - SHORT];
+            ref = (int)v.x;
+            bound = __uint_as_float(v.y);
+        }
+    }
+};
 
-// Depth-first k-nearest search: a node is entered when q lies in its radius box (only then
-// can one of its entries have |q - p| <= r) and, once k candidates are held, when the lower
-// bound of its entries' distances does not exceed the k-th distance (an entry farther than
+// Depth-first k-nearest search over the entry LBVH.  A child is entered when the distance bound
+// of its positions is within the largest radius of its entries (only then can one of them have
+// |q - p| <= r) and, once k candidates are held, within the k-th distance (an entry farther than
 // that cannot enter the set; one at exactly that distance can, through the index tie-break).
-// The nearer child (by that bound) is visited first.  The set found is the same as with an
-// exhaustive visit of all candidates.
-template <int KR>
-__device__ __forceinline__ uint32_t entries_query(const EntryTree& T, V3 q, uint32_t k, int* stack, int stride,
-                                                  uint32_t* outIdx, float* outDist) {
+// The nearer child (by that bound) is visited first, the other deferred with its bound and
+// dropped when it is popped if the k-th distance found since has passed it.  The set found is
+// the same as with an exhaustive visit of all candidates.
+template <int KR, typename Stack>
+__device__ __forceinline__ uint32_t entries_query(const EntryTree& T, V3 q, uint32_t k, Stack& st, uint32_t* outIdx,
+                                                  float* outDist) {
     TopK<KR> top;
     top.init();
     const uint32_t n = *T.numValid;
     if (k == 0 || n == 0 || isnan(q.x) || isnan(q.y) || isnan(q.z)) return 0;
-    const uint32_t leafBase = (1u << *T.activeLevels) - 1u;
     float kth = INFINITY;   // k-th smallest distance once k candidates are held
-    auto admits = [&](uint32_t node, float& bound) -> bool {
-        if (!in_box(q, T.lo[node], T.hi[node])) return false;
-        bound = box_distance(q, T.plo[node], T.phi[node]);
-        return bound <= kth;
-    };
-    float b0 = 0.0f;
-    if (!admits(0u, b0)) return 0;
-    int sp = 0;
-    uint32_t node = 0;
-    while (true) {
-        if (node >= leafBase) {
-            const uint32_t j = node - leafBase;
-            const uint32_t b = j * T.leafSize, e = min(b + T.leafSize, n);
-            for (uint32_t s = b; s < e; s++) {
-                const float4 ep = T.entrySorted[s];
-                const float dd = distance(q, V3{ep.x, ep.y, ep.z});
-                if (dd > ep.w || dd > kth) continue;
-                top.insert(dd, T.entryPixel[s], k);
-                if (top.count == k) {
+    auto scan = [&](uint32_t first, uint32_t cnt) {
+        for (uint32_t s = first; s < first + cnt; s++) {
+            const float4 ep = T.entrySorted[s];
+            const float dd = distance(q, V3{ep.x, ep.y, ep.z});
+            if (!(dd <= ep.w) || dd > kth) continue;   // a NaN distance or radius is no candidate
+            top.insert(dd, T.entryPixel[s], k);
+            if (top.count == k) {
 #pragma unroll
-                    for (int m = 0; m < KR; m++)
-                        if ((uint32_t)m + 1u == k) kth = top.d[m];
+                for (int m = 0; m < KR; m++)
+                    if ((uint32_t)m + 1u == k) kth = top.d[m];
+            }
+        }
+    };
+    if (n <= ENTRY_LEAF_MAX) {
+        scan(0u, n);
+    } else {
+        int sp = 0;
+        int ref = 0;
+        while (true) {
+            if (ref >= 0) {
+                const float4* nd = T.nodes + 4 * (size_t)ref;
+                const float4 lo0 = nd[0], hi0 = nd[1], lo1 = nd[2], hi1 = nd[3];
+                const float b0 = box_distance(q, lo0, hi0), b1 = box_distance(q, lo1, hi1);
+                const bool ok0 = b0 <= lo0.w && b0 <= kth, ok1 = b1 <= lo1.w && b1 <= kth;
+                const int r0 = __float_as_int(hi0.w), r1 = __float_as_int(hi1.w);
+                if (ok0 && ok1) {
+                    const bool near0 = b0 <= b1;
+                    st.put(sp, near0 ? r1 : r0, near0 ? b1 : b0);
+                    ++sp;
+                    ref = near0 ? r0 : r1;
+                    continue;
                 }
+                if (ok0 || ok1) {
+                    ref = ok0 ? r0 : r1;
+                    continue;
+                }
+            } else {
+                const uint32_t code = ~(uint32_t)ref;
+                scan(code >> 4, code & 15u);
             }
-        } else {
-            const uint32_t c0 = 2 * node + 1, c1 = 2 * node + 2;
-            float bd0 = 0.0f, bd1 = 0.0f;
-            const bool ok0 = admits(c0, bd0), ok1 = admits(c1, bd1);
-            if (ok0 && ok1) {
-                const bool near0 = bd0 <= bd1;
-                stack[sp * stride] = (int)(near0 ? c1 : c0);
-                ++sp;
-                node = near0 ? c0 : c1;
-                continue;
+            bool found = false;
+            while (sp > 0 && !found) {
+                --sp;
+                float b;
+                st.get(sp, ref, b);
+                found = b <= kth;
             }
-            if (ok0 || ok1) {
-                node = ok0 ? c0 : c1;
-                continue;
-            }
+            if (!found) break;
         }
-        // Pop the next deferred sibling that the k-th distance found since has not ruled out.
-        bool found = false;
-        while (sp > 0 && !found) {
-            --sp;
-            node = (uint32_t)stack[sp * stride];
-            found = box_distance(q, T.plo[node], T.phi[node]) <= kth;
-        }
-        if (!found) break;
     }
 #pragma unroll
     for (int j = 0; j < KR; j++) {

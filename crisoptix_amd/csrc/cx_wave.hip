@@ -777,12 +777,19 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_init(WaveArgs W) {
     }
 }
 
+// Deferred siblings of the gather's entry query kept in LDS per thread; deeper ones (rare: the
+// walk defers one per level only where both children stay in reach) go to private memory.
+static constexpr int GATHER_QUERY_SHORT = 8;
+
 template <int KR>
 __device__ __forceinline__ uint32_t query_neighbors(const WaveArgs& W, uint32_t p, V3 q, int* stack, int stride) {
     const uint32_t k = W.A.k;
     uint32_t* nIdx = W.A.neighIdx + (size_t)p * k;
     float* nDist = W.A.neighDist + (size_t)p * k;
-    const uint32_t count = entries_query<KR>(W.A.tree, q, k, stack, stride, nIdx, nDist);
+    QueryStack<GATHER_QUERY_SHORT> st;
+    st.lds = stack;
+    st.stride = stride;
+    const uint32_t count = entries_query<KR>(W.A.tree, q, k, st, nIdx, nDist);
     W.A.neighCount[p] = count;
     return count;
 }
@@ -796,9 +803,8 @@ __device__ __forceinline__ void neighbor_stats(const WaveArgs& W, uint32_t count
 // (final_gather.cu:209-246).  Writes the
 // neighbours, the mask of those with a valid suffix (validMask) and pixFlags bit 3 (queried).
 // The tree walk is a chain of dependent fetches, so it runs apart from the register-heavy shift
-// starts (k_w_gather_setup), at more waves per SIMD.  The depth-first walk defers at most one
-// sibling per internal level of the path it is on, so its LDS stack (dynamic shared memory,
-// gather_query_lds bytes) holds the tree's allocated levels, not a fixed 32 entries.
+// starts (k_w_gather_setup), at more waves per SIMD; its deferred siblings take GATHER_QUERY_SHORT
+// stack entries of LDS per thread (dynamic shared memory, gather_query_lds bytes).
 template <int KR>
 __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(KR <= 8 ? 8 : 1, 8))) void k_w_gather_query(
     WaveArgs W) {
@@ -1216,11 +1222,8 @@ hipError_t wave_tail(int step, const WaveArgs& W, uint32_t maxSteps, int pass, h
     }
 }
 
-// LDS of one k_w_gather_query block: a stack entry per level of the entry tree's allocation
-// (activeLevels never exceeds it) for each thread.
-static size_t gather_query_lds(const WaveArgs& W) {
-    return (size_t)std::max(1u, W.A.tree.levels) * WBLOCK * sizeof(int);
-}
+// LDS of one k_w_gather_query block: GATHER_QUERY_SHORT stack entries (two ints each) per thread.
+static size_t gather_query_lds(const WaveArgs&) { return (size_t)2 * GATHER_QUERY_SHORT * WBLOCK * sizeof(int); }
 
 hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStream_t s) {
     const dim3 b(WBLOCK);

@@ -322,7 +322,7 @@ int cx_entries_query(cx_ctx* ctx, const cx_entries* entries, const float* d_quer
 /* No counterpart in the reference: cx_entries_build followed by cx_entries_query in one call that
  * allocates no device memory, for callers that cannot allocate (code under hipGraph capture,
  * serving loops, harnesses that forbid hipMalloc).  The entry records, sort buffers and tree
- * live in a fixed device array of the library (about 4.4 MB per device, held until the process
+ * live in a fixed device array of the library (about 8.4 MB per device, held until the process
  * ends) whose address cx_ctx_create takes on its device; the context's device must be current.
  * Arguments mean what they mean for cx_entries_build / cx_entries_query, and the outputs are
  * theirs bit for bit.  n above 65536 (a 256 x 256 frame's entries) returns CX_ERR_INVALID_ARG.
