@@ -201,9 +201,11 @@ int entries_alloc(cx_ctx* ctx, uint32_t capacity, cx_entries** out) {
     A(dalloc(en->allocs, &en->T.entrySorted, n));
     A(dalloc(en->allocs, &en->T.entryPixel, n));
     A(dalloc(en->allocs, &en->T.nodes, 4 * (size_t)n));
-    A(dalloc(en->allocs, &en->T.parent, n));
-    A(dalloc(en->allocs, &en->T.leafParent, n));
-    A(dalloc(en->allocs, &en->T.arrivals, n));
+    en->T.segLevelsCap = entry_seg_levels(n);
+    const size_t segNodes = (size_t(2) << en->T.segLevelsCap) - 1;
+    A(dalloc(en->allocs, &en->T.segLo, segNodes));
+    A(dalloc(en->allocs, &en->T.segHi, segNodes));
+    A(dalloc(en->allocs, &en->T.segLevels, 1));
     en->T.numValid = en->E.numValid;
     en->T.sortedKeys = en->E.keys;   // the sort leaves its output in keys / vals
     en->sortTempBytes = sort_temp_bytes(n);

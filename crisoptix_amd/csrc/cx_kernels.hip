@@ -613,7 +613,7 @@ __device__ __forceinline__ int lbvh_delta(const uint32_t* keys, uint32_t n, int 
     return __clz(a ^ b);
 }
 
-// Sorted slot i: its entry record and pixel; the arrival counter of internal node i cleared.
+// Sorted slot i: its entry record and pixel.
 __global__ void k_lbvh_slots(EntryTree T, const uint32_t* sortedVals, const float4* entryPos) {
     const uint32_t n = *T.numValid;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -621,12 +621,81 @@ __global__ void k_lbvh_slots(EntryTree T, const uint32_t* sortedVals, const floa
     const uint32_t pix = sortedVals[i];
     T.entrySorted[i] = entryPos[pix];
     T.entryPixel[i] = pix;
-    T.arrivals[i] = 0u;
 }
 
-// Internal node i: its key range and split (the paper's determineRange and findSplit), the
-// children's references (a range of at most ENTRY_LEAF_MAX entries becomes a leaf range) and the
-// parent links of the full binary tree that the bounds pass climbs.
+__device__ __forceinline__ void grow_entry(float4& lo, float4& hi, float4 p) {
+    lo = float4{fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z), fmaxf(lo.w, p.w)};
+    hi = float4{fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z), 0.0f};
+}
+
+__device__ __forceinline__ void grow_box(float4& lo, float4& hi, float4 l2, float4 h2) {
+    lo = float4{fminf(lo.x, l2.x), fminf(lo.y, l2.y), fminf(lo.z, l2.z), fmaxf(lo.w, l2.w)};
+    hi = float4{fmaxf(hi.x, h2.x), fmaxf(hi.y, h2.y), fmaxf(hi.z, h2.z), 0.0f};
+}
+
+// Levels of the range tree over the valid entries.
+__global__ void k_seg_prep(EntryTree T) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *T.segLevels = entry_seg_levels(*T.numValid);
+}
+
+// Range-tree leaf j: sorted entries [8j, 8j + 8) that are valid (an empty leaf keeps an empty box).
+__global__ void k_seg_leaves(EntryTree T, uint32_t leavesCap) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t leaves = 1u << *T.segLevels;
+    if (j >= leavesCap || j >= leaves) return;
+    const uint32_t n = *T.numValid;
+    float4 lo = float4{INFINITY, INFINITY, INFINITY, -INFINITY}, hi = float4{-INFINITY, -INFINITY, -INFINITY, 0.0f};
+    const uint32_t b = j * ENTRY_SEG_LEAF, e = min(b + ENTRY_SEG_LEAF, n);
+    for (uint32_t i = b; i < e; i++) grow_entry(lo, hi, T.entrySorted[i]);
+    T.segLo[leaves - 1u + j] = lo;
+    T.segHi[leaves - 1u + j] = hi;
+}
+
+// Range-tree nodes of level l (heap indices [2^l - 1, 2^(l+1) - 1)) above the leaf level.
+__global__ void k_seg_level(EntryTree T, uint32_t level, uint32_t count) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count || level >= *T.segLevels) return;
+    const uint32_t node = (1u << level) - 1u + j;
+    float4 lo = T.segLo[2 * node + 1], hi = T.segHi[2 * node + 1];
+    grow_box(lo, hi, T.segLo[2 * node + 2], T.segHi[2 * node + 2]);
+    T.segLo[node] = lo;
+    T.segHi[node] = hi;
+}
+
+// Bounds (and largest radius) of the sorted entries [a, b], b >= a: short ranges are scanned;
+// otherwise the partial groups at both ends are scanned and the whole groups between them are
+// covered bottom-up by the classic iterative segment-tree walk (at most two nodes per level).
+__device__ __forceinline__ void range_bounds(const EntryTree& T, uint32_t levels, uint32_t a, uint32_t b, float4& lo,
+                                             float4& hi) {
+    lo = float4{INFINITY, INFINITY, INFINITY, -INFINITY};
+    hi = float4{-INFINITY, -INFINITY, -INFINITY, 0.0f};
+    const uint32_t ga = a / ENTRY_SEG_LEAF, gb = b / ENTRY_SEG_LEAF;
+    if (gb - ga <= 1u) {
+        for (uint32_t i = a; i <= b; i++) grow_entry(lo, hi, T.entrySorted[i]);
+        return;
+    }
+    for (uint32_t i = a; i < (ga + 1u) * ENTRY_SEG_LEAF; i++) grow_entry(lo, hi, T.entrySorted[i]);
+    for (uint32_t i = gb * ENTRY_SEG_LEAF; i <= b; i++) grow_entry(lo, hi, T.entrySorted[i]);
+    // whole groups [ga + 1, gb) as 1-based heap indices [l, r)
+    uint32_t l = (1u << levels) + ga + 1u, r = (1u << levels) + gb;
+    while (l < r) {
+        if (l & 1u) {
+            grow_box(lo, hi, T.segLo[l - 1u], T.segHi[l - 1u]);
+            l++;
+        }
+        if (r & 1u) {
+            --r;
+            grow_box(lo, hi, T.segLo[r - 1u], T.segHi[r - 1u]);
+        }
+        l >>= 1;
+        r >>= 1;
+    }
+}
+
+// Internal node i: its key range and split (the paper's determineRange and findSplit), then the
+// complete 64-byte record: each child's bounds and largest radius from range_bounds, and its
+// reference (a range of at most ENTRY_LEAF_MAX entries becomes a leaf range).  Nodes are
+// independent, so the build needs no synchronisation between threads.
 __global__ void k_lbvh_nodes(EntryTree T) {
     const uint32_t n = *T.numValid;
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -651,44 +720,15 @@ __global__ void k_lbvh_nodes(EntryTree T) {
     const uint32_t nl = (uint32_t)(g - first + 1), nr = (uint32_t)(last - g);
     const int refL = nl <= ENTRY_LEAF_MAX ? (int)~(((uint32_t)first << 4) | nl) : g;
     const int refR = nr <= ENTRY_LEAF_MAX ? (int)~(((uint32_t)(g + 1) << 4) | nr) : g + 1;
+    const uint32_t levels = *T.segLevels;
+    float4 lo0, hi0, lo1, hi1;
+    range_bounds(T, levels, (uint32_t)first, (uint32_t)g, lo0, hi0);
+    range_bounds(T, levels, (uint32_t)g + 1u, (uint32_t)last, lo1, hi1);
     float4* nd = T.nodes + 4 * (size_t)i;
-    nd[1].w = __int_as_float(refL);
-    nd[3].w = __int_as_float(refR);
-    const uint32_t up = (uint32_t)i << 1;
-    if (first == g) T.leafParent[g] = up; else T.parent[g] = up;
-    if (last == g + 1) T.leafParent[g + 1] = up | 1u; else T.parent[g + 1] = up | 1u;
-    if (i == 0) T.parent[0] = 0xFFFFFFFFu;
-}
-
-// Bounds, bottom-up: a thread per sorted entry writes its box into its parent's child slot and
-// climbs while it is the second child to arrive at a node (the first one stops there), so every
-// node's two slots are complete when its box is formed.  The fence before each arrival makes
-// the slot visible device-wide before the counter; the fence after it makes the sibling's slot
-// visible to this thread.  Min and max are exact, so the result does not depend on the order.
-__global__ void k_lbvh_bounds(EntryTree T) {
-    const uint32_t n = *T.numValid;
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n < 2 || e >= n) return;
-    const float4 p = T.entrySorted[e];
-    float4 lo = float4{p.x, p.y, p.z, p.w}, hi = float4{p.x, p.y, p.z, 0.0f};
-    uint32_t slot = T.leafParent[e];
-    while (true) {
-        const uint32_t node = slot >> 1, side = slot & 1u;
-        float4* nd = T.nodes + 4 * (size_t)node;
-        nd[2 * side] = lo;   // lo.xyz and the largest radius
-        float* h = (float*)(nd + 2 * side + 1);
-        h[0] = hi.x;         // hi.xyz; .w keeps the child reference
-        h[1] = hi.y;
-        h[2] = hi.z;
-        __threadfence();
-        if (atomicAdd(&T.arrivals[node], 1u) == 0u) return;
-        __threadfence();
-        const float4 a0 = nd[0], a1 = nd[1], c0 = nd[2], c1 = nd[3];
-        lo = float4{fminf(a0.x, c0.x), fminf(a0.y, c0.y), fminf(a0.z, c0.z), fmaxf(a0.w, c0.w)};
-        hi = float4{fmaxf(a1.x, c1.x), fmaxf(a1.y, c1.y), fmaxf(a1.z, c1.z), 0.0f};
-        slot = T.parent[node];
-        if (slot == 0xFFFFFFFFu) return;
-    }
+    nd[0] = lo0;
+    nd[1] = float4{hi0.x, hi0.y, hi0.z, __int_as_float(refL)};
+    nd[2] = lo1;
+    nd[3] = float4{hi1.x, hi1.y, hi1.z, __int_as_float(refR)};
 }
 
 template <int KR>
@@ -1022,8 +1062,10 @@ hipError_t launch_key_sort(uint32_t* keys, uint32_t* vals, uint32_t* keysAlt, ui
     return hipGetLastError();
 }
 
-// Sorts the valid entries by Morton key and builds the LBVH over them: slots, nodes, then the
-// bounds pass.  Grids cover the capacity n; every kernel reads the valid count on the device.
+// Sorts the valid entries by Morton key and builds the LBVH over them: the sorted slots, the
+// range tree (one launch per level; levels above the valid entries' tree return at once), then
+// every internal node at once.  Grids cover the capacity n; kernels read the valid count on the
+// device.
 hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n, void* temp, size_t tempBytes,
                              hipStream_t s) {
     if (n == 0) return hipGetLastError();   // numValid was cleared when the entries were written
@@ -1031,8 +1073,14 @@ hipError_t launch_tree_build(const EntryBuild& E, const EntryTree& T, uint32_t n
     if (e != hipSuccess) return e;
     const dim3 g((n + 255) / 256), b(256);
     hipLaunchKernelGGL(k_lbvh_slots, g, b, 0, s, T, E.vals, E.pos);
+    hipLaunchKernelGGL(k_seg_prep, dim3(1), dim3(64), 0, s, T);
+    const uint32_t leavesCap = 1u << T.segLevelsCap;
+    hipLaunchKernelGGL(k_seg_leaves, dim3((leavesCap + 255) / 256), b, 0, s, T, leavesCap);
+    for (int l = (int)T.segLevelsCap - 1; l >= 0; l--) {
+        const uint32_t cnt = 1u << l;
+        hipLaunchKernelGGL(k_seg_level, dim3((cnt + 255) / 256), b, 0, s, T, (uint32_t)l, cnt);
+    }
     hipLaunchKernelGGL(k_lbvh_nodes, g, b, 0, s, T);
-    hipLaunchKernelGGL(k_lbvh_bounds, g, b, 0, s, T);
     return hipGetLastError();
 }
 

@@ -24,7 +24,7 @@ constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 // Byte offsets of the buffers inside the array.
 struct Layout {
     size_t pos, valid, keys, vals, keysAlt, valsAlt, numValid;
-    size_t entrySorted, entryPixel, nodes, parent, leafParent, arrivals, sortTemp, sortTempBytes, total;
+    size_t entrySorted, entryPixel, nodes, segLo, segHi, segLevels, sortTemp, sortTempBytes, total;
 };
 
 constexpr Layout make_layout() {
@@ -48,9 +48,9 @@ constexpr Layout make_layout() {
     L.entrySorted = take(n * sizeof(float4));
     L.entryPixel = take(n * sizeof(uint32_t));
     L.nodes = take(4 * n * sizeof(float4));
-    L.parent = take(n * sizeof(uint32_t));
-    L.leafParent = take(n * sizeof(uint32_t));
-    L.arrivals = take(n * sizeof(uint32_t));
+    L.segLo = take(((size_t(2) << entry_seg_levels(n)) - 1) * sizeof(float4));
+    L.segHi = take(((size_t(2) << entry_seg_levels(n)) - 1) * sizeof(float4));
+    L.segLevels = take(sizeof(uint32_t));
     L.sortTemp = take(sortBytes);
     L.sortTempBytes = sortBytes;
     L.total = o;
@@ -124,16 +124,16 @@ extern "C" int cx_entries_search_static(cx_ctx* ctx, const float* pos, const flo
     T.entrySorted = (float4*)(b + LAYOUT.entrySorted);
     T.entryPixel = (uint32_t*)(b + LAYOUT.entryPixel);
     T.nodes = (float4*)(b + LAYOUT.nodes);
-    T.parent = (uint32_t*)(b + LAYOUT.parent);
-    T.leafParent = (uint32_t*)(b + LAYOUT.leafParent);
-    T.arrivals = (uint32_t*)(b + LAYOUT.arrivals);
+    T.segLo = (float4*)(b + LAYOUT.segLo);
+    T.segHi = (float4*)(b + LAYOUT.segHi);
+    T.segLevels = (uint32_t*)(b + LAYOUT.segLevels);
+    T.segLevelsCap = entry_seg_levels(n > 0 ? n : 1u);   // <= the layout's levels: the arrays fit
     T.sortedKeys = E.keys;   // the sort leaves its output in keys / vals
     T.numValid = E.numValid;
     const hipStream_t s = (hipStream_t)stream;
     // every value the kernels read is written earlier in this call: the entry records for all
     // n slots, the valid count (cleared by a fill kernel, then set by the sort's first pass), the
-    // sorted slots, the arrival counters (cleared by the slot kernel) and the nodes, parents and
-    // bounds of the tree over the valid entries
+    // sorted slots, the range tree's levels and nodes over the valid entries, and the LBVH nodes
     hipError_t e = launch_entries_from_arrays(pos, radius, valid, n, E, s);
     if (e == hipSuccess) e = launch_tree_build(E, T, n, b + LAYOUT.sortTemp, LAYOUT.sortTempBytes, s);
     if (e == hipSuccess) e = launch_entries_query(T, query, nq, k, idx, dist, count, s);

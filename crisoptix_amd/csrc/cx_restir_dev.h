@@ -181,16 +181,31 @@ static constexpr uint32_t ENTRY_LEAF_MAX = 8;
 // 62 internal nodes and a depth-first walk defers at most one sibling per node of its path.
 static constexpr int ENTRY_STACK = 64;
 
+// The bounds of a node come from a range query over an implicit complete binary tree of the
+// sorted entries in groups of ENTRY_SEG_LEAF (heap order, children 2i+1 and 2i+2): each node
+// holds the position bounds of its groups and the largest radius, so any range of sorted entries
+// is the union of at most two groups' worth of scanned entries and two tree nodes per level.
+static constexpr uint32_t ENTRY_SEG_LEAF = 8;
+
 struct EntryTree {
     float4* entrySorted;   // [capacity] pos.xyz, radius, in key order
     uint32_t* entryPixel;  // [capacity] entry index (pixel) of each sorted slot
     float4* nodes;         // [capacity][4] internal nodes (n - 1 of them are used)
-    uint32_t* parent;      // [capacity] internal node -> (parent << 1) | side; the root 0xFFFFFFFF
-    uint32_t* leafParent;  // [capacity] sorted slot -> (parent << 1) | side
-    uint32_t* arrivals;    // [capacity] children whose bounds have reached each internal node
+    float4* segLo;         // [2^(segLevelsCap+1) - 1] lo.xyz, largest radius of each range node
+    float4* segHi;         //   hi.xyz
+    uint32_t* segLevels;   // device scalar: levels of the range tree over the valid entries
+    uint32_t segLevelsCap; // levels at capacity (sizes of the range-tree arrays)
     const uint32_t* sortedKeys;  // [capacity] keys in sorted order (the sort's output)
     const uint32_t* numValid;
 };
+
+// Levels of the range tree over `entries` sorted entries.
+CX_HD constexpr uint32_t entry_seg_levels(uint32_t entries) {
+    const uint32_t groups = entries > ENTRY_SEG_LEAF ? (entries + ENTRY_SEG_LEAF - 1) / ENTRY_SEG_LEAF : 1u;
+    uint32_t l = 0;
+    while ((1u << l) < groups) l++;
+    return l;
+}
 
 // ---- frame parameters passed by value to every pass kernel ----
 
