@@ -444,11 +444,12 @@ void trace_rec(cx_renderer* r, hipStream_t s) {
     (void)hipEventRecord(r->traceEv[r->traceEvUsed++], s);
 }
 
-// Wavefront rounds before the tail kernel by machine.  Measured on the 1080p Conditional ReSTIR
-// frame: suffix generation 1 / 2 / 3 rounds -> 8.62 / 8.81 / 8.99 ms per frame (its walks have
-// few long survivors after the first bounce); prefix generation 1 / 2 / 3 -> 9.05 / 8.81 / 8.83;
-// shifts 1 / 2 / 3 -> 8.80 / 8.81 / 8.85.
-uint32_t default_wave_rounds(int step) { return step == WK_SG_STEP ? 1u : 2u; }
+// Wavefront rounds before the tail kernel, every machine.  Measured on the 1080p Conditional
+// ReSTIR frame (tent of traced lights, 14.6 rays/px): every machine 1 (suffix generation) or 2
+// (the others) / 2 / 3 / 4 / 6 rounds, or all rounds as traversal + step (0) -> 29.28 / 28.77 /
+// 28.91 / 29.15 / 29.82 / 29.88 ms per frame; the tail traces in place at 3 waves per SIMD, so
+// long suffix walks are cheaper in the queue rounds.
+uint32_t default_wave_rounds(int) { return 2u; }
 
 hipError_t run_machine(cx_renderer* r, WaveArgs W, int setup, int step, uint32_t cap, uint32_t rounds,
                        int pass, hipStream_t s) {
