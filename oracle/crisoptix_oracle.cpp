@@ -20,10 +20,14 @@
 // Parity pins (tests/test_oracle_*.py): PCG32 against the PCG reference demo output and an
 // independent restatement from the PCG paper, TEA against the published TEA test vector,
 // GGX closed forms and Monte-Carlo normalisation, brute-force closest hit and occlusion,
-// brute-force neighbour sets, closed forms of the Jacobian and the pairwise MIS weights,
-// the identity-shift invariant, and a converged FP64 reference of the path-tracer
-// estimator.  The reservoir passes as a whole are "parity unpinned" against the
-// reference itself (it cannot be built here: no OptiX SDK, CUDA toolkit or glm).
+// brute-force neighbour sets, closed forms of the Jacobian and the pairwise MIS weights, the
+// pairwise weights' partition of unity for reciprocal Jacobians, the prefix identity shift and
+// the suffix identity shift at an NEE reconnection (J = 1), the final gather's combination in
+// its limits (unreachable neighbours give the canonical estimate), a converged FP64 reference of
+// the path-tracer estimator, and the canonical prefix + suffix chain of Conditional ReSTIR
+// against that FP64 reference one vertex shorter.  Shifted-path reuse as a whole (which
+// neighbour suffixes are combined, through which replayed vertices) stays "parity unpinned"
+// against the reference itself (it cannot be built here: no OptiX SDK, CUDA toolkit or glm).
 
 #include "orc_math.h"
 
@@ -1607,6 +1611,41 @@ static void renderer_entries(Renderer& R, EntrySet& E) {
 // Final gather (shader/final_gather.cu:69-356)
 // ---------------------------------------------------------------------------------------
 
+// One valid neighbour of the canonical gather: its suffix shifted onto the canonical prefix
+// (value, Jacobian), its own value, reservoir weight and confidence, and the canonical suffix
+// shifted onto its prefix (value, Jacobian).
+struct GatherTerm {
+    V3 fFromCanonOfNeigh;
+    float jNeighToCanon = 0.0f;
+    V3 neighF;
+    float neighWSum = 0.0f, neighConf = 0.0f;
+    V3 fFromNeighOfCanon;
+    float jCanonToNeigh = 0.0f;
+};
+
+// The pairwise-MIS combination of final_gather.cu:135-187 over `n` valid neighbours.
+static V3 gather_combine(const V3& canonPreF, float canonPreP, const V3& canonSufF, const V3& canonContrib,
+                         const GatherTerm* t, uint32_t n) {
+    V3 suffixContrib(0.0f);
+    const float pairwiseK = (float)n;
+    float canonMisWeight = 1.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        float neighMisWeight = neighbor_pairwise_mis(t[i].fFromCanonOfNeigh, t[i].neighF, t[i].jNeighToCanon,
+                                                     pairwiseK, 1.0f, t[i].neighConf);
+        if (std::isnan(neighMisWeight) || std::isinf(neighMisWeight)) neighMisWeight = 0.0f;
+        const V3 neighContrib = neighMisWeight * t[i].fFromCanonOfNeigh * t[i].neighWSum * t[i].jNeighToCanon;
+        if (!any_nan_inf(neighContrib)) suffixContrib += neighContrib;
+        canonMisWeight += canonical_pairwise_mis(canonSufF, t[i].fFromNeighOfCanon, t[i].jCanonToNeigh, pairwiseK,
+                                                 1.0f, t[i].neighConf);
+    }
+    V3 gatherContrib = suffixContrib * canonPreF / canonPreP;
+    gatherContrib /= (float)(n + 1);
+    V3 result = gatherContrib + (canonContrib * canonMisWeight);
+    result /= pairwiseK + 1.0f;
+    if (any_nan_inf(result)) return V3(0.0f);
+    return result;
+}
+
 static V3 final_gather_canon(Renderer& R, const EntrySet& E, uint32_t p, const V3& origin, const V3& dir,
                              uint32_t k, PCG32& rng) {   // final_gather.cu:69-188
     const Scene& s = *R.scene;
@@ -1631,31 +1670,19 @@ static V3 final_gather_canon(Renderer& R, const EntrySet& E, uint32_t p, const V
         ++validNeighCount;
         validNeighFlags |= 1u << i;
     }
-    V3 suffixContrib(0.0f);
-    const float pairwiseK = (float)validNeighCount;
-    float canonMisWeight = 1.0f;
+    GatherTerm terms[32];
+    uint32_t n = 0;
     for (uint32_t i = 0; i < neighCount; ++i) {
         if (!(validNeighFlags & (1u << i))) continue;
         const Reservoir<Suffix>& nres = R.suffixRes[2 * nb[i].pixelIdx + P.front];
-        const Suffix& neigh = nres.sample;
-        float jNeighToCanon = 0.0f;
-        const V3 fFromCanonOfNeigh = shift_suffix(s, neigh, canonSuffix, jNeighToCanon);
-        float jCanonToNeigh = 0.0f;
-        const V3 fFromNeighOfCanon = shift_suffix(s, canonSuffix, neigh, jCanonToNeigh);
-        float neighMisWeight = neighbor_pairwise_mis(fFromCanonOfNeigh, neigh.f, jNeighToCanon, pairwiseK, 1.0f,
-                                                     nres.confidence);
-        if (std::isnan(neighMisWeight) || std::isinf(neighMisWeight)) neighMisWeight = 0.0f;
-        const V3 neighContrib = neighMisWeight * fFromCanonOfNeigh * nres.wSum * jNeighToCanon;
-        if (!any_nan_inf(neighContrib)) suffixContrib += neighContrib;
-        canonMisWeight += canonical_pairwise_mis(canonSuffix.f, fFromNeighOfCanon, jCanonToNeigh, pairwiseK, 1.0f,
-                                                 nres.confidence);
+        GatherTerm& t = terms[n++];
+        t.fFromCanonOfNeigh = shift_suffix(s, nres.sample, canonSuffix, t.jNeighToCanon);
+        t.fFromNeighOfCanon = shift_suffix(s, canonSuffix, nres.sample, t.jCanonToNeigh);
+        t.neighF = nres.sample.f;
+        t.neighWSum = nres.wSum;
+        t.neighConf = nres.confidence;
     }
-    V3 gatherContrib = suffixContrib * canonPrefix.f / canonPrefix.p;
-    gatherContrib /= (float)(validNeighCount + 1);
-    V3 result = gatherContrib + (canonContrib * canonMisWeight);
-    result /= pairwiseK + 1.0f;
-    if (any_nan_inf(result)) return V3(0.0f);
-    return result;
+    return gather_combine(canonPrefix.f, canonPrefix.p, canonSuffix.f, canonContrib, terms, n);
 }
 
 static V3 final_gather_not_canon(Renderer& R, const EntrySet& E, uint32_t p, const V3& origin, const V3& dir,
@@ -2671,6 +2698,30 @@ void orc_identity_shift_suffix(void* sp, const cx_launch_params* lp, const float
     v3out(f, out + 3);
     out[6] = J;
     out[7] = suf.valid() ? 1.0f : 0.0f;
+    out[8] = (float)suf.reconIdx();
+    out[9] = (float)suf.length();
+}
+
+// The final gather's combination (final_gather.cu:135-187) over injected neighbour terms, each
+// 13 floats: fFromCanonOfNeigh[3], jNeighToCanon, neighF[3], neighWSum, neighConf,
+// fFromNeighOfCanon[3], jCanonToNeigh.
+void orc_gather_combine(const float* canonPreF, float canonPreP, const float* canonSufF, const float* canonContrib,
+                        const float* terms, uint32_t n, float* out) {
+    std::vector<GatherTerm> t(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const float* x = terms + 13 * (size_t)i;
+        t[i].fFromCanonOfNeigh = V3(x[0], x[1], x[2]);
+        t[i].jNeighToCanon = x[3];
+        t[i].neighF = V3(x[4], x[5], x[6]);
+        t[i].neighWSum = x[7];
+        t[i].neighConf = x[8];
+        t[i].fFromNeighOfCanon = V3(x[9], x[10], x[11]);
+        t[i].jCanonToNeigh = x[12];
+    }
+    const V3 r = gather_combine(V3(canonPreF[0], canonPreF[1], canonPreF[2]), canonPreP,
+                                V3(canonSufF[0], canonSufF[1], canonSufF[2]),
+                                V3(canonContrib[0], canonContrib[1], canonContrib[2]), t.data(), n);
+    v3out(r, out);
 }
 
 void* orc_renderer_create(void* sp, uint32_t w, uint32_t h) {

@@ -45,6 +45,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, u32, u64, f32, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float, C.c_int
         L.orc_scene_create.restype = vp
+        L.orc_gather_combine.argtypes = [vp, C.c_float, vp, vp, vp, u32, vp]
         L.orc_scene_create.argtypes = [C.POINTER(abi.SceneDesc), i32]
         L.orc_scene_destroy.argtypes = [vp]
         L.orc_scene_size.restype = f32
@@ -232,7 +233,7 @@ class OracleScene:
     def identity_shift(self, params: abi.LaunchParams, origin, direction, rng_state: int, which="prefix"):
         o = np.ascontiguousarray(origin, np.float32)
         d = np.ascontiguousarray(direction, np.float32)
-        out = np.zeros(8, np.float32)
+        out = np.zeros(10, np.float32)   # f, shifted f, J, valid; the suffix's reconIdx and length
         fn = lib().orc_identity_shift_prefix if which == "prefix" else lib().orc_identity_shift_suffix
         fn(self.h, C.byref(params), _p(o), _p(d), rng_state, _p(out))
         return out
@@ -268,6 +269,17 @@ def pairwise_mis(f_basis, f_neigh, J, K, canon_m, neigh_m):
     nw = C.c_float()
     lib().orc_pairwise_mis(_p(a), _p(b), J, K, canon_m, neigh_m, C.byref(cw), C.byref(nw))
     return cw.value, nw.value
+
+
+def gather_combine(canon_pre_f, canon_pre_p, canon_suf_f, canon_contrib, terms):
+    """The final gather's pairwise-MIS combination (final_gather.cu:135-187) over injected neighbour
+    terms: rows of fFromCanonOfNeigh[3], jNeighToCanon, neighF[3], neighWSum, neighConf,
+    fFromNeighOfCanon[3], jCanonToNeigh."""
+    t = np.ascontiguousarray(terms, np.float32).reshape(-1, 13)
+    a = [np.ascontiguousarray(x, np.float32) for x in (canon_pre_f, canon_suf_f, canon_contrib)]
+    out = np.zeros(3, np.float32)
+    lib().orc_gather_combine(_p(a[0]), C.c_float(canon_pre_p), _p(a[1]), _p(a[2]), _p(t), t.shape[0], _p(out))
+    return out
 
 
 def reservoir_stream(weights, confidences, rng_state: int):

@@ -3,6 +3,7 @@
 import math
 
 import numpy as np
+import pytest
 
 from crisoptix_amd import abi, scenes
 
@@ -146,3 +147,102 @@ def test_unclamped_jacobian_can_make_reservoirs_infinite(orc):
         rec = front[px]
         assert np.isposinf(rec["wSum"])
         assert np.isfinite(rec["f"]).all() and 0 < float(np.max(rec["f"])) < 1e-3
+
+
+def test_pairwise_mis_partition_of_unity(orc):
+    """For a path x of the canonical domain and y = T^-1(x) of a neighbour's, the canonical weight
+    of x in that pair (ComputeCanonicalPairwiseMISWeight with J_{x->y}) and the weight x gets as
+    the neighbour's shifted sample (ComputeNeighborPairwiseMISWeight with J_{y->x} = 1 / J_{x->y})
+    sum to one for any target values, confidences, K and Jacobian (path_gen.h:15-48): the pairwise
+    MIS of the spatial passes and the final gather is a partition of unity pair by pair.  A
+    Jacobian applied in the wrong direction, K on the wrong side or swapped confidences breaks it."""
+    rng = np.random.default_rng(17)
+    worst = 0.0
+    for _ in range(2000):
+        fc = rng.uniform(0.01, 5.0, 3).astype(np.float32)
+        fn = rng.uniform(0.01, 5.0, 3).astype(np.float32)
+        J = float(np.exp(rng.normal(0.0, 1.5)))
+        K = float(rng.integers(1, 9))
+        cM = float(rng.uniform(0.5, 3.0))
+        nM = float(rng.uniform(1.0, 50.0))
+        cw, _ = orc.pairwise_mis(fc, fn, J, K, cM, nM)
+        _, nw = orc.pairwise_mis(fc, fn, np.float32(1.0 / J), K, cM, nM)
+        worst = max(worst, abs(cw + nw - 1.0))
+    assert worst < 1e-5, worst
+    # the same two calls with the Jacobian not inverted are not a partition
+    cw, _ = orc.pairwise_mis(np.ones(3, np.float32), np.ones(3, np.float32), 4.0, 1.0, 1.0, 1.0)
+    _, nw = orc.pairwise_mis(np.ones(3, np.float32), np.ones(3, np.float32), 4.0, 1.0, 1.0, 1.0)
+    assert abs(cw + nw - 1.0) > 0.5
+
+
+@pytest.mark.parametrize("max_len", [3, 4])
+def test_suffix_identity_shift_at_nee_reconnection(orc, max_len):
+    """A suffix that took NEE at its first vertex (the prefix's last vertex) is reconnected there:
+    reconIdx = 1 = its length (path_gen.h:320-341).  Shifting it onto its own last prefix vertex
+    replays no vertex (suffix_reuse.h:34), so the reconnection Jacobian compares a vertex with
+    itself and is exactly 1 (path_gen.h:59-72), and the shifted value brdf(to the reconnection
+    point) * colour / d^2 is the suffix's f up to the rounding of the reconnection hit point
+    (suffix_reuse.h:63-104).  Light traced as geometry (C9'), so the reconnection ray ends on the
+    light as in the reference's scenes."""
+    sc = scenes.cornell_box(1, 4, trace_emitters=True)
+    os_ = orc.OracleScene(sc)
+    cam = scenes.cornell_camera(32, 32).data()
+    rng = np.random.default_rng(53 + max_len)
+    p = abi.workload_params("conditional_restir", 32, 32, os_.size)
+    p.maxPathLen = max_len
+    n = 0
+    for i in range(4000):
+        d = cam["W"] + cam["U"] * rng.uniform(-1, 1) + cam["V"] * rng.uniform(-1, 1)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        out = os_.identity_shift(p, cam["pos"], d, int(rng.integers(0, 2**63)), which="suffix")
+        if not out[7] or out[8] != 1 or out[9] != 1:
+            continue
+        f, fs, J = out[0:3].astype(np.float64), out[3:6].astype(np.float64), out[6]
+        if f.sum() <= 0:
+            continue
+        n += 1
+        assert J == 1.0
+        assert np.allclose(fs, f, rtol=1e-4, atol=0.0), (f, fs)
+    assert n > 300
+
+
+def test_gather_combination_limits(orc):
+    """The final gather's combination (final_gather.cu:135-187), fed injected neighbour terms:
+    * neighbours that cannot produce the canonical path and that the canonical suffix cannot reach
+      (both shifted values zero) leave the canonical estimate unchanged: each pair's canonical
+      weight is 1, so canonMisWeight = K + 1 cancels the final 1/(K + 1), for K = 1..8;
+    * a zero neighbour weight removes that neighbour's own term but keeps its pairwise share of the
+      canonical weight;
+    * one neighbour identical to the canonical suffix (shift J = 1, confidence 1, W = 1/p) gives
+      (1/4 + 3/2)/2 = 7/8 of the canonical estimate: the neighbour term is divided by N + 1 = 2 at
+      line 181 and again by K + 1 = 2 at line 184, where the pair's weights (1/2 each) alone would
+      return the canonical estimate; the oracle keeps the reference's double division."""
+    pre_f, pre_p = np.asarray([0.6, 0.5, 0.4], np.float32), 0.7
+    suf_f, suf_p = np.asarray([0.3, 0.2, 0.25], np.float32), 0.9
+    canon = (pre_f * suf_f / np.float32(pre_p * suf_p)).astype(np.float32)
+    rng = np.random.default_rng(3)
+    for K in range(1, 9):
+        t = np.zeros((K, 13), np.float32)
+        t[:, 3] = rng.uniform(0.5, 2.0, K)        # Jacobians
+        t[:, 4:7] = rng.uniform(0.1, 1.0, (K, 3))  # neighbours' own values
+        t[:, 7] = rng.uniform(0.5, 3.0, K)        # reservoir weights
+        t[:, 8] = rng.uniform(1.0, 50.0, K)       # confidences
+        t[:, 12] = rng.uniform(0.5, 2.0, K)
+        r = orc.gather_combine(pre_f, pre_p, suf_f, canon, t)
+        assert np.allclose(r, canon, rtol=2e-7, atol=0), (K, r, canon)
+    # a neighbour with zero reservoir weight next to one that reaches the canonical path
+    t = np.zeros((2, 13), np.float32)
+    t[:, 0:3] = suf_f
+    t[:, 3] = 1.0
+    t[:, 4:7] = suf_f
+    t[:, 8] = 1.0
+    t[:, 9:12] = suf_f
+    t[:, 12] = 1.0
+    t[0, 7] = 1.0 / suf_p
+    with_zero = orc.gather_combine(pre_f, pre_p, suf_f, canon, t)
+    t[1, 7] = 1.0 / suf_p
+    both = orc.gather_combine(pre_f, pre_p, suf_f, canon, t)
+    assert np.all(both > with_zero)
+    # one neighbour identical to the canonical suffix
+    r1 = orc.gather_combine(pre_f, pre_p, suf_f, canon, t[:1])
+    assert np.allclose(r1, canon * 0.875, rtol=1e-6), (r1 / canon)

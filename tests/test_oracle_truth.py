@@ -10,6 +10,7 @@ reference's PathTracer estimator (path_gen.h:389-472):
 import math
 
 import numpy as np
+import pytest
 
 from crisoptix_amd import abi, scenes
 
@@ -84,3 +85,56 @@ def test_float_path_tracer_agrees_with_truth(orc):
     # Per pixel: standardised differences are consistent with noise.
     z = (img - truth) / np.sqrt(tvar + sample_var / frames + 1e-30)
     assert np.mean(z ** 2) < 2.0
+
+
+def _canonical_chain(orc, sc, cam, W, H, L, frames, seed):
+    """Frames of Conditional ReSTIR with every reuse off and gatherM = 1: the final gather's output is
+    then one canonical path's f/p per pixel and frame (final_gather.cu:76-91)."""
+    os_ = orc.OracleScene(sc)
+    p = abi.workload_params("conditional_restir", W, H, os_.size)
+    p.maxPathLen = L
+    p.gatherM = 1
+    p.prefixEnableTemporal = p.prefixEnableSpatial = 0
+    p.suffixEnableTemporal = p.suffixEnableSpatial = 0
+    ren = orc.OracleRenderer(os_, W, H, p, seed=seed)
+    imgs = np.stack([ren.frame(cam, f == 0).astype(np.float64).copy() for f in range(frames)])
+    return os_, imgs
+
+
+@pytest.mark.parametrize("name", ["cornell", "tent"])
+def test_conditional_restir_canonical_chain_is_unbiased(orc, name):
+    """Pins TracePrefix, TraceSuffix and the canonical combination (path_gen.h:74-387,
+    final_gather.cu:76-91) against the FP64 truth.  The prefix stops at its last vertex without an
+    NEE draw; the suffix starts there and holds at most maxPathLen - prefixLen vertices, the last
+    with NEE forced and p not multiplied by neeProb (an NEE probability of 1 there), so its paths
+    carry light from vertices 1 .. maxPathLen - 1: the expectation is the path tracer's at
+    maxPathLen - 1.  No material of these scenes emits, so the length-1 emission term of
+    path_gen.h:204 never applies.  The test statistic is each 4x4 block's mean against the truth
+    (5 standard errors from the frames' own variance); a dropped 1/neeProb or 1/(1 - neeProb), an
+    extra vertex or a missing NEE term each shift them by far more."""
+    if name == "cornell":
+        W, H, L = 8, 8, 4
+        sc = scenes.cornell_box(1, 4)
+        camd = scenes.cornell_camera(W, H).data()
+        frames = 4096
+    else:
+        W, H, L = 8, 8, 6
+        sc = scenes.object_grid(grid=3, seed=5, ground_quads=20, ico_subdiv=2, num_lights=16)
+        camd = scenes.orbit_camera(sc, W, H).data()
+        frames = 8192
+    cam = abi.CameraData.from_dict(camd)
+    os_, imgs = _canonical_chain(orc, sc, cam, W, H, L, frames, seed=21)
+    truth, tvar = orc.truth_render(os_, cam, W, H, L - 1, 0.3, 16384, seed=13)
+    mean = imgs.mean(0)
+    var_mean = imgs.var(0, ddof=1) / frames
+    bm = mean.reshape(H // 4, 4, W // 4, 4, 3).mean((1, 3))
+    bt = truth.reshape(H // 4, 4, W // 4, 4, 3).mean((1, 3))
+    bse = np.sqrt((var_mean + tvar).reshape(H // 4, 4, W // 4, 4, 3).sum((1, 3))) / 16
+    z = (bm - bt) / (bse + 1e-30)
+    print(f"{name}: image mean {mean.mean():.5f} vs truth(L-1) {truth.mean():.5f}; block z {np.round(z.ravel(), 2)}")
+    assert np.all(np.abs(z) < 5.0)
+    # the truth one vertex longer is far off: the comparison can tell the path lengths apart
+    longer, lvar = orc.truth_render(os_, cam, W, H, L, 0.3, 16384, seed=14)
+    zl = (mean.mean() - longer.mean()) / (np.sqrt((var_mean + lvar).sum()) / mean.size)
+    print(f"{name}: against the truth at maxPathLen {L}: z {zl:.1f}")
+    assert abs(zl) > 5.0
