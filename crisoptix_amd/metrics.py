@@ -1,6 +1,7 @@
 """Image-quality measurements of rendered frames.
 
-rel-MSE = mean over pixels and channels of (I - I_ref)^2 / (I_ref^2 + 1e-2).
+rel-MSE = mean over pixels and channels of (I - I_ref)^2 / (I_ref^2 + 1e-2), reported beside the
+rel-MSE of an all-black image and with the exposure normalised to a unit-mean truth.
 
 The reference's ReSTIR estimators are heavy-tailed: the prefix shift evaluates the reconnection
 Jacobian with the source's primary-hit position in place of the vertex before the reconnection
@@ -51,15 +52,37 @@ def frame_batches(renderer, cam: abi.CameraData, frames: int, batches: int) -> n
     return out
 
 
-def batch_report(batch_means: np.ndarray, truth: np.ndarray) -> dict:
+def batch_report(batch_means: np.ndarray, truth: np.ndarray, truth_var: np.ndarray | None = None) -> dict:
     """rel-MSE of the plain mean (over finite pixels; non-finite ones counted) and of the
-    per-pixel median of the batch means."""
+    per-pixel median of the batch means, each beside the figures that make it readable:
+      * black_rel_mse: the rel-MSE of an all-black image (what a renderer that outputs nothing
+        scores; with radiance far below sqrt(1e-2) it is tiny, so rel-MSE alone says little);
+      * *_exposure: the same with image and truth divided by the truth's mean (unit-mean
+        exposure), where the 1e-2 of the denominator is small against the signal;
+      * noise_rel_mse_exposure: the exposure-normalised rel-MSE that the measured noise alone
+        predicts for the plain mean: the variance of the mean of the batches (their spread
+        divided by their number) plus the truth's own variance, over (truth^2 + 1e-2);
+      * mean_ratio, mean_ratio_median_of_means: image mean over truth mean."""
+    b = batch_means.shape[0]
     mean = batch_means.mean(0)
     finite = np.isfinite(mean).all(-1)
     mom = np.median(batch_means, axis=0)
+    t = np.asarray(truth, np.float64)
+    s = 1.0 / float(t.mean()) if t.mean() > 0 else 1.0
+    var_mean = np.var(batch_means, axis=0, ddof=1) / b if b > 1 else np.zeros_like(t)
+    tv = np.zeros_like(t) if truth_var is None else np.asarray(truth_var, np.float64)
+    ok = finite[..., None] & np.isfinite(var_mean)
+    noise = float(np.mean(np.where(ok, (var_mean + tv) * s * s / ((t * s) ** 2 + 1e-2), 0.0)))
     return {
-        "rel_mse": rel_mse(mean[finite], truth[finite]),
+        "rel_mse": rel_mse(mean[finite], t[finite]),
         "nonfinite_px": int((~finite).sum()),
-        "rel_mse_median_of_means": rel_mse(mom, truth),
-        "batches": int(batch_means.shape[0]),
+        "rel_mse_median_of_means": rel_mse(mom, t),
+        "black_rel_mse": rel_mse(np.zeros_like(t), t),
+        "rel_mse_exposure": rel_mse(mean[finite] * s, t[finite] * s),
+        "rel_mse_median_of_means_exposure": rel_mse(mom * s, t * s),
+        "black_rel_mse_exposure": rel_mse(np.zeros_like(t), t * s),
+        "noise_rel_mse_exposure": noise,
+        "mean_ratio": float(mean[finite].mean() / t[finite].mean()) if finite.any() else float("nan"),
+        "mean_ratio_median_of_means": float(mom.mean() / t.mean()),
+        "batches": int(b),
     }
