@@ -282,6 +282,9 @@ struct WideBuilder {
     std::vector<float> cost;
     std::vector<uint8_t> use, split;
     bool greedy = false;   // the greedy rule instead of the programme's tables
+    // Set when the tables describe no valid spread (a split outside [1, j), a slot count outside
+    // [1, i] or more than `width` children); the collapse is then abandoned for the next rule.
+    bool bad = false;
 
     float child_cost(const Child& c, int i) const {
         return c.ref < 0 ? 0.0f : cost[(size_t)c.ref * (width + 1) + i];
@@ -316,6 +319,7 @@ struct WideBuilder {
             float D[9];
             for (int j = 2; j <= width; j++) {
                 D[j] = INFINITY;
+                split[base + j] = 1;   // kept when no cost compares below +inf (overflowed or NaN areas)
                 for (int k = 1; k < j; k++) {
                     const float c = child_cost(l, k) + child_cost(r, j - k);
                     if (c < D[j]) {
@@ -339,16 +343,26 @@ struct WideBuilder {
     }
 
     // Children of a wide node: binary node `node` spread over at most j slots.
-    void distribute(int32_t node, int j, Child* ch, int& n) const {
+    void distribute(int32_t node, int j, Child* ch, int& n) {
+        if (bad) return;
         Child l, r;
         children_of(node, l, r);
-        const int k = split[(size_t)node * (width + 1) + j];
+        const int k = j >= 2 && j <= width ? split[(size_t)node * (width + 1) + j] : 0;
+        if (k < 1 || k >= j) {
+            bad = true;
+            return;
+        }
         place(l, k, ch, n);
         place(r, j - k, ch, n);
     }
 
-    void place(const Child& c, int i, Child* ch, int& n) const {
-        const int u = c.ref < 0 ? 1 : use[(size_t)c.ref * (width + 1) + i];
+    void place(const Child& c, int i, Child* ch, int& n) {
+        if (bad) return;
+        const int u = c.ref < 0 ? 1 : (i >= 1 && i <= width ? use[(size_t)c.ref * (width + 1) + i] : 0);
+        if (u < 1 || u > i || (u == 1 && n >= width)) {
+            bad = true;
+            return;
+        }
         if (u == 1) ch[n++] = c;
         else distribute(c.ref, u, ch, n);
     }
@@ -377,6 +391,7 @@ struct WideBuilder {
             }
         } else {
             distribute(node, width, ch, n);
+            if (bad) return -1;
         }
         const uint32_t stack = depthStack + (uint32_t)(n - 1);
         maxStack = std::max(maxStack, stack);
@@ -385,8 +400,10 @@ struct WideBuilder {
         out.resize(out.size() + nodeWords);
         int32_t refs[8];
         for (int i = 0; i < width; i++) refs[i] = i < n ? ch[i].ref : WIDE_EMPTY;
-        for (int i = 0; i < n; i++)
+        for (int i = 0; i < n; i++) {
             if (ch[i].ref >= 0) refs[i] = emit(ch[i].ref, stack);
+            if (bad) return -1;
+        }
         float4* w = &out[(size_t)idx * nodeWords];
         for (int g = 0; g < groups; g++) {
             float v[6][4];
@@ -427,7 +444,7 @@ bool collapse_with(const BvhHost& bin, int width, bool greedy, uint32_t stackLim
     }
     root = B.emit(bin.root, 0);
     stackBound = B.maxStack;
-    return B.maxStack + 1 <= stackLimit;
+    return !B.bad && B.maxStack + 1 <= stackLimit;
 }
 
 }  // namespace

@@ -660,10 +660,9 @@ int cx_scene_build(cx_ctx* ctx, const cx_scene_desc* d, cx_scene** out) {
     if (d->bvhBuilder != CX_BVH_BUILD_HOST && d->bvhBuilder != CX_BVH_BUILD_DEVICE)
         return set_err(ctx, CX_ERR_INVALID_ARG, "unknown bvhBuilder");
     const bool onDevice = d->bvhBuilder == CX_BVH_BUILD_DEVICE;
-    if (onDevice)   // the clustering compares box areas: it needs finite coordinates
-        for (size_t i = 0; i < 3 * (size_t)d->numVertices; i++)
-            if (!std::isfinite(d->positions[i]))
-                return set_err(ctx, CX_ERR_INVALID_ARG, "non-finite vertex position (device BVH build)");
+    // Both builders compare box areas and centroids: they need finite coordinates.
+    for (size_t i = 0; i < 3 * (size_t)d->numVertices; i++)
+        if (!std::isfinite(d->positions[i])) return set_err(ctx, CX_ERR_INVALID_ARG, "non-finite vertex position");
     const uint32_t maxLeaf = d->bvhMaxLeafSize ? d->bvhMaxLeafSize : DEFAULT_MAX_LEAF;
     const auto tBuild = std::chrono::steady_clock::now();
     BvhHost bvh;
@@ -888,6 +887,8 @@ int cx_bvh_host_trace(const cx_scene_desc* d, const float* rays, uint32_t n, int
     if (!d || (n && (!rays || !prim || (!anyHit && (!t || !u || !v))))) return CX_ERR_INVALID_ARG;
     const uint32_t wantWidth = d->bvhWidth ? d->bvhWidth : 4u;
     if (wantWidth != 2 && wantWidth != 4 && wantWidth != 8) return CX_ERR_INVALID_ARG;
+    for (size_t i = 0; i < 3 * (size_t)d->numVertices; i++)
+        if (!std::isfinite(d->positions[i])) return CX_ERR_INVALID_ARG;
     BvhHost bvh;
     std::string err;
     if (!build_bvh(d->positions, d->indices, d->numTriangles, d->bvhMaxLeafSize ? d->bvhMaxLeafSize : DEFAULT_MAX_LEAF,

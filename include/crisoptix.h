@@ -145,8 +145,8 @@ typedef struct cx_material_tex {
  *   HOST:   binned SAH (32 bins) on the host, collapsed to the requested width, uploaded.
  *   DEVICE: on the GPU from the uploaded vertices: 30-bit Morton sort of the triangle
  *           centroids, PLOC clustering (Meister and Bittner 2018, radius 64), SAH leaf
- *           formation, collapse to the requested width; deterministic.  Vertex positions must
- *           be finite.  A tree deeper than the traversal stack is replaced by the host build
+ *           formation, collapse to the requested width; deterministic.  A tree deeper than
+ *           the traversal stack is replaced by the host build
  *           (cx_scene_info.bvhBuilder then reports HOST). */
 typedef enum cx_bvh_builder {
     CX_BVH_BUILD_HOST = 0,
@@ -182,7 +182,7 @@ enum {
     CX_VIS_ALL = 3
 };
 typedef struct cx_scene_desc {
-    const float* positions;        /* host, [numVertices][3] */
+    const float* positions;        /* host, [numVertices][3], finite (CX_ERR_INVALID_ARG otherwise) */
     const float* normals;          /* host, [numVertices][3] */
     uint32_t numVertices;
     const uint32_t* indices;       /* host, [numTriangles][3] */

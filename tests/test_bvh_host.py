@@ -124,3 +124,41 @@ def test_traced_emitters_are_invisible_to_occlusion(width, orc):
     same = pt == pp
     assert (same | panel).all()
     assert (tt[panel & (pp != 0xFFFFFFFF)] <= tp[panel & (pp != 0xFFFFFFFF)]).all()
+
+
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_overflowing_box_areas_and_non_finite_positions(width, orc):
+    """Box areas that overflow to +inf leave no collapse cost below +inf (the least-area collapse
+    then has no split to follow): the collapse must still produce a tree, or fall back, whose
+    answers are the oracle's.  Two scenes: an object grid scaled by 3e18 (every area is +inf; the
+    triangle test overflows as well, in the oracle too, so every ray misses), and the unscaled
+    grid with two small triangles 1e19 away on either side (the upper levels' areas are +inf,
+    rays through the grid hit).  A NaN or infinite vertex coordinate is refused."""
+    import dataclasses
+    base = scenes.object_grid(4, 1)
+    huge = dataclasses.replace(base, positions=(base.positions.astype(np.float64) * 3e18).astype(np.float32))
+    far = np.asarray([[1e19, 0, 0], [1e19, 1, 0], [1e19, 0, 1], [-1e19, 0, 0], [-1e19, 1, 0], [-1e19, 0, 1]],
+                     np.float32)
+    nv = len(base.positions)
+    spread = dataclasses.replace(
+        base, positions=np.concatenate([base.positions, far]),
+        normals=np.concatenate([base.normals, np.tile([[1, 0, 0]], (6, 1)).astype(np.float32)]),
+        indices=np.concatenate([base.indices, np.asarray([[nv, nv + 1, nv + 2], [nv + 3, nv + 4, nv + 5]],
+                                                         np.uint32)]),
+        tri_material=np.concatenate([base.tri_material, base.tri_material[:2]]),
+        tri_mask=None if base.tri_mask is None else np.concatenate([base.tri_mask, [3, 3]]).astype(np.uint8))
+    for sc, rays, hit_frac in ((huge, random_rays(huge, 2000, seed=33), 0.0),
+                               (spread, random_rays(base, 2000, seed=34), 0.3)):
+        rays[:, 7] = 1e30
+        po, to, uo, vo = orc.OracleScene(sc).trace_closest(rays)
+        pg, tg, ug, vg, used = runtime.host_trace(sc, rays, width=width)
+        assert used in (2, width)
+        assert np.mean(po != 0xFFFFFFFF) >= hit_frac
+        assert np.array_equal(po, pg)
+        for a, b in ((to, tg), (uo, ug), (vo, vg)):
+            assert np.array_equal(bits(a), bits(b))
+    for bad in (np.nan, np.inf, -np.inf):
+        pos = base.positions.copy()
+        pos[5, 1] = bad
+        with pytest.raises(runtime.CxError):
+            runtime.host_trace(dataclasses.replace(base, positions=pos), random_rays(base, 10, seed=1), width=width)
