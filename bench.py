@@ -97,29 +97,31 @@ def traffic_from_profiles(kernel, width):
 
 def rel_mse_vs_truth(runtime, ctx, gscene, scene, frames, exec_mode):
     """rel-MSE = mean over pixels and channels of (I - I_ref)^2 / (I_ref^2 + 1e-2) of HIP frames
-    of a static view against the committed FP64 truth of the PathTracer estimator
-    (tests/golden/truth_objgrid16_96x54.npz, written by tests/golden/make_truth.py).  The frames
-    are kept as 16 batch means: the rel-MSE of their plain mean (finite pixels; non-finite ones
-    counted) and of their per-pixel median are reported, since the reference's ReSTIR
-    estimators are heavy-tailed (crisoptix_amd/metrics.py)."""
+    of a static view against committed FP64 truths of the PathTracer estimator (written by
+    tests/golden/make_truth.py): the path tracer at maxPathLen 8 against the truth at 8, and
+    Conditional ReSTIR at maxPathLen 8 against the truth at 7, the path space it covers (its
+    suffix stops one vertex early).  The frames are kept as 16 batch means: the rel-MSE of their
+    plain mean (finite pixels; non-finite ones counted) and of their per-pixel median, each beside
+    the rel-MSE of an all-black image and with the exposure normalised to a unit-mean truth
+    (crisoptix_amd/metrics.py), since the reference's ReSTIR estimators are heavy-tailed."""
     from crisoptix_amd import abi, metrics, scenes
-    path = os.path.join(ROOT, "tests", "golden", "truth_objgrid16_96x54.npz")
-    with np.load(path, allow_pickle=False) as z:
-        truth = z["mean"]
-        tcam = z["camera"]
-        spp = int(z["spp"])
-    h, w = truth.shape[:2]
-    cam = scenes.orbit_camera(scene, w, h)
-    cd = abi.CameraData.from_dict(cam.data())
-    if not np.array_equal(np.concatenate([np.asarray(getattr(cd, k), np.float32) for k in ("pos", "U", "V", "W")]),
-                          tcam):
-        raise RuntimeError("truth fixture was rendered from another view")
-    out = {"frames": frames, "view": f"{w}x{h} static, orbit camera frame 0", "truth_spp": spp}
-    for mode in ("conditional_restir", "path_tracer"):
+    out = {"frames": frames}
+    for mode, name in (("conditional_restir", "objgrid16_96x54_L7"), ("path_tracer", "objgrid16_96x54")):
+        path = os.path.join(ROOT, "tests", "golden", f"truth_{name}.npz")
+        with np.load(path, allow_pickle=False) as z:
+            truth, tvar, tcam, spp = z["mean"], z["var"], z["camera"], int(z["spp"])
+        h, w = truth.shape[:2]
+        cam = scenes.orbit_camera(scene, w, h)
+        cd = abi.CameraData.from_dict(cam.data())
+        if not np.array_equal(np.concatenate([np.asarray(getattr(cd, k), np.float32) for k in ("pos", "U", "V", "W")]),
+                              tcam):
+            raise RuntimeError(f"truth fixture {name} was rendered from another view")
         p = abi.workload_params(mode, w, h, gscene.size)
         ren = runtime.Renderer(w, h, ctx, cam, gscene, params=p, seed=3)
         ren.set_exec_mode(exec_mode)
-        out[mode] = metrics.batch_report(metrics.frame_batches(ren, cd, frames, 16), truth)
+        rep = metrics.batch_report(metrics.frame_batches(ren, cd, frames, 16), truth, tvar)
+        rep.update({"truth": name, "truth_spp": spp, "view": f"{w}x{h} static, orbit camera frame 0"})
+        out[mode] = {k: (round(v, 7) if isinstance(v, float) else v) for k, v in rep.items()}
     return out
 
 

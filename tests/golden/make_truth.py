@@ -32,6 +32,10 @@ TRUTHS = {
                           maxPathLen=4, neeProb=0.3, spp=65536),
     "objgrid16_96x54": dict(scene=("object_grid", dict(grid=16, seed=2, lights="tent", trace_emitters=True)),
                             camera="orbit", W=96, H=54, maxPathLen=8, neeProb=0.3, spp=65536),
+    # Conditional ReSTIR at maxPathLen 8 carries light from vertices 1..7 (its suffix stops one
+    # vertex early): the same path space as the path tracer at 7
+    "objgrid16_96x54_L7": dict(scene=("object_grid", dict(grid=16, seed=2, lights="tent", trace_emitters=True)),
+                               camera="orbit", W=96, H=54, maxPathLen=7, neeProb=0.3, spp=65536),
 }
 
 
