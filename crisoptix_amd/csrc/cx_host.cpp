@@ -913,15 +913,36 @@ int cx_bvh_host_trace(const cx_scene_desc* d, const float* rays, uint32_t n, int
         void put(int i, int x) { e[i] = x; }
         int get(int i) const { return e[i]; }
     };
+    // The stepped traversal of the queue-trace kernel (trav_init / trav_step), one ray at a time.
+    auto run = [&](auto widthTag, auto anyTag, TravRay& tr, HostStack& st) {
+        constexpr int Wd = decltype(widthTag)::value;
+        constexpr bool Any = decltype(anyTag)::value;
+        while (!trav_step<Any, Wd>(S, tr, st)) {
+        }
+    };
     for (uint32_t i = 0; i < n; i++) {
         const float* r = rays + 8 * (size_t)i;
         HostStack st;
-        Hit h;
+        TravRay tr;
         const V3 o{r[0], r[1], r[2]}, dir{r[3], r[4], r[5]};
+        auto go = [&](auto anyTag) {
+            if (width == 4) {
+                trav_init<4>(S, tr, o, dir, r[6], r[7]);
+                run(std::integral_constant<int, 4>{}, anyTag, tr, st);
+            } else if (width == 8) {
+                trav_init<8>(S, tr, o, dir, r[6], r[7]);
+                run(std::integral_constant<int, 8>{}, anyTag, tr, st);
+            } else {
+                trav_init<2>(S, tr, o, dir, r[6], r[7]);
+                run(std::integral_constant<int, 2>{}, anyTag, tr, st);
+            }
+        };
         if (anyHit) {
-            prim[i] = traverse_s<true>(S, o, dir, r[6], r[7], h, st) ? 1u : 0u;
+            go(std::true_type{});
+            prim[i] = tr.best.prim != INVALID_PRIM ? 1u : 0u;
         } else {
-            traverse_s<false>(S, o, dir, r[6], r[7], h, st);
+            go(std::false_type{});
+            const Hit& h = tr.best;
             const bool hit = h.prim != INVALID_PRIM;
             prim[i] = h.prim;
             t[i] = hit ? h.t : 0.0f;

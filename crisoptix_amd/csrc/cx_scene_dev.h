@@ -229,9 +229,83 @@ CX_HD void cswap(float& ta, int& ra, float& tb, int& rb) {
     ra = r;
 }
 
-// Wide traversal: test the N child boxes, sort the hit ones by entry distance (a fixed
-// compare-exchange network on registers, misses at +inf), continue with the nearest and push
-// the rest farthest first.
+// One wide node: test the N child boxes, sort the hit ones by entry distance (a fixed
+// compare-exchange network on registers, misses at +inf), push all but the nearest, farthest
+// first, and return the nearest in ref; false (ref unchanged) when no child box is hit.
+template <int N, typename Stack>
+CX_HD bool wide_visit(const DScene& S, int& ref, V3 o, V3 inv, float tmin, float tcap, Stack& st, int& sp) {
+    const float4* n = S.wnodes + (size_t)ref * (2 * N);
+    float tn[N];
+    int cr[N];
+#pragma unroll
+    for (int g = 0; g < N / 4; g++) {
+        const float4 lx = n[8 * g + 0], hx = n[8 * g + 1], ly = n[8 * g + 2], hy = n[8 * g + 3];
+        const float4 lz = n[8 * g + 4], hz = n[8 * g + 5];
+        const int4 rr = __builtin_bit_cast(int4, n[8 * g + 6]);
+        const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+        const int ra[4] = {rr.x, rr.y, rr.z, rr.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            bool h;
+            const float t = box1(lxa[j], hxa[j], lya[j], hya[j], lza[j], hza[j], o, inv, tmin, tcap, h);
+            const bool use = h && ra[j] != WIDE_EMPTY;
+            tn[4 * g + j] = use ? t : INFINITY;
+            cr[4 * g + j] = ra[j];
+        }
+    }
+    if constexpr (N == 4) {
+        cswap(tn[0], cr[0], tn[1], cr[1]);
+        cswap(tn[2], cr[2], tn[3], cr[3]);
+        cswap(tn[0], cr[0], tn[2], cr[2]);
+        cswap(tn[1], cr[1], tn[3], cr[3]);
+        cswap(tn[1], cr[1], tn[2], cr[2]);
+    } else {
+        // Batcher's odd-even merge sort of 8 keys (19 compare-exchanges).
+#define CX_CS(a, b) cswap(tn[a], cr[a], tn[b], cr[b])
+        CX_CS(0, 1); CX_CS(2, 3); CX_CS(4, 5); CX_CS(6, 7);
+        CX_CS(0, 2); CX_CS(1, 3); CX_CS(4, 6); CX_CS(5, 7);
+        CX_CS(1, 2); CX_CS(5, 6);
+        CX_CS(0, 4); CX_CS(1, 5); CX_CS(2, 6); CX_CS(3, 7);
+        CX_CS(2, 4); CX_CS(3, 5);
+        CX_CS(1, 2); CX_CS(3, 4); CX_CS(5, 6);
+#undef CX_CS
+    }
+    if (tn[0] == INFINITY) return false;
+#pragma unroll
+    for (int i = N - 1; i >= 1; i--) {
+        if (tn[i] != INFINITY) {
+            st.put(sp, cr[i]);
+            ++sp;
+        }
+    }
+    ref = cr[0];
+    return true;
+}
+
+// One binary node: the nearer hit child in ref (the farther one pushed); false when neither
+// child box is hit.
+template <typename Stack>
+CX_HD bool bin_visit(const DScene& S, int& ref, V3 o, V3 inv, float tmin, float tcap, Stack& st, int& sp) {
+    const float4* n = S.nodes + 4 * (size_t)ref;
+    const float4 a = n[0], b = n[1], c = n[2];
+    const int4 dd = __builtin_bit_cast(int4, n[3]);
+    bool h0, h1;
+    float t0, t1;
+    box2(a, b, c, o, inv, tmin, tcap, h0, h1, t0, t1);
+    if (h0 && h1) {
+        const bool first0 = t0 <= t1;
+        st.put(sp, first0 ? dd.y : dd.x);
+        ++sp;
+        ref = first0 ? dd.x : dd.y;
+        return true;
+    }
+    if (h0) { ref = dd.x; return true; }
+    if (h1) { ref = dd.y; return true; }
+    return false;
+}
+
 template <bool ANY, int N, typename Stack>
 CX_HD bool traverse_wide(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit& best,
                                               Stack& st) {
@@ -241,55 +315,7 @@ CX_HD bool traverse_wide(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hi
     int ref = S.wroot;
     while (true) {
         if (ref >= 0) {
-            const float4* n = S.wnodes + (size_t)ref * (2 * N);
-            float tn[N];
-            int cr[N];
-#pragma unroll
-            for (int g = 0; g < N / 4; g++) {
-                const float4 lx = n[8 * g + 0], hx = n[8 * g + 1], ly = n[8 * g + 2], hy = n[8 * g + 3];
-                const float4 lz = n[8 * g + 4], hz = n[8 * g + 5];
-                const int4 rr = __builtin_bit_cast(int4, n[8 * g + 6]);
-                const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-                const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-                const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-                const int ra[4] = {rr.x, rr.y, rr.z, rr.w};
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    bool h;
-                    const float t = box1(lxa[j], hxa[j], lya[j], hya[j], lza[j], hza[j], o, inv, tmin, best.t, h);
-                    const bool use = h && ra[j] != WIDE_EMPTY;
-                    tn[4 * g + j] = use ? t : INFINITY;
-                    cr[4 * g + j] = ra[j];
-                }
-            }
-            if constexpr (N == 4) {
-                cswap(tn[0], cr[0], tn[1], cr[1]);
-                cswap(tn[2], cr[2], tn[3], cr[3]);
-                cswap(tn[0], cr[0], tn[2], cr[2]);
-                cswap(tn[1], cr[1], tn[3], cr[3]);
-                cswap(tn[1], cr[1], tn[2], cr[2]);
-            } else {
-                // Batcher's odd-even merge sort of 8 keys (19 compare-exchanges).
-#define CX_CS(a, b) cswap(tn[a], cr[a], tn[b], cr[b])
-                CX_CS(0, 1); CX_CS(2, 3); CX_CS(4, 5); CX_CS(6, 7);
-                CX_CS(0, 2); CX_CS(1, 3); CX_CS(4, 6); CX_CS(5, 7);
-                CX_CS(1, 2); CX_CS(5, 6);
-                CX_CS(0, 4); CX_CS(1, 5); CX_CS(2, 6); CX_CS(3, 7);
-                CX_CS(2, 4); CX_CS(3, 5);
-                CX_CS(1, 2); CX_CS(3, 4); CX_CS(5, 6);
-#undef CX_CS
-            }
-            if (tn[0] != INFINITY) {
-#pragma unroll
-                for (int i = N - 1; i >= 1; i--) {
-                    if (tn[i] != INFINITY) {
-                        st.put(sp, cr[i]);
-                        ++sp;
-                    }
-                }
-                ref = cr[0];
-                continue;
-            }
+            if (wide_visit<N>(S, ref, o, inv, tmin, best.t, st, sp)) continue;
         } else {
             if (leaf_hits<ANY>(S, ref, o, rp, tmin, tmax, best)) return true;
         }
@@ -314,21 +340,7 @@ CX_HD bool traverse_bin(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit
     int ref = S.root;
     while (true) {
         if (ref >= 0) {
-            const float4* n = S.nodes + 4 * (size_t)ref;
-            const float4 a = n[0], b = n[1], c = n[2];
-            const int4 dd = __builtin_bit_cast(int4, n[3]);
-            bool h0, h1;
-            float t0, t1;
-            box2(a, b, c, o, inv, tmin, best.t, h0, h1, t0, t1);
-            if (h0 && h1) {
-                const bool first0 = t0 <= t1;
-                st.put(sp, first0 ? dd.y : dd.x);
-                ++sp;
-                ref = first0 ? dd.x : dd.y;
-                continue;
-            }
-            if (h0) { ref = dd.x; continue; }
-            if (h1) { ref = dd.y; continue; }
+            if (bin_visit(S, ref, o, inv, tmin, best.t, st, sp)) continue;
         } else {
             if (leaf_hits<ANY>(S, ref, o, rp, tmin, tmax, best)) return true;
         }
@@ -337,6 +349,51 @@ CX_HD bool traverse_bin(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit
         ref = st.get(sp);
     }
     return best.prim != INVALID_PRIM;
+}
+
+// The traversal loop above as a resumable state, advanced one node or one leaf per step, for
+// kernels that interleave several rays per lane (a lane whose ray ends takes the next one
+// without waiting for the rest of its wavefront).  The answer is the loop's: the same nodes in
+// the same order.
+struct TravRay {
+    V3 o, inv;
+    RayPre rp;
+    float tmin, tmax;
+    Hit best;
+    int ref;
+    int sp;
+};
+
+template <int WIDTH>
+CX_HD void trav_init(const DScene& S, TravRay& r, V3 o, V3 d, float tmin, float tmax) {
+    r.o = o;
+    r.inv = V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    r.rp = ray_pre(d);
+    r.tmin = tmin;
+    r.tmax = tmax;
+    r.best = Hit{INVALID_PRIM, tmax, 0.0f, 0.0f};
+    r.ref = WIDTH == 2 ? S.root : S.wroot;
+    r.sp = 0;
+}
+
+// One step; true when the ray is answered: closest hit in r.best (prim INVALID_PRIM: a miss),
+// occlusion as r.best.prim != INVALID_PRIM.  A scene without triangles answers at once.
+template <bool ANY, int WIDTH, typename Stack>
+CX_HD bool trav_step(const DScene& S, TravRay& r, Stack& st) {
+    if (S.numTris == 0) return true;
+    if (r.ref >= 0) {
+        if constexpr (WIDTH == 2) {
+            if (bin_visit(S, r.ref, r.o, r.inv, r.tmin, r.best.t, st, r.sp)) return false;
+        } else {
+            if (wide_visit<WIDTH>(S, r.ref, r.o, r.inv, r.tmin, r.best.t, st, r.sp)) return false;
+        }
+    } else {
+        if (leaf_hits<ANY>(S, r.ref, r.o, r.rp, r.tmin, r.tmax, r.best)) return true;
+    }
+    if (r.sp == 0) return true;
+    --r.sp;
+    r.ref = st.get(r.sp);
+    return false;
 }
 
 // Traversal at a BVH width fixed at compile time (WIDTH = S.width; the caller dispatches).

@@ -7,6 +7,9 @@
 // pixel in the reference's order.  The traversal kernels use a 16-entry LDS stack per thread
 // (deeper entries spill to private memory), so their occupancy is set by registers.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "cx_internal.h"
 
@@ -20,14 +23,62 @@ __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + thr
 
 // ---- traversal of a queue ----
 
-// Answers every ray of qin: closest-hit rays in [0, nc), shadow rays in [cap - ns, cap).
-// Grid-stride over the device-side counts, so the host never waits for them.  Also clears the
-// counts of qout (the next step kernel's output queue) and adds the rays to the pass counters.
-// The 2- and 4-wide kernels are held to 64 VGPRs (8 waves per SIMD): the leaf loop's prefetch of
-// the next triangle would otherwise take the 4-wide kernel to 66 and 7 waves (measured on the
-// 1080p frame: traversal 4.71 ms at 8 waves, 4.78-4.80 at 7, 4.79-4.81 without the prefetch).
+// The rays of one contiguous range [begin, end) of qin's index space, traced by one wavefront
+// whose lanes each hold one ray at a time: a lane whose ray is answered writes the result and
+// idles until at least TRACE_REFILL lanes of the wavefront (or as many as rays remain) are idle,
+// and then those lanes take the range's next rays together.  A wavefront so stays busy until its
+// range is drained, instead of waiting at every ray for its slowest lane; refilling in groups
+// keeps the wavefront from stalling on a ray load at every step.
+static constexpr uint32_t TRACE_REFILL = 16;
+
+template <bool ANY, int WIDTH, typename Stack>
+__device__ __forceinline__ void trace_range(const DScene& S, const Queue& qin, const QIndex& qi, uint32_t begin,
+                                            uint32_t end, Stack& st) {
+    const uint64_t below = (1ull << (threadIdx.x & 63)) - 1ull;
+    uint32_t cursor = begin;   // wavefront-uniform
+    bool has = false;
+    uint32_t slot = 0;
+    TravRay r;
+    while (true) {
+        const uint64_t idle = __ballot(!has);
+        if (cursor < end) {
+            const uint32_t nIdle = (uint32_t)__popcll(idle), left = end - cursor;
+            if (nIdle >= (left < TRACE_REFILL ? left : TRACE_REFILL)) {
+                if (!has) {
+                    const uint32_t i = cursor + (uint32_t)__popcll(idle & below);
+                    if (i < end) {
+                        slot = in_slot(qin, qi, i);
+                        const float4 a = qin.ray[slot], b = qin.ray[(size_t)qin.cap + slot];
+                        trav_init<WIDTH>(S, r, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w);
+                        has = true;
+                    }
+                }
+                cursor += nIdle;
+            }
+        }
+        if (!has) {
+            if (cursor >= end) break;
+            continue;
+        }
+        if (trav_step<ANY, WIDTH>(S, r, st)) {
+            const Hit& h = r.best;
+            const bool hit = h.prim != INVALID_PRIM;
+            if (ANY) qin.res[slot] = uint4{hit ? 1u : 0u, 0u, 0u, 0u};
+            else qin.res[slot] = uint4{h.prim, fb(hit ? h.t : 0.0f), fb(hit ? h.u : 0.0f), fb(hit ? h.v : 0.0f)};
+            has = false;
+        }
+    }
+}
+
+// Answers every ray of qin in place (closest hit: prim, t, u, v; occlusion: the flag in x),
+// zeroes the stripe counts of qout (the next step kernel's output queue) and adds the rays to
+// the pass counters.  The closest range and then the shadow range are cut into one contiguous
+// range per wavefront, each traced by trace_range, and the grid is what fits on the device at
+// once.  The 2- and 4-wide kernels run at 5 waves per SIMD (102 VGPRs, no spills; measured on the
+// 1080p frame, per launch: 0.96 ms at 5 waves, 0.98 at 6, 1.05 at 4, 1.37 at 8 with 27 VGPRs
+// spilled; 1.04 ms for the former one-ray-per-iteration kernel at 8 waves).
 template <int WIDTH>
-__global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH == 8 ? 1 : 8, 8))) void k_q_trace(
+__global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH == 8 ? 1 : 5, 8))) void k_q_trace(
     DScene S, Queue qin, Queue qout, unsigned long long* counters) {
     __shared__ int s_st[SHORT_STACK * WBLOCK];
     __shared__ QIndex qi;
@@ -44,20 +95,11 @@ __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH ==
     TravStack<SHORT_STACK> st;
     st.lds = s_st + threadIdx.x;
     st.stride = WBLOCK;
-    const uint32_t n = nc + ns;
-    for (uint32_t i = blockIdx.x * WBLOCK + threadIdx.x; i < n; i += gridDim.x * WBLOCK) {
-        const uint32_t slot = in_slot(qin, qi, i);
-        const float4 a = qin.ray[slot], b = qin.ray[(size_t)qin.cap + slot];
-        Hit h;
-        if (i < nc) {
-            traverse_w<false, WIDTH>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
-            const bool hit = h.prim != INVALID_PRIM;
-            qin.res[slot] = uint4{h.prim, fb(hit ? h.t : 0.0f), fb(hit ? h.u : 0.0f), fb(hit ? h.v : 0.0f)};
-        } else {
-            const bool occ = traverse_w<true, WIDTH>(S, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, h, st);
-            qin.res[slot] = uint4{occ ? 1u : 0u, 0u, 0u, 0u};
-        }
-    }
+    const uint32_t waves = gridDim.x * (WBLOCK / 64), w = blockIdx.x * (WBLOCK / 64) + (threadIdx.x >> 6);
+    const uint32_t perC = (nc + waves - 1) / waves, perS = (ns + waves - 1) / waves;
+    const uint32_t c0 = min(w * perC, nc), s0 = min(w * perS, ns);
+    trace_range<false, WIDTH>(S, qin, qi, c0, min(c0 + perC, nc), st);
+    trace_range<true, WIDTH>(S, qin, qi, nc + s0, nc + min(s0 + perS, ns), st);
 }
 
 // Grid-stride loop over the jobs of qin whose bound is uniform per block, so every lane of a
@@ -1188,6 +1230,33 @@ static dim3 grid_for(uint32_t n) { return dim3((n + WBLOCK - 1) / WBLOCK); }
 // (the renderer's queue-grid setting, 8192 by default: 32 blocks of 128 threads per CU).
 static dim3 stride_grid(const Queue& q) { return dim3(queue_grid_blocks(q.jobs, q.maxBlocks)); }
 
+// Blocks of WBLOCK threads of `kernel` resident on the current device at once (its occupancy
+// times the CU count), cached per kernel and device: the grid of the kernels that cut their
+// input into one range per wavefront, whose ranges shrink as the grid grows past it.
+template <typename K>
+static uint32_t resident_blocks(K kernel) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, uint32_t> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1u << 20;
+    const auto key = std::make_pair(reinterpret_cast<const void*>(kernel), dev);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int perCu = 0, cus = 0;
+    uint32_t v = 1u << 20;   // unknown: leave the queue's own grid
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, WBLOCK, 0) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && perCu > 0 && cus > 0)
+        v = (uint32_t)perCu * (uint32_t)cus;
+    cache.emplace(key, v);
+    return v;
+}
+
+template <typename K>
+static dim3 range_grid(const Queue& q, K kernel) {
+    return dim3(std::max(1u, std::min(stride_grid(q).x, resident_blocks(kernel))));
+}
+
 hipError_t wave_zero_counts(uint32_t* c, hipStream_t s) {
     hipLaunchKernelGGL(k_w_zero2, dim3(1), dim3(64), 0, s, c);
     return hipGetLastError();
@@ -1195,10 +1264,10 @@ hipError_t wave_zero_counts(uint32_t* c, hipStream_t s) {
 
 hipError_t wave_trace(const DScene& S, const Queue& qin, const Queue& qout, unsigned long long* counters,
                       hipStream_t s) {
-    const dim3 g = stride_grid(qin), b(WBLOCK);
-    if (S.width == 4) hipLaunchKernelGGL(k_q_trace<4>, g, b, 0, s, S, qin, qout, counters);
-    else if (S.width == 8) hipLaunchKernelGGL(k_q_trace<8>, g, b, 0, s, S, qin, qout, counters);
-    else hipLaunchKernelGGL(k_q_trace<2>, g, b, 0, s, S, qin, qout, counters);
+    const dim3 b(WBLOCK);
+    if (S.width == 4) hipLaunchKernelGGL(k_q_trace<4>, range_grid(qin, k_q_trace<4>), b, 0, s, S, qin, qout, counters);
+    else if (S.width == 8) hipLaunchKernelGGL(k_q_trace<8>, range_grid(qin, k_q_trace<8>), b, 0, s, S, qin, qout, counters);
+    else hipLaunchKernelGGL(k_q_trace<2>, range_grid(qin, k_q_trace<2>), b, 0, s, S, qin, qout, counters);
     return hipGetLastError();
 }
 

@@ -15,7 +15,7 @@ replayed estimate along seeded random directions and for the largest gradients. 
 JSON line with timings and agreement.
 
 optimize: 8 views at 640 x 480 (orbit camera at 8 angles) of the object grid with 16 x 16
-objects (icospheres of subdivision 3 and boxes, about 330 k triangles, 256 object materials +
+objects on a finely tessellated ground (optimize_scene: 407 k triangles, 256 object materials +
 ground).  Ground-truth parameters are the scene's; the loop starts from albedo and roughness
 U[0.3, 0.7] and runs Adam (lr 1e-2, torch.optim) on the summed L2 loss of the 8 views; each step
 renders per view one frame for the residual and an independent frame whose adjoint gives the
@@ -41,6 +41,13 @@ import time
 import numpy as np
 
 from . import abi, runtime, scenes
+
+
+def optimize_scene():
+    """The inverse-rendering scene: 16 x 16 objects (icospheres of subdivision 3 and boxes, one
+    material each) on a 290 x 290-quad ground, 407,120 triangles and 256 object materials +
+    ground, lit by 16 light panels."""
+    return scenes.object_grid(16, seed=3, ico_subdiv=3, ground_quads=290, trace_emitters=False, lights="panels")
 
 
 def _params(size, w, h):
@@ -296,7 +303,7 @@ def optimize(a) -> dict:
     from .autograd import DiffRenderer
 
     ctx = runtime.Context(0)
-    sc = scenes.object_grid(16, seed=3, ico_subdiv=3, trace_emitters=False, lights="panels")
+    sc = optimize_scene()
     gs = runtime.Scene(ctx, sc)
     W, H = a.width, a.height
     truth = sc.materials.astype(np.float32)
@@ -370,7 +377,7 @@ def optimize(a) -> dict:
         with torch.no_grad():
             albedo.clamp_(0.01, 0.99)
             rough.clamp_(0.02, 1.0)
-        if step % 20 == 0 or step == a.steps - 1:
+        if step % a.log_every == 0 or step == a.steps - 1:
             hist.append({"step": step, "loss": total, "rel_err": rel_err(), "rel_err_observed": rel_err(observed)})
             print(f"step {step}: loss {total:.6f}, rel err albedo {hist[-1]['rel_err'][0]:.4f} "
                   f"roughness {hist[-1]['rel_err'][1]:.4f}; observed materials: albedo "
@@ -422,7 +429,7 @@ def _optimize_device(a, sc, gs, views, targets, clips, truth, albedo, rough, obs
             ds.replay()
         else:
             ds.step()
-        if step % 20 == 0 or step == a.steps - 1:
+        if step % a.log_every == 0 or step == a.steps - 1:
             hist.append({"step": step, "loss": ds.loss(), "rel_err": rel_err(), "rel_err_observed": rel_err(observed)})
             print(f"step {step}: loss {hist[-1]['loss']:.6f}, rel err albedo {hist[-1]['rel_err'][0]:.4f} "
                   f"roughness {hist[-1]['rel_err'][1]:.4f}", file=sys.stderr, flush=True)
@@ -459,6 +466,7 @@ def main(argv=None):
     ap.add_argument("--views", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-2)
     ap.add_argument("--min-pixels", type=int, default=200)
+    ap.add_argument("--log-every", type=int, default=20, help="optimize: steps between history entries")
     ap.add_argument("--exec", choices=["host", "device", "graph"], default="graph",
                     help="optimize: host round trips per view (host), a device-resident step (device), or that "
                          "step captured once and replayed (graph)")
