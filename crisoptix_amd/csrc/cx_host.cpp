@@ -86,6 +86,7 @@ struct cx_renderer {
         uint4* prefixScratch = nullptr;
         uint4* suffixScratch = nullptr;
         uint4* packedScratch = nullptr;
+        uint4* suffixAos = nullptr;   // record-major copy of the front suffix reservoirs (final gather)
         uint64_t* rngScratch = nullptr;
         uint32_t* pixFlags = nullptr;
         uint32_t* validMask = nullptr;
@@ -330,6 +331,7 @@ int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
         A(dalloc(w.pixelAllocs, &w.prefixScratch, (size_t)PREFIX_CHUNKS * P));
         A(dalloc(w.pixelAllocs, &w.suffixScratch, (size_t)SUFFIX_CHUNKS * P));
         A(dalloc(w.pixelAllocs, &w.packedScratch, 2 * P));
+        A(dalloc(w.pixelAllocs, &w.suffixAos, (size_t)SUFFIX_AOS_STRIDE * P));
         A(dalloc(w.pixelAllocs, &w.rngScratch, P));
         A(dalloc(w.pixelAllocs, &w.pixFlags, P));
         A(dalloc(w.pixelAllocs, &w.validMask, P));
@@ -415,6 +417,7 @@ WaveArgs make_wave_args(const cx_renderer* r, const FrameArgs& A) {
     W.prefixScratch = w.prefixScratch;
     W.suffixScratch = w.suffixScratch;
     W.packedScratch = w.packedScratch;
+    W.suffixAos = w.suffixAos;
     W.rngScratch = w.rngScratch;
     W.shiftRes = w.shiftRes;
     W.pixFlags = w.pixFlags;
@@ -532,7 +535,10 @@ hipError_t wave_pass(cx_renderer* r, int pass, const FrameArgs& A, hipStream_t s
         case CX_PASS_FINAL_GATHER: {
             const bool more = A.k > 0 && lp.gatherN > 1;
             hipStream_t side = r->fgSide;
-            e = run_machine(r, W, WK_GATHER_INIT, WK_PG_STEP, P, pgRounds, pass, s);
+            // the neighbours' records, read at random by the shifts and the combination
+            e = wave_launch(WK_SUFFIX_AOS, W, P, s);
+            W.R.suffixFrontAos = W.suffixAos;
+            if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_INIT, WK_PG_STEP, P, pgRounds, pass, s);
             if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SUFFIX_INIT, WK_SG_STEP, P, sgRounds, pass, s);
             // the entry store and tree build, forked to `side`, join before the query
             if (e == hipSuccess && side) {

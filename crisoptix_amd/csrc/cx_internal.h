@@ -87,6 +87,7 @@ enum WaveKernel {
     WK_GATHER_COMBINE_LAST,
     WK_FG_INIT,
     WK_FG_STEP,
+    WK_SUFFIX_AOS,
 };
 
 hipError_t wave_zero_counts(uint32_t* c, hipStream_t s);

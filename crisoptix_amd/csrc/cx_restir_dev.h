@@ -123,9 +123,8 @@ __device__ __forceinline__ void store_prefix(uint4* b, uint32_t P, uint32_t p, c
 
 __device__ __forceinline__ uint32_t load_suffix_flags(const uint4* b, uint32_t P, uint32_t p) { return b[p].z; }
 
-__device__ __forceinline__ SuffixRes load_suffix(const uint4* b, uint32_t P, uint32_t p) {
-    const uint4 c0 = ld4(b, P, 0, p), c1 = ld4(b, P, 1, p), c2 = ld4(b, P, 2, p), c3 = ld4(b, P, 3, p);
-    const uint4 c4 = ld4(b, P, 4, p), c5 = ld4(b, P, 5, p), c6 = ld4(b, P, 6, p);
+__device__ __forceinline__ SuffixRes suffix_from_chunks(uint4 c0, uint4 c1, uint4 c2, uint4 c3, uint4 c4, uint4 c5,
+                                                        uint4 c6) {
     SuffixRes r;
     r.s.rng = (uint64_t)c0.x | ((uint64_t)c0.y << 32);
     r.s.flags = c0.z;
@@ -138,6 +137,26 @@ __device__ __forceinline__ SuffixRes load_suffix(const uint4* b, uint32_t P, uin
     r.s.recon = Packed{c3.w, bf(c4.x), bf(c4.y), V3{bf(c4.z), bf(c4.w), bf(c5.x)}};
     r.s.lastPrefix = Packed{c5.y, bf(c5.z), bf(c5.w), V3{bf(c6.x), bf(c6.y), bf(c6.z)}};
     return r;
+}
+
+__device__ __forceinline__ SuffixRes load_suffix(const uint4* b, uint32_t P, uint32_t p) {
+    return suffix_from_chunks(ld4(b, P, 0, p), ld4(b, P, 1, p), ld4(b, P, 2, p), ld4(b, P, 3, p), ld4(b, P, 4, p),
+                              ld4(b, P, 5, p), ld4(b, P, 6, p));
+}
+
+// Record-major copy of suffix reservoirs for random access: the SUFFIX_CHUNKS chunks of pixel p
+// at b[p * SUFFIX_AOS_STRIDE + c], one 128-byte line per record (the last word is padding), so a
+// gather that reads a neighbour's record touches one line instead of SUFFIX_CHUNKS.
+static constexpr uint32_t SUFFIX_AOS_STRIDE = 8;
+static_assert(SUFFIX_CHUNKS < (int)SUFFIX_AOS_STRIDE, "a suffix record fits one 128-byte line");
+
+__device__ __forceinline__ SuffixRes load_suffix_aos(const uint4* b, uint32_t p) {
+    const uint4* r = b + (size_t)p * SUFFIX_AOS_STRIDE;
+    return suffix_from_chunks(r[0], r[1], r[2], r[3], r[4], r[5], r[6]);
+}
+
+__device__ __forceinline__ uint32_t load_suffix_flags_aos(const uint4* b, uint32_t p) {
+    return b[(size_t)p * SUFFIX_AOS_STRIDE].z;
 }
 
 __device__ __forceinline__ void store_suffix(uint4* b, uint32_t P, uint32_t p, const SuffixRes& r) {

@@ -794,6 +794,7 @@ struct RecordSet {
     const uint4* prefixScratch;   // canonical prefix records (PrefixRes layout)
     const uint4* suffixScratch;   // canonical suffix records (SuffixRes layout)
     const uint4* packedScratch;   // [P] x 2 chunks: Packed of a traced interaction
+    const uint4* suffixFrontAos;  // record-major copy of suffixRes[front], or null (final gather only)
     uint32_t front, back, P;
 };
 
@@ -806,6 +807,7 @@ __device__ __forceinline__ Prefix load_prefix_ref(const RecordSet& R, uint32_t r
 
 __device__ __forceinline__ Suffix load_suffix_ref(const RecordSet& R, uint32_t ref) {
     const uint32_t buf = ref >> 28, p = ref & 0x0FFFFFFFu;
+    if (buf == REF_SUFFIX_FRONT && R.suffixFrontAos) return load_suffix_aos(R.suffixFrontAos, p).s;
     const uint4* b = buf == REF_SUFFIX_FRONT ? R.suffixRes[R.front]
                                              : (buf == REF_SUFFIX_BACK ? R.suffixRes[R.back] : R.suffixScratch);
     return load_suffix(b, R.P, p).s;
@@ -833,6 +835,7 @@ struct WaveArgs {
     uint4* prefixScratch;   // [PREFIX_CHUNKS][P] canonical prefix of the pass
     uint4* suffixScratch;   // [SUFFIX_CHUNKS][P] canonical suffix of the pass
     uint4* packedScratch;   // [P][2] last vertex of a final-gather prefix (Packed)
+    uint4* suffixAos;       // [P][SUFFIX_AOS_STRIDE] record-major copy of the front suffix reservoirs
     uint64_t* rngScratch;   // [P] pixel generator after path generation / neighbour selection
     float4* shiftRes;       // [jobs] f.xyz, Jacobian of each finished shift
     uint32_t* pixFlags;     // [P] per-pass pixel flags

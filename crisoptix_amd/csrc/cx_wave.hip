@@ -840,6 +840,21 @@ __device__ __forceinline__ void neighbor_stats(const WaveArgs& W, uint32_t count
     if (W.A.lp.trackPrefixStats) neighbor_stat(W.A.counters, count);
 }
 
+// Record-major copy of the front suffix reservoirs (the final gather's neighbours, read at
+// random: one 128-byte line per record instead of one line per chunk).  The final gather reads
+// the neighbours through it and writes no suffix reservoir, so the copy stays exact.
+__global__ __launch_bounds__(WBLOCK) void k_w_suffix_aos(WaveArgs W) {
+    const FrameArgs& A = W.A;
+    const uint32_t p = gid();
+    if (p >= A.P) return;
+    const uint4* b = A.suffixRes[A.front];
+    uint4* o = W.suffixAos + (size_t)p * SUFFIX_AOS_STRIDE;
+#pragma unroll
+    for (int c = 0; c < SUFFIX_CHUNKS; c++) o[c] = ld4(b, A.P, c, p);
+#pragma unroll
+    for (int c = SUFFIX_CHUNKS; c < (int)SUFFIX_AOS_STRIDE; c++) o[c] = uint4{0u, 0u, 0u, 0u};
+}
+
 // After the canonical prefix and suffix: the neighbour query and the valid-neighbour mask
 // (final_gather.cu:90-133); gIdx > 0: the same for a gather prefix of FinalGatherNotCanon
 // (final_gather.cu:209-246).  Writes the
@@ -877,7 +892,7 @@ __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(KR <= 8 
             count = query_neighbors<KR>(W, p, q, s_st + threadIdx.x, WBLOCK);
             neighbor_stats(W, count);
             for (uint32_t i = 0; i < count; ++i) {
-                if (!((load_suffix_flags(A.suffixRes[A.front], A.P, A.neighIdx[(size_t)p * k + i]) >> 16) & 1u))
+                if (!((load_suffix_flags_aos(W.R.suffixFrontAos, A.neighIdx[(size_t)p * k + i]) >> 16) & 1u))
                     continue;
                 validFlags |= 1u << i;
             }
@@ -958,7 +973,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_combine(WaveArgs W, int las
                         V3 suffixContrib = v3(0.0f);
                         for (uint32_t i = 0; i < neighCount; ++i) {
                             if (!((validFlags >> i) & 1u)) continue;
-                            const SuffixRes nres = load_suffix(A.suffixRes[A.front], A.P, A.neighIdx[(size_t)p * k + i]);
+                            const SuffixRes nres = load_suffix_aos(W.R.suffixFrontAos, A.neighIdx[(size_t)p * k + i]);
                             const float4 a = W.shiftRes[((size_t)p * k + i) * 2];
                             const float4 b = W.shiftRes[((size_t)p * k + i) * 2 + 1];
                             const V3 fFromCanonOfNeigh = V3{a.x, a.y, a.z};
@@ -986,7 +1001,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_gather_combine(WaveArgs W, int las
             V3 suffixContrib = v3(0.0f);
             for (uint32_t i = 0; i < neighCount; ++i) {
                 if (!((validFlags >> i) & 1u)) continue;
-                const SuffixRes nres = load_suffix(A.suffixRes[A.front], A.P, A.neighIdx[(size_t)p * k + i]);
+                const SuffixRes nres = load_suffix_aos(W.R.suffixFrontAos, A.neighIdx[(size_t)p * k + i]);
                 const float4 a = W.shiftRes[((size_t)p * k + i) * 2];
                 const V3 f = V3{a.x, a.y, a.z};
                 const float jNC = a.w;
@@ -1324,6 +1339,7 @@ hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStrea
             else hipLaunchKernelGGL(k_w_gather_query<32>, g, b, gather_query_lds(W), s, W);
             hipLaunchKernelGGL(k_w_gather_setup, g, b, 0, s, W);
             break;
+        case WK_SUFFIX_AOS: hipLaunchKernelGGL(k_w_suffix_aos, g, b, 0, s, W); break;
         case WK_GATHER_COMBINE: hipLaunchKernelGGL(k_w_gather_combine, g, b, 0, s, W, 0); break;
         case WK_GATHER_COMBINE_LAST: hipLaunchKernelGGL(k_w_gather_combine, g, b, 0, s, W, 1); break;
         case WK_FG_INIT: hipLaunchKernelGGL(k_w_fg_init, g, b, 0, s, W); break;
