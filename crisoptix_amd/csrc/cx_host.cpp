@@ -536,7 +536,7 @@ hipError_t wave_pass(cx_renderer* r, int pass, const FrameArgs& A, hipStream_t s
             const bool more = A.k > 0 && lp.gatherN > 1;
             hipStream_t side = r->fgSide;
             // the neighbours' records, read at random by the shifts and the combination
-            e = wave_launch(WK_SUFFIX_AOS, W, P, s);
+            e = wave_launch(WK_SUFFIX_AOS, W, P * SUFFIX_AOS_STRIDE, s);
             W.R.suffixFrontAos = W.suffixAos;
             if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_INIT, WK_PG_STEP, P, pgRounds, pass, s);
             if (e == hipSuccess) e = run_machine(r, W, WK_GATHER_SUFFIX_INIT, WK_SG_STEP, P, sgRounds, pass, s);

@@ -55,7 +55,9 @@ CX_HD uint32_t queue_grid_blocks(uint32_t jobs, uint32_t maxBlocks) {
 // Slots per stripe for a machine of at most `jobs` jobs over P pixels: the most appends one
 // stripe can receive.  Wavefront w appends to stripe w mod 64, at most 64 rays per append
 // call.  A setup kernel (one thread per pixel, 2 wavefronts per block) makes ceil(jobs / P)
-// append calls per lane; a step kernel makes one per loop pass of its grid-stride loop.
+// append calls per lane; one with a thread per job (the final gather's shift starts) makes one
+// call per lane over ceil(jobs / P) times the wavefronts, which the same bound covers; a step
+// kernel makes one per loop pass of its grid-stride loop.
 CX_HD uint32_t queue_stripe_cap(uint32_t jobs, uint32_t P, uint32_t maxBlocks) {
     auto cdiv = [](uint64_t a, uint64_t b) { return (a + b - 1) / b; };
     const uint64_t setupWaves = 2 * cdiv(P, QUEUE_BLOCK);

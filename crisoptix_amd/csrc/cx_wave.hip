@@ -841,18 +841,16 @@ __device__ __forceinline__ void neighbor_stats(const WaveArgs& W, uint32_t count
 }
 
 // Record-major copy of the front suffix reservoirs (the final gather's neighbours, read at
-// random: one 128-byte line per record instead of one line per chunk).  The final gather reads
-// the neighbours through it and writes no suffix reservoir, so the copy stays exact.
+// random: one 128-byte line per record instead of one line per chunk).  One thread per 16-byte
+// word of the copy, so a wavefront writes 1 KiB of whole lines and reads 8 runs of 8 records
+// from the chunk arrays.  The final gather reads the neighbours through it and writes no suffix
+// reservoir, so the copy stays exact.
 __global__ __launch_bounds__(WBLOCK) void k_w_suffix_aos(WaveArgs W) {
     const FrameArgs& A = W.A;
-    const uint32_t p = gid();
-    if (p >= A.P) return;
-    const uint4* b = A.suffixRes[A.front];
-    uint4* o = W.suffixAos + (size_t)p * SUFFIX_AOS_STRIDE;
-#pragma unroll
-    for (int c = 0; c < SUFFIX_CHUNKS; c++) o[c] = ld4(b, A.P, c, p);
-#pragma unroll
-    for (int c = SUFFIX_CHUNKS; c < (int)SUFFIX_AOS_STRIDE; c++) o[c] = uint4{0u, 0u, 0u, 0u};
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)A.P * SUFFIX_AOS_STRIDE) return;
+    const uint32_t p = (uint32_t)(i / SUFFIX_AOS_STRIDE), c = (uint32_t)(i % SUFFIX_AOS_STRIDE);
+    W.suffixAos[i] = c < (uint32_t)SUFFIX_CHUNKS ? ld4(A.suffixRes[A.front], A.P, (int)c, p) : uint4{0u, 0u, 0u, 0u};
 }
 
 // After the canonical prefix and suffix: the neighbour query and the valid-neighbour mask
@@ -904,42 +902,38 @@ __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(KR <= 8 
 
 // The two shifts per valid neighbour of a queried pixel (final_gather.cu:135-177); gIdx > 0: one
 // shift per neighbour onto the gather prefix's traced last vertex (final_gather.cu:248-279).
-// Held to 5 waves per SIMD: 92 VGPRs without spilling, against 104 and 4 waves unconstrained.
+// One thread per shift (pixel p, neighbour slot i, direction d), so each thread makes one chain
+// of dependent record and vertex fetches instead of a pixel's k x 2 in a row (1.32 ms per 1080p
+// frame with one thread per pixel).  Held to 5 waves per SIMD (92 VGPRs without spilling).
 __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_w_gather_setup(WaveArgs W) {
     const FrameArgs& A = W.A;
-    const uint32_t p = gid();
-    const uint32_t k = A.k;
-    bool run = false;
-    uint32_t validFlags = 0;
+    const uint32_t k = A.k, dirs = W.gIdx == 0 ? 2u : 1u;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, per = (uint64_t)k * dirs;
+    const uint32_t p = (uint32_t)min(t / per, (uint64_t)A.P);   // per > 0: the host launches none for k = 0
+    const uint32_t i = (uint32_t)((t % per) / dirs), d = (uint32_t)(t % dirs);
+    bool want = false;
     if (p < A.P) {
         const uint32_t pf = W.pixFlags[p];
-        run = (pf & 1u) && (pf & 8u);
-        if (run) validFlags = W.validMask[p];
+        want = (pf & 1u) && (pf & 8u) && ((W.validMask[p] >> i) & 1u);
     }
-    const uint32_t dirs = W.gIdx == 0 ? 2u : 1u;
-    for (uint32_t i = 0; i < k; ++i) {
-        for (uint32_t d = 0; d < dirs; d++) {
-            Emit e = emit_none();
-            ShState s;
-            const uint32_t job = ((uint32_t)p * k + i) * 2 + d;
-            const bool want = run && ((validFlags >> i) & 1u);
-            if (want) {
-                const uint32_t n = A.neighIdx[(size_t)p * k + i];
-                const uint32_t neigh = make_ref(REF_SUFFIX_FRONT, n);
-                if (W.gIdx == 0) {
-                    const uint32_t canon = make_ref(REF_SUFFIX_SCRATCH, p);
-                    e = d == 0 ? sh_start_suffix(A.S, W.R, s, neigh, canon) : sh_start_suffix(A.S, W.R, s, canon, neigh);
-                } else {
-                    e = sh_start_suffix(A.S, W.R, s, neigh, make_ref(REF_PACKED_TRACED, p));
-                }
-                if (e.kind == EMIT_NONE) shift_finish(W, job, v3(0.0f), 0.0f);
-            }
-            const uint32_t slot = append(W.qout, e.kind);
-            if (want && e.kind != EMIT_NONE) {
-                store_ctx(W.qout, slot, sh_pack(s));
-                emit_job(W.qout, slot, job, e);
-            }
+    Emit e = emit_none();
+    ShState s;
+    const uint32_t job = ((uint32_t)p * k + i) * 2 + d;
+    if (want) {
+        const uint32_t n = A.neighIdx[(size_t)p * k + i];
+        const uint32_t neigh = make_ref(REF_SUFFIX_FRONT, n);
+        if (W.gIdx == 0) {
+            const uint32_t canon = make_ref(REF_SUFFIX_SCRATCH, p);
+            e = d == 0 ? sh_start_suffix(A.S, W.R, s, neigh, canon) : sh_start_suffix(A.S, W.R, s, canon, neigh);
+        } else {
+            e = sh_start_suffix(A.S, W.R, s, neigh, make_ref(REF_PACKED_TRACED, p));
         }
+        if (e.kind == EMIT_NONE) shift_finish(W, job, v3(0.0f), 0.0f);
+    }
+    const uint32_t slot = append(W.qout, e.kind);
+    if (want && e.kind != EMIT_NONE) {
+        store_ctx(W.qout, slot, sh_pack(s));
+        emit_job(W.qout, slot, job, e);
     }
 }
 
@@ -1334,11 +1328,14 @@ hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStrea
         case WK_SUFFIX_SPATIAL_SETUP: hipLaunchKernelGGL(k_w_spatial_setup<true>, g, b, 0, s, W); break;
         case WK_SUFFIX_SPATIAL_RESAMPLE: hipLaunchKernelGGL(k_w_suffix_spatial_resample, g, b, 0, s, W); break;
         case WK_GATHER_INIT: hipLaunchKernelGGL(k_w_gather_init, g, b, 0, s, W); break;
-        case WK_GATHER_SETUP:
+        case WK_GATHER_SETUP: {
             if (W.A.k <= 8) hipLaunchKernelGGL(k_w_gather_query<8>, g, b, gather_query_lds(W), s, W);
             else hipLaunchKernelGGL(k_w_gather_query<32>, g, b, gather_query_lds(W), s, W);
-            hipLaunchKernelGGL(k_w_gather_setup, g, b, 0, s, W);
+            const uint64_t shifts = (uint64_t)threads * W.A.k * (W.gIdx == 0 ? 2u : 1u);
+            if (shifts > 0)
+                hipLaunchKernelGGL(k_w_gather_setup, dim3((uint32_t)((shifts + WBLOCK - 1) / WBLOCK)), b, 0, s, W);
             break;
+        }
         case WK_SUFFIX_AOS: hipLaunchKernelGGL(k_w_suffix_aos, g, b, 0, s, W); break;
         case WK_GATHER_COMBINE: hipLaunchKernelGGL(k_w_gather_combine, g, b, 0, s, W, 0); break;
         case WK_GATHER_COMBINE_LAST: hipLaunchKernelGGL(k_w_gather_combine, g, b, 0, s, W, 1); break;
