@@ -90,6 +90,8 @@ struct cx_renderer {
         uint64_t* rngScratch = nullptr;
         uint32_t* pixFlags = nullptr;
         uint32_t* validMask = nullptr;
+        uint32_t* gatherList = nullptr;
+        uint32_t* gatherBlocks = nullptr;
         uint32_t* nbr = nullptr;
         float4* radScratch = nullptr;
         float4* fgScratch = nullptr;
@@ -335,6 +337,8 @@ int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
         A(dalloc(w.pixelAllocs, &w.rngScratch, P));
         A(dalloc(w.pixelAllocs, &w.pixFlags, P));
         A(dalloc(w.pixelAllocs, &w.validMask, P));
+        A(dalloc(w.pixelAllocs, &w.gatherList, P));
+        A(dalloc(w.pixelAllocs, &w.gatherBlocks, (P + QUEUE_BLOCK - 1) / QUEUE_BLOCK + 1));
         A(dalloc(w.pixelAllocs, &w.nbr, 16 * P));
         A(dalloc(w.pixelAllocs, &w.radScratch, P));
         A(dalloc(w.pixelAllocs, &w.fgScratch, P));
@@ -422,6 +426,8 @@ WaveArgs make_wave_args(const cx_renderer* r, const FrameArgs& A) {
     W.shiftRes = w.shiftRes;
     W.pixFlags = w.pixFlags;
     W.validMask = w.validMask;
+    W.gatherList = w.gatherList;
+    W.gatherBlocks = w.gatherBlocks;
     W.nbr = w.nbr;
     W.radScratch = w.radScratch;
     W.fgScratch = w.fgScratch;

@@ -842,6 +842,8 @@ struct WaveArgs {
     float4* shiftRes;       // [jobs] f.xyz, Jacobian of each finished shift
     uint32_t* pixFlags;     // [P] per-pass pixel flags
     uint32_t* validMask;    // [P] valid-neighbour mask of the final gather
+    uint32_t* gatherList;   // [P] the pixels that query neighbours, in pixel order
+    uint32_t* gatherBlocks; // [ceil(P / WBLOCK) + 1] querying pixels per block, then their offsets and total
     uint32_t* nbr;          // [P][16] neighbour pixel of each spatial slot
     float4* radScratch;     // [P] radiance summed over the final-gather prefixes
     float4* fgScratch;      // [P] final-gather prefix throughput xyz, pdf

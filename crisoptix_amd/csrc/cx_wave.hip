@@ -853,69 +853,131 @@ __global__ __launch_bounds__(WBLOCK) void k_w_suffix_aos(WaveArgs W) {
     W.suffixAos[i] = c < (uint32_t)SUFFIX_CHUNKS ? ld4(A.suffixRes[A.front], A.P, (int)c, p) : uint4{0u, 0u, 0u, 0u};
 }
 
-// After the canonical prefix and suffix: the neighbour query and the valid-neighbour mask
-// (final_gather.cu:90-133); gIdx > 0: the same for a gather prefix of FinalGatherNotCanon
-// (final_gather.cu:209-246).  Writes the
-// neighbours, the mask of those with a valid suffix (validMask) and pixFlags bit 3 (queried).
-// The tree walk is a chain of dependent fetches, so it runs apart from the register-heavy shift
-// starts (k_w_gather_setup), at more waves per SIMD; its deferred siblings take GATHER_QUERY_SHORT
-// stack entries of LDS per thread (dynamic shared memory, gather_query_lds bytes).
+// Whether pixel p queries neighbours in this gather: gIdx 0, a valid canonical prefix that did
+// not end in NEE and a valid canonical suffix (final_gather.cu:90-133); gIdx > 0, a valid gather
+// prefix (final_gather.cu:209-246).
+__device__ __forceinline__ bool gather_runs(const WaveArgs& W, uint32_t p, uint32_t pf) {
+    const FrameArgs& A = W.A;
+    if (!(pf & 1u) || A.k == 0) return false;
+    if (W.gIdx == 0) {
+        const uint32_t f = load_prefix_flags(W.prefixScratch, A.P, p);
+        return ((f >> 16) & 1u) && !((f >> 17) & 1u) && ((load_suffix_flags(W.suffixScratch, A.P, p) >> 16) & 1u);
+    }
+    return (pf & 4u) != 0u;
+}
+
+// The querying pixels, listed in pixel order so the query and the shift starts run on them
+// alone (a third of the pixels on the 1080p workload).  Pass 1 marks them (pixFlags bit 3,
+// "queried"; validMask 0 for the others) and counts them per block; a one-block scan turns the
+// counts into offsets (gatherBlocks[nb] = total); pass 2 writes each one at its block's offset
+// plus its rank in the block.
+__global__ __launch_bounds__(WBLOCK) void k_w_gather_mark(WaveArgs W) {
+    __shared__ uint32_t s_n[WBLOCK / 64];
+    const uint32_t p = gid();
+    bool run = false;
+    if (p < W.A.P) {
+        const uint32_t pf = W.pixFlags[p];
+        if (pf & 1u) {
+            run = gather_runs(W, p, pf);
+            W.pixFlags[p] = (pf & 7u) | (run ? 8u : 0u);
+            if (!run) W.validMask[p] = 0u;
+        }
+    }
+    const uint64_t m = __ballot(run);
+    if ((threadIdx.x & 63) == 0) s_n[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0;
+        for (int w = 0; w < WBLOCK / 64; w++) c += s_n[w];
+        W.gatherBlocks[blockIdx.x] = c;
+    }
+}
+
+// Exclusive scan of blocks[0, n) in place, total in blocks[n]; one block of 1024 threads, each
+// summing a run of ceil(n / 1024) counts.
+__global__ __launch_bounds__(1024) void k_w_gather_scan(uint32_t* blocks, uint32_t n) {
+    __shared__ uint32_t s[1024];
+    const uint32_t t = threadIdx.x, per = (n + 1023u) / 1024u, b0 = min(t * per, n), b1 = min(b0 + per, n);
+    uint32_t sum = 0;
+    for (uint32_t i = b0; i < b1; i++) sum += blocks[i];
+    s[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t v = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[t] - sum;
+    for (uint32_t i = b0; i < b1; i++) {
+        const uint32_t c = blocks[i];
+        blocks[i] = run;
+        run += c;
+    }
+    if (t == 1023u) blocks[n] = s[1023];
+}
+
+__global__ __launch_bounds__(WBLOCK) void k_w_gather_list(WaveArgs W) {
+    __shared__ uint32_t s_n[WBLOCK / 64];
+    const uint32_t p = gid();
+    const bool run = p < W.A.P && (W.pixFlags[p] & 9u) == 9u;
+    const uint64_t m = __ballot(run);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) s_n[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = W.gatherBlocks[blockIdx.x];
+    for (uint32_t i = 0; i < w; i++) base += s_n[i];
+    if (run) W.gatherList[base + (uint32_t)__popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull))] = p;
+}
+
+__device__ __forceinline__ uint32_t gather_total(const WaveArgs& W) {
+    return W.gatherBlocks[(W.A.P + WBLOCK - 1) / WBLOCK];
+}
+
+// The neighbour query of each listed pixel and its valid-neighbour mask (final_gather.cu:90-133;
+// gIdx > 0: final_gather.cu:209-246), one thread per listed pixel.  The tree walk is a chain of
+// dependent fetches, so it runs apart from the register-heavy shift starts (k_w_gather_setup), at
+// more waves per SIMD; its deferred siblings take GATHER_QUERY_SHORT stack entries of LDS per
+// thread (dynamic shared memory, gather_query_lds bytes).
 template <int KR>
 __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(KR <= 8 ? 8 : 1, 8))) void k_w_gather_query(
     WaveArgs W) {
     extern __shared__ int s_st[];
     const FrameArgs& A = W.A;
-    const uint32_t p = gid();
-    const bool px = p < A.P;
-    const uint32_t k = A.k;
-    bool run = false;
-    uint32_t validFlags = 0, count = 0;
-    if (px && (W.pixFlags[p] & 1u)) {
-        V3 q = v3(0.0f);
-        if (W.gIdx == 0) {
-            const uint32_t pf = load_prefix_flags(W.prefixScratch, A.P, p);
-            if (((pf >> 16) & 1u) && !((pf >> 17) & 1u) && ((load_suffix_flags(W.suffixScratch, A.P, p) >> 16) & 1u) &&
-                k > 0) {
-                const Inter last = rehydrate(A.S, load_suffix(W.suffixScratch, A.P, p).s.lastPrefix);
-                q = last.pos;
-                run = true;
-            }
-        } else if ((W.pixFlags[p] & 4u) && k > 0) {
-            const uint4 a = W.packedScratch[2 * (size_t)p], b = W.packedScratch[2 * (size_t)p + 1];
-            const Packed pk = Packed{a.x, bf(a.y), bf(a.z), V3{bf(a.w), bf(b.x), bf(b.y)}};
-            q = inter_restore(A.S, pk, true).pos;
-            run = true;
-        }
-        if (run) {
-            count = query_neighbors<KR>(W, p, q, s_st + threadIdx.x, WBLOCK);
-            neighbor_stats(W, count);
-            for (uint32_t i = 0; i < count; ++i) {
-                if (!((load_suffix_flags_aos(W.R.suffixFrontAos, A.neighIdx[(size_t)p * k + i]) >> 16) & 1u))
-                    continue;
-                validFlags |= 1u << i;
-            }
-        }
-        W.pixFlags[p] = (W.pixFlags[p] & 7u) | (run ? 8u : 0u);
-        W.validMask[p] = validFlags;
+    const uint32_t j = gid();
+    if (j >= gather_total(W)) return;
+    const uint32_t p = W.gatherList[j], k = A.k;
+    V3 q;
+    if (W.gIdx == 0) {
+        q = rehydrate(A.S, load_suffix(W.suffixScratch, A.P, p).s.lastPrefix).pos;
+    } else {
+        const uint4 a = W.packedScratch[2 * (size_t)p], b = W.packedScratch[2 * (size_t)p + 1];
+        const Packed pk = Packed{a.x, bf(a.y), bf(a.z), V3{bf(a.w), bf(b.x), bf(b.y)}};
+        q = inter_restore(A.S, pk, true).pos;
     }
+    const uint32_t count = query_neighbors<KR>(W, p, q, s_st + threadIdx.x, WBLOCK);
+    neighbor_stats(W, count);
+    uint32_t validFlags = 0;
+    for (uint32_t i = 0; i < count; ++i)
+        if ((load_suffix_flags_aos(W.R.suffixFrontAos, A.neighIdx[(size_t)p * k + i]) >> 16) & 1u) validFlags |= 1u << i;
+    W.validMask[p] = validFlags;
 }
 
 // The two shifts per valid neighbour of a queried pixel (final_gather.cu:135-177); gIdx > 0: one
 // shift per neighbour onto the gather prefix's traced last vertex (final_gather.cu:248-279).
-// One thread per shift (pixel p, neighbour slot i, direction d), so each thread makes one chain
-// of dependent record and vertex fetches instead of a pixel's k x 2 in a row (1.32 ms per 1080p
-// frame with one thread per pixel).  Held to 5 waves per SIMD (92 VGPRs without spilling).
+// One thread per shift of a listed pixel (list entry j, neighbour slot i, direction d), so each
+// thread makes one chain of dependent record and vertex fetches instead of a pixel's k x 2 in a
+// row (1.32 ms per 1080p frame with one thread per pixel).  Held to 5 waves per SIMD.
 __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_w_gather_setup(WaveArgs W) {
     const FrameArgs& A = W.A;
     const uint32_t k = A.k, dirs = W.gIdx == 0 ? 2u : 1u;
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, per = (uint64_t)k * dirs;
-    const uint32_t p = (uint32_t)min(t / per, (uint64_t)A.P);   // per > 0: the host launches none for k = 0
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, per = (uint64_t)k * dirs;   // per > 0
+    const uint64_t total = gather_total(W);
+    if ((t & ~63ull) >= total * per) return;   // the whole wavefront is past the listed pixels' shifts
+    const uint64_t j = t / per;
+    const uint32_t p = j < total ? W.gatherList[j] : A.P;
     const uint32_t i = (uint32_t)((t % per) / dirs), d = (uint32_t)(t % dirs);
-    bool want = false;
-    if (p < A.P) {
-        const uint32_t pf = W.pixFlags[p];
-        want = (pf & 1u) && (pf & 8u) && ((W.validMask[p] >> i) & 1u);
-    }
+    const bool want = p < A.P && ((W.validMask[p] >> i) & 1u);
     Emit e = emit_none();
     ShState s;
     const uint32_t job = ((uint32_t)p * k + i) * 2 + d;
@@ -1329,6 +1391,9 @@ hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStrea
         case WK_SUFFIX_SPATIAL_RESAMPLE: hipLaunchKernelGGL(k_w_suffix_spatial_resample, g, b, 0, s, W); break;
         case WK_GATHER_INIT: hipLaunchKernelGGL(k_w_gather_init, g, b, 0, s, W); break;
         case WK_GATHER_SETUP: {
+            hipLaunchKernelGGL(k_w_gather_mark, g, b, 0, s, W);
+            hipLaunchKernelGGL(k_w_gather_scan, dim3(1), dim3(1024), 0, s, W.gatherBlocks, g.x);
+            hipLaunchKernelGGL(k_w_gather_list, g, b, 0, s, W);
             if (W.A.k <= 8) hipLaunchKernelGGL(k_w_gather_query<8>, g, b, gather_query_lds(W), s, W);
             else hipLaunchKernelGGL(k_w_gather_query<32>, g, b, gather_query_lds(W), s, W);
             const uint64_t shifts = (uint64_t)threads * W.A.k * (W.gIdx == 0 ? 2u : 1u);
