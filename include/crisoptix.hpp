@@ -7,6 +7,8 @@
  *   Renderer(w, h, ctx, Camera&, const Scene&), LaunchFrame(out), GetFrameIdx()
  *       include/graph/Renderer.h:8-21
  *   Scene(ctx, model instances, explicit emitters), GetSceneSize()   include/model/Scene.h:12-24
+ *   ModelInstance(model, transform), Emitter(mesh, transform, color)
+ *       include/model/ModelInstance.h, include/model/Emitter.h:58-73, src/Emitter.cpp:3-62
  *   Camera(pos, viewDir, up, aspectRatio, fov), GetData(), RotateAroundOrigin()
  *       include/graph/Camera.h:25-79, src/Camera.cpp:20-38
  *   Pipeline::AddRaygenShader / CreatePipeline                        include/graph/Pipeline.h:116-156
@@ -132,10 +134,60 @@ struct Mesh {
     uint8_t visibility = CX_VIS_ALL;
 };
 
-// Emitter (include/model/Emitter.h:58-73, src/Emitter.cpp:3-62): world-space faces, one colour.
+// glm::mat4: column-major, m[4 * column + row].
+struct Mat4 {
+    float m[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    // (M * vec4(p, 1)).xyz, summed column by column in float as glm's operator* does.
+    Vec3 TransformPoint(const float* p) const {
+        float o[3];
+        for (int r = 0; r < 3; r++) o[r] = ((m[r] * p[0] + m[4 + r] * p[1]) + m[8 + r] * p[2]) + m[12 + r];
+        return Vec3{o[0], o[1], o[2]};
+    }
+    // mat3(M) * d
+    Vec3 TransformDir(const float* d) const {
+        float o[3];
+        for (int r = 0; r < 3; r++) o[r] = (m[r] * d[0] + m[4 + r] * d[1]) + m[8 + r] * d[2];
+        return Vec3{o[0], o[1], o[2]};
+    }
+};
+
+// Model (include/model/Model.h): its meshes, in the model's own space.  Loading from files is the
+// caller's (crisoptix_amd/assets.py reads OBJ/MTL as the reference's assimp path does).
+struct Model {
+    std::vector<Mesh> meshes;
+};
+
+// ModelInstance(model, transform) (include/model/ModelInstance.h): a model placed in the world.
+class ModelInstance {
+public:
+    ModelInstance(const Model& model, const Mat4& transform) : m_Model(model), m_Transform(transform) {}
+    const Model& GetModel() const { return m_Model; }
+    const Mat4& GetTransform() const { return m_Transform; }
+    void SetTransform(const Mat4& transform) { m_Transform = transform; }
+
+private:
+    const Model& m_Model;
+    Mat4 m_Transform;
+};
+
+// Emitter (include/model/Emitter.h:58-73, src/Emitter.cpp:3-62): world-space faces and one
+// colour; built from a mesh and its transform as the reference's constructor is (the vertices
+// transformed on the host), or from faces already in world space.
 struct Emitter {
     std::vector<float> faces;       // [faces][3 vertices][3]
     Vec3 color{1.0f, 1.0f, 1.0f};
+
+    Emitter() = default;
+    Emitter(std::vector<float> worldFaces, const Vec3& c) : faces(std::move(worldFaces)), color(c) {}
+    Emitter(const Mesh* mesh, const Mat4& transform, const Vec3& c) : color(c) {
+        if (!mesh || mesh->indices.size() % 3 != 0) throw Error(CX_ERR_INVALID_ARG, "Emitter: mesh without triangles");
+        const size_t nv = mesh->positions.size() / 3;
+        for (uint32_t i : mesh->indices) {
+            if (i >= nv) throw Error(CX_ERR_INVALID_ARG, "Emitter: vertex index out of range");
+            const Vec3 w = transform.TransformPoint(&mesh->positions[3 * (size_t)i]);
+            faces.insert(faces.end(), {w.x, w.y, w.z});
+        }
+    }
 };
 
 // An RGBA8 texture (src/Texture.cpp:7-76).
@@ -154,9 +206,64 @@ struct SceneOptions {
 // the meshes into one triangle list with per-triangle material ids and builds the BVH.
 class Scene {
 public:
+    // Scene(ctx, modelInstances, explicitEmitters) (include/model/Scene.h:12-24): every instance's
+    // meshes flattened into one instance with the transforms applied on the host (positions by
+    // M, normals by mat3(M), as rehydration reads one transform, include/graph/Interaction.h:143).
+    Scene(const Context& ctx, const std::vector<const ModelInstance*>& instances, const ShaderBindingTable& sbt,
+          const std::vector<const Emitter*>& emitters, const std::vector<Texture>& textures = {},
+          const SceneOptions& opt = {}) {
+        std::vector<Mesh> world;
+        for (const ModelInstance* inst : instances) {
+            if (!inst) throw Error(CX_ERR_INVALID_ARG, "Scene: null model instance");
+            const Mat4& M = inst->GetTransform();
+            for (const Mesh& m : inst->GetModel().meshes) {
+                Mesh w = m;
+                for (size_t v = 0; 3 * v + 2 < m.positions.size(); v++) {
+                    const Vec3 p = M.TransformPoint(&m.positions[3 * v]);
+                    w.positions[3 * v] = p.x;
+                    w.positions[3 * v + 1] = p.y;
+                    w.positions[3 * v + 2] = p.z;
+                }
+                for (size_t v = 0; 3 * v + 2 < m.normals.size(); v++) {
+                    const Vec3 n = M.TransformDir(&m.normals[3 * v]);
+                    w.normals[3 * v] = n.x;
+                    w.normals[3 * v + 1] = n.y;
+                    w.normals[3 * v + 2] = n.z;
+                }
+                world.push_back(std::move(w));
+            }
+        }
+        std::vector<Emitter> em;
+        for (const Emitter* e : emitters) {
+            if (!e) throw Error(CX_ERR_INVALID_ARG, "Scene: null emitter");
+            em.push_back(*e);
+        }
+        Build(ctx, world, sbt, em, textures, opt);
+    }
+
+    // The same from meshes and emitters already in world space.
     Scene(const Context& ctx, const std::vector<Mesh>& meshes, const ShaderBindingTable& sbt,
           const std::vector<Emitter>& emitters, const std::vector<Texture>& textures = {},
           const SceneOptions& opt = {}) {
+        Build(ctx, meshes, sbt, emitters, textures, opt);
+    }
+
+    ~Scene() { cx_scene_destroy(m_Scene); }
+    Scene(const Scene&) = delete;
+    Scene& operator=(const Scene&) = delete;
+
+    // Scene::GetSceneSize (src/Scene.cpp:48-71): diagonal of the vertices' box and the origin.
+    float GetSceneSize() const { return GetInfo().sceneSize; }
+    cx_scene_info GetInfo() const {
+        cx_scene_info info{};
+        Check(cx_scene_get_info(m_Scene, &info), m_Ctx, "cx_scene_get_info");
+        return info;
+    }
+    cx_scene* Get() const { return m_Scene; }
+
+private:
+    void Build(const Context& ctx, const std::vector<Mesh>& meshes, const ShaderBindingTable& sbt,
+               const std::vector<Emitter>& emitters, const std::vector<Texture>& textures, const SceneOptions& opt) {
         if (!sbt.Created()) throw Error(CX_ERR_INVALID_ARG, "Scene: ShaderBindingTable::CreateSBT not called");
         std::vector<float> pos, nrm, uv;
         std::vector<uint32_t> idx, triMat;
@@ -215,20 +322,7 @@ public:
         Check(cx_scene_build(ctx.Get(), &d, &m_Scene), ctx.Get(), "cx_scene_build");
         m_Ctx = ctx.Get();
     }
-    ~Scene() { cx_scene_destroy(m_Scene); }
-    Scene(const Scene&) = delete;
-    Scene& operator=(const Scene&) = delete;
 
-    // Scene::GetSceneSize (src/Scene.cpp:48-71): diagonal of the vertices' box and the origin.
-    float GetSceneSize() const { return GetInfo().sceneSize; }
-    cx_scene_info GetInfo() const {
-        cx_scene_info info{};
-        Check(cx_scene_get_info(m_Scene, &info), m_Ctx, "cx_scene_get_info");
-        return info;
-    }
-    cx_scene* Get() const { return m_Scene; }
-
-private:
     cx_scene* m_Scene = nullptr;
     cx_ctx* m_Ctx = nullptr;
 };

@@ -4,7 +4,7 @@
 // frame of both runs for the test that compares them with the oracle.
 //
 //   render_frames --abi                      prints "abi <header> <library>" (no GPU needed)
-//   render_frames SCENE OUT W H FRAMES SEED [orbit=0|1] [field=value ...]
+//   render_frames SCENE OUT W H FRAMES SEED [orbit=0|1] [instanced=0|1] [field=value ...]
 //
 // SCENE (little-endian): "CXS1", u32 numVertices numTriangles numMaterials numEmitters
 //   numEmitterFaces numMeshes hasUv; f32 positions[V][3] normals[V][3] (uvs[V][2]);
@@ -14,6 +14,11 @@
 // OUT: f32 sceneSize, u32 frames; per frame cx_camera, u32 frameIdx, f32 image[H][W][3] of
 //   LaunchFrame, f32 image[H][W][3] of the Pipeline run.
 // field=value overrides a cx_launch_params field of the Renderer defaults (src/Renderer.cpp:34-58).
+// instanced=1: the file's vertices and emitter faces are in a model space turned by -90 degrees
+// about y; the scene is then built from one Model placed by a ModelInstance whose transform turns
+// it back (exactly: the matrix holds only 0 and +-1), and each emitter from a mesh of its faces
+// with the same transform, as Scene(ctx, instances, emitters) and Emitter(mesh, transform, colour)
+// are used in the reference.
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
@@ -125,11 +130,12 @@ int run(int argc, char** argv) {
     const uint32_t W = (uint32_t)std::atoi(argv[3]), H = (uint32_t)std::atoi(argv[4]);
     const int frames = std::atoi(argv[5]);
     const uint64_t seed = std::strtoull(argv[6], nullptr, 10);
-    bool orbit = false;
+    bool orbit = false, instanced = false;
     std::vector<std::string> overrides;
     for (int i = 7; i < argc; i++) {
         const std::string a = argv[i];
         if (a.rfind("orbit=", 0) == 0) orbit = a == "orbit=1";
+        else if (a.rfind("instanced=", 0) == 0) instanced = a == "instanced=1";
         else overrides.push_back(a);
     }
 
@@ -180,11 +186,30 @@ int run(int argc, char** argv) {
         me.visibility = (uint8_t)m[5];
     }
     std::vector<crisoptix::Emitter> emitters(ne);
+    crisoptix::Mat4 turn;   // +90 degrees about y: (x, y, z) -> (z, y, -x)
+    const float cols[16] = {0, 0, -1, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 0, 1};
+    std::memcpy(turn.m, cols, sizeof(cols));
+    std::vector<crisoptix::Mesh> emitterMeshes(ne);
     for (uint32_t e = 0; e < ne; e++) {
-        emitters[e].faces.assign(faces.begin() + 9 * (size_t)faceOff[e], faces.begin() + 9 * (size_t)faceOff[e + 1]);
-        emitters[e].color = {ecol[3 * e], ecol[3 * e + 1], ecol[3 * e + 2]};
+        const crisoptix::Vec3 color{ecol[3 * e], ecol[3 * e + 1], ecol[3 * e + 2]};
+        std::vector<float> f(faces.begin() + 9 * (size_t)faceOff[e], faces.begin() + 9 * (size_t)faceOff[e + 1]);
+        if (instanced) {
+            crisoptix::Mesh& em = emitterMeshes[e];
+            em.positions = f;
+            for (uint32_t v = 0; v < f.size() / 3; v++) em.indices.push_back(v);
+            emitters[e] = crisoptix::Emitter(&em, turn, color);
+        } else {
+            emitters[e] = crisoptix::Emitter(std::move(f), color);
+        }
     }
-    crisoptix::Scene scene(ctx, meshes, sbt, emitters);
+    crisoptix::Model model;
+    model.meshes = meshes;
+    const crisoptix::ModelInstance instance(model, turn);
+    std::vector<const crisoptix::Emitter*> emitterPtrs;
+    for (const crisoptix::Emitter& e : emitters) emitterPtrs.push_back(&e);
+    crisoptix::Scene scene = instanced ? crisoptix::Scene(ctx, std::vector<const crisoptix::ModelInstance*>{&instance},
+                                                          sbt, emitterPtrs)
+                                       : crisoptix::Scene(ctx, meshes, sbt, emitters);
 
     // An invalid renderer size reaches the caller as crisoptix::Error with the library's message.
     {

@@ -4,7 +4,8 @@ Renderer::LaunchFrame and, on a second renderer, through a Pipeline of raygen pa
 order of src/Renderer.cpp:259-371.
 
 CPU: the header compiles on its own, and the program links against the library and reports the
-ABI version it was built for.  GPU: every LaunchFrame frame matches the oracle within 1e-4
+ABI version it was built for; the instanced case's model-space scene turns back to the world
+scene bit for bit.  GPU: every LaunchFrame frame matches the oracle within 1e-4
 relative per channel (as the frame-level parity tests), every Pipeline frame equals its
 LaunchFrame frame bit for bit, and the camera basis the C++ Camera computes is the one the
 Python Camera computes (within one float32 ulp: the C++ class keeps the field of view and
@@ -25,6 +26,12 @@ cpp_build = importlib.util.module_from_spec(_spec)
 _spec.loader.exec_module(cpp_build)
 
 LIB = os.path.join(ROOT, "crisoptix_amd", "libcrisoptix_amd.so")
+
+
+def test_instanced_scene_files_turn_back_exactly():
+    for sc in (scenes.cornell_box(1, 8), tent_grid_scene()):
+        for v in (sc.positions, sc.normals, sc.emitter_faces):
+            model_space(v)
 
 
 def test_header_compiles_alone(tmp_path):
@@ -62,21 +69,39 @@ def mesh_runs(sc):
     return runs
 
 
-def write_scene(path, sc, cam):
+def model_space(v):
+    """Points or directions [..., 3] in the model space of the instanced case: turned by -90
+    degrees about y, (x, y, z) -> (-z, y, x), so that the instance transform (x, y, z) -> (z, y, -x)
+    turns them back.  Checked to come back bit for bit through the C++ Mat4's float arithmetic
+    (((m0 x + m4 y) + m8 z) + m12 per row)."""
+    v = np.asarray(v, np.float32)
+    f = np.stack([-v[..., 2], v[..., 1], v[..., 0]], -1).astype(np.float32)
+    one, zero = np.float32(1), np.float32(0)
+    back = np.stack([((zero * f[..., 0] + zero * f[..., 1]) + one * f[..., 2]) + zero,
+                     ((zero * f[..., 0] + one * f[..., 1]) + zero * f[..., 2]) + zero,
+                     ((-one * f[..., 0] + zero * f[..., 1]) + zero * f[..., 2]) + zero], -1).astype(np.float32)
+    assert np.array_equal(back.view(np.uint32), v.view(np.uint32)), "the turn does not come back exactly"
+    return f
+
+
+def write_scene(path, sc, cam, instanced=False):
     runs = mesh_runs(sc)
     has_uv = sc.uvs is not None
     assert not sc.textures
+    pos, nrm, faces = sc.positions, sc.normals, sc.emitter_faces
+    if instanced:
+        pos, nrm, faces = model_space(pos), model_space(nrm), model_space(faces)
     with open(path, "wb") as f:
         f.write(b"CXS1")
         f.write(np.asarray([len(sc.positions), sc.num_triangles, len(sc.materials), len(sc.emitter_color),
                             len(sc.emitter_faces), len(runs), int(has_uv)], "<u4").tobytes())
-        for a, dt in ((sc.positions, "<f4"), (sc.normals, "<f4")):
+        for a, dt in ((pos, "<f4"), (nrm, "<f4")):
             f.write(np.ascontiguousarray(a, dt).tobytes())
         if has_uv:
             f.write(np.ascontiguousarray(sc.uvs, "<f4").tobytes())
         f.write(np.ascontiguousarray(sc.indices, "<u4").tobytes())
         f.write(np.ascontiguousarray(sc.materials, "<f4").tobytes())
-        f.write(np.ascontiguousarray(sc.emitter_faces, "<f4").tobytes())
+        f.write(np.ascontiguousarray(faces, "<f4").tobytes())
         f.write(np.ascontiguousarray(sc.emitter_face_offset, "<u4").tobytes())
         f.write(np.ascontiguousarray(sc.emitter_color, "<f4").tobytes())
         f.write(np.asarray(runs, "<u4").tobytes())
@@ -116,6 +141,10 @@ CASES = {
     # traced lights (meshes of visibility CX_VIS_SURFACE) and a turning camera
     "cond_tent_orbit": dict(scene="tent16", mode="conditional_restir", W=48, H=32, frames=3, orbit=True,
                             over=dict(gatherRadius=0.6)),
+    # the same scene built from a Model placed by a ModelInstance and emitters from meshes with
+    # the instance's transform
+    "cond_tent_instanced": dict(scene="tent16", mode="conditional_restir", W=48, H=32, frames=2, orbit=True,
+                                over=dict(gatherRadius=0.6), instanced=True),
 }
 
 
@@ -132,9 +161,10 @@ def test_cpp_frames_match_oracle(case, tmp_path, orc, cx_runtime):
     for k, v in c.get("over", {}).items():
         setattr(p, k, v)
     scene_file, out_file = tmp_path / "scene.bin", tmp_path / "frames.bin"
-    write_scene(scene_file, sc, cam)
+    write_scene(scene_file, sc, cam, instanced=c.get("instanced", False))
     args = [cpp_build.OUT, str(scene_file), str(out_file), str(W), str(H), str(c["frames"]), "4321",
-            f"orbit={int(bool(c.get('orbit')))}", *overrides(p, abi.default_params(W, H, oscene.size))]
+            f"orbit={int(bool(c.get('orbit')))}", f"instanced={int(bool(c.get('instanced')))}",
+            *overrides(p, abi.default_params(W, H, oscene.size))]
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
