@@ -75,7 +75,7 @@ def parse():
                     help="binned SAH on the host or PLOC on the GPU (same hits, different trees)")
     ap.add_argument("--wave-rounds", type=int, default=None,
                     help="trace/step rounds of every machine before the in-place tail kernel (0: all rounds; "
-                         "default: 2 for every machine)")
+                         "default: 3 for suffix walks, 2 for the other machines)")
     ap.add_argument("--queue-grid", type=int, default=0,
                     help="most blocks of the grid-stride queue kernels (0: the library's default, 8192)")
     ap.add_argument("--no-overlap", action="store_true",
@@ -305,7 +305,7 @@ def main():
             "exec_mode": a.exec,
             "final_phase_overlap": not a.no_overlap,
             "queue_grid": a.queue_grid or 8192,
-            "wave_rounds": a.wave_rounds if a.wave_rounds is not None else "2 (every machine)",
+            "wave_rounds": a.wave_rounds if a.wave_rounds is not None else "3 for suffix walks, 2 for the others",
             "resolution": [W, H],
             "triangles": scene.num_triangles,
             "max_path_len": int(params.maxPathLen),

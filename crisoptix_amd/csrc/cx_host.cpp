@@ -106,7 +106,7 @@ struct cx_renderer {
     // Most blocks of the grid-stride queue kernels.  1080p Conditional ReSTIR frame: 2048 / 4096 /
     // 8192 / 16384 -> 9.53 / 8.59-8.62 / 8.45 / 8.47 ms; a machine of fewer jobs launches fewer.
     uint32_t gridLimit = 8192;
-    uint32_t waveRounds = CX_WAVE_ROUNDS_DEFAULT;   // rounds before the tail kernel, every machine (0: all)
+    uint32_t waveRounds = CX_WAVE_ROUNDS_DEFAULT;   // rounds before the tail kernel, every machine (0: all; default per machine)
     uint64_t allocEpoch = 0;     // bumped whenever a buffer a frame uses is reallocated
     // Graph mode: a frame is one hipGraphLaunch of a graph captured from the eager frame for
     // each front-buffer parity; the camera, random and frame index come from `dyn`.
@@ -453,12 +453,14 @@ void trace_rec(cx_renderer* r, hipStream_t s) {
     (void)hipEventRecord(r->traceEv[r->traceEvUsed++], s);
 }
 
-// Wavefront rounds before the tail kernel, every machine.  Measured on the 1080p Conditional
-// ReSTIR frame (tent of traced lights, 14.6 rays/px): every machine 1 (suffix generation) or 2
-// (the others) / 2 / 3 / 4 / 6 rounds, or all rounds as traversal + step (0) -> 29.28 / 28.77 /
-// 28.91 / 29.15 / 29.82 / 29.88 ms per frame; the tail traces in place at 3 waves per SIMD, so
-// long suffix walks are cheaper in the queue rounds.
-uint32_t default_wave_rounds(int) { return 2u; }
+// Wavefront rounds before the tail kernel, per machine: 3 for suffix walks, 2 for the others.
+// Measured on the 1080p Conditional ReSTIR frame (tent of traced lights, 14.6 rays/px): every
+// machine 1 (suffix walks) or 2 (the others) / 2 / 3 / 4 / 6 rounds, or all rounds as traversal
+// + step (0) -> 29.28 / 28.77 / 28.91 / 29.15 / 29.82 / 29.88 ms per frame; then, with the
+// range-per-wavefront traversal, suffix walks 3 / 4 / 5 rounds against 2 -> 25.43 / 25.69 / 25.98
+// against 25.77 ms, shifts 1 or 3 and prefix walks 1 or 3 against 2 no better.  The tail traces
+// in place at 3 waves per SIMD, so long suffix walks are cheaper in the queue rounds.
+uint32_t default_wave_rounds(int machine) { return machine == WK_SG_STEP ? 3u : 2u; }
 
 hipError_t run_machine(cx_renderer* r, WaveArgs W, int setup, int step, uint32_t cap, uint32_t rounds,
                        int pass, hipStream_t s) {

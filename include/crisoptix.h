@@ -391,7 +391,7 @@ int cx_renderer_set_queue_grid(cx_renderer* r, uint32_t maxBlocks);
  * launch and one step launch, then finishes its remaining jobs in one kernel that traces each
  * job's rays in place (the same traversal, so the same answers and ray counts).  `rounds` applies
  * to every machine; 0 runs every round as traversal + step; CX_WAVE_ROUNDS_DEFAULT (the initial
- * setting) gives every machine 2 rounds.  Results are identical for
+ * setting) gives suffix walks 3 rounds and the other machines 2.  Results are identical for
  * every value; a tuning knob (crisoptix_amd/tune.py). */
 #define CX_WAVE_ROUNDS_DEFAULT 0xFFFFFFFFu
 int cx_renderer_set_wave_rounds(cx_renderer* r, uint32_t rounds);
