@@ -1,8 +1,8 @@
 // cx_entries_search_static: the prefix-entry structure built and queried in one call with no
 // device allocation (include/crisoptix.h).  It runs the kernels of cx_entries_build followed by
-// cx_entries_query (entry records, the memset-free radix sort, the implicit tree, the
-// k-nearest query); the buffers those keep in a cx_entries object live here in one file-scope
-// device array of fixed size instead.
+// cx_entries_query (entry records, the memset-free radix sort, the linear BVH over the sorted
+// entries with its range-tree bounds, the k-nearest query); the buffers those keep in a
+// cx_entries object live here in one file-scope device array of fixed size instead.
 //
 // The array's name is a host-side stand-in in host code, not a device address: its address on
 // each device is taken once with hipGetSymbolAddress (noalloc_setup, called by cx_ctx_create
