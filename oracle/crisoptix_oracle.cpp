@@ -900,8 +900,11 @@ static void trace_prefix(const Scene& s, Prefix& prefix, const V3& origin, const
     prefix.lastInt = interaction.pack();
 }
 
+// reconVertex (a test diagnostic, not part of the reference's path): receives the suffix vertex
+// at which a BSDF-sampled reconnection was chosen (its length at that point), 0 otherwise.
 static void trace_suffix(const Scene& s, Suffix& suffix, const Prefix& prefix, PCG32& rng,
-                         const Params& P) {   // path_gen.h:252-387
+                         const Params& P, uint32_t* reconVertex = nullptr) {   // path_gen.h:252-387
+    if (reconVertex) *reconVertex = 0;
     suffix.f = V3(1.0f);
     suffix.p = 1.0f;
     suffix.postReconF = V3(1.0f);
@@ -962,6 +965,7 @@ static void trace_suffix(const Scene& s, Suffix& suffix, const Prefix& prefix, P
             suffix.setReconIdx(prefix.length());   // :378, reference quirk kept
             suffix.reconOutDir = bs.outDir;
             postRecon = true;
+            if (reconVertex) *reconVertex = suffix.length();
         }
         lastDir = bs.outDir;
         suffix.setLength(suffix.length() + 1);
@@ -2691,7 +2695,8 @@ void orc_identity_shift_suffix(void* sp, const cx_launch_params* lp, const float
     Prefix pre;
     trace_prefix(s, pre, V3(origin[0], origin[1], origin[2]), V3(dir[0], dir[1], dir[2]), rng, P);
     Suffix suf;
-    trace_suffix(s, suf, pre, rng, P);
+    uint32_t reconVertex = 0;
+    trace_suffix(s, suf, pre, rng, P, &reconVertex);
     float J = 0.0f;
     const V3 f = shift_suffix(s, suf, suf, J);
     v3out(suf.f, out);
@@ -2700,6 +2705,8 @@ void orc_identity_shift_suffix(void* sp, const cx_launch_params* lp, const float
     out[7] = suf.valid() ? 1.0f : 0.0f;
     out[8] = (float)suf.reconIdx();
     out[9] = (float)suf.length();
+    out[10] = (float)reconVertex;                  // BSDF reconnection: the suffix vertex chosen, else 0
+    out[11] = (float)pre.length();
 }
 
 // The final gather's combination (final_gather.cu:135-187) over injected neighbour terms, each

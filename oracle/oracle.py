@@ -233,7 +233,9 @@ class OracleScene:
     def identity_shift(self, params: abi.LaunchParams, origin, direction, rng_state: int, which="prefix"):
         o = np.ascontiguousarray(origin, np.float32)
         d = np.ascontiguousarray(direction, np.float32)
-        out = np.zeros(10, np.float32)   # f, shifted f, J, valid; the suffix's reconIdx and length
+        # f, shifted f, J, valid; the suffix's reconIdx and length, the suffix vertex of a BSDF
+        # reconnection (0: none) and the prefix length
+        out = np.zeros(12, np.float32)
         fn = lib().orc_identity_shift_prefix if which == "prefix" else lib().orc_identity_shift_suffix
         fn(self.h, C.byref(params), _p(o), _p(d), rng_state, _p(out))
         return out
