@@ -22,12 +22,14 @@
 // GGX closed forms and Monte-Carlo normalisation, brute-force closest hit and occlusion,
 // brute-force neighbour sets, closed forms of the Jacobian and the pairwise MIS weights, the
 // pairwise weights' partition of unity for reciprocal Jacobians, the prefix identity shift and
-// the suffix identity shift at an NEE reconnection (J = 1), the final gather's combination in
-// its limits (unreachable neighbours give the canonical estimate), a converged FP64 reference of
-// the path-tracer estimator, and the canonical prefix + suffix chain of Conditional ReSTIR
-// against that FP64 reference one vertex shorter.  Shifted-path reuse as a whole (which
-// neighbour suffixes are combined, through which replayed vertices) stays "parity unpinned"
-// against the reference itself (it cannot be built here: no OptiX SDK, CUDA toolkit or glm).
+// the suffix identity shift at an NEE reconnection and at a BSDF reconnection on the first
+// suffix vertex with prefix length 1 (J = 1, the suffix's own value), the final gather's
+// combination in its limits (unreachable neighbours give the canonical estimate), a converged
+// FP64 reference of the path-tracer estimator, and the canonical prefix + suffix chain of
+// Conditional ReSTIR against that FP64 reference one vertex shorter.  Shifts whose reconIdx
+// (path_gen.h:378: the prefix length) differs from the reconnection vertex's index stay "parity
+// unpinned" against the reference itself (it cannot be built here: no OptiX SDK, CUDA toolkit or
+// glm); no identity fixes their values.
 
 #include "orc_math.h"
 
