@@ -1,11 +1,12 @@
 // Wavefront kernels of the frame loop.
 //
 // Per pass: an init/setup kernel creates one job per pixel (or per pixel x neighbour x shift
-// direction) and appends the first rays; then, until no job is left, the closest-hit and
-// shadow traversal kernels answer the queued rays and a step kernel advances every job to its
-// next ray or to its end; finally a resample kernel runs the reference's reservoir updates per
-// pixel in the reference's order.  The traversal kernels use a 16-entry LDS stack per thread
-// (deeper entries spill to private memory), so their occupancy is set by registers.
+// direction) and appends the first rays; then a few rounds in which the traversal kernel answers
+// the queued closest-hit and shadow rays and a step kernel advances every job to its next ray or
+// to its end; then a tail kernel in which every remaining job traces its rays in place until it
+// ends; finally a resample kernel runs the reference's reservoir updates per pixel in the
+// reference's order.  The traversal code uses a 16-entry LDS stack per thread (deeper entries
+// spill to private memory), so occupancy is set by registers.
 #include <algorithm>
 #include <map>
 #include <mutex>
