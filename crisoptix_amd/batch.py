@@ -12,16 +12,23 @@ plain:   for each view, cx_renderer_frame on the default stream, one after anoth
 batched: cx_batch_frame, every view on its own stream of the context's pool, so the small views'
          launch-bound passes overlap with the others' work; with --graph each view's frame is
          one replay of its captured hipGraph (cx_renderer_set_graph_mode).
+The HIP runtime spreads a process's streams over GPU_MAX_HW_QUEUES hardware queues (HIP's default
+is 4); the 16 per-view streams overlap best on 16 queues, so the command line sets
+GPU_MAX_HW_QUEUES to --hw-queues (default 16; 4 gives HIP's default) before the first HIP call,
+over any value in the environment.  Both legs run in the one process, so both see the same queues.
+Both legs run without the final-phase side stream (cx_renderer_set_overlap): over these 16 views it
+gained nothing in either leg at 4 queues, and at 16 queues it slowed the plain leg by a tenth.
 Both run the same renderers' code on the same seeds: the per-view images are compared bit for
 bit every frame (equal images, so equal rel-MSE against any reference).
 
-    python -m crisoptix_amd.batch [--frames 8] [--tune] [--graph]
+    python -m crisoptix_amd.batch [--frames 8] [--tune] [--graph] [--hw-queues 16]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import math
+import os
 import sys
 import time
 
@@ -55,7 +62,7 @@ def make_params(w, h, size):
 
 
 class Views:
-    def __init__(self, ctx, hosts, gscenes, specs, settings=None, graph=False):
+    def __init__(self, ctx, hosts, gscenes, specs, settings=None, graph=False, overlap=True):
         self.cams, self.rens, self.specs = [], [], specs
         for i, s in enumerate(specs):
             sc, gs = hosts[s["scene"]], gscenes[s["scene"]]
@@ -71,6 +78,7 @@ class Views:
                 if st.get("wave_rounds") is not None:
                     ren.set_wave_rounds(st["wave_rounds"])
             ren.set_graph_mode(graph)
+            ren.set_overlap(overlap)
             self.cams.append(cam)
             self.rens.append(ren)
 
@@ -107,8 +115,8 @@ def run(a) -> dict:
         settings = [tuning[GRIDS[s["scene"]]] for s in specs]
     gscenes = [runtime.Scene(ctx, sc, width=widths[i]) for i, sc in enumerate(hosts)]
     t_build = time.perf_counter() - t0
-    plain = Views(ctx, hosts, gscenes, specs, settings)
-    batched = Views(ctx, hosts, gscenes, specs, settings, graph=a.graph)
+    plain = Views(ctx, hosts, gscenes, specs, settings, overlap=False)
+    batched = Views(ctx, hosts, gscenes, specs, settings, graph=a.graph, overlap=False)
     batch = runtime.Batch(ctx, batched.rens)
     P = sum(s["width"] * s["height"] for s in specs)
     mismatches = 0
@@ -149,6 +157,8 @@ def run(a) -> dict:
         "image_mismatches": mismatches,
         "tuned": tuning,
         "graph_mode": bool(a.graph),
+        "final_phase_overlap": False,
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4 (HIP default)"),
         "build_seconds": t_build,
     }
 
@@ -162,7 +172,13 @@ def main(argv=None):
     ap.add_argument("--graph", action="store_true", help="batched views replay captured frame graphs")
     ap.add_argument("--tune-frames", type=int, default=4)
     ap.add_argument("--tune-cache")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (1-32, over the environment's value); "
+                         "takes effect only before the process's first HIP call")
     a = ap.parse_args(argv)
+    if not 1 <= a.hw_queues <= 32:
+        ap.error("--hw-queues must be between 1 and 32")
+    os.environ["GPU_MAX_HW_QUEUES"] = str(a.hw_queues)
     print(json.dumps(run(a)), flush=True)
 
 

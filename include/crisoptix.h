@@ -378,7 +378,9 @@ int cx_renderer_get_exec_mode(const cx_renderer* r);
  * returns: view i runs cx_renderer_frame(renderers[i], &cams[i], cameraChanged[i], outs[i])
  * on streams[i], or, when streams is NULL, on the i-th stream of the context's pool, so the
  * views' kernels overlap on the GPU.  Each view's output is its plain frame's, bit for bit.
- * cx_batch_sync waits for the pool's streams. */
+ * cx_batch_sync waits for the pool's streams.  The HIP runtime spreads streams over
+ * GPU_MAX_HW_QUEUES hardware queues (default 4); with 16 views, 16 queues overlap them best,
+ * and the final-phase side stream (cx_renderer_set_overlap) is best turned off there. */
 int cx_batch_frame(cx_ctx* ctx, cx_renderer* const* renderers, const cx_camera* cams, const int* cameraChanged,
                    float* const* outs, uint32_t n, void* const* streams);
 int cx_batch_sync(cx_ctx* ctx);
