@@ -90,8 +90,14 @@ struct cx_renderer {
         uint64_t* rngScratch = nullptr;
         uint32_t* pixFlags = nullptr;
         uint32_t* validMask = nullptr;
+        uint32_t* gatherKeys = nullptr;      // Morton-ordered querying pixels of the final gather
         uint32_t* gatherList = nullptr;
-        uint32_t* gatherBlocks = nullptr;
+        uint32_t* gatherKeysAlt = nullptr;
+        uint32_t* gatherListAlt = nullptr;
+        uint32_t* gatherCount = nullptr;
+        uint32_t* gatherSortTemp = nullptr;
+        size_t gatherSortBytes = 0;
+        float4* gatherPos = nullptr;
         uint32_t* nbr = nullptr;
         float4* radScratch = nullptr;
         float4* fgScratch = nullptr;
@@ -337,8 +343,14 @@ int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
         A(dalloc(w.pixelAllocs, &w.rngScratch, P));
         A(dalloc(w.pixelAllocs, &w.pixFlags, P));
         A(dalloc(w.pixelAllocs, &w.validMask, P));
+        A(dalloc(w.pixelAllocs, &w.gatherKeys, P));
         A(dalloc(w.pixelAllocs, &w.gatherList, P));
-        A(dalloc(w.pixelAllocs, &w.gatherBlocks, (P + QUEUE_BLOCK - 1) / QUEUE_BLOCK + 1));
+        A(dalloc(w.pixelAllocs, &w.gatherKeysAlt, P));
+        A(dalloc(w.pixelAllocs, &w.gatherListAlt, P));
+        A(dalloc(w.pixelAllocs, &w.gatherCount, 1));
+        w.gatherSortBytes = sort_temp_bytes((uint32_t)P);
+        A(dalloc(w.pixelAllocs, &w.gatherSortTemp, w.gatherSortBytes / sizeof(uint32_t)));
+        A(dalloc(w.pixelAllocs, &w.gatherPos, P));
         A(dalloc(w.pixelAllocs, &w.nbr, 16 * P));
         A(dalloc(w.pixelAllocs, &w.radScratch, P));
         A(dalloc(w.pixelAllocs, &w.fgScratch, P));
@@ -426,8 +438,20 @@ WaveArgs make_wave_args(const cx_renderer* r, const FrameArgs& A) {
     W.shiftRes = w.shiftRes;
     W.pixFlags = w.pixFlags;
     W.validMask = w.validMask;
+    W.gatherKeys = w.gatherKeys;
     W.gatherList = w.gatherList;
-    W.gatherBlocks = w.gatherBlocks;
+    W.gatherKeysAlt = w.gatherKeysAlt;
+    W.gatherListAlt = w.gatherListAlt;
+    W.gatherCount = w.gatherCount;
+    W.gatherSortTemp = w.gatherSortTemp;
+    W.gatherSortBytes = w.gatherSortBytes;
+    W.gatherPos = w.gatherPos;
+    {   // the query points are quantised over the scene's bounds, as the entries are
+        EntryBuild q{};
+        set_quantisation(q, r->scene->info.boundsMin, r->scene->info.boundsMax);
+        W.mortonLo = q.lo;
+        W.mortonScale = q.scale;
+    }
     W.nbr = w.nbr;
     W.radScratch = w.radScratch;
     W.fgScratch = w.fgScratch;

@@ -395,21 +395,6 @@ __global__ __launch_bounds__(BLOCK) void k_suffix_spatial(FrameArgs A) {
 // prefix_store_entries (shader/prefix_store_entries.cu:9-53) + the Morton keys of the entry
 // tree build.  Entry radius = half-width of the entry's box (prefix_search_hit.cu:33-34).
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
-    v = (v * 0x00010001u) & 0xFF0000FFu;
-    v = (v * 0x00000101u) & 0x0F00F00Fu;
-    v = (v * 0x00000011u) & 0xC30C30C3u;
-    v = (v * 0x00000005u) & 0x49249249u;
-    return v;
-}
-
-__device__ __forceinline__ uint32_t morton30(V3 q, V3 lo, V3 scale) {
-    const float fx = fminf(fmaxf((q.x - lo.x) * scale.x, 0.0f), 1023.0f);
-    const float fy = fminf(fmaxf((q.y - lo.y) * scale.y, 0.0f), 1023.0f);
-    const float fz = fminf(fmaxf((q.z - lo.z) * scale.z, 0.0f), 1023.0f);
-    return (expand_bits((uint32_t)fx) << 2) | (expand_bits((uint32_t)fy) << 1) | expand_bits((uint32_t)fz);
-}
-
 __global__ __launch_bounds__(BLOCK) void k_store_entries(FrameArgs A, EntryBuild E) {
     uint32_t x, y, p;
     if (!tile_pixel(A, x, y, p)) return;

@@ -171,6 +171,23 @@ __device__ __forceinline__ void store_suffix(uint4* b, uint32_t P, uint32_t p, c
 
 // ---- neighbour-search structures ----
 
+// 30-bit Morton key of a point quantised to a 1024^3 grid: (q - lo) * scale clamped to [0, 1023]
+// per axis (a NaN coordinate maps to 0), the bits interleaved x, y, z from the top.
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton30(V3 q, V3 lo, V3 scale) {
+    const float fx = fminf(fmaxf((q.x - lo.x) * scale.x, 0.0f), 1023.0f);
+    const float fy = fminf(fmaxf((q.y - lo.y) * scale.y, 0.0f), 1023.0f);
+    const float fz = fminf(fmaxf((q.z - lo.z) * scale.z, 0.0f), 1023.0f);
+    return (expand_bits((uint32_t)fx) << 2) | (expand_bits((uint32_t)fy) << 1) | expand_bits((uint32_t)fz);
+}
+
 // Per-frame entry records before sorting (one slot per pixel / input entry).
 struct EntryBuild {
     float4* pos;          // [n] pos.xyz, radius

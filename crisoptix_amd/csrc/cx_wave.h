@@ -842,8 +842,18 @@ struct WaveArgs {
     float4* shiftRes;       // [jobs] f.xyz, Jacobian of each finished shift
     uint32_t* pixFlags;     // [P] per-pass pixel flags
     uint32_t* validMask;    // [P] valid-neighbour mask of the final gather
-    uint32_t* gatherList;   // [P] the pixels that query neighbours, in pixel order
-    uint32_t* gatherBlocks; // [ceil(P / WBLOCK) + 1] querying pixels per block, then their offsets and total
+    // The final gather's querying pixels in Morton order of their query points: keys [P] (per
+    // pixel, 0xFFFFFFFF where it does not query), then sorted with the pixel as value into
+    // gatherList[0, *gatherCount); the Alt arrays and sortTemp are the sort's scratch.
+    uint32_t* gatherKeys;
+    uint32_t* gatherList;
+    uint32_t* gatherKeysAlt;
+    uint32_t* gatherListAlt;
+    uint32_t* gatherCount;
+    void* gatherSortTemp;
+    size_t gatherSortBytes;
+    float4* gatherPos;      // [P] query point of each querying pixel
+    V3 mortonLo, mortonScale;   // quantisation of the query points (the scene's bounds)
     uint32_t* nbr;          // [P][16] neighbour pixel of each spatial slot
     float4* radScratch;     // [P] radiance summed over the final-gather prefixes
     float4* fgScratch;      // [P] final-gather prefix throughput xyz, pdf
