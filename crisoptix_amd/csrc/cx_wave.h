@@ -908,7 +908,11 @@ __device__ __forceinline__ Emit sh_continue(const DScene& S, const RecordSet& R,
         s.state = SH_TRACE;
         return emit_closest(curr.pos, bs.outDir);
     }
-    const Packed reconP = s.isSuffix ? load_suffix_ref(R, s.srcRef).recon : load_prefix_ref(R, s.srcRef).recon;
+    // an if, not ?: over the two records: the conditional operator on whole records made the
+    // compiler load both into private memory and read the field back from there
+    Packed reconP;
+    if (s.isSuffix) reconP = load_suffix_ref(R, s.srcRef).recon;
+    else reconP = load_prefix_ref(R, s.srcRef).recon;
     const Inter srcRecon = rehydrate(S, reconP);
     const V3 reconVec = srcRecon.pos - curr.pos;
     const float reconLen = length(reconVec);
