@@ -207,6 +207,48 @@ CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float
     return false;
 }
 
+// leaf_hits with the leaf's first triangle in q[0..2] and, when it has more than one, its second
+// in q[3..5], already loaded; later triangles are fetched one ahead as there.
+template <bool ANY>
+CX_HD bool leaf_hits_q(const DScene& S, uint32_t code, const float4* q, V3 o, const RayPre& rp, float tmin,
+                       float tmax, Hit& best) {
+    const uint32_t first = code >> 4, cnt = code & 15u;
+    const float4* tv = S.tris + 3 * (size_t)first;
+    float4 n0 = q[0], n1 = q[1], n2 = q[2];
+    for (uint32_t i = 0; i < cnt; i++) {
+        const float4 p0 = n0, p1 = n1, p2 = n2;
+        if (i + 1 < cnt) {
+            if (i == 0) {
+                n0 = q[3];
+                n1 = q[4];
+                n2 = q[5];
+            } else {
+                const float4* nv = tv + 3 * (size_t)(i + 1);
+                n0 = nv[0];
+                n1 = nv[1];
+                n2 = nv[2];
+            }
+        }
+        if (__builtin_bit_cast(uint32_t, p1.w) & (ANY ? 2u : 1u)) continue;   // invisible to this ray kind
+        float t, u, v;
+        if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, tmax, t, u, v))
+            continue;
+        const uint32_t prim = __builtin_bit_cast(uint32_t, p0.w);
+        if (ANY) {
+            best.prim = prim;
+            best.t = t;
+            return true;
+        }
+        if (t < best.t || (t == best.t && prim < best.prim)) {
+            best.prim = prim;
+            best.t = t;
+            best.u = u;
+            best.v = v;
+        }
+    }
+    return false;
+}
+
 // Slab test of one child of a wide node (same arithmetic and slack as box2).
 CX_HD float box1(float lx, float hx, float ly, float hy, float lz, float hz, V3 o, V3 inv,
                                       float tmin, float tcap, bool& hit) {
@@ -231,17 +273,18 @@ CX_HD void cswap(float& ta, int& ra, float& tb, int& rb) {
 
 // One wide node: test the N child boxes, sort the hit ones by entry distance (a fixed
 // compare-exchange network on registers, misses at +inf), push all but the nearest, farthest
-// first, and return the nearest in ref; false (ref unchanged) when no child box is hit.
+// first, and return the nearest in ref; false (ref unchanged) when no child box is hit.  q holds
+// the node's first 7 float4 per group of 4 children (bounds lo.x, hi.x, lo.y, hi.y, lo.z, hi.z
+// of the 4 children, then their references), already loaded.
 template <int N, typename Stack>
-CX_HD bool wide_visit(const DScene& S, int& ref, V3 o, V3 inv, float tmin, float tcap, Stack& st, int& sp) {
-    const float4* n = S.wnodes + (size_t)ref * (2 * N);
+CX_HD bool wide_visit_q(const float4* q, int& ref, V3 o, V3 inv, float tmin, float tcap, Stack& st, int& sp) {
     float tn[N];
     int cr[N];
 #pragma unroll
     for (int g = 0; g < N / 4; g++) {
-        const float4 lx = n[8 * g + 0], hx = n[8 * g + 1], ly = n[8 * g + 2], hy = n[8 * g + 3];
-        const float4 lz = n[8 * g + 4], hz = n[8 * g + 5];
-        const int4 rr = __builtin_bit_cast(int4, n[8 * g + 6]);
+        const float4 lx = q[7 * g + 0], hx = q[7 * g + 1], ly = q[7 * g + 2], hy = q[7 * g + 3];
+        const float4 lz = q[7 * g + 4], hz = q[7 * g + 5];
+        const int4 rr = __builtin_bit_cast(int4, q[7 * g + 6]);
         const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
         const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
         const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
@@ -282,6 +325,20 @@ CX_HD bool wide_visit(const DScene& S, int& ref, V3 o, V3 inv, float tmin, float
     }
     ref = cr[0];
     return true;
+}
+
+// wide_visit_q on node `ref` of the wide array (2 N float4 per node, the 8th of each group of
+// 4 children unused).
+template <int N, typename Stack>
+CX_HD bool wide_visit(const DScene& S, int& ref, V3 o, V3 inv, float tmin, float tcap, Stack& st, int& sp) {
+    const float4* n = S.wnodes + (size_t)ref * (2 * N);
+    float4 q[7 * (N / 4)];
+#pragma unroll
+    for (int g = 0; g < N / 4; g++) {
+#pragma unroll
+        for (int k = 0; k < 7; k++) q[7 * g + k] = n[8 * g + k];
+    }
+    return wide_visit_q<N>(q, ref, o, inv, tmin, tcap, st, sp);
 }
 
 // One binary node: the nearer hit child in ref (the farther one pushed); false when neither
@@ -378,10 +435,30 @@ CX_HD void trav_init(const DScene& S, TravRay& r, V3 o, V3 d, float tmin, float 
 
 // One step; true when the ray is answered: closest hit in r.best (prim INVALID_PRIM: a miss),
 // occlusion as r.best.prim != INVALID_PRIM.  A scene without triangles answers at once.
+//
+// At width 4 a step makes one fetch for both kinds of lane: a lane at an inner node loads the
+// node's 7 float4, a lane at a leaf its first two triangles (6 float4, the seventh load repeating
+// the last), in the same load instructions, before either kind is processed.  A wavefront whose
+// lanes are split between nodes and leaves so waits once per step instead of once for the node
+// fetches and again for the triangle fetches.
 template <bool ANY, int WIDTH, typename Stack>
 CX_HD bool trav_step(const DScene& S, TravRay& r, Stack& st) {
     if (S.numTris == 0) return true;
-    if (r.ref >= 0) {
+    if constexpr (WIDTH == 4) {
+        const bool inner = r.ref >= 0;
+        const uint32_t code = ~(uint32_t)r.ref;
+        // a leaf holds 1..15 triangles; its loads stay within its first two
+        const uint32_t last = inner ? 6u : ((code & 15u) >= 2u ? 5u : 2u);
+        const float4* base = inner ? S.wnodes + (size_t)r.ref * 8 : S.tris + 3 * (size_t)(code >> 4);
+        float4 q[7];
+#pragma unroll
+        for (uint32_t k = 0; k < 7; k++) q[k] = base[k < last ? k : last];
+        if (inner) {
+            if (wide_visit_q<4>(q, r.ref, r.o, r.inv, r.tmin, r.best.t, st, r.sp)) return false;
+        } else {
+            if (leaf_hits_q<ANY>(S, code, q, r.o, r.rp, r.tmin, r.tmax, r.best)) return true;
+        }
+    } else if (r.ref >= 0) {
         if constexpr (WIDTH == 2) {
             if (bin_visit(S, r.ref, r.o, r.inv, r.tmin, r.best.t, st, r.sp)) return false;
         } else {
