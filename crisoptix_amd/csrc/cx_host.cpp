@@ -366,9 +366,9 @@ int ensure_wave(cx_renderer* r, const cx_launch_params& lp) {
     const size_t capP = (size_t)QSTRIPES * queue_stripe_cap(r->P, r->P, r->gridLimit);
     const size_t capShift = (size_t)QSTRIPES * queue_stripe_cap((uint32_t)shift_jobs(r, lp), r->P, r->gridLimit);
     const size_t jobs = std::max(capP, capShift);
-    const size_t words = std::max<size_t>((size_t)PG_CHUNKS * capP, (size_t)SH_CHUNKS * capShift);
-    static_assert(PG_CHUNKS >= SG_CHUNKS && PG_CHUNKS >= CP_CHUNKS && PG_CHUNKS >= FG_CHUNKS,
-                  "path machines fit the prefix-generation context");
+    // the largest context of the per-pixel path machines
+    constexpr size_t pathChunks = std::max({PG_CHUNKS, SG_CHUNKS, CP_CHUNKS, FG_CHUNKS});
+    const size_t words = std::max<size_t>(pathChunks * capP, (size_t)SH_CHUNKS * capShift);
     if (jobs <= w.jobCap && words <= w.ctxWords) return CX_OK;
     if (r->callerCapture)
         return set_err(r->ctx, CX_ERR_INVALID_ARG,

@@ -279,7 +279,13 @@ struct Rd {
 // ======================================================================================
 
 static constexpr uint32_t PG_TRACE = 1, PG_NEE = 2;
-static constexpr int PG_CHUNKS = 15;
+// The queued context of a walk (11 chunks, 176 bytes): the prefix record, the generator, the
+// state word, and the traced interaction and emitter colour of a pending NEE ray.  The rest of
+// PgState is rebuilt from the job's queued ray (pg_unpack): while a closest-hit ray is pending,
+// cur and dir are its origin and direction; while an NEE ray is pending, lightDir and dist are its
+// direction and tmax, and dir (the direction that reached the interaction) is the interaction's
+// stored incoming direction.
+static constexpr int PG_CHUNKS = 11;
 
 struct PgState {
     Prefix pre;
@@ -298,11 +304,12 @@ __device__ __forceinline__ Ctx<PG_CHUNKS> pg_pack(const PgState& s) {
     w.u64(p.rng); w.u(p.flags); w.f(p.p); w.v(p.f); w.f(p.pathLen); w.v(p.postReconF); w.v(p.reconOutDir);
     w.pk(p.primary); w.pk(p.recon); w.pk(p.last);
     w.u64(s.rng); w.u(s.traceIdx | (s.postRecon << 8) | (s.state << 16));
-    w.v(s.cur); w.v(s.dir); w.pk(s.it); w.v(s.lightDir); w.f(s.dist); w.v(s.color);
+    w.pk(s.it); w.v(s.color);
     return w.ctx();
 }
 
-__device__ __forceinline__ PgState pg_unpack(const Ctx<PG_CHUNKS>& c) {
+// ray: the job's queued ray {origin, tmin}, {direction, tmax}.
+__device__ __forceinline__ PgState pg_unpack(const Ctx<PG_CHUNKS>& c, float4 ra, float4 rb) {
     Rd<PG_CHUNKS> r(c);
     PgState s;
     Prefix& p = s.pre;
@@ -313,7 +320,17 @@ __device__ __forceinline__ PgState pg_unpack(const Ctx<PG_CHUNKS>& c) {
     s.traceIdx = m & 0xFFu;
     s.postRecon = (m >> 8) & 1u;
     s.state = m >> 16;
-    s.cur = r.v(); s.dir = r.v(); s.it = r.pk(); s.lightDir = r.v(); s.dist = r.f(); s.color = r.v();
+    s.it = r.pk(); s.color = r.v();
+    s.cur = V3{ra.x, ra.y, ra.z};
+    if (s.state == PG_NEE) {
+        s.dir = s.it.dir;
+        s.lightDir = V3{rb.x, rb.y, rb.z};
+        s.dist = rb.w;
+    } else {
+        s.dir = V3{rb.x, rb.y, rb.z};
+        s.lightDir = v3(0.0f);
+        s.dist = 0.0f;
+    }
     return s;
 }
 
@@ -453,7 +470,12 @@ __device__ __forceinline__ Emit pg_on_shadow(const DScene& S, const cx_launch_pa
 // ======================================================================================
 
 static constexpr uint32_t SG_TRACE = 1, SG_NEE = 2, SG_RECON = 3;
-static constexpr int SG_CHUNKS = 13;
+// The queued context of a suffix walk (10 chunks): the suffix record, the generator, the state
+// words and the emitter colour, then 6 words whose use depends on the pending ray: the NEE ray's
+// interaction (SG_NEE), or the BSDF value and the light distance (SG_RECON).  The rest of SgState
+// is rebuilt from the job's queued ray (sg_unpack): lastDir is a closest-hit ray's direction
+// (SG_TRACE), lightDir the NEE or reconnection ray's direction, lightLen the NEE ray's tmax.
+static constexpr int SG_CHUNKS = 10;
 
 struct SgState {
     Suffix suf;
@@ -475,11 +497,17 @@ __device__ __forceinline__ Ctx<SG_CHUNKS> sg_pack(const SgState& s) {
     w.u64(s.rng);
     w.u(s.vertIdx | (s.maxLen << 8) | (s.prefixLen << 16) | (s.postRecon << 24) | (s.itTraced << 25));
     w.u(s.state);
-    w.pk(s.it); w.v(s.lastDir); w.v(s.lightDir); w.f(s.lightLen); w.v(s.color); w.v(s.brdf);
+    w.v(s.color);
+    if (s.state == SG_RECON) {
+        w.v(s.brdf); w.f(s.lightLen);
+    } else {
+        w.pk(s.it);
+    }
     return w.ctx();
 }
 
-__device__ __forceinline__ SgState sg_unpack(const Ctx<SG_CHUNKS>& c) {
+// ra, rb: the job's queued ray {origin, tmin}, {direction, tmax}.
+__device__ __forceinline__ SgState sg_unpack(const Ctx<SG_CHUNKS>& c, float4 ra, float4 rb) {
     Rd<SG_CHUNKS> r(c);
     SgState s;
     Suffix& u = s.suf;
@@ -493,7 +521,19 @@ __device__ __forceinline__ SgState sg_unpack(const Ctx<SG_CHUNKS>& c) {
     s.postRecon = (m >> 24) & 1u;
     s.itTraced = (m >> 25) & 1u;
     s.state = r.u();
-    s.it = r.pk(); s.lastDir = r.v(); s.lightDir = r.v(); s.lightLen = r.f(); s.color = r.v(); s.brdf = r.v();
+    s.color = r.v();
+    const V3 d = V3{rb.x, rb.y, rb.z};
+    s.lastDir = d;
+    s.lightDir = d;
+    s.brdf = v3(0.0f);
+    s.lightLen = rb.w;
+    if (s.state == SG_RECON) {
+        s.brdf = r.v(); s.lightLen = r.f();
+        s.it = packed_invalid();
+    } else {
+        s.it = r.pk();
+    }
+    (void)ra;
     return s;
 }
 
@@ -683,7 +723,7 @@ __device__ __forceinline__ PgState pg_load(const Queue& q, uint32_t slot, const 
     c.w[0] = c0;
 #pragma unroll
     for (int i = 1; i < PG_CHUNKS; i++) c.w[i] = q.ctx[(size_t)i * q.cap + slot];
-    return pg_unpack(c);
+    return pg_unpack(c, q.ray[slot], q.ray[(size_t)q.cap + slot]);
 }
 
 __device__ __forceinline__ CpState cp_load(const Queue& q, uint32_t slot) {

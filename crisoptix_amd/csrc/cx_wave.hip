@@ -691,7 +691,7 @@ __global__ __launch_bounds__(WBLOCK) void k_w_sg_step(WaveArgs W) {
     if (active) {
         const uint32_t slot = in_slot(W.qin, _qi, i);
         job = W.qin.job[slot];
-        s = sg_unpack(load_ctx<SG_CHUNKS>(W.qin, slot));
+        s = sg_unpack(load_ctx<SG_CHUNKS>(W.qin, slot), W.qin.ray[slot], W.qin.ray[(size_t)W.qin.cap + slot]);
         V3 o, d;
         get_ray(W.qin, slot, o, d);
         e = sg_advance(W, job, s, W.qin.res[slot], o, d);
@@ -1177,7 +1177,9 @@ struct Mach<WK_CP_STEP> {
 template <>
 struct Mach<WK_SG_STEP> {
     using S = SgState;
-    __device__ static S load(const WaveArgs& W, uint32_t slot) { return sg_unpack(load_ctx<SG_CHUNKS>(W.qin, slot)); }
+    __device__ static S load(const WaveArgs& W, uint32_t slot) {
+        return sg_unpack(load_ctx<SG_CHUNKS>(W.qin, slot), W.qin.ray[slot], W.qin.ray[(size_t)W.qin.cap + slot]);
+    }
     __device__ static Emit advance(const WaveArgs& W, uint32_t job, S& s, uint4 r, V3 o, V3 d) {
         return sg_advance(W, job, s, r, o, d);
     }
