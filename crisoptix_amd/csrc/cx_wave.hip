@@ -93,9 +93,10 @@ __device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, 
 // Answers every ray of qin in place (closest hit: prim, t, u, v; occlusion: the flag in x),
 // zeroes the stripe counts of qout (the next step kernel's output queue) and adds the rays to
 // the pass counters.  Each wavefront traces its chunks of the closest rays and then of the
-// shadow rays (trace_chunks), and the grid is what fits on the device at once.  The 2- and 4-wide kernels run at 5 waves per SIMD (102 VGPRs, no spills; measured on the
-// 1080p frame, per launch: 0.96 ms at 5 waves, 0.98 at 6, 1.05 at 4, 1.37 at 8 with 27 VGPRs
-// spilled; 1.04 ms for the former one-ray-per-iteration kernel at 8 waves).
+// shadow rays (trace_chunks), and the grid is what fits on the device at once.  The 2- and
+// 4-wide kernels are held to 5 waves per SIMD (82-84 VGPRs, no spills; at 4 or 6 waves the
+// 1080p frame is slower, and at 8 the kernel spilled 27 VGPRs: 1.37 against 0.96 ms per launch
+// when measured on an earlier version of the traversal).
 template <int WIDTH>
 __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH == 8 ? 1 : 5, 8))) void k_q_trace(
     DScene S, Queue qin, Queue qout, unsigned long long* counters) {
