@@ -25,14 +25,16 @@ __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + thr
 // ---- traversal of a queue ----
 
 // The rays of wavefront w among `waves`, out of the n rays [base, base + n) of qin's index space:
-// the range is cut into chunks of TRACE_CHUNK rays and wavefront w takes chunks w, w + waves,
-// w + 2 waves, ...  A chunk holds the appends of one or two producing wavefronts, so its rays
-// start close together, and the chunks of one wavefront come from all over the queue: a
-// contiguous range per wavefront held rays of one region of the image (the stripes keep
-// neighbouring producers together), and the wavefronts given the expensive regions finished
-// last (1080p frame, measured together: 24.9 ms with 32-ray chunks, 25.6 ms with contiguous
-// ranges; 16-64 rays per chunk time alike, while 128 and 256 leave a whole chunk more for some
-// wavefronts than for others, 25.5 and 27.9 ms).
+// the range is cut into chunks of ch rays and wavefront w takes chunks w, w + waves, w + 2 waves,
+// ...  A chunk holds the appends of one producing wavefront or a part of one, so its rays start
+// close together, and the chunks of one wavefront come from all over the queue: a contiguous
+// range per wavefront held rays of one region of the image (the stripes keep neighbouring
+// producers together), and the wavefronts given the expensive regions finished last (1080p
+// frame, measured together: 24.9 ms with 32-ray chunks, 25.6 ms with contiguous ranges).  Some
+// wavefronts get one chunk more than others, so a chunk is at most a quarter of a wavefront's
+// share, ch = min(TRACE_CHUNK, n / (4 waves)), at least 1: with 32-ray chunks throughout, a 512²
+// frame (a few chunks per wavefront) took 2.10 ms instead of 1.86, and 128- or 256-ray chunks
+// made the 1080p frame 25.5 and 27.9 ms; 16-64-ray chunks time alike there.
 // The wavefront's lanes each hold one ray at a time: a lane whose ray is answered writes the
 // result and idles until at least TRACE_REFILL lanes of the wavefront (or as many as rays
 // remain) are idle, and then those lanes take the next rays together.  A wavefront so stays
@@ -45,13 +47,15 @@ template <bool ANY, int WIDTH, typename Stack>
 __device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, const QIndex& qi, uint32_t base,
                                              uint32_t n, uint32_t w, uint32_t waves, Stack& st) {
     // cnt = rays of this wavefront: its chunks w + m waves < nch, the last chunk of the range
-    // holding n - (nch - 1) TRACE_CHUNK rays
-    const uint32_t nch = (n + TRACE_CHUNK - 1) / TRACE_CHUNK;
+    // holding n - (nch - 1) ch rays
+    const uint32_t share = n / (waves * 4u);
+    const uint32_t ch = share < 1u ? 1u : (share < TRACE_CHUNK ? share : TRACE_CHUNK);
+    const uint32_t nch = (n + ch - 1) / ch;
     uint32_t cnt = 0;
     if (w < nch) {
         const uint32_t mine = (nch - 1 - w) / waves + 1;
-        cnt = mine * TRACE_CHUNK;
-        if (w + (mine - 1) * waves == nch - 1) cnt -= nch * TRACE_CHUNK - n;
+        cnt = mine * ch;
+        if (w + (mine - 1) * waves == nch - 1) cnt -= nch * ch - n;
     }
     const uint64_t below = (1ull << (threadIdx.x & 63)) - 1ull;
     uint32_t cursor = 0;   // wavefront-uniform, counts this wavefront's rays
@@ -66,7 +70,7 @@ __device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, 
                 if (!has) {
                     const uint32_t v = cursor + (uint32_t)__popcll(idle & below);
                     if (v < cnt) {
-                        const uint32_t i = base + (w + (v / TRACE_CHUNK) * waves) * TRACE_CHUNK + v % TRACE_CHUNK;
+                        const uint32_t i = base + (w + (v / ch) * waves) * ch + v % ch;
                         slot = in_slot(qin, qi, i);
                         const float4 a = qin.ray[slot], b = qin.ray[(size_t)qin.cap + slot];
                         trav_init<WIDTH>(S, r, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w);
