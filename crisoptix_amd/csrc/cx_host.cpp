@@ -112,6 +112,9 @@ struct cx_renderer {
     // Most blocks of the grid-stride queue kernels.  1080p Conditional ReSTIR frame: 2048 / 4096 /
     // 8192 / 16384 -> 9.53 / 8.59-8.62 / 8.45 / 8.47 ms; a machine of fewer jobs launches fewer.
     uint32_t gridLimit = 8192;
+    // Set by cx_batch_frame while it enqueues this renderer's frame beside other views: the
+    // traversal then keeps whole 32-ray chunks (wave_trace minChunk); results are unchanged.
+    bool concurrent = false;
     uint32_t waveRounds = CX_WAVE_ROUNDS_DEFAULT;   // rounds before the tail kernel, every machine (0: all; default per machine)
     uint64_t allocEpoch = 0;     // bumped whenever a buffer a frame uses is reallocated
     // Graph mode: a frame is one hipGraphLaunch of a graph captured from the eager frame for
@@ -499,7 +502,7 @@ hipError_t run_machine(cx_renderer* r, WaveArgs W, int setup, int step, uint32_t
     for (uint32_t i = 0; i < waveRounds && e == hipSuccess; i++) {
         const Queue qin = queue_view(r, cur, cap), qout = queue_view(r, 1 - cur, cap);
         trace_rec(r, s);
-        e = wave_trace(W.A.S, qin, qout, W.A.counters + 2 * pass, s);
+        e = wave_trace(W.A.S, qin, qout, W.A.counters + 2 * pass, r->concurrent ? ~0u : 1u, s);
         trace_rec(r, s);
         W.qin = qin;
         W.qout = qout;
@@ -1527,7 +1530,9 @@ int cx_batch_frame(cx_ctx* ctx, cx_renderer* const* rs, const cx_camera* cams, c
     }
     for (uint32_t i = 0; i < n; i++) {
         void* s = streams ? streams[i] : (void*)ctx->pool[i];
+        rs[i]->concurrent = n > 1;
         const int rc = cx_renderer_frame(rs[i], &cams[i], cameraChanged[i], outs[i], s);
+        rs[i]->concurrent = false;
         if (rc != CX_OK) {
             if (rs[i]->ctx != ctx) set_err(ctx, rc, cx_last_error(rs[i]->ctx));
             return rc;

@@ -92,8 +92,10 @@ enum WaveKernel {
 
 hipError_t wave_zero_counts(uint32_t* c, hipStream_t s);
 hipError_t wave_tail(int step, const WaveArgs& W, uint32_t maxSteps, int pass, hipStream_t s);
+// minChunk: the fewest rays per traversal chunk (1 for a frame that has the device alone, the
+// largest chunk for one that runs beside others; results are the same either way).
 hipError_t wave_trace(const DScene& S, const Queue& qin, const Queue& qout, unsigned long long* counters,
-                      hipStream_t s);
+                      uint32_t minChunk, hipStream_t s);
 hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStream_t s);
 
 // Most triangles per BVH leaf when cx_scene_desc.bvhMaxLeafSize is 0.  Measured on the

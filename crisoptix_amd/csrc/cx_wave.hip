@@ -32,9 +32,13 @@ __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + thr
 // producers together), and the wavefronts given the expensive regions finished last (1080p
 // frame, measured together: 24.9 ms with 32-ray chunks, 25.6 ms with contiguous ranges).  Some
 // wavefronts get one chunk more than others, so a chunk is at most a quarter of a wavefront's
-// share, ch = min(TRACE_CHUNK, n / (4 waves)), at least 1: with 32-ray chunks throughout, a 512²
-// frame (a few chunks per wavefront) took 2.10 ms instead of 1.86, and 128- or 256-ray chunks
-// made the 1080p frame 25.5 and 27.9 ms; 16-64-ray chunks time alike there.
+// share, ch = min(TRACE_CHUNK, n / (4 waves)), at least minChunk: with 32-ray chunks throughout,
+// a 512² frame (a few chunks per wavefront) took 2.10 ms instead of 1.86, and 128- or 256-ray
+// chunks made the 1080p frame 25.5 and 27.9 ms; 16-64-ray chunks time alike there.  A frame that
+// runs beside other views (cx_batch_frame) passes minChunk = TRACE_CHUNK: the others fill the
+// CUs its late wavefronts leave, and whole chunks keep rays that start together on one
+// wavefront (16 views: 129.6-131.1 ms per batch frame with 32-ray chunks against 133.5-134.2
+// with a quarter share).
 // The wavefront's lanes each hold one ray at a time: a lane whose ray is answered writes the
 // result and idles until at least TRACE_REFILL lanes of the wavefront (or as many as rays
 // remain) are idle, and then those lanes take the next rays together.  A wavefront so stays
@@ -45,11 +49,11 @@ static constexpr uint32_t TRACE_CHUNK = 32;
 
 template <bool ANY, int WIDTH, typename Stack>
 __device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, const QIndex& qi, uint32_t base,
-                                             uint32_t n, uint32_t w, uint32_t waves, Stack& st) {
+                                             uint32_t n, uint32_t w, uint32_t waves, uint32_t minChunk, Stack& st) {
     // cnt = rays of this wavefront: its chunks w + m waves < nch, the last chunk of the range
     // holding n - (nch - 1) ch rays
     const uint32_t share = n / (waves * 4u);
-    const uint32_t ch = share < 1u ? 1u : (share < TRACE_CHUNK ? share : TRACE_CHUNK);
+    const uint32_t ch = share < minChunk ? minChunk : (share < TRACE_CHUNK ? share : TRACE_CHUNK);
     const uint32_t nch = (n + ch - 1) / ch;
     uint32_t cnt = 0;
     if (w < nch) {
@@ -103,7 +107,7 @@ __device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, 
 // when measured on an earlier version of the traversal).
 template <int WIDTH>
 __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH == 8 ? 1 : 5, 8))) void k_q_trace(
-    DScene S, Queue qin, Queue qout, unsigned long long* counters) {
+    DScene S, Queue qin, Queue qout, unsigned long long* counters, uint32_t minChunk) {
     __shared__ int s_st[SHORT_STACK * WBLOCK];
     __shared__ QIndex qi;
     qindex_load(qin, qi);
@@ -120,8 +124,8 @@ __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH ==
     st.lds = s_st + threadIdx.x;
     st.stride = WBLOCK;
     const uint32_t waves = gridDim.x * (WBLOCK / 64), w = blockIdx.x * (WBLOCK / 64) + (threadIdx.x >> 6);
-    trace_chunks<false, WIDTH>(S, qin, qi, 0u, nc, w, waves, st);
-    trace_chunks<true, WIDTH>(S, qin, qi, nc, ns, w, waves, st);
+    trace_chunks<false, WIDTH>(S, qin, qi, 0u, nc, w, waves, minChunk, st);
+    trace_chunks<true, WIDTH>(S, qin, qi, nc, ns, w, waves, minChunk, st);
 }
 
 // Grid-stride loop over the jobs of qin whose bound is uniform per block, so every lane of a
@@ -1324,11 +1328,14 @@ hipError_t wave_zero_counts(uint32_t* c, hipStream_t s) {
 }
 
 hipError_t wave_trace(const DScene& S, const Queue& qin, const Queue& qout, unsigned long long* counters,
-                      hipStream_t s) {
+                      uint32_t minChunk, hipStream_t s) {
     const dim3 b(WBLOCK);
-    if (S.width == 4) hipLaunchKernelGGL(k_q_trace<4>, range_grid(qin, k_q_trace<4>), b, 0, s, S, qin, qout, counters);
-    else if (S.width == 8) hipLaunchKernelGGL(k_q_trace<8>, range_grid(qin, k_q_trace<8>), b, 0, s, S, qin, qout, counters);
-    else hipLaunchKernelGGL(k_q_trace<2>, range_grid(qin, k_q_trace<2>), b, 0, s, S, qin, qout, counters);
+    const uint32_t mc = std::max(1u, std::min(minChunk, TRACE_CHUNK));
+    if (S.width == 4)
+        hipLaunchKernelGGL(k_q_trace<4>, range_grid(qin, k_q_trace<4>), b, 0, s, S, qin, qout, counters, mc);
+    else if (S.width == 8)
+        hipLaunchKernelGGL(k_q_trace<8>, range_grid(qin, k_q_trace<8>), b, 0, s, S, qin, qout, counters, mc);
+    else hipLaunchKernelGGL(k_q_trace<2>, range_grid(qin, k_q_trace<2>), b, 0, s, S, qin, qout, counters, mc);
     return hipGetLastError();
 }
 
