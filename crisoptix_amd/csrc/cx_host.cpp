@@ -1634,6 +1634,9 @@ int cx_renderer_adjoint(cx_renderer* r, const float* dLdI, double* gAlbedo, doub
     G.status = status;
     G.evalMats = r->scene->evalMats;
     G.evalEmit = r->scene->evalEmit;
+    G.numMaterials = r->scene->numMaterials;
+    G.numEmitters = r->scene->d.numEmitters;
+    G.ldsSlots = 0;
     CX_HIP(r->ctx, launch_adjoint(A, G, (hipStream_t)stream));
     return CX_OK;
 }

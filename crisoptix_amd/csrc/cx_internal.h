@@ -59,6 +59,10 @@ struct AdjArgs {
     uint32_t* status;              // [3] replayed, replay differs from the record, skipped
     const DMaterial* evalMats;     // values and derivatives use these materials
     const float* evalEmit;         // and these emitter colours
+    uint32_t numMaterials, numEmitters;
+    // Gradient slots summed per block in LDS before one global add each (0: every term is a
+    // global add): albedo [0, 3M), roughness [3M, 4M), emitter colour [4M, 4M + 3E).
+    uint32_t ldsSlots;
 };
 hipError_t launch_adjoint(const FrameArgs& A, const AdjArgs& G, hipStream_t s);
 hipError_t launch_tonemap(const float* hdr, uint8_t* ldr, uint32_t n, hipStream_t s);
