@@ -190,3 +190,37 @@ def test_torch_autograd_function(cx_runtime, cx_ctx):
     assert np.allclose(albedo.grad.numpy(), gA, rtol=1e-5, atol=1e-6 * np.abs(gA).max())
     assert np.allclose(rough.grad.numpy(), gR, rtol=1e-5, atol=1e-6 * np.abs(gR).max())
     assert np.allclose(emit.grad.numpy(), gE, rtol=1e-5, atol=1e-6 * np.abs(gE).max())
+
+
+def test_block_sums_equal_global_adds(cx_runtime, cx_ctx):
+    """The adjoint sums its gradient terms per block in LDS when the table is small (4 M + 3 E
+    doubles within 24 KiB) and adds every term to the global arrays otherwise.  The same frames
+    with 800 unused materials appended to the table (so 4 M + 3 E doubles exceed 24 KiB) take the
+    second path: the gradients of the used materials agree with the first path's within FP64
+    summation-order rounding, the unused materials get none, and the replay is the same."""
+    import dataclasses
+    sc = scenes.inverse_scene()
+    pad = np.repeat(sc.materials[:1], 800, axis=0)
+    padded = dataclasses.replace(sc, materials=np.concatenate([sc.materials, pad]).astype(np.float32))
+    M, E = sc.materials.shape[0], sc.emitter_color.shape[0]
+    assert 8 * (4 * M + 3 * E) <= 24 * 1024 < 8 * (4 * padded.materials.shape[0] + 3 * E)
+    g = np.random.default_rng(9).standard_normal((H, W, 3)).astype(np.float32)
+    res = []
+    for s in (sc, padded):
+        gs = cx_runtime.Scene(cx_ctx, s)
+        p = abi.workload_params("restir_pt_cornell", W, H, gs.size)
+        p.maxPathLen = 4
+        cam = scenes.orbit_camera(s, W, H)
+        cd = abi.CameraData.from_dict(cam.data())
+        ren = cx_runtime.Renderer(W, H, cx_ctx, cam, gs, params=p, seed=11)
+        for f in range(FRAMES):
+            ren.launch_frame(cd, f == 0)
+        res.append(gpu_adjoint(cx_runtime, s, ren, g))
+    (a0, r0, e0, rep0, st0), (a1, r1, e1, rep1, st1) = res
+    assert np.array_equal(rep0.view(np.uint32), rep1.view(np.uint32))
+    assert np.array_equal(st0, st1) and st0[0] > 0
+    scale = max(np.abs(a0).max(), np.abs(r0).max(), np.abs(e0).max())
+    assert scale > 0
+    for x, y in ((a0, a1[:M]), (r0, r1[:M]), (e0, e1)):
+        np.testing.assert_allclose(y, x, rtol=1e-10, atol=1e-13 * scale)
+    assert not np.any(a1[M:]) and not np.any(r1[M:])
