@@ -170,9 +170,8 @@ struct TravStack {
 // an any-hit query is answered.  Triangles whose skip bits hide them from the ray kind (closest
 // hit: bit 0, any hit: bit 1) are passed over, as OptiX passes over an instance whose
 // visibilityMask shares no bit with the ray's.
-template <bool ANY>
-CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float tmin,
-                                          float tmax, Hit& best) {
+CX_HD bool leaf_hits_rt(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float tmin, float tmax, Hit& best,
+                       bool any) {
     const uint32_t code = ~(uint32_t)ref;
     const uint32_t first = code >> 4, cnt = code & 15u;
     // the next triangle's vertices are fetched before the current one is tested, so a leaf's
@@ -187,12 +186,12 @@ CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float
             n1 = nv[1];
             n2 = nv[2];
         }
-        if (__builtin_bit_cast(uint32_t, p1.w) & (ANY ? 2u : 1u)) continue;   // invisible to this ray kind
+        if (__builtin_bit_cast(uint32_t, p1.w) & (any ? 2u : 1u)) continue;   // invisible to this ray kind
         float t, u, v;
         if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, tmax, t, u, v))
             continue;
         const uint32_t prim = __builtin_bit_cast(uint32_t, p0.w);
-        if (ANY) {
+        if (any) {
             best.prim = prim;
             best.t = t;
             return true;
@@ -207,11 +206,15 @@ CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float
     return false;
 }
 
+template <bool ANY>
+CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float tmin, float tmax, Hit& best) {
+    return leaf_hits_rt(S, ref, o, rp, tmin, tmax, best, ANY);
+}
+
 // leaf_hits with the leaf's first triangle in q[0..2] and, when it has more than one, its second
 // in q[3..5], already loaded; later triangles are fetched one ahead as there.
-template <bool ANY>
 CX_HD bool leaf_hits_q(const DScene& S, uint32_t code, const float4* q, V3 o, const RayPre& rp, float tmin,
-                       float tmax, Hit& best) {
+                       float tmax, Hit& best, bool any) {
     const uint32_t first = code >> 4, cnt = code & 15u;
     const float4* tv = S.tris + 3 * (size_t)first;
     float4 n0 = q[0], n1 = q[1], n2 = q[2];
@@ -229,12 +232,12 @@ CX_HD bool leaf_hits_q(const DScene& S, uint32_t code, const float4* q, V3 o, co
                 n2 = nv[2];
             }
         }
-        if (__builtin_bit_cast(uint32_t, p1.w) & (ANY ? 2u : 1u)) continue;   // invisible to this ray kind
+        if (__builtin_bit_cast(uint32_t, p1.w) & (any ? 2u : 1u)) continue;   // invisible to this ray kind
         float t, u, v;
         if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, tmax, t, u, v))
             continue;
         const uint32_t prim = __builtin_bit_cast(uint32_t, p0.w);
-        if (ANY) {
+        if (any) {
             best.prim = prim;
             best.t = t;
             return true;
@@ -441,8 +444,8 @@ CX_HD void trav_init(const DScene& S, TravRay& r, V3 o, V3 d, float tmin, float 
 // the last), in the same load instructions, before either kind is processed.  A wavefront whose
 // lanes are split between nodes and leaves so waits once per step instead of once for the node
 // fetches and again for the triangle fetches.
-template <bool ANY, int WIDTH, typename Stack>
-CX_HD bool trav_step(const DScene& S, TravRay& r, Stack& st) {
+template <int WIDTH, typename Stack>
+CX_HD bool trav_step_rt(const DScene& S, TravRay& r, Stack& st, bool any) {
     if (S.numTris == 0) return true;
     if constexpr (WIDTH == 4) {
         const bool inner = r.ref >= 0;
@@ -456,7 +459,7 @@ CX_HD bool trav_step(const DScene& S, TravRay& r, Stack& st) {
         if (inner) {
             if (wide_visit_q<4>(q, r.ref, r.o, r.inv, r.tmin, r.best.t, st, r.sp)) return false;
         } else {
-            if (leaf_hits_q<ANY>(S, code, q, r.o, r.rp, r.tmin, r.tmax, r.best)) return true;
+            if (leaf_hits_q(S, code, q, r.o, r.rp, r.tmin, r.tmax, r.best, any)) return true;
         }
     } else if (r.ref >= 0) {
         if constexpr (WIDTH == 2) {
@@ -465,12 +468,17 @@ CX_HD bool trav_step(const DScene& S, TravRay& r, Stack& st) {
             if (wide_visit<WIDTH>(S, r.ref, r.o, r.inv, r.tmin, r.best.t, st, r.sp)) return false;
         }
     } else {
-        if (leaf_hits<ANY>(S, r.ref, r.o, r.rp, r.tmin, r.tmax, r.best)) return true;
+        if (leaf_hits_rt(S, r.ref, r.o, r.rp, r.tmin, r.tmax, r.best, any)) return true;
     }
     if (r.sp == 0) return true;
     --r.sp;
     r.ref = st.get(r.sp);
     return false;
+}
+
+template <bool ANY, int WIDTH, typename Stack>
+CX_HD bool trav_step(const DScene& S, TravRay& r, Stack& st) {
+    return trav_step_rt<WIDTH>(S, r, st, ANY);
 }
 
 // Traversal at a BVH width fixed at compile time (WIDTH = S.width; the caller dispatches).

@@ -47,23 +47,48 @@ __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + thr
 static constexpr uint32_t TRACE_REFILL = 16;
 static constexpr uint32_t TRACE_CHUNK = 32;
 
-template <bool ANY, int WIDTH, typename Stack>
-__device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, const QIndex& qi, uint32_t base,
-                                             uint32_t n, uint32_t w, uint32_t waves, uint32_t minChunk, Stack& st) {
+// Wavefront w's chunks of the n rays [base, base + n): cnt rays in chunks of ch.
+struct ChunkRange {
+    uint32_t base, ch, cnt;
+};
+
+__device__ __forceinline__ ChunkRange chunk_range(uint32_t base, uint32_t n, uint32_t w, uint32_t waves,
+                                                  uint32_t minChunk) {
     // cnt = rays of this wavefront: its chunks w + m waves < nch, the last chunk of the range
     // holding n - (nch - 1) ch rays
+    ChunkRange c;
     const uint32_t share = n / (waves * 4u);
-    const uint32_t ch = share < minChunk ? minChunk : (share < TRACE_CHUNK ? share : TRACE_CHUNK);
-    const uint32_t nch = (n + ch - 1) / ch;
-    uint32_t cnt = 0;
+    c.base = base;
+    c.ch = share < minChunk ? minChunk : (share < TRACE_CHUNK ? share : TRACE_CHUNK);
+    const uint32_t nch = (n + c.ch - 1) / c.ch;
+    c.cnt = 0;
     if (w < nch) {
         const uint32_t mine = (nch - 1 - w) / waves + 1;
-        cnt = mine * ch;
-        if (w + (mine - 1) * waves == nch - 1) cnt -= nch * ch - n;
+        c.cnt = mine * c.ch;
+        if (w + (mine - 1) * waves == nch - 1) c.cnt -= nch * c.ch - n;
     }
+    return c;
+}
+
+// Index in qin of the wavefront's v-th ray of range c.
+__device__ __forceinline__ uint32_t chunk_ray(const ChunkRange& c, uint32_t v, uint32_t w, uint32_t waves) {
+    return c.base + (w + (v / c.ch) * waves) * c.ch + v % c.ch;
+}
+
+// The wavefront's closest-hit chunks and then its shadow chunks as one sequence, so a lane whose
+// closest-hit ray is answered takes a shadow ray at the next refill instead of idling until its
+// wavefront has finished every closest-hit ray.  A lane's ray kind is a runtime flag: the two
+// kinds step through the same code, which differs only in the leaf's early exit and the result
+// written.
+template <int WIDTH, typename Stack>
+__device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, const QIndex& qi, uint32_t nc,
+                                             uint32_t ns, uint32_t w, uint32_t waves, uint32_t minChunk,
+                                             Stack& st) {
+    const ChunkRange cc = chunk_range(0u, nc, w, waves, minChunk), cs = chunk_range(nc, ns, w, waves, minChunk);
+    const uint32_t cnt = cc.cnt + cs.cnt;
     const uint64_t below = (1ull << (threadIdx.x & 63)) - 1ull;
     uint32_t cursor = 0;   // wavefront-uniform, counts this wavefront's rays
-    bool has = false;
+    bool has = false, any = false;
     uint32_t slot = 0;
     TravRay r;
     while (true) {
@@ -74,7 +99,8 @@ __device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, 
                 if (!has) {
                     const uint32_t v = cursor + (uint32_t)__popcll(idle & below);
                     if (v < cnt) {
-                        const uint32_t i = base + (w + (v / ch) * waves) * ch + v % ch;
+                        any = v >= cc.cnt;
+                        const uint32_t i = any ? chunk_ray(cs, v - cc.cnt, w, waves) : chunk_ray(cc, v, w, waves);
                         slot = in_slot(qin, qi, i);
                         const float4 a = qin.ray[slot], b = qin.ray[(size_t)qin.cap + slot];
                         trav_init<WIDTH>(S, r, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w);
@@ -88,11 +114,11 @@ __device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, 
             if (cursor >= cnt) break;
             continue;
         }
-        if (trav_step<ANY, WIDTH>(S, r, st)) {
+        if (trav_step_rt<WIDTH>(S, r, st, any)) {
             const Hit& h = r.best;
             const bool hit = h.prim != INVALID_PRIM;
-            if (ANY) qin.res[slot] = uint4{hit ? 1u : 0u, 0u, 0u, 0u};
-            else qin.res[slot] = uint4{h.prim, fb(hit ? h.t : 0.0f), fb(hit ? h.u : 0.0f), fb(hit ? h.v : 0.0f)};
+            qin.res[slot] = any ? uint4{hit ? 1u : 0u, 0u, 0u, 0u}
+                                : uint4{h.prim, fb(hit ? h.t : 0.0f), fb(hit ? h.u : 0.0f), fb(hit ? h.v : 0.0f)};
             has = false;
         }
     }
@@ -101,10 +127,11 @@ __device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, 
 // Answers every ray of qin in place (closest hit: prim, t, u, v; occlusion: the flag in x),
 // zeroes the stripe counts of qout (the next step kernel's output queue) and adds the rays to
 // the pass counters.  Each wavefront traces its chunks of the closest rays and then of the
-// shadow rays (trace_chunks), and the grid is what fits on the device at once.  The 2- and
-// 4-wide kernels are held to 5 waves per SIMD (82-84 VGPRs, no spills; at 4 or 6 waves the
-// 1080p frame is slower, and at 8 the kernel spilled 27 VGPRs: 1.37 against 0.96 ms per launch
-// when measured on an earlier version of the traversal).
+// shadow rays in one refill loop (trace_chunks), and the grid is what fits on the device at
+// once.  The 2- and 4-wide kernels are allowed 5 to 8 waves per SIMD; the 4-wide one needs 74
+// VGPRs and so runs at 6 (1080p frame 22.9 ms; held to exactly 5 waves, 25.6 ms).  Forcing 8
+// waves spilled 27 VGPRs on an earlier version of the traversal (1.37 against 0.96 ms per
+// launch).
 template <int WIDTH>
 __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH == 8 ? 1 : 5, 8))) void k_q_trace(
     DScene S, Queue qin, Queue qout, unsigned long long* counters, uint32_t minChunk) {
@@ -124,8 +151,7 @@ __global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH ==
     st.lds = s_st + threadIdx.x;
     st.stride = WBLOCK;
     const uint32_t waves = gridDim.x * (WBLOCK / 64), w = blockIdx.x * (WBLOCK / 64) + (threadIdx.x >> 6);
-    trace_chunks<false, WIDTH>(S, qin, qi, 0u, nc, w, waves, minChunk, st);
-    trace_chunks<true, WIDTH>(S, qin, qi, nc, ns, w, waves, minChunk, st);
+    trace_chunks<WIDTH>(S, qin, qi, nc, ns, w, waves, minChunk, st);
 }
 
 // Grid-stride loop over the jobs of qin whose bound is uniform per block, so every lane of a
