@@ -128,12 +128,12 @@ __device__ __forceinline__ void trace_chunks(const DScene& S, const Queue& qin, 
 // zeroes the stripe counts of qout (the next step kernel's output queue) and adds the rays to
 // the pass counters.  Each wavefront traces its chunks of the closest rays and then of the
 // shadow rays in one refill loop (trace_chunks), and the grid is what fits on the device at
-// once.  The 2- and 4-wide kernels are allowed 5 to 8 waves per SIMD; the 4-wide one needs 74
-// VGPRs and so runs at 6 (1080p frame 22.9 ms; held to exactly 5 waves, 25.6 ms).  Forcing 8
-// waves spilled 27 VGPRs on an earlier version of the traversal (1.37 against 0.96 ms per
-// launch).
+// once.  The 4-wide kernel is held to at least 7 waves per SIMD: 72 VGPRs, 8 bytes per lane
+// more of private stack than at its natural 74 VGPRs and 6 waves (1080p frame 22.75-22.82 ms at
+// 7 waves, 22.91 at 6, 23.57-23.61 at 8, 25.6 held to exactly 5).  The 2-wide kernel keeps at
+// least 5.
 template <int WIDTH>
-__global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH == 8 ? 1 : 5, 8))) void k_q_trace(
+__global__ __launch_bounds__(WBLOCK) __attribute__((amdgpu_waves_per_eu(WIDTH == 8 ? 1 : (WIDTH == 4 ? 7 : 5), 8))) void k_q_trace(
     DScene S, Queue qin, Queue qout, unsigned long long* counters, uint32_t minChunk) {
     __shared__ int s_st[SHORT_STACK * WBLOCK];
     __shared__ QIndex qi;
