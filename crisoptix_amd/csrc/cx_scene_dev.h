@@ -422,6 +422,21 @@ struct TravRay {
     Hit best;
     int ref;
     int sp;
+    int top;   // stack entry sp - 1 (valid when sp > 0); the stack holds entries 0 .. sp - 2
+};
+
+// The traversal stack of a TravRay with its top entry in a register: a pop takes the register,
+// and the stack read that refills it is needed only at the next push or pop, so it is not on
+// the path from a pop to the fetch of the popped node.
+template <typename Stack>
+struct TopStack {
+    Stack& st;
+    int& top;
+    // push of entry i (= the current depth)
+    CX_HD void put(int i, int v) {
+        if (i > 0) st.put(i - 1, top);
+        top = v;
+    }
 };
 
 template <int WIDTH>
@@ -434,6 +449,7 @@ CX_HD void trav_init(const DScene& S, TravRay& r, V3 o, V3 d, float tmin, float 
     r.best = Hit{INVALID_PRIM, tmax, 0.0f, 0.0f};
     r.ref = WIDTH == 2 ? S.root : S.wroot;
     r.sp = 0;
+    r.top = 0;
 }
 
 // One step; true when the ray is answered: closest hit in r.best (prim INVALID_PRIM: a miss),
@@ -445,8 +461,9 @@ CX_HD void trav_init(const DScene& S, TravRay& r, V3 o, V3 d, float tmin, float 
 // lanes are split between nodes and leaves so waits once per step instead of once for the node
 // fetches and again for the triangle fetches.
 template <int WIDTH, typename Stack>
-CX_HD bool trav_step_rt(const DScene& S, TravRay& r, Stack& st, bool any) {
+CX_HD bool trav_step_rt(const DScene& S, TravRay& r, Stack& stack, bool any) {
     if (S.numTris == 0) return true;
+    TopStack<Stack> st{stack, r.top};
     if constexpr (WIDTH == 4) {
         const bool inner = r.ref >= 0;
         const uint32_t code = ~(uint32_t)r.ref;
@@ -472,7 +489,8 @@ CX_HD bool trav_step_rt(const DScene& S, TravRay& r, Stack& st, bool any) {
     }
     if (r.sp == 0) return true;
     --r.sp;
-    r.ref = st.get(r.sp);
+    r.ref = r.top;
+    if (r.sp > 0) r.top = stack.get(r.sp - 1);
     return false;
 }
 
