@@ -166,6 +166,20 @@ struct TravStack {
     }
 };
 
+// A traversal stack with its top entry in a register: a pop takes the register,
+// and the stack read that refills it is needed only at the next push or pop, so it is not on
+// the path from a pop to the fetch of the popped node.
+template <typename Stack>
+struct TopStack {
+    Stack& st;
+    int& top;
+    // push of entry i (= the current depth)
+    CX_HD void put(int i, int v) {
+        if (i > 0) st.put(i - 1, top);
+        top = v;
+    }
+};
+
 // Closest-hit / any-hit leaf loop shared by the binary and wide traversals.  Returns true when
 // an any-hit query is answered.  Triangles whose skip bits hide them from the ray kind (closest
 // hit: bit 0, any hit: bit 1) are passed over, as OptiX passes over an instance whose
@@ -425,19 +439,6 @@ struct TravRay {
     int top;   // stack entry sp - 1 (valid when sp > 0); the stack holds entries 0 .. sp - 2
 };
 
-// The traversal stack of a TravRay with its top entry in a register: a pop takes the register,
-// and the stack read that refills it is needed only at the next push or pop, so it is not on
-// the path from a pop to the fetch of the popped node.
-template <typename Stack>
-struct TopStack {
-    Stack& st;
-    int& top;
-    // push of entry i (= the current depth)
-    CX_HD void put(int i, int v) {
-        if (i > 0) st.put(i - 1, top);
-        top = v;
-    }
-};
 
 template <int WIDTH>
 CX_HD void trav_init(const DScene& S, TravRay& r, V3 o, V3 d, float tmin, float tmax) {
