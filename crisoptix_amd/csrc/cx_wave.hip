@@ -1344,8 +1344,9 @@ static dim3 grid_for(uint32_t n) { return dim3((n + WBLOCK - 1) / WBLOCK); }
 static dim3 stride_grid(const Queue& q) { return dim3(queue_grid_blocks(q.jobs, q.maxBlocks)); }
 
 // Blocks of WBLOCK threads of `kernel` resident on the current device at once (its occupancy
-// times the CU count), cached per kernel and device: the grid of the kernels that cut their
-// input into one range per wavefront, whose ranges shrink as the grid grows past it.
+// times the CU count), cached per kernel and device: the grid of the kernels that share their
+// input out among their wavefronts (the traversal's chunks), whose shares shrink as the grid
+// grows past it.
 template <typename K>
 static uint32_t resident_blocks(K kernel) {
     static std::mutex mu;
