@@ -508,6 +508,10 @@ __device__ __forceinline__ uint32_t rs_block_scan(uint32_t v, uint32_t& total, u
 }
 
 // One block per digit d: exclusive scan of counts[d][0, tiles) in place; totals[d] = the sum.
+// Each thread takes RS_SCAN_ITEMS consecutive tiles, so up to 2048 tiles (2 M elements) take
+// one block-wide scan instead of a chain of them.
+static constexpr int RS_SCAN_ITEMS = 8;
+
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* counts, uint32_t stride, uint32_t cap,
                                                        const uint32_t* numValid, int first, uint32_t* totals) {
     __shared__ uint32_t s_w[RS_WAVES];
@@ -515,12 +519,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scan(uint32_t* counts, uint32
     const uint32_t tiles = (n + RS_TILE - 1) / RS_TILE;
     uint32_t* row = counts + (size_t)blockIdx.x * stride;
     uint32_t carry = 0u;
-    for (uint32_t b0 = 0; b0 < tiles; b0 += RS_THREADS) {
-        const uint32_t b = b0 + threadIdx.x;
-        const uint32_t v = b < tiles ? row[b] : 0u;
+    for (uint32_t b0 = 0; b0 < tiles; b0 += RS_THREADS * RS_SCAN_ITEMS) {
+        const uint32_t bt = b0 + threadIdx.x * RS_SCAN_ITEMS;
+        uint32_t v[RS_SCAN_ITEMS], sum = 0u;
+#pragma unroll
+        for (int j = 0; j < RS_SCAN_ITEMS; j++) {
+            v[j] = bt + j < tiles ? row[bt + j] : 0u;
+            sum += v[j];
+        }
         uint32_t total;
-        const uint32_t ex = rs_block_scan(v, total, s_w);
-        if (b < tiles) row[b] = carry + ex;
+        uint32_t run = carry + rs_block_scan(sum, total, s_w);
+#pragma unroll
+        for (int j = 0; j < RS_SCAN_ITEMS; j++) {
+            if (bt + j < tiles) row[bt + j] = run;
+            run += v[j];
+        }
         carry += total;
     }
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
