@@ -183,9 +183,11 @@ struct TopStack {
 // Closest-hit / any-hit leaf loop shared by the binary and wide traversals.  Returns true when
 // an any-hit query is answered.  Triangles whose skip bits hide them from the ray kind (closest
 // hit: bit 0, any hit: bit 1) are passed over, as OptiX passes over an instance whose
-// visibilityMask shares no bit with the ray's.
-CX_HD bool leaf_hits_rt(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float tmin, float tmax, Hit& best,
-                       bool any) {
+// visibilityMask shares no bit with the ray's.  The interval tested is [tmin, best.t]: best.t
+// starts at the ray's tmax and only falls, so a triangle behind the current closest hit is
+// rejected before its barycentric divisions, and one at exactly best.t still reaches the
+// (t, primitive) tie-break.
+CX_HD bool leaf_hits_rt(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float tmin, Hit& best, bool any) {
     const uint32_t code = ~(uint32_t)ref;
     const uint32_t first = code >> 4, cnt = code & 15u;
     // the next triangle's vertices are fetched before the current one is tested, so a leaf's
@@ -202,7 +204,7 @@ CX_HD bool leaf_hits_rt(const DScene& S, int32_t ref, V3 o, const RayPre& rp, fl
         }
         if (__builtin_bit_cast(uint32_t, p1.w) & (any ? 2u : 1u)) continue;   // invisible to this ray kind
         float t, u, v;
-        if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, tmax, t, u, v))
+        if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, best.t, t, u, v))
             continue;
         const uint32_t prim = __builtin_bit_cast(uint32_t, p0.w);
         if (any) {
@@ -221,14 +223,14 @@ CX_HD bool leaf_hits_rt(const DScene& S, int32_t ref, V3 o, const RayPre& rp, fl
 }
 
 template <bool ANY>
-CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float tmin, float tmax, Hit& best) {
-    return leaf_hits_rt(S, ref, o, rp, tmin, tmax, best, ANY);
+CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float tmin, Hit& best) {
+    return leaf_hits_rt(S, ref, o, rp, tmin, best, ANY);
 }
 
 // leaf_hits with the leaf's first triangle in q[0..2] and, when it has more than one, its second
 // in q[3..5], already loaded; later triangles are fetched one ahead as there.
 CX_HD bool leaf_hits_q(const DScene& S, uint32_t code, const float4* q, V3 o, const RayPre& rp, float tmin,
-                       float tmax, Hit& best, bool any) {
+                       Hit& best, bool any) {
     const uint32_t first = code >> 4, cnt = code & 15u;
     const float4* tv = S.tris + 3 * (size_t)first;
     float4 n0 = q[0], n1 = q[1], n2 = q[2];
@@ -248,7 +250,7 @@ CX_HD bool leaf_hits_q(const DScene& S, uint32_t code, const float4* q, V3 o, co
         }
         if (__builtin_bit_cast(uint32_t, p1.w) & (any ? 2u : 1u)) continue;   // invisible to this ray kind
         float t, u, v;
-        if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, tmax, t, u, v))
+        if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, best.t, t, u, v))
             continue;
         const uint32_t prim = __builtin_bit_cast(uint32_t, p0.w);
         if (any) {
@@ -391,7 +393,7 @@ CX_HD bool traverse_wide(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hi
         if (ref >= 0) {
             if (wide_visit<N>(S, ref, o, inv, tmin, best.t, st, sp)) continue;
         } else {
-            if (leaf_hits<ANY>(S, ref, o, rp, tmin, tmax, best)) return true;
+            if (leaf_hits<ANY>(S, ref, o, rp, tmin, best)) return true;
         }
         if (sp == 0) break;
         --sp;
@@ -416,7 +418,7 @@ CX_HD bool traverse_bin(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit
         if (ref >= 0) {
             if (bin_visit(S, ref, o, inv, tmin, best.t, st, sp)) continue;
         } else {
-            if (leaf_hits<ANY>(S, ref, o, rp, tmin, tmax, best)) return true;
+            if (leaf_hits<ANY>(S, ref, o, rp, tmin, best)) return true;
         }
         if (sp == 0) break;
         --sp;
@@ -432,8 +434,8 @@ CX_HD bool traverse_bin(const DScene& S, V3 o, V3 d, float tmin, float tmax, Hit
 struct TravRay {
     V3 o, inv;
     RayPre rp;
-    float tmin, tmax;
-    Hit best;
+    float tmin;
+    Hit best;   // best.t: the ray's tmax until a hit lowers it
     int ref;
     int sp;
     int top;   // stack entry sp - 1 (valid when sp > 0); the stack holds entries 0 .. sp - 2
@@ -446,7 +448,6 @@ CX_HD void trav_init(const DScene& S, TravRay& r, V3 o, V3 d, float tmin, float 
     r.inv = V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     r.rp = ray_pre(d);
     r.tmin = tmin;
-    r.tmax = tmax;
     r.best = Hit{INVALID_PRIM, tmax, 0.0f, 0.0f};
     r.ref = WIDTH == 2 ? S.root : S.wroot;
     r.sp = 0;
@@ -477,7 +478,7 @@ CX_HD bool trav_step_rt(const DScene& S, TravRay& r, Stack& stack, bool any) {
         if (inner) {
             if (wide_visit_q<4>(q, r.ref, r.o, r.inv, r.tmin, r.best.t, st, r.sp)) return false;
         } else {
-            if (leaf_hits_q(S, code, q, r.o, r.rp, r.tmin, r.tmax, r.best, any)) return true;
+            if (leaf_hits_q(S, code, q, r.o, r.rp, r.tmin, r.best, any)) return true;
         }
     } else if (r.ref >= 0) {
         if constexpr (WIDTH == 2) {
@@ -486,7 +487,7 @@ CX_HD bool trav_step_rt(const DScene& S, TravRay& r, Stack& stack, bool any) {
             if (wide_visit<WIDTH>(S, r.ref, r.o, r.inv, r.tmin, r.best.t, st, r.sp)) return false;
         }
     } else {
-        if (leaf_hits_rt(S, r.ref, r.o, r.rp, r.tmin, r.tmax, r.best, any)) return true;
+        if (leaf_hits_rt(S, r.ref, r.o, r.rp, r.tmin, r.best, any)) return true;
     }
     if (r.sp == 0) return true;
     --r.sp;
