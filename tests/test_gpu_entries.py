@@ -33,6 +33,26 @@ def test_neighbor_sets_bit_exact(k, orc, cx_runtime, cx_ctx):
         assert np.array_equal(do[j, :c].view(np.uint32), dg[j, :c].view(np.uint32)), j
 
 
+def test_large_set_scan_spans_several_chunks(orc, cx_runtime, cx_ctx):
+    """3 M entry slots, 80 % valid: every pass of the radix sort has more than 2048 tiles of 1024
+    keys, so its per-digit scan carries across several block-wide chunks.  The oracle's grid
+    search (pinned against brute force in test_oracle_knn.py) gives the expected sets."""
+    pos, radius, valid, q = make_entries(3_000_000, seed=5)
+    q = q[:4000]
+    assert int(valid.sum()) > 2048 * 1024
+    io, do, co = orc.entries_query(pos, radius, valid, q, 8, brute=False)
+    E = cx_runtime.Entries(cx_ctx, pos.shape[0])
+    E.build(pos, radius, valid, pos.min(0), pos.max(0))
+    ig, dg, cg = E.query(q, 8)
+    print(f"3 M entries: mean neighbours {co.mean():.2f}, full {np.mean(co == 8):.3f}")
+    assert co.mean() > 7.5
+    assert np.array_equal(co, cg)
+    for j in range(q.shape[0]):
+        c = co[j]
+        assert np.array_equal(io[j, :c], ig[j, :c]), j
+        assert np.array_equal(do[j, :c].view(np.uint32), dg[j, :c].view(np.uint32)), j
+
+
 def test_empty_and_all_invalid(cx_runtime, cx_ctx):
     pos = np.zeros((10, 3), np.float32)
     E = cx_runtime.Entries(cx_ctx, 10)
