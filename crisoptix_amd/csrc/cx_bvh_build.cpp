@@ -223,6 +223,7 @@ bool build_bvh(const float* pos, const uint32_t* idx, uint32_t numTris, uint32_t
     }
     out.root = B.build(0, numTris, 0);
     out.maxDepth = B.maxDepth;
+    out.tris.resize(out.tris.size() + TRI_PAD, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     if (out.maxDepth + 1 >= (uint32_t)STACK_DEPTH) {
         err = "BVH deeper than the traversal stack";
         return false;
@@ -460,7 +461,7 @@ bool collapse_bvh(const BvhHost& bin, int width, uint32_t stackLimit, std::vecto
               collapse_with(bin, width, true, stackLimit, out, root, stackBound);
     // Every reference must be empty, an inner node of `out` or a leaf inside the triangle slots.
     const size_t nodeWords = 2 * (size_t)width, numNodes = out.size() / nodeWords;
-    const uint64_t slots = bin.tris.size() / 3;
+    const uint64_t slots = bin.tris.size() >= TRI_PAD ? (bin.tris.size() - TRI_PAD) / 3 : 0;
     for (size_t i = 0; ok && i < numNodes; i++)
         for (int g = 0; g < width / 4; g++) {
             int32_t r4[4];

@@ -390,12 +390,15 @@ __global__ __launch_bounds__(BB) void k_td(uint32_t n, uint32_t lo, uint32_t hi,
 }
 
 // Triangle of leaf s to its slot, 3 float4 per triangle (vertex 0's w = the primitive index,
-// vertex 1's w = the skip bits, the complement of the triangle's CX_VIS_* mask).
+// vertex 1's w = the skip bits, the complement of the triangle's CX_VIS_* mask); thread 0 also
+// clears the TRI_PAD float4 after the last slot.
 __global__ __launch_bounds__(BB) void k_tri_slots(const uint32_t* vals, const uint32_t* off, uint32_t n,
                                                   const float* pos, const uint32_t* idx, const uint8_t* mask,
                                                   float4* tris, uint32_t* bad) {
     const uint32_t s = tid();
     if (s >= n) return;
+    if (s == 0)
+        for (size_t k = 0; k < TRI_PAD; k++) tris[3 * (size_t)n + k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const uint32_t t = vals[s], slot = off[s];
     if (t >= n || slot >= n) {
         atomicOr(bad, 4u);
@@ -928,7 +931,7 @@ bool build_bvh_device(const float* pos, const uint32_t* idx, const uint8_t* mask
         allocs.push_back(p);
         return p;
     };
-    out.tris = (float4*)alloc(3 * (size_t)n * sizeof(float4));
+    out.tris = (float4*)alloc((3 * (size_t)n + TRI_PAD) * sizeof(float4));
     out.nodes = (float4*)alloc(4 * (size_t)numNodes * sizeof(float4));
     if (herr != hipSuccess) return fail(herr, "allocation of the node arrays failed");
     uint32_t* maxDepth = B.scalars;

@@ -108,9 +108,14 @@ hipError_t wave_launch(int kernel, const WaveArgs& W, uint32_t threads, hipStrea
 static constexpr uint32_t DEFAULT_MAX_LEAF = 2;
 
 // Host BVH build output (binary SAH BVH in the node format of cx_scene_dev.h).
+// Zero float4 after the last triangle slot of every triangle array the traversal reads: the
+// 4-wide traversal step loads 7 float4 from a leaf's first slot whatever the leaf's size, so the
+// loads of a last leaf of one triangle end 4 float4 past the slots.
+static constexpr size_t TRI_PAD = 4;
+
 struct BvhHost {
     std::vector<float4> nodes;   // 4 per inner node
-    std::vector<float4> tris;    // 3 per triangle slot
+    std::vector<float4> tris;    // 3 per triangle slot, then TRI_PAD zero float4 (none without triangles)
     int32_t root = 0;
     uint32_t maxDepth = 0;
 };

@@ -458,8 +458,9 @@ CX_HD void trav_init(const DScene& S, TravRay& r, V3 o, V3 d, float tmin, float 
 // occlusion as r.best.prim != INVALID_PRIM.  A scene without triangles answers at once.
 //
 // At width 4 a step makes one fetch for both kinds of lane: a lane at an inner node loads the
-// node's 7 float4, a lane at a leaf its first two triangles (6 float4, the seventh load repeating
-// the last), in the same load instructions, before either kind is processed.  A wavefront whose
+// node's 7 float4, a lane at a leaf the 7 float4 from its first slot (its first two triangles
+// and one more float4; a leaf of one triangle reads the next slot's, unused), in the same load
+// instructions, before either kind is processed.  A wavefront whose
 // lanes are split between nodes and leaves so waits once per step instead of once for the node
 // fetches and again for the triangle fetches.
 template <int WIDTH, typename Stack>
@@ -469,12 +470,12 @@ CX_HD bool trav_step_rt(const DScene& S, TravRay& r, Stack& stack, bool any) {
     if constexpr (WIDTH == 4) {
         const bool inner = r.ref >= 0;
         const uint32_t code = ~(uint32_t)r.ref;
-        // a leaf holds 1..15 triangles; its loads stay within its first two
-        const uint32_t last = inner ? 6u : ((code & 15u) >= 2u ? 5u : 2u);
+        // a leaf's 7 loads run past its first two triangles into the next slots, or into the
+        // TRI_PAD float4 after the last slot; one base address, seven immediate offsets
         const float4* base = inner ? S.wnodes + (size_t)r.ref * 8 : S.tris + 3 * (size_t)(code >> 4);
         float4 q[7];
 #pragma unroll
-        for (uint32_t k = 0; k < 7; k++) q[k] = base[k < last ? k : last];
+        for (uint32_t k = 0; k < 7; k++) q[k] = base[k];
         if (inner) {
             if (wide_visit_q<4>(q, r.ref, r.o, r.inv, r.tmin, r.best.t, st, r.sp)) return false;
         } else {
