@@ -227,43 +227,25 @@ CX_HD bool leaf_hits(const DScene& S, int32_t ref, V3 o, const RayPre& rp, float
     return leaf_hits_rt(S, ref, o, rp, tmin, best, ANY);
 }
 
-// leaf_hits with the leaf's first triangle in q[0..2] and, when it has more than one, its second
-// in q[3..5], already loaded; later triangles are fetched one ahead as there.
-CX_HD bool leaf_hits_q(const DScene& S, uint32_t code, const float4* q, V3 o, const RayPre& rp, float tmin,
-                       Hit& best, bool any) {
-    const uint32_t first = code >> 4, cnt = code & 15u;
-    const float4* tv = S.tris + 3 * (size_t)first;
-    float4 n0 = q[0], n1 = q[1], n2 = q[2];
-    for (uint32_t i = 0; i < cnt; i++) {
-        const float4 p0 = n0, p1 = n1, p2 = n2;
-        if (i + 1 < cnt) {
-            if (i == 0) {
-                n0 = q[3];
-                n1 = q[4];
-                n2 = q[5];
-            } else {
-                const float4* nv = tv + 3 * (size_t)(i + 1);
-                n0 = nv[0];
-                n1 = nv[1];
-                n2 = nv[2];
-            }
-        }
-        if (__builtin_bit_cast(uint32_t, p1.w) & (any ? 2u : 1u)) continue;   // invisible to this ray kind
-        float t, u, v;
-        if (!tri_hit(o, rp, V3{p0.x, p0.y, p0.z}, V3{p1.x, p1.y, p1.z}, V3{p2.x, p2.y, p2.z}, tmin, best.t, t, u, v))
-            continue;
-        const uint32_t prim = __builtin_bit_cast(uint32_t, p0.w);
-        if (any) {
-            best.prim = prim;
-            best.t = t;
-            return true;
-        }
-        if (t < best.t || (t == best.t && prim < best.prim)) {
-            best.prim = prim;
-            best.t = t;
-            best.u = u;
-            best.v = v;
-        }
+// One triangle of a leaf, its slot already loaded in q[0..2]: leaf_hits' test of that triangle.
+// Returns true when an any-hit query is answered.
+CX_HD bool tri_hits_q(const float4* q, V3 o, const RayPre& rp, float tmin, Hit& best, bool any) {
+    if (__builtin_bit_cast(uint32_t, q[1].w) & (any ? 2u : 1u)) return false;   // invisible to this ray kind
+    float t, u, v;
+    if (!tri_hit(o, rp, V3{q[0].x, q[0].y, q[0].z}, V3{q[1].x, q[1].y, q[1].z}, V3{q[2].x, q[2].y, q[2].z}, tmin,
+                 best.t, t, u, v))
+        return false;
+    const uint32_t prim = __builtin_bit_cast(uint32_t, q[0].w);
+    if (any) {
+        best.prim = prim;
+        best.t = t;
+        return true;
+    }
+    if (t < best.t || (t == best.t && prim < best.prim)) {
+        best.prim = prim;
+        best.t = t;
+        best.u = u;
+        best.v = v;
     }
     return false;
 }
@@ -458,9 +440,10 @@ CX_HD void trav_init(const DScene& S, TravRay& r, V3 o, V3 d, float tmin, float 
 // occlusion as r.best.prim != INVALID_PRIM.  A scene without triangles answers at once.
 //
 // At width 4 a step makes one fetch for both kinds of lane: a lane at an inner node loads the
-// node's 7 float4, a lane at a leaf the 7 float4 from its first slot (its first two triangles
-// and one more float4; a leaf of one triangle reads the next slot's, unused), in the same load
-// instructions, before either kind is processed.  A wavefront whose
+// node's 7 float4, a lane at a leaf the 7 float4 from the slot of its next triangle (that
+// triangle's 3, then 4 unused), in the same load instructions, before either kind is processed.
+// A leaf step tests one triangle and leaves the lane on the rest of the leaf, so the leaf branch
+// of a wavefront split between nodes and leaves costs one triangle test per step.  A wavefront whose
 // lanes are split between nodes and leaves so waits once per step instead of once for the node
 // fetches and again for the triangle fetches.
 template <int WIDTH, typename Stack>
@@ -470,7 +453,7 @@ CX_HD bool trav_step_rt(const DScene& S, TravRay& r, Stack& stack, bool any) {
     if constexpr (WIDTH == 4) {
         const bool inner = r.ref >= 0;
         const uint32_t code = ~(uint32_t)r.ref;
-        // a leaf's 7 loads run past its first two triangles into the next slots, or into the
+        // a leaf's 7 loads run past its next triangle into the following slots, or into the
         // TRI_PAD float4 after the last slot; one base address, seven immediate offsets
         const float4* base = inner ? S.wnodes + (size_t)r.ref * 8 : S.tris + 3 * (size_t)(code >> 4);
         float4 q[7];
@@ -479,7 +462,11 @@ CX_HD bool trav_step_rt(const DScene& S, TravRay& r, Stack& stack, bool any) {
         if (inner) {
             if (wide_visit_q<4>(q, r.ref, r.o, r.inv, r.tmin, r.best.t, st, r.sp)) return false;
         } else {
-            if (leaf_hits_q(S, code, q, r.o, r.rp, r.tmin, r.best, any)) return true;
+            if (tri_hits_q(q, r.o, r.rp, r.tmin, r.best, any)) return true;
+            if ((code & 15u) > 1u) {
+                r.ref = (int32_t)~(code + 15u);   // the rest of the leaf, from its next slot
+                return false;
+            }
         }
     } else if (r.ref >= 0) {
         if constexpr (WIDTH == 2) {
